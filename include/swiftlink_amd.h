@@ -1,0 +1,158 @@
+/*
+ * swiftlink_amd: MI355X (gfx950) library for SwiftLink's MCMC linkage hot path.
+ *
+ * C ABI: plain pointers, sizes and status codes.  The library copies every
+ * descriptor at sl_create, owns all device memory, writes results into caller
+ * buffers, never aborts or exits, and keeps no global state (one context per
+ * host thread; a context serialises its own work on one HIP stream).
+ *
+ * Conventions (identical to the reference CPU path):
+ *   persons     internal order, founders first (pedigree.cc:154-162)
+ *   traits      phased order UU=0, AA=1, AU=2, UA=3 (trait.h:21-27)
+ *   genotypes   UNTYPED=0, HETERO=1, HOMOZ_A=2, HOMOZ_B=3 (genotype.h:59-64)
+ *   peel ops    CHILD=1, PARENT=2, PARTNER=3, LAST=4 (peeling.h:13-19)
+ *   graphs      host layout int32 [M][N][2] (descent_graph.h:35-37), or the
+ *               device-packed layout uint32 [M][W], W = ceil(2(N-F)/32),
+ *               bit b = 2*(person-F) + parent.
+ *   uniforms    Philox4x32-10, key (seed_lo, seed_hi), counter
+ *               (draw, step, sweep, chain); u = (x0>>5 * 2^26 + x1>>6) * 2^-53.
+ */
+#ifndef SWIFTLINK_AMD_H
+#define SWIFTLINK_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SL_OK 0
+#define SL_E_INVALID (-1)         /* bad argument or descriptor */
+#define SL_E_UNSUPPORTED (-2)     /* pedigree/peel table outside what the kernels handle */
+#define SL_E_DEVICE (-3)          /* HIP runtime error */
+#define SL_E_ZERO_LIKELIHOOD (-4) /* a peel returned likelihood <= 0 (peeler.cc:92-95) */
+#define SL_E_ILLEGAL_GRAPH (-5)   /* descent graph inconsistent with genotypes */
+#define SL_E_CAPACITY (-6)        /* request exceeds a stated maximum */
+
+/* arithmetic of the peel and sampler kernels */
+#define SL_FP64 0            /* plain double precision */
+#define SL_FP32_RESCALED 1   /* float tables with per-table power-of-two exponents */
+
+typedef struct sl_ctx sl_ctx;
+
+typedef struct {
+    int32_t n_people;
+    int32_t n_founders;
+    int32_t n_markers;
+    const int32_t* mother;        /* [N], -1 for founders */
+    const int32_t* father;        /* [N], -1 for founders */
+    const int32_t* typed;         /* [N], Person::istyped */
+    const uint8_t* genotypes;     /* [N][M] unphased genotype */
+    const double* disease_prob;   /* [N][4] Person::get_disease_prob */
+    const double* geno_prob;      /* [N][M][4] Person::get_trait_probability */
+    const uint8_t* legal;         /* [N][M] genotype elimination, bit t = phased trait t legal */
+} sl_pedigree_desc;
+
+typedef struct {
+    const double* theta;          /* [M-1] GeneticMap::get_theta */
+    const double* partial_theta;  /* [M-1] GeneticMap::get_theta_partial(l, 1) */
+    const double* minor_freq;     /* [M]   GeneticMap::get_minor */
+    int32_t lods_per_interval;    /* GeneticMap::get_lodscore_count (-n) */
+} sl_map_desc;
+
+typedef struct {
+    int32_t n_ops;
+    const int32_t* type;          /* [n_ops] PeelOperation::get_type */
+    const int32_t* peel_node;     /* [n_ops] get_peelnode */
+    const int32_t* cut_offsets;   /* [n_ops+1] CSR into cutset */
+    const int32_t* cutset;        /* get_cutset, in order */
+    const int32_t* prev_offsets;  /* [n_ops+1] */
+    const int32_t* prev;          /* get_prevfunctions */
+    const int32_t* child_offsets; /* [n_ops+1] */
+    const int32_t* children;      /* get_children */
+} sl_peel_desc;
+
+typedef struct {
+    int32_t precision;            /* SL_FP64 or SL_FP32_RESCALED */
+    int32_t device;               /* HIP device ordinal */
+    int32_t block_threads;        /* LOD kernel workgroup size; 0 = autotuned */
+    int32_t graphs_per_block;     /* graphs folded into one LOD workgroup; 0 = default */
+} sl_options;
+
+/* Replaces GPULodscores::GPULodscores + gpu_init (gpu_lodscores.h:63-80,
+ * gpu_lodscores.cc:119-161): validates and copies the pedigree, map and peel
+ * table, compiles the level-scheduled peel program and uploads it. */
+int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel_desc* peel,
+              const sl_options* opts, sl_ctx** out);
+
+/* Replaces ~GPULodscores (gpu_lodscores.h:95-99); frees everything the
+ * context allocated. */
+void sl_destroy(sl_ctx* ctx);
+
+/* Message of the last failure on ctx (or of the last failed sl_create on this
+ * thread when ctx is NULL). */
+const char* sl_last_error(const sl_ctx* ctx);
+
+/* log P(T): the trait peel with no descent graph, computed on the device.
+ * Replaces Peeler::calc_trait_prob (peeler.cc:65-73) as used at
+ * markov_chain.cc:51. */
+int sl_trait_prob(sl_ctx* ctx, double* log_trait_prob);
+
+/* Sets every accumulator to log 0 and the graph count to 0.  Replaces
+ * run_gpu_lodscoreinit_kernel (cuda_common.h:250, cuda_lodscore.cu:505-509). */
+int sl_lod_reset(sl_ctx* ctx);
+
+/* Adds n_graphs descent graphs (host, reference int32 layout [G][M][2N]) to
+ * the LOD accumulators: for every interval and trait position, the trait peel
+ * gives ln L - R - T which is log-summed into the accumulator.  Replaces
+ * GPULodscores::calculate (gpu_lodscores.h:118, gpu_lodscores.cc:598-607) and
+ * run_gpu_lodscore_kernel (cuda_common.h:248). Asynchronous w.r.t. the host
+ * after the upload. */
+int sl_lod_accumulate(sl_ctx* ctx, const int32_t* graphs, int32_t n_graphs);
+
+/* Same for graphs already on the device in the packed layout
+ * (uint32 [G][M][W]); no host transfer. */
+int sl_lod_accumulate_packed(sl_ctx* ctx, const uint32_t* dev_graphs, int32_t n_graphs);
+
+/* Waits for queued work.  Replaces GPULodscores::block_until_finished
+ * (gpu_lodscores.h:119). */
+int sl_sync(sl_ctx* ctx);
+
+/* Copies out acc_log[(M-1)*n] (natural-log accumulators), the graph count and
+ * lod[(M-1)*n] = (acc - ln count - ln P(T)) / ln 10 (lod_score.h:86-88).
+ * Any output pointer may be NULL.  Replaces GPULodscores::get_results
+ * (gpu_lodscores.h:120, gpu_lodscores.cc:621-637) with
+ * lodscorenormalise_kernel (cuda_lodscore.cu:511-516). */
+int sl_lod_results(sl_ctx* ctx, double* acc_log, int64_t* count, double* lod);
+
+/* Per-graph terms ln L - R - T for every interval and position, into
+ * out[G][M-1][n] (host).  Test hook; does not touch the accumulators.
+ * Counterpart of Peeler::process (peeler.cc:79-103) per graph. */
+int sl_lod_terms(sl_ctx* ctx, const int32_t* graphs, int32_t n_graphs, double* out);
+
+/* Kernel timing with HIP events on the context's stream (enable != 0 starts
+ * recording and clears the totals).  *ms and *launches cover the LOD peel
+ * kernel since the last enable. */
+int sl_profile(sl_ctx* ctx, int32_t enable);
+int sl_profile_read(sl_ctx* ctx, int32_t kernel, double* ms, int64_t* launches);
+
+#define SL_KERNEL_LOD 0
+#define SL_KERNEL_LSAMPLER 1
+#define SL_KERNEL_MSAMPLER 2
+
+/* Algorithmic work of one LOD evaluation (one graph, one interval, one
+ * position) from the compiled program: flops and LDS bytes. */
+int sl_lod_work(sl_ctx* ctx, double* flops_per_eval, double* lds_bytes_per_eval);
+
+/* Compiles a peel table into the level-scheduled program the LOD kernel
+ * runs and serialises it into buf (int32 words; *needed gets the length).
+ * Host only, no device work: lets the program's indexing be checked on a
+ * machine without a GPU.  No counterpart in the reference. */
+int sl_program_export(const sl_pedigree_desc* ped, const sl_peel_desc* peel, int32_t npos, int32_t* buf,
+                      int64_t cap, int64_t* needed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -569,32 +569,55 @@ static void cmd_graphs(Case& c, const std::string& out, uint64_t seed, int G, in
     write_bin(out + "/graphs.i32", &all[0], all.size() * 4);
 }
 
-// CPU baseline timing of the reference path with OpenMP threads:
-// MarkovChain::step with burn-in beyond the window (pure L or pure M sweeps),
-// and Peeler::process over all intervals with one Peeler per thread as in
-// MarkovChain::run (markov_chain.cc:375-383).
+// CPU baseline timing of the reference path with OpenMP threads.
+//  L-sweeps: one LocusSampler per thread, loci of one parity in an omp for,
+//            then the other parity (non-adjacent loci run concurrently, as in
+//            MarkovChain::run_old_lsampler, markov_chain.cc:228-267).
+//  M-sweeps: MeiosisSampler reset + one step per non-ignorable meiosis; its
+//            per-locus likelihoods are OpenMP loops (meiosis_sampler.cc:21-37).
+//  LOD:      one Peeler per thread over the intervals (markov_chain.cc:375-383).
+// (MarkovChain::step is not used: its thread_num is shared between threads.)
 static void cmd_bench(Case& c, const std::string& out, const std::string& gfile, int threads, int nL, int nM,
                       int nG) {
     omp_set_num_threads(threads);
     std::vector<char> gb = read_bin(gfile);
     DescentGraph dg(c.p, c.m, c.sex_linked);
     load_graph(dg, (const int32_t*)&gb[0], c.N, c.M);
-    struct mcmc_options o = c.h->opts();
-    o.burnin = 1 << 30;
-    o.sex_linked = c.sex_linked;
     double tL = 0, tM = 0, tG = 0;
     if (nL > 0) {
-        o.lsampler_prob = 1.0;
-        MarkovChain mc(c.p, c.m, c.psg, o, 0);
+        std::vector<LocusSampler*> ls;
+        for (int t = 0; t < threads; ++t) ls.push_back(new LocusSampler(c.p, c.m, c.psg, 0, c.sex_linked));
         double t0 = wall();
-        mc.step(dg, 0, nL);
+        for (int s = 0; s < nL; ++s)
+            for (int parity = 0; parity < 2; ++parity) {
+#pragma omp parallel
+                {
+                    int tn = omp_get_thread_num();
+                    stream_set(4242, (uint32_t)tn, (uint32_t)s, 0);
+#pragma omp for schedule(dynamic, 4)
+                    for (int l = parity; l < c.M; l += 2) {
+                        g_pos.step = (uint32_t)l;
+                        g_pos.draw = 0;
+                        ls[tn]->set_locus_minimal(l);
+                        ls[tn]->step(dg, l);
+                    }
+                }
+            }
         tL = wall() - t0;
+        for (int t = 0; t < threads; ++t) delete ls[t];
     }
     if (nM > 0) {
-        o.lsampler_prob = 0.0;
-        MarkovChain mc(c.p, c.m, c.psg, o, 0);
+        MeiosisSampler ms(c.p, c.m, c.sex_linked);
+        std::vector<int> mo = meiosis_list(c);
         double t0 = wall();
-        mc.step(dg, 0, nM);
+        for (int s = 0; s < nM; ++s) {
+            sched_mshuffle(4242, 0, (uint32_t)s, mo);
+            ms.reset(dg, mo[0]);
+            for (size_t j = 0; j < mo.size(); ++j) {
+                stream_set(4242, 0, (uint32_t)s, (uint32_t)j);
+                ms.step(dg, mo[j]);
+            }
+        }
         tM = wall() - t0;
     }
     if (nG > 0) {

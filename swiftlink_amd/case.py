@@ -170,3 +170,16 @@ class Case:
                 other = self.father[ch] if self.mother[ch] == p else self.mother[ch]
                 if other not in cut:
                     raise ValueError(f"op {o}: child {ch}'s other parent is not in the cutset")
+
+
+def pack_graphs(case: "Case", graphs: np.ndarray) -> np.ndarray:
+    """Reference-layout graphs int32 [G][M][2N] -> the device layout uint32
+    [G][M][W], bit b = 2*(person-F) + parent, W = ceil(2(N-F)/32)."""
+    g = np.asarray(graphs).reshape(-1, case.M, 2 * case.N)
+    bits = (g[:, :, 2 * case.F:] & 1).astype(np.uint8)
+    nm = 2 * (case.N - case.F)
+    W = (nm + 31) // 32
+    pad = np.zeros((g.shape[0], case.M, 32 * W), np.uint8)
+    pad[:, :, :nm] = bits
+    packed = np.packbits(pad, axis=2, bitorder="little")
+    return np.ascontiguousarray(packed).view(np.uint32).reshape(g.shape[0], case.M, W)

@@ -1,0 +1,474 @@
+// C-ABI layer: context lifetime, descriptor validation, device buffers and
+// kernel launches.  Every entry point returns a status code; nothing aborts.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/swiftlink_amd.h"
+#include "kernels.h"
+#include "program.h"
+
+using sl::PeelProgram;
+
+namespace {
+thread_local std::string g_create_error;
+}
+
+struct sl_ctx {
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int precision = SL_FP64;
+    int N = 0, F = 0, M = 0, W = 0, npos = 0;
+    PeelProgram prog;
+    std::vector<double> theta, ptheta, minor;
+    // device program
+    sl::OpDesc* d_ops = nullptr;
+    sl::PrevDesc* d_prevs = nullptr;
+    sl::ChildDesc* d_children = nullptr;
+    uint32_t* d_tasks = nullptr;
+    int32_t* d_level_base = nullptr;
+    double* d_trait_d = nullptr;
+    float* d_trait_f = nullptr;
+    double* d_ptheta = nullptr;
+    double* d_logt = nullptr;
+    double* d_loga = nullptr;
+    double* d_acc = nullptr;
+    double* d_scalar = nullptr;
+    // growable work buffers
+    double* d_partial = nullptr;
+    size_t partial_cap = 0;
+    uint32_t* d_packed = nullptr;
+    size_t packed_cap = 0;
+    int32_t* d_stage = nullptr;
+    size_t stage_cap = 0;
+    double* d_terms = nullptr;
+    size_t terms_cap = 0;
+    int64_t count = 0;
+    double trait_prob = 0.0;
+    bool trait_prob_valid = false;
+    int block = 128;
+    int gpw = 4;
+    size_t lds = 0;
+    int arena_bytes = 0, maxbits_bytes = 0;
+    // profiling
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+    double prof_ms[3] = {0, 0, 0};
+    int64_t prof_n[3] = {0, 0, 0};
+};
+
+namespace {
+
+int fail(sl_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+int hip_fail(sl_ctx* c, hipError_t e, const char* where) {
+    return fail(c, SL_E_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define SL_HIP(c, call)                                      \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return hip_fail((c), e_, #call); \
+    } while (0)
+
+template <typename T>
+int upload(sl_ctx* c, T** dst, const T* src, size_t n) {
+    if (n == 0) n = 1;
+    SL_HIP(c, hipMalloc((void**)dst, n * sizeof(T)));
+    if (src) SL_HIP(c, hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return SL_OK;
+}
+
+template <typename T>
+int ensure(sl_ctx* c, T** buf, size_t* cap, size_t n) {
+    if (n <= *cap) return SL_OK;
+    if (*buf) SL_HIP(c, hipFree(*buf));
+    *buf = nullptr;
+    SL_HIP(c, hipMalloc((void**)buf, n * sizeof(T)));
+    *cap = n;
+    return SL_OK;
+}
+
+sl::LodArgs lod_args(sl_ctx* c) {
+    sl::LodArgs A{};
+    A.ops = c->d_ops;
+    A.prevs = c->d_prevs;
+    A.children = c->d_children;
+    A.tasks = c->d_tasks;
+    A.level_base = c->d_level_base;
+    A.n_levels = c->prog.n_levels;
+    A.last_op = c->prog.last_op;
+    A.n_ops = c->prog.n_ops;
+    A.npos = c->npos;
+    A.trait_d = c->d_trait_d;
+    A.trait_f = c->d_trait_f;
+    A.partial_theta = c->d_ptheta;
+    A.log_theta = c->d_logt;
+    A.log_anti = c->d_loga;
+    A.N = c->N;
+    A.F = c->F;
+    A.M = c->M;
+    A.W = c->W;
+    A.trans = std::log(0.5) * (2.0 * (c->N - c->F));
+    A.gpw = c->gpw;
+    A.arena_bytes = c->arena_bytes;
+    A.maxbits_bytes = c->maxbits_bytes;
+    return A;
+}
+
+hipEvent_t take_event(sl_ctx* c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int check_desc(const sl_pedigree_desc* p, const sl_map_desc* m, const sl_peel_desc* k) {
+    if (!p || !m || !k) return fail(nullptr, SL_E_INVALID, "null descriptor");
+    if (p->n_people < 1 || p->n_founders < 1 || p->n_founders > p->n_people)
+        return fail(nullptr, SL_E_INVALID, "bad person or founder count");
+    if (p->n_markers < 2) return fail(nullptr, SL_E_INVALID, "need at least two markers");
+    if (!p->mother || !p->father || !p->typed || !p->genotypes || !p->disease_prob || !p->geno_prob || !p->legal)
+        return fail(nullptr, SL_E_INVALID, "null pedigree array");
+    if (!m->theta || !m->partial_theta || !m->minor_freq) return fail(nullptr, SL_E_INVALID, "null map array");
+    if (m->lods_per_interval < 1 || m->lods_per_interval > sl::kMaxPos)
+        return fail(nullptr, SL_E_INVALID, "lods_per_interval outside 1..16");
+    if (k->n_ops != p->n_people) return fail(nullptr, SL_E_INVALID, "peel table must have one op per person");
+    const int N = p->n_people, F = p->n_founders;
+    for (int i = 0; i < N; ++i) {
+        const bool founder = i < F;
+        if (founder && (p->mother[i] != -1 || p->father[i] != -1))
+            return fail(nullptr, SL_E_INVALID, "founders must come first and have no parents");
+        if (!founder && (p->mother[i] < 0 || p->mother[i] >= N || p->father[i] < 0 || p->father[i] >= N ||
+                         p->mother[i] == i || p->father[i] == i))
+            return fail(nullptr, SL_E_INVALID, "non-founder with a missing or out-of-range parent");
+    }
+    for (int l = 0; l + 1 < p->n_markers; ++l)
+        if (!(m->theta[l] > 0.0 && m->theta[l] < 0.5 + 1e-12))
+            return fail(nullptr, SL_E_INVALID, "recombination fraction must lie in (0, 0.5]");
+    return SL_OK;
+}
+
+int release_all(sl_ctx* c) {
+    void* ptrs[] = {c->d_ops, c->d_prevs, c->d_children, c->d_tasks, c->d_level_base, c->d_trait_d,
+                    c->d_trait_f, c->d_ptheta, c->d_logt, c->d_loga, c->d_acc, c->d_scalar,
+                    c->d_partial, c->d_packed, c->d_stage, c->d_terms};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
+    for (auto& u : c->ev_used) {
+        (void)hipEventDestroy(u.second.first);
+        (void)hipEventDestroy(u.second.second);
+    }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    return SL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sl_last_error(const sl_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel_desc* peel,
+              const sl_options* opts, sl_ctx** out) {
+    if (!out) return fail(nullptr, SL_E_INVALID, "null output pointer");
+    *out = nullptr;
+    int rc = check_desc(ped, map, peel);
+    if (rc) return rc;
+    sl_ctx* c = new sl_ctx();
+    c->precision = opts ? opts->precision : SL_FP64;
+    if (c->precision != SL_FP64 && c->precision != SL_FP32_RESCALED) {
+        delete c;
+        return fail(nullptr, SL_E_INVALID, "unknown precision");
+    }
+    c->device = opts ? opts->device : 0;
+    c->N = ped->n_people;
+    c->F = ped->n_founders;
+    c->M = ped->n_markers;
+    c->W = (2 * (c->N - c->F) + 31) / 32;
+    c->npos = map->lods_per_interval;
+    if (opts && opts->block_threads > 0) c->block = opts->block_threads;
+    if (opts && opts->graphs_per_block > 0) c->gpw = opts->graphs_per_block;
+
+    sl::PedigreeView pv{c->N, c->F, ped->mother, ped->father, ped->disease_prob};
+    sl::PeelView kv{peel->n_ops, peel->type, peel->peel_node, peel->cut_offsets, peel->cutset,
+                    peel->prev_offsets, peel->prev, peel->child_offsets, peel->children};
+    std::string why = sl::compile_lod_program(pv, kv, c->npos, &c->prog);
+    if (!why.empty()) {
+        delete c;
+        return fail(nullptr, SL_E_UNSUPPORTED, why);
+    }
+    const size_t esz = c->precision == SL_FP64 ? 8 : 4;
+    c->arena_bytes = (int)((c->prog.arena_elems * esz + 15) / 16 * 16);
+    c->maxbits_bytes = c->precision == SL_FP64 ? 0 : (int)((c->prog.n_ops * c->npos * 4 + 15) / 16 * 16);
+    c->lds = (size_t)c->arena_bytes + c->maxbits_bytes + ((c->N + 15) / 16 * 16);
+    if (c->lds > 160 * 1024) {
+        delete c;
+        return fail(nullptr, SL_E_UNSUPPORTED, "peel program needs more than 160 KiB of LDS");
+    }
+
+    c->theta.assign(map->theta, map->theta + c->M - 1);
+    c->ptheta.assign(map->partial_theta, map->partial_theta + c->M - 1);
+    c->minor.assign(map->minor_freq, map->minor_freq + c->M);
+
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) {
+        std::string m = std::string("HIP device setup failed: ") + hipGetErrorString(e);
+        release_all(c);
+        delete c;
+        return fail(nullptr, SL_E_DEVICE, m);
+    }
+    e = sl::configure_lod_lds(c->precision, c->lds);
+    if (e != hipSuccess) {
+        std::string m = std::string("LDS configuration failed: ") + hipGetErrorString(e);
+        release_all(c);
+        delete c;
+        return fail(nullptr, SL_E_DEVICE, m);
+    }
+    std::vector<double> logt(c->M - 1), loga(c->M - 1);
+    for (int l = 0; l + 1 < c->M; ++l) {
+        logt[l] = std::log(c->theta[l]);
+        loga[l] = std::log(1.0 - c->theta[l]);
+    }
+    std::vector<float> tf(4 * (size_t)c->N);
+    for (size_t i = 0; i < tf.size(); ++i) tf[i] = (float)ped->disease_prob[i];
+    const int nacc = (c->M - 1) * c->npos;
+    if ((rc = upload(c, &c->d_ops, c->prog.ops.data(), c->prog.ops.size())) ||
+        (rc = upload(c, &c->d_prevs, c->prog.prevs.data(), c->prog.prevs.size())) ||
+        (rc = upload(c, &c->d_children, c->prog.children.data(), c->prog.children.size())) ||
+        (rc = upload(c, &c->d_tasks, c->prog.tasks.data(), c->prog.tasks.size())) ||
+        (rc = upload(c, &c->d_level_base, c->prog.level_base.data(), c->prog.level_base.size())) ||
+        (rc = upload(c, &c->d_trait_d, ped->disease_prob, 4 * (size_t)c->N)) ||
+        (rc = upload(c, &c->d_trait_f, tf.data(), tf.size())) ||
+        (rc = upload(c, &c->d_ptheta, c->ptheta.data(), c->ptheta.size())) ||
+        (rc = upload(c, &c->d_logt, logt.data(), logt.size())) ||
+        (rc = upload(c, &c->d_loga, loga.data(), loga.size())) ||
+        (rc = upload<double>(c, &c->d_acc, nullptr, nacc)) || (rc = upload<double>(c, &c->d_scalar, nullptr, 4))) {
+        std::string m = c->err;
+        release_all(c);
+        delete c;
+        return fail(nullptr, rc, m);
+    }
+    e = sl::launch_fill(c->d_acc, -INFINITY, nacc, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        std::string m = std::string("initialisation failed: ") + hipGetErrorString(e);
+        release_all(c);
+        delete c;
+        return fail(nullptr, SL_E_DEVICE, m);
+    }
+    *out = c;
+    return SL_OK;
+}
+
+void sl_destroy(sl_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    release_all(ctx);
+    delete ctx;
+}
+
+int sl_sync(sl_ctx* c) {
+    if (!c) return SL_E_INVALID;
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    return SL_OK;
+}
+
+int sl_trait_prob(sl_ctx* c, double* out) {
+    if (!c || !out) return SL_E_INVALID;
+    if (!c->trait_prob_valid) {
+        sl::LodArgs A = lod_args(c);
+        SL_HIP(c, sl::launch_trait_prob(A, c->precision, c->block, c->lds, c->d_scalar, c->stream));
+        SL_HIP(c, hipMemcpyAsync(&c->trait_prob, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        SL_HIP(c, hipStreamSynchronize(c->stream));
+        c->trait_prob_valid = true;
+    }
+    *out = c->trait_prob;
+    return SL_OK;
+}
+
+int sl_lod_reset(sl_ctx* c) {
+    if (!c) return SL_E_INVALID;
+    SL_HIP(c, sl::launch_fill(c->d_acc, -INFINITY, (c->M - 1) * c->npos, c->stream));
+    c->count = 0;
+    return SL_OK;
+}
+
+static int lod_run(sl_ctx* c, const uint32_t* dev_graphs, int G, double* terms_dev) {
+    const int nchunks = (G + c->gpw - 1) / c->gpw;
+    const size_t nacc = (size_t)(c->M - 1) * c->npos;
+    int rc = ensure(c, &c->d_partial, &c->partial_cap, nacc * nchunks);
+    if (rc) return rc;
+    sl::LodArgs A = lod_args(c);
+    A.graphs = dev_graphs;
+    A.G = G;
+    A.partial = c->d_partial;
+    A.terms = terms_dev;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->prof) {
+        e0 = take_event(c);
+        e1 = take_event(c);
+        if (e0 && e1) SL_HIP(c, hipEventRecord(e0, c->stream));
+    }
+    SL_HIP(c, sl::launch_lod(A, c->precision, c->block, c->lds, nchunks, c->stream));
+    if (c->prof && e0 && e1) {
+        SL_HIP(c, hipEventRecord(e1, c->stream));
+        c->ev_used.push_back({SL_KERNEL_LOD, {e0, e1}});
+    }
+    if (!terms_dev) SL_HIP(c, sl::launch_lod_merge(c->d_acc, c->d_partial, nchunks, (int)nacc, c->stream));
+    return SL_OK;
+}
+
+static int stage_graphs(sl_ctx* c, const int32_t* graphs, int G) {
+    const size_t nin = (size_t)G * c->M * 2 * c->N;
+    const size_t nout = (size_t)G * c->M * c->W;
+    int rc = ensure(c, &c->d_stage, &c->stage_cap, nin);
+    if (rc) return rc;
+    rc = ensure(c, &c->d_packed, &c->packed_cap, nout);
+    if (rc) return rc;
+    SL_HIP(c, hipMemcpyAsync(c->d_stage, graphs, nin * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    SL_HIP(c, sl::launch_pack(c->d_stage, c->d_packed, G, c->M, c->N, c->F, c->W, c->stream));
+    return SL_OK;
+}
+
+int sl_lod_accumulate(sl_ctx* c, const int32_t* graphs, int32_t G) {
+    if (!c || (!graphs && G > 0) || G < 0) return SL_E_INVALID;
+    if (G == 0) return SL_OK;
+    int rc = stage_graphs(c, graphs, G);
+    if (rc) return rc;
+    rc = lod_run(c, c->d_packed, G, nullptr);
+    if (rc) return rc;
+    c->count += G;
+    return SL_OK;
+}
+
+int sl_lod_accumulate_packed(sl_ctx* c, const uint32_t* dev_graphs, int32_t G) {
+    if (!c || (!dev_graphs && G > 0) || G < 0) return SL_E_INVALID;
+    if (G == 0) return SL_OK;
+    int rc = lod_run(c, dev_graphs, G, nullptr);
+    if (rc) return rc;
+    c->count += G;
+    return SL_OK;
+}
+
+int sl_lod_results(sl_ctx* c, double* acc_log, int64_t* count, double* lod) {
+    if (!c) return SL_E_INVALID;
+    const size_t n = (size_t)(c->M - 1) * c->npos;
+    std::vector<double> acc(n);
+    SL_HIP(c, hipMemcpyAsync(acc.data(), c->d_acc, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    if (acc_log) memcpy(acc_log, acc.data(), n * sizeof(double));
+    if (count) *count = c->count;
+    if (lod) {
+        double tp = 0.0;
+        int rc = sl_trait_prob(c, &tp);
+        if (rc) return rc;
+        const double lc = std::log((double)c->count);
+        for (size_t i = 0; i < n; ++i) lod[i] = (acc[i] - lc - tp) / std::log(10.0);
+    }
+    return SL_OK;
+}
+
+int sl_lod_terms(sl_ctx* c, const int32_t* graphs, int32_t G, double* out) {
+    if (!c || !graphs || !out || G <= 0) return SL_E_INVALID;
+    int rc = stage_graphs(c, graphs, G);
+    if (rc) return rc;
+    const size_t n = (size_t)G * (c->M - 1) * c->npos;
+    rc = ensure(c, &c->d_terms, &c->terms_cap, n);
+    if (rc) return rc;
+    rc = lod_run(c, c->d_packed, G, c->d_terms);
+    if (rc) return rc;
+    SL_HIP(c, hipMemcpyAsync(out, c->d_terms, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    return SL_OK;
+}
+
+int sl_profile(sl_ctx* c, int32_t enable) {
+    if (!c) return SL_E_INVALID;
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    for (auto& u : c->ev_used) {
+        c->ev_pool.push_back(u.second.first);
+        c->ev_pool.push_back(u.second.second);
+    }
+    c->ev_used.clear();
+    for (int i = 0; i < 3; ++i) {
+        c->prof_ms[i] = 0.0;
+        c->prof_n[i] = 0;
+    }
+    c->prof = enable != 0;
+    return SL_OK;
+}
+
+int sl_profile_read(sl_ctx* c, int32_t kernel, double* ms, int64_t* launches) {
+    if (!c || kernel < 0 || kernel > 2) return SL_E_INVALID;
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    for (auto& u : c->ev_used) {
+        float t = 0.0f;
+        SL_HIP(c, hipEventElapsedTime(&t, u.second.first, u.second.second));
+        c->prof_ms[u.first] += t;
+        c->prof_n[u.first] += 1;
+        c->ev_pool.push_back(u.second.first);
+        c->ev_pool.push_back(u.second.second);
+    }
+    c->ev_used.clear();
+    if (ms) *ms = c->prof_ms[kernel];
+    if (launches) *launches = c->prof_n[kernel];
+    return SL_OK;
+}
+
+int sl_lod_work(sl_ctx* c, double* flops, double* lds_bytes) {
+    if (!c) return SL_E_INVALID;
+    if (flops) *flops = c->prog.flops_per_eval;
+    if (lds_bytes) *lds_bytes = c->prog.lds_bytes_per_eval * (c->precision == SL_FP64 ? 8.0 : 4.0);
+    return SL_OK;
+}
+
+int sl_program_export(const sl_pedigree_desc* ped, const sl_peel_desc* peel, int32_t npos, int32_t* buf,
+                      int64_t cap, int64_t* needed) {
+    if (!ped || !peel || !needed) return fail(nullptr, SL_E_INVALID, "null argument");
+    sl::PeelProgram P;
+    sl::PedigreeView pv{ped->n_people, ped->n_founders, ped->mother, ped->father, ped->disease_prob};
+    sl::PeelView kv{peel->n_ops, peel->type, peel->peel_node, peel->cut_offsets, peel->cutset,
+                    peel->prev_offsets, peel->prev, peel->child_offsets, peel->children};
+    std::string why = sl::compile_lod_program(pv, kv, npos, &P);
+    if (!why.empty()) return fail(nullptr, SL_E_UNSUPPORTED, why);
+    std::vector<int32_t> o;
+    o.push_back(P.n_ops);
+    o.push_back(P.n_levels);
+    o.push_back(P.last_op);
+    o.push_back(P.arena_elems);
+    o.push_back((int32_t)P.prevs.size());
+    o.push_back((int32_t)P.children.size());
+    o.push_back((int32_t)P.tasks.size());
+    const int32_t* w = reinterpret_cast<const int32_t*>(P.ops.data());
+    o.insert(o.end(), w, w + P.ops.size() * (sizeof(sl::OpDesc) / 4));
+    w = reinterpret_cast<const int32_t*>(P.prevs.data());
+    o.insert(o.end(), w, w + P.prevs.size() * (sizeof(sl::PrevDesc) / 4));
+    w = reinterpret_cast<const int32_t*>(P.children.data());
+    o.insert(o.end(), w, w + P.children.size() * (sizeof(sl::ChildDesc) / 4));
+    for (uint32_t t : P.tasks) o.push_back((int32_t)t);
+    o.insert(o.end(), P.level_base.begin(), P.level_base.end());
+    *needed = (int64_t)o.size();
+    if (buf && cap >= (int64_t)o.size()) memcpy(buf, o.data(), o.size() * 4);
+    return SL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,40 @@
+// Kernel argument blocks and launchers shared by the API layer and the
+// kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "program.h"
+
+namespace sl {
+
+struct LodArgs {
+    const OpDesc* ops;
+    const PrevDesc* prevs;
+    const ChildDesc* children;
+    const uint32_t* tasks;
+    const int32_t* level_base;
+    int n_levels, last_op, n_ops, npos;
+    const double* trait_d;       // [N][4] disease probabilities (FP64 program)
+    const float* trait_f;        // [N][4] (FP32 program)
+    const double* partial_theta; // [M-1]
+    const double* log_theta;     // [M-1] ln theta
+    const double* log_anti;      // [M-1] ln(1 - theta)
+    int N, F, M, W;
+    double trans;                // 2(N-F) ln 1/2
+    const uint32_t* graphs;      // packed [G][M][W]
+    int G, gpw;
+    double* partial;             // [chunks][M-1][npos] chunk log-sums
+    double* terms;               // optional [G][M-1][npos]
+    int arena_bytes, maxbits_bytes;
+};
+
+hipError_t launch_lod(const LodArgs& A, int precision, int block, size_t lds, int nchunks, hipStream_t s);
+hipError_t launch_trait_prob(const LodArgs& A, int precision, int block, size_t lds, double* out, hipStream_t s);
+hipError_t launch_lod_merge(double* acc, const double* partial, int nchunks, int n, hipStream_t s);
+hipError_t launch_fill(double* p, double v, int n, hipStream_t s);
+hipError_t configure_lod_lds(int precision, size_t lds);
+hipError_t launch_pack(const int32_t* in, uint32_t* out, int G, int M, int N, int F, int W, hipStream_t s);
+
+}  // namespace sl

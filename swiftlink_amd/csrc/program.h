@@ -1,0 +1,106 @@
+// Peel programs: the host-compiled form of a peel-operation table that the
+// trait-peel (LOD) kernels execute.
+//
+// The reference evaluates each PeelOperation with per-cell virtual dispatch
+// and std::vector<int> assignment lookups (rfunction.cc:144-198,
+// peel_matrix.h:37-49).  Here the operation table is compiled once into flat
+// arrays:
+//   * ops are grouped into levels (an op's level is one more than the deepest
+//     table it consumes), so a workgroup evaluates every op of a level in one
+//     pass and needs one barrier per level, not per op;
+//   * each level has a task list: one 32-bit word (op << 16 | dense cell) per
+//     evaluated cell, in ascending cell order per op (the reference's
+//     lod_indices order, peel_sequence_generator.cc:167-186);
+//   * tables are stored compactly in a mixed-radix layout over the legal trait
+//     values of each cutset member (digit 0 fastest), so a table holds exactly
+//     the cells the reference evaluates;
+//   * consumers address a previous table by a per-(op, prev) digit map:
+//     (source digit, rank byte of that person, stride), no search;
+//   * tables live in an LDS arena whose slots are reused once the consuming
+//     level has run.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace sl {
+
+constexpr int kMaxCut = 7;       // cutset size limit: k+1 digits fit 16 bits
+constexpr int kMaxPrev = 8;
+constexpr int kMaxPos = 16;      // trait positions per interval
+
+enum OpType : int32_t { kChild = 1, kParent = 2, kPartner = 3, kLast = 4 };
+
+struct OpDesc {
+    int32_t type;
+    int32_t k;            // cutset size
+    int32_t peel;         // peel-node person
+    int32_t n_prev;
+    int32_t tab_off;      // arena offset (elements) of position 0
+    int32_t size;         // compact cells per position
+    int32_t mat_digit;    // CHILD: cutset digit of the mother
+    int32_t pat_digit;    // CHILD: cutset digit of the father
+    int32_t n_child;      // PARENT
+    int32_t child_off;    // into children[]
+    int32_t prev_off;     // into prevs[]
+    int32_t task_base;    // global index of the op's first task
+};
+
+struct PrevDesc {
+    int32_t op;
+    int32_t tab_off;
+    int32_t size;
+    int32_t n_digits;
+    // word: src digit (bits 0-3) | rank byte (bits 4-11) | stride (bits 12-31)
+    uint32_t digit[kMaxCut + 1];
+};
+
+struct ChildDesc {
+    int32_t person;
+    int32_t child_digit;
+    int32_t other_digit;
+    int32_t peel_is_mother;
+};
+
+// Host-side compiled program.
+struct PeelProgram {
+    int n_people = 0, n_founders = 0;
+    int n_ops = 0;
+    int n_levels = 0;
+    int last_op = -1;
+    std::vector<OpDesc> ops;
+    std::vector<PrevDesc> prevs;
+    std::vector<ChildDesc> children;
+    std::vector<uint32_t> tasks;        // all levels, concatenated
+    std::vector<int32_t> level_base;    // [n_levels+1] into tasks
+    std::vector<uint8_t> lod_mask;      // [N] trait values with non-zero disease prob
+    int arena_elems = 0;                // per position-set (all positions)
+    double flops_per_eval = 0.0;        // algorithmic flops per (graph, interval, position)
+    double lds_bytes_per_eval = 0.0;    // table reads per evaluation, in elements
+};
+
+struct PedigreeView {
+    int N, F;
+    const int32_t* mother;
+    const int32_t* father;
+    const double* disease_prob;  // [N][4]
+};
+
+struct PeelView {
+    int n_ops;
+    const int32_t* type;
+    const int32_t* peel;
+    const int32_t* cut_off;
+    const int32_t* cut;
+    const int32_t* prev_off;
+    const int32_t* prev;
+    const int32_t* child_off;
+    const int32_t* child;
+};
+
+// Compiles `peel` for `ped` with `npos` positions folded into each table slot.
+// Returns an empty string on success, else the reason the table is rejected.
+std::string compile_lod_program(const PedigreeView& ped, const PeelView& peel, int npos, PeelProgram* out);
+
+}  // namespace sl

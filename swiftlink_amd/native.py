@@ -1,0 +1,221 @@
+"""ctypes binding of libswiftlink_amd.so (the C ABI in include/swiftlink_amd.h).
+
+There is no fallback: if the library is missing or a call fails, this module
+raises.  The library is built in-tree by `make -C swiftlink_amd` (or
+__graft_entry__.build()).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from .case import Case
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libswiftlink_amd.so")
+
+SL_FP64 = 0
+SL_FP32_RESCALED = 1
+SL_KERNEL_LOD, SL_KERNEL_LSAMPLER, SL_KERNEL_MSAMPLER = 0, 1, 2
+
+i32p = C.POINTER(C.c_int32)
+u32p = C.POINTER(C.c_uint32)
+u8p = C.POINTER(C.c_uint8)
+f64p = C.POINTER(C.c_double)
+i64p = C.POINTER(C.c_int64)
+
+
+class PedigreeDesc(C.Structure):
+    _fields_ = [("n_people", C.c_int32), ("n_founders", C.c_int32), ("n_markers", C.c_int32),
+                ("mother", i32p), ("father", i32p), ("typed", i32p), ("genotypes", u8p),
+                ("disease_prob", f64p), ("geno_prob", f64p), ("legal", u8p)]
+
+
+class MapDesc(C.Structure):
+    _fields_ = [("theta", f64p), ("partial_theta", f64p), ("minor_freq", f64p), ("lods_per_interval", C.c_int32)]
+
+
+class PeelDesc(C.Structure):
+    _fields_ = [("n_ops", C.c_int32), ("type", i32p), ("peel_node", i32p), ("cut_offsets", i32p),
+                ("cutset", i32p), ("prev_offsets", i32p), ("prev", i32p), ("child_offsets", i32p),
+                ("children", i32p)]
+
+
+class Options(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("device", C.c_int32), ("block_threads", C.c_int32),
+                ("graphs_per_block", C.c_int32)]
+
+
+class SlError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "sl_create": (C.c_int, [C.POINTER(PedigreeDesc), C.POINTER(MapDesc), C.POINTER(PeelDesc), C.POINTER(Options),
+                            C.POINTER(C.c_void_p)]),
+    "sl_destroy": (None, [C.c_void_p]),
+    "sl_last_error": (C.c_char_p, [C.c_void_p]),
+    "sl_trait_prob": (C.c_int, [C.c_void_p, f64p]),
+    "sl_lod_reset": (C.c_int, [C.c_void_p]),
+    "sl_lod_accumulate": (C.c_int, [C.c_void_p, i32p, C.c_int32]),
+    "sl_lod_accumulate_packed": (C.c_int, [C.c_void_p, u32p, C.c_int32]),
+    "sl_sync": (C.c_int, [C.c_void_p]),
+    "sl_lod_results": (C.c_int, [C.c_void_p, f64p, i64p, f64p]),
+    "sl_lod_terms": (C.c_int, [C.c_void_p, i32p, C.c_int32, f64p]),
+    "sl_profile": (C.c_int, [C.c_void_p, C.c_int32]),
+    "sl_profile_read": (C.c_int, [C.c_void_p, C.c_int32, f64p, i64p]),
+    "sl_lod_work": (C.c_int, [C.c_void_p, f64p, f64p]),
+    "sl_program_export": (C.c_int, [C.POINTER(PedigreeDesc), C.POINTER(PeelDesc), C.c_int32, i32p, C.c_int64,
+                                    i64p]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise SlError(f"{LIBPATH} is not built (run `make -C {HERE}` or __graft_entry__.build())")
+        l = C.CDLL(LIBPATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+class _Descs:
+    """Holds numpy buffers alive while descriptors point at them."""
+
+    def __init__(self, case: Case):
+        c = case
+        z = lambda a, dt: np.ascontiguousarray(a, dtype=dt) if np.size(a) else np.zeros(1, dt)
+        self.bufs = dict(
+            mother=z(c.mother, np.int32), father=z(c.father, np.int32), typed=z(c.typed, np.int32),
+            geno=z(c.geno, np.uint8), dprob=z(c.disease_prob, np.float64), gprob=z(c.geno_prob, np.float64),
+            legal=z(c.legal, np.uint8), theta=z(c.theta, np.float64), ptheta=z(c.partial_theta, np.float64),
+            minor=z(c.minor, np.float64), type=z(c.ops.type, np.int32), peel=z(c.ops.peel, np.int32),
+            cut_off=z(c.ops.cut_off, np.int32), cut=z(c.ops.cut, np.int32), prev_off=z(c.ops.prev_off, np.int32),
+            prev=z(c.ops.prev, np.int32), child_off=z(c.ops.child_off, np.int32), child=z(c.ops.child, np.int32))
+        b = self.bufs
+        self.ped = PedigreeDesc(c.N, c.F, c.M, _ptr(b["mother"], i32p), _ptr(b["father"], i32p),
+                                _ptr(b["typed"], i32p), _ptr(b["geno"], u8p), _ptr(b["dprob"], f64p),
+                                _ptr(b["gprob"], f64p), _ptr(b["legal"], u8p))
+        self.map = MapDesc(_ptr(b["theta"], f64p), _ptr(b["ptheta"], f64p), _ptr(b["minor"], f64p), c.nlod)
+        self.peel = PeelDesc(c.ops.n, _ptr(b["type"], i32p), _ptr(b["peel"], i32p), _ptr(b["cut_off"], i32p),
+                             _ptr(b["cut"], i32p), _ptr(b["prev_off"], i32p), _ptr(b["prev"], i32p),
+                             _ptr(b["child_off"], i32p), _ptr(b["child"], i32p))
+
+
+def program_export(case: Case, npos: Optional[int] = None) -> np.ndarray:
+    """Compiled LOD program as int32 words (host only, no GPU needed)."""
+    d = _Descs(case)
+    need = C.c_int64(0)
+    npos = case.nlod if npos is None else npos
+    rc = lib().sl_program_export(C.byref(d.ped), C.byref(d.peel), npos, None, 0, C.byref(need))
+    if rc:
+        raise SlError(lib().sl_last_error(None).decode())
+    buf = np.zeros(need.value, np.int32)
+    rc = lib().sl_program_export(C.byref(d.ped), C.byref(d.peel), npos, _ptr(buf, i32p), buf.size, C.byref(need))
+    if rc:
+        raise SlError(lib().sl_last_error(None).decode())
+    return buf
+
+
+class Context:
+    """One sl_ctx: a pedigree, its map and peel program on one GPU."""
+
+    def __init__(self, case: Case, precision: int = SL_FP64, device: int = 0, block: int = 0, gpw: int = 0):
+        self.case = case
+        self._d = _Descs(case)
+        opts = Options(precision, device, block, gpw)
+        h = C.c_void_p()
+        rc = lib().sl_create(C.byref(self._d.ped), C.byref(self._d.map), C.byref(self._d.peel), C.byref(opts),
+                             C.byref(h))
+        if rc:
+            raise SlError(f"sl_create failed ({rc}): {lib().sl_last_error(None).decode()}")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc:
+            raise SlError(f"{what} failed ({rc}): {lib().sl_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- LOD ----
+    def trait_prob(self) -> float:
+        v = C.c_double(0)
+        self._check(lib().sl_trait_prob(self.h, C.byref(v)), "sl_trait_prob")
+        return v.value
+
+    def lod_reset(self):
+        self._check(lib().sl_lod_reset(self.h), "sl_lod_reset")
+
+    def lod_accumulate(self, graphs: np.ndarray):
+        g = np.ascontiguousarray(graphs, np.int32).reshape(-1, self.case.M, 2 * self.case.N)
+        self._check(lib().sl_lod_accumulate(self.h, _ptr(g, i32p), g.shape[0]), "sl_lod_accumulate")
+
+    def lod_accumulate_packed(self, dev_ptr: int, n: int):
+        self._check(lib().sl_lod_accumulate_packed(self.h, C.cast(C.c_void_p(dev_ptr), u32p), n),
+                    "sl_lod_accumulate_packed")
+
+    def sync(self):
+        self._check(lib().sl_sync(self.h), "sl_sync")
+
+    def lod_results(self):
+        n = (self.case.M - 1) * self.case.nlod
+        acc = np.zeros(n)
+        lod = np.zeros(n)
+        cnt = C.c_int64(0)
+        self._check(lib().sl_lod_results(self.h, _ptr(acc, f64p), C.byref(cnt), _ptr(lod, f64p)), "sl_lod_results")
+        return acc, cnt.value, lod
+
+    def lod_terms(self, graphs: np.ndarray) -> np.ndarray:
+        g = np.ascontiguousarray(graphs, np.int32).reshape(-1, self.case.M, 2 * self.case.N)
+        out = np.zeros((g.shape[0], self.case.M - 1, self.case.nlod))
+        self._check(lib().sl_lod_terms(self.h, _ptr(g, i32p), g.shape[0], _ptr(out, f64p)), "sl_lod_terms")
+        return out
+
+    def profile(self, enable: bool):
+        self._check(lib().sl_profile(self.h, 1 if enable else 0), "sl_profile")
+
+    def profile_read(self, kernel: int = SL_KERNEL_LOD):
+        ms = C.c_double(0)
+        n = C.c_int64(0)
+        self._check(lib().sl_profile_read(self.h, kernel, C.byref(ms), C.byref(n)), "sl_profile_read")
+        return ms.value, n.value
+
+    def lod_work(self):
+        f = C.c_double(0)
+        b = C.c_double(0)
+        self._check(lib().sl_lod_work(self.h, C.byref(f), C.byref(b)), "sl_lod_work")
+        return f.value, b.value

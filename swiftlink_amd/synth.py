@@ -230,6 +230,7 @@ def gene_drop(ped: SynthPedigree, pos_cm: np.ndarray, maf: np.ndarray, disease_p
     th = _haldane(np.diff(loc_cm) / 100.0)
     L = M + 1
     hap = np.zeros((N, 2, L), dtype=np.int8)
+    inherit = np.zeros((N, 2, L), dtype=np.int8)   # meiosis indicators of the drop
     origin = np.full((N, 2), -1, dtype=np.int64)   # founder haplotype id at the disease locus
     order = sorted(range(N), key=lambda i: ped.generation[i])
     for i in order:
@@ -244,11 +245,12 @@ def gene_drop(ped: SynthPedigree, pos_cm: np.ndarray, maf: np.ndarray, disease_p
             which = np.concatenate([[int(rng.integers(2))], sw.astype(np.int64)])
             which = np.cumsum(which) % 2
             hap[i, s] = hap[par, which, np.arange(L)]
+            inherit[i, s] = which
             origin[i, s] = origin[par, which[disease_pos]] if origin[par, which[disease_pos]] >= 0 else -1
             # origin is tracked through the disease column only
     alleles = np.delete(hap, disease_pos, axis=2)
     carry = origin == disease_founder_hap
-    return alleles, carry
+    return alleles, carry, np.delete(inherit, disease_pos, axis=2)
 
 
 def generate(spec: SynthSpec, max_drops: int = 400):
@@ -265,17 +267,45 @@ def generate(spec: SynthSpec, max_drops: int = 400):
     for attempt in range(max_drops):
         drng = np.random.default_rng([spec.seed, attempt])
         fh = int(drng.integers(4))
-        alleles, carry = gene_drop(ped, pos, maf, disease_pos, fh, drng)
+        alleles, carry, inherit = gene_drop(ped, pos, maf, disease_pos, fh, drng)
         affected = carry[:, 0] & carry[:, 1]
         naff = int(affected.sum())
         if best is None or naff > best[0]:
-            best = (naff, alleles, affected, drng)
+            best = (naff, alleles, affected, drng, inherit)
         if naff >= spec.min_affected:
             break
-    naff, alleles, affected, drng = best
+    naff, alleles, affected, drng, inherit = best
     missing = drng.random((N, M)) < spec.missing
     return dict(spec=spec, ped=ped, pos_cm=pos, maf=maf, alleles=alleles, missing=missing,
-                affected=affected, disease_pos=disease_pos)
+                affected=affected, disease_pos=disease_pos, inherit=inherit)
+
+
+def true_graph(data: dict, case) -> np.ndarray:
+    """The gene drop's meiosis indicators as a descent graph in the case's
+    internal order, reference layout int32 [M][2N] (founder slots 0).  It is
+    legal by construction: the genotypes were dropped along it."""
+    ped: SynthPedigree = data["ped"]
+    inh = data["inherit"]
+    pos = {pid: i for i, pid in enumerate(ped.ids)}
+    M = inh.shape[2]
+    g = np.zeros((M, 2 * case.N), np.int32)
+    for j, pid in enumerate(case.ids):
+        i = pos[pid]
+        if ped.father[i] < 0:
+            continue
+        g[:, 2 * j] = inh[i, 0]
+        g[:, 2 * j + 1] = inh[i, 1]
+    return g
+
+
+def make_case(name: str, outdir: str, nlod: int = 5, **overrides):
+    """Generate config `name`, write its LINKAGE files and load them.
+    Returns (case, data)."""
+    from . import linkage
+    spec = dataclasses.replace(CONFIGS[name], **overrides) if overrides else CONFIGS[name]
+    data = generate(spec)
+    p, m, d = write_linkage(data, outdir, name)
+    return linkage.load(p, m, d, nlod=nlod), data
 
 
 def write_linkage(data: dict, outdir: str, name: str = "case") -> Tuple[str, str, str]:

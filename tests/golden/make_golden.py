@@ -1,0 +1,98 @@
+"""Writes the golden vectors under tests/golden/ from the compiled reference.
+
+Run in a container where the reference sources exist (REFSRC, default
+/root/reference/src): it builds oracle/_ref/refharness (oracle/Makefile,
+target `ref`), runs it on the reference's example pedigrees and on a seeded
+synthetic pedigree, and stores inputs and outputs as compressed .npz files.
+Each fixture holds:
+  case_*      the pedigree/map/peel tables exactly as the reference built them
+  graphs      legal descent graphs (reference int32 layout [G][M][2N])
+  lod_terms   per-graph ln L - R - T for every interval/position (Peeler::process)
+  lod_acc     LODscores accumulators over those graphs, lod_final their LOD
+  fag         FounderAlleleGraph4 likelihood per graph and locus
+  lsweep_*    one L-sweep (LocusSampler::step over a locus order) under the
+              Philox stream (seed, chain 0, sweep 5)
+  msweep_*    one M-sweep (MeiosisSampler reset + steps) under (seed, 0, sweep 6)
+  chain_*     a 40-sweep chain in the shared schedule with scoring
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from swiftlink_amd.case import Case  # noqa: E402
+from swiftlink_amd import synth  # noqa: E402
+
+REF = os.environ.get("REFSRC", "/root/reference/src")
+EXAMPLES = os.path.join(os.path.dirname(REF), "examples")
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "refharness")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def run(*args):
+    subprocess.run([HARNESS, *map(str, args)], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+
+
+def make(name, base, work):
+    p, m, d = base + ".ped", base + ".map", base + ".dat"
+    run("dump", p, m, d, work)
+    run("graphs", p, m, d, work, 7, 4, 3)
+    G = 4
+    run("lod", p, m, d, work, os.path.join(work, "graphs.i32"), G)
+    run("fag", p, m, d, work, os.path.join(work, "graphs.i32"), G)
+    case = Case.from_harness_dir(work)
+    M, N = case.M, case.N
+    order = np.array(list(range(0, M, 2)) + list(range(1, M, 2)), np.int32)
+    order.tofile(os.path.join(work, "lorder.i32"))
+    ml = case.meioses()
+    morder = np.random.default_rng(3).permutation(ml).astype(np.int32)
+    morder.tofile(os.path.join(work, "morder.i32"))
+    gi = os.path.join(work, "graph_init.i32")
+    for sub in ("ls", "ms", "ch"):
+        os.makedirs(os.path.join(work, sub), exist_ok=True)
+    run("lsweep", p, m, d, os.path.join(work, "ls"), gi, os.path.join(work, "lorder.i32"), 99, 0, 5)
+    run("msweep", p, m, d, os.path.join(work, "ms"), gi, os.path.join(work, "morder.i32"), 99, 0, 6)
+    run("chain", p, m, d, os.path.join(work, "ch"), gi, 77, 0, 40, 10, 3, 0.5)
+    rd = lambda f, dt: np.fromfile(os.path.join(work, f), dt)
+    fx = {"case_" + k: v for k, v in case.to_npz_dict().items()}
+    fx.update(
+        graphs=rd("graphs.i32", np.int32).reshape(-1, M, 2 * N)[:G],
+        graph_init=rd("graph_init.i32", np.int32).reshape(M, 2 * N),
+        lod_terms=rd("lod_terms.f64", np.float64).reshape(G, M - 1, case.nlod),
+        lod_acc=rd("lod_acc.f64", np.float64), lod_final=rd("lod_final.f64", np.float64),
+        lod_count=np.array([json.load(open(os.path.join(work, "lod_meta.json")))["count"]]),
+        fag=rd("fag.f64", np.float64).reshape(G, M),
+        lsweep_order=order, lsweep_seed=np.array([99, 0, 5]),
+        lsweep_out=rd("ls/graph_out.i32", np.int32).reshape(M, 2 * N), lsweep_draws=rd("ls/draws.i32", np.int32),
+        msweep_order=morder, msweep_seed=np.array([99, 0, 6]),
+        msweep_out=rd("ms/graph_out.i32", np.int32).reshape(M, 2 * N),
+        chain_params=np.array([77, 0, 40, 10, 3]), chain_lprob=np.array([0.5]),
+        chain_out=rd("ch/graph_out.i32", np.int32).reshape(M, 2 * N),
+        chain_kinds=rd("ch/sweep_kinds.i32", np.int32), chain_acc=rd("ch/lod_acc.f64", np.float64),
+        chain_count=np.array([json.load(open(os.path.join(work, "ch/chain_meta.json")))["count"]]),
+    )
+    path = os.path.join(OUT, name + ".npz")
+    np.savez_compressed(path, **fx)
+    print(f"{name}: N={N} M={M} ops={case.ops.n} -> {os.path.getsize(path)} bytes")
+
+
+def main():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REFSRC={REF}"], check=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        for ex in ("loop", "east"):
+            make(ex, os.path.join(EXAMPLES, ex), os.path.join(tmp, ex))
+        sdir = os.path.join(tmp, "synth")
+        synth.write_config("ped30_snp100", sdir)
+        make("ped30_snp100", os.path.join(sdir, "ped30_snp100"), os.path.join(tmp, "ped30"))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,117 @@
+"""LOD trait peel on the GPU against the oracle (parity tests proper).
+
+FP64: per-graph terms within 1e-12 relative of the oracle (same operation
+order; only log and the recombination sum differ in rounding), accumulated
+LOD within 1e-6 absolute.  FP32 with per-table exponents: |dLOD| <= 1e-4.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden, GOLDEN_CASES
+from oracle_lib import Oracle
+from swiftlink_amd import native, synth
+
+pytestmark = pytest.mark.gpu
+
+FP32_LOD_TOL = 1e-4   # |LOD_fp32 - LOD_fp64 oracle|
+FP64_LOD_TOL = 1e-6
+
+
+def _lod(acc, count, trait_prob):
+    return (acc - np.log(count) - trait_prob) / np.log(10.0)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_trait_prob_fp64(name):
+    case, fx = load_golden(name)
+    with native.Context(case, native.SL_FP64) as ctx:
+        np.testing.assert_allclose(ctx.trait_prob(), case.trait_prob, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_trait_prob_fp32(name):
+    case, fx = load_golden(name)
+    with native.Context(case, native.SL_FP32_RESCALED) as ctx:
+        assert abs(ctx.trait_prob() - case.trait_prob) < 1e-4
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_terms_fp64_match_reference(name):
+    case, fx = load_golden(name)
+    with native.Context(case, native.SL_FP64) as ctx:
+        terms = ctx.lod_terms(fx["graphs"])
+    np.testing.assert_allclose(terms, fx["lod_terms"], rtol=1e-12, atol=1e-10)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_terms_fp32(name):
+    case, fx = load_golden(name)
+    with native.Context(case, native.SL_FP32_RESCALED) as ctx:
+        terms = ctx.lod_terms(fx["graphs"])
+    assert np.abs(terms - fx["lod_terms"]).max() / np.log(10.0) < FP32_LOD_TOL
+
+
+@pytest.mark.parametrize("precision,tol", [(native.SL_FP64, FP64_LOD_TOL), (native.SL_FP32_RESCALED, FP32_LOD_TOL)])
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_accumulate_matches_reference(name, precision, tol):
+    case, fx = load_golden(name)
+    with native.Context(case, precision, gpw=3) as ctx:
+        ctx.lod_accumulate(fx["graphs"][:3])
+        ctx.lod_accumulate(fx["graphs"][3:])
+        acc, count, lod = ctx.lod_results()
+    assert count == len(fx["graphs"])
+    assert np.abs(lod - fx["lod_final"]).max() < tol
+    assert np.abs(acc - fx["lod_acc"]).max() < tol * np.log(10.0)
+
+
+def test_reset_and_empty():
+    case, fx = load_golden("loop")
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.lod_accumulate(fx["graphs"])
+        ctx.lod_reset()
+        acc, count, _ = ctx.lod_results()
+        assert count == 0 and np.all(np.isneginf(acc))
+        ctx.lod_accumulate(fx["graphs"][:0])
+        ctx.lod_accumulate(fx["graphs"][:1])
+        acc, count, lod = ctx.lod_results()
+    np.testing.assert_allclose(acc, fx["lod_terms"][0].reshape(-1), rtol=1e-12)
+
+
+@pytest.fixture(scope="module")
+def ped150(tmp_path_factory):
+    d = tmp_path_factory.mktemp("ped150")
+    case, data = synth.make_case("ped150_snp1000", str(d))
+    g0 = synth.true_graph(data, case)
+    o = Oracle(case)
+    graphs = [g0]
+    g = g0
+    for s in range(3):  # a few oracle L-sweeps to get distinct legal graphs
+        for l in list(range(0, case.M, 2)) + list(range(1, case.M, 2)):
+            g, _ = o.lstep(g, l, 11, 0, s)
+        graphs.append(g)
+    return case, np.stack(graphs), o
+
+
+def test_ped150_fp64_terms(ped150):
+    """150 people, 3 loops, 1000 SNPs: spot intervals against the oracle."""
+    case, graphs, o = ped150
+    with native.Context(case, native.SL_FP64) as ctx:
+        terms = ctx.lod_terms(graphs)
+    for gi in range(len(graphs)):
+        for l in (0, 1, 499, 500, case.M - 2):
+            np.testing.assert_allclose(terms[gi, l], o.lod_terms(graphs[gi], l), rtol=1e-12)
+
+
+def test_ped150_fp32_lod_within_tolerance(ped150):
+    case, graphs, o = ped150
+    ref = o.lod_accumulate(graphs)
+    tp = o.trait_prob()
+    lod_ref = _lod(ref, len(graphs), tp)
+    with native.Context(case, native.SL_FP32_RESCALED) as ctx:
+        ctx.lod_accumulate(graphs)
+        acc, count, lod = ctx.lod_results()
+    assert np.abs(lod - lod_ref).max() < FP32_LOD_TOL
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.lod_accumulate(graphs)
+        acc, count, lod = ctx.lod_results()
+    assert np.abs(lod - lod_ref).max() < FP64_LOD_TOL

@@ -1,0 +1,75 @@
+"""Library checks that need no GPU: it loads, exports every entry point the
+header declares, rejects bad descriptors, and its compiled peel program (the
+indexing the LOD kernel uses) reproduces the oracle when executed on the CPU."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, load_golden, GOLDEN_CASES
+from oracle_lib import Oracle
+from program_emu import Program
+from swiftlink_amd import native
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "swiftlink_amd.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(sl_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_header_symbols():
+    lib = native.lib()
+    names = header_symbols()
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(native.exported_symbols()) == names
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_program_bounds_and_fp64_bit_exact(name):
+    case, fx = load_golden(name)
+    P = Program(native.program_export(case))
+    P.check_bounds(case.nlod)
+    o = Oracle(case)
+    assert P.run(case, None, 0)[0] == o.trait_prob()
+    g = fx["graphs"][0]
+    loci = range(case.M - 1) if case.M < 8 else (0, 1, case.M // 2, case.M - 2)
+    for l in loci:
+        lnL = P.run(case, g, l)
+        ref = [np.log(o.trait_likelihood(g, l, off)) for off in range(1, case.nlod + 1)]
+        assert np.array_equal(lnL, ref)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_program_fp32_rescaled(name):
+    case, fx = load_golden(name)
+    P = Program(native.program_export(case))
+    o = Oracle(case)
+    g = fx["graphs"][1]
+    for l in (0, case.M - 2):
+        lnL = P.run(case, g, l, fp32=True)
+        ref = [np.log(o.trait_likelihood(g, l, off)) for off in range(1, case.nlod + 1)]
+        assert np.abs(lnL - ref).max() / np.log(10) < 1e-5
+
+
+def test_program_rejects_bad_tables():
+    case, fx = load_golden("loop")
+    bad = case.ops.prev.copy()
+    if bad.size:
+        bad[0] = case.ops.n + 3
+        case.ops.prev = bad
+        with pytest.raises(native.SlError):
+            native.program_export(case)
+
+
+def test_pack_graphs_layout():
+    from swiftlink_amd.case import pack_graphs
+    case, fx = load_golden("east")
+    g = fx["graphs"][:2]
+    p = pack_graphs(case, g)
+    for gi in range(2):
+        for l in (0, 57):
+            for b in range(2 * (case.N - case.F)):
+                assert ((p[gi, l, b // 32] >> (b % 32)) & 1) == g[gi, l, 2 * case.F + b]
