@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--gpw", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp32-only", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -84,6 +87,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": 4 * n, "launches": 5}))
+        return
     from swiftlink_amd import native, synth
     from swiftlink_amd.case import pack_graphs
 
@@ -117,7 +129,10 @@ def main():
         return dt, kms / max(nl, 1), lod, flops
 
     dt32, k32, lod32, flops = run(native.SL_FP32_RESCALED, args.steps, args.warmup)
-    dt64, k64, lod64, _ = run(native.SL_FP64, max(2, args.steps // 4), args.warmup)
+    if args.fp32_only:
+        dt64, k64, lod64 = float("nan"), float("nan"), lod32
+    else:
+        dt64, k64, lod64, _ = run(native.SL_FP64, max(2, args.steps // 4), args.warmup)
     # same graphs, same count of accumulations -> comparable LOD
     value = args.steps * evals_per_step / dt32
     fp64_value = max(2, args.steps // 4) * evals_per_step / dt64
@@ -140,7 +155,7 @@ def main():
                    "graphs_per_step": B, "evals_per_step": evals_per_step},
         "fp64_lod_evals_per_s": fp64_value,
         "fp32_vs_fp64_speedup": value / fp64_value,
-        "max_abs_dlod_fp32_vs_fp64": float(np.abs(lod32 - lod64).max()),
+        "max_abs_dlod_fp32_vs_fp64": None if args.fp32_only else float(np.abs(lod32 - lod64).max()),
         "roofline": {"bound": "mfma", "compute": "fp32 VALU (no MFMA: tables of 4^k cells)",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,

@@ -31,7 +31,11 @@ struct sl_ctx {
     sl::PrevDesc* d_prevs = nullptr;
     sl::ChildDesc* d_children = nullptr;
     uint32_t* d_tasks = nullptr;
+    uint32_t* d_tinfo = nullptr;
+    uint16_t* d_idx = nullptr;
     int32_t* d_level_base = nullptr;
+    int32_t* d_sched_off = nullptr;
+    int32_t* d_sched_ops = nullptr;
     double* d_trait_d = nullptr;
     float* d_trait_f = nullptr;
     double* d_ptheta = nullptr;
@@ -105,7 +109,12 @@ sl::LodArgs lod_args(sl_ctx* c) {
     A.prevs = c->d_prevs;
     A.children = c->d_children;
     A.tasks = c->d_tasks;
+    A.tinfo = c->d_tinfo;
+    A.idx = c->d_idx;
     A.level_base = c->d_level_base;
+    A.sched_off = c->d_sched_off;
+    A.sched_ops = c->d_sched_ops;
+    A.waves = c->prog.waves;
     A.n_levels = c->prog.n_levels;
     A.last_op = c->prog.last_op;
     A.n_ops = c->prog.n_ops;
@@ -164,7 +173,9 @@ int check_desc(const sl_pedigree_desc* p, const sl_map_desc* m, const sl_peel_de
 }
 
 int release_all(sl_ctx* c) {
-    void* ptrs[] = {c->d_ops, c->d_prevs, c->d_children, c->d_tasks, c->d_level_base, c->d_trait_d,
+    void* ptrs[] = {c->d_ops, c->d_prevs, c->d_children, c->d_tasks, c->d_tinfo, c->d_idx, c->d_level_base,
+                    c->d_sched_off,
+                    c->d_sched_ops, c->d_trait_d,
                     c->d_trait_f, c->d_ptheta, c->d_logt, c->d_loga, c->d_acc, c->d_scalar,
                     c->d_partial, c->d_packed, c->d_stage, c->d_terms};
     for (void* p : ptrs)
@@ -205,20 +216,24 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
     c->W = (2 * (c->N - c->F) + 31) / 32;
     c->npos = map->lods_per_interval;
     if (opts && opts->block_threads > 0) c->block = opts->block_threads;
+    if (c->block % 64 != 0 || c->block < 64 || c->block > 1024) {
+        delete c;
+        return fail(nullptr, SL_E_INVALID, "block_threads must be a multiple of 64 in 64..1024");
+    }
     if (opts && opts->graphs_per_block > 0) c->gpw = opts->graphs_per_block;
 
     sl::PedigreeView pv{c->N, c->F, ped->mother, ped->father, ped->disease_prob};
     sl::PeelView kv{peel->n_ops, peel->type, peel->peel_node, peel->cut_offsets, peel->cutset,
                     peel->prev_offsets, peel->prev, peel->child_offsets, peel->children};
-    std::string why = sl::compile_lod_program(pv, kv, c->npos, &c->prog);
+    std::string why = sl::compile_lod_program(pv, kv, c->npos, c->block / 64, &c->prog);
     if (!why.empty()) {
         delete c;
         return fail(nullptr, SL_E_UNSUPPORTED, why);
     }
     const size_t esz = c->precision == SL_FP64 ? 8 : 4;
     c->arena_bytes = (int)((c->prog.arena_elems * esz + 15) / 16 * 16);
-    c->maxbits_bytes = c->precision == SL_FP64 ? 0 : (int)((c->prog.n_ops * c->npos * 4 + 15) / 16 * 16);
-    c->lds = (size_t)c->arena_bytes + c->maxbits_bytes + ((c->N + 15) / 16 * 16);
+    c->maxbits_bytes = c->precision == SL_FP64 ? 0 : (int)((c->prog.n_ops * 4 + 15) / 16 * 16);
+    c->lds = (size_t)c->arena_bytes + c->maxbits_bytes + 4 * sl::kMaxPos * esz + ((c->N + 15) / 16 * 16);
     if (c->lds > 160 * 1024) {
         delete c;
         return fail(nullptr, SL_E_UNSUPPORTED, "peel program needs more than 160 KiB of LDS");
@@ -255,7 +270,11 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
         (rc = upload(c, &c->d_prevs, c->prog.prevs.data(), c->prog.prevs.size())) ||
         (rc = upload(c, &c->d_children, c->prog.children.data(), c->prog.children.size())) ||
         (rc = upload(c, &c->d_tasks, c->prog.tasks.data(), c->prog.tasks.size())) ||
+        (rc = upload(c, &c->d_tinfo, c->prog.tinfo.data(), c->prog.tinfo.size())) ||
+        (rc = upload(c, &c->d_idx, c->prog.idx.data(), c->prog.idx.size())) ||
         (rc = upload(c, &c->d_level_base, c->prog.level_base.data(), c->prog.level_base.size())) ||
+        (rc = upload(c, &c->d_sched_off, c->prog.sched_off.data(), c->prog.sched_off.size())) ||
+        (rc = upload(c, &c->d_sched_ops, c->prog.sched_ops.data(), c->prog.sched_ops.size())) ||
         (rc = upload(c, &c->d_trait_d, ped->disease_prob, 4 * (size_t)c->N)) ||
         (rc = upload(c, &c->d_trait_f, tf.data(), tf.size())) ||
         (rc = upload(c, &c->d_ptheta, c->ptheta.data(), c->ptheta.size())) ||
@@ -448,7 +467,7 @@ int sl_program_export(const sl_pedigree_desc* ped, const sl_peel_desc* peel, int
     sl::PedigreeView pv{ped->n_people, ped->n_founders, ped->mother, ped->father, ped->disease_prob};
     sl::PeelView kv{peel->n_ops, peel->type, peel->peel_node, peel->cut_offsets, peel->cutset,
                     peel->prev_offsets, peel->prev, peel->child_offsets, peel->children};
-    std::string why = sl::compile_lod_program(pv, kv, npos, &P);
+    std::string why = sl::compile_lod_program(pv, kv, npos, 2, &P);
     if (!why.empty()) return fail(nullptr, SL_E_UNSUPPORTED, why);
     std::vector<int32_t> o;
     o.push_back(P.n_ops);
@@ -466,6 +485,13 @@ int sl_program_export(const sl_pedigree_desc* ped, const sl_peel_desc* peel, int
     o.insert(o.end(), w, w + P.children.size() * (sizeof(sl::ChildDesc) / 4));
     for (uint32_t t : P.tasks) o.push_back((int32_t)t);
     o.insert(o.end(), P.level_base.begin(), P.level_base.end());
+    o.push_back(P.waves);
+    o.insert(o.end(), P.sched_off.begin(), P.sched_off.end());
+    o.insert(o.end(), P.sched_ops.begin(), P.sched_ops.end());
+    o.push_back((int32_t)P.tinfo.size());
+    for (uint32_t t : P.tinfo) o.push_back((int32_t)t);
+    o.push_back((int32_t)P.idx.size());
+    for (uint16_t t : P.idx) o.push_back((int32_t)t);
     *needed = (int64_t)o.size();
     if (buf && cap >= (int64_t)o.size()) memcpy(buf, o.data(), o.size() * 4);
     return SL_OK;

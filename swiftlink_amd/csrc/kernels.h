@@ -14,8 +14,12 @@ struct LodArgs {
     const PrevDesc* prevs;
     const ChildDesc* children;
     const uint32_t* tasks;
+    const uint32_t* tinfo;       // [cells*4] precompiled term info
+    const uint16_t* idx;         // precompiled consumed-table indices
     const int32_t* level_base;
-    int n_levels, last_op, n_ops, npos;
+    const int32_t* sched_off;    // [n_levels*waves+1]
+    const int32_t* sched_ops;
+    int n_levels, last_op, n_ops, npos, waves;
     const double* trait_d;       // [N][4] disease probabilities (FP64 program)
     const float* trait_f;        // [N][4] (FP32 program)
     const double* partial_theta; // [M-1]

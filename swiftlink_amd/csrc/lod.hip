@@ -1,19 +1,23 @@
 // Trait-peel (LOD score) kernels for gfx950.
 //
-// Work decomposition: one workgroup per (interval l, chunk of graphs).  For
-// each graph of its chunk the workgroup evaluates the whole peel program for
-// all n trait positions of the interval at once: the positions are folded
-// into the lane dimension (task t of a level covers (position, cell)), so a
-// level with few cells still fills the wavefronts.  Levels are separated by
-// one barrier; tables live in an LDS arena laid out by the host compiler.
+// Work decomposition: one workgroup per (interval l, chunk of graphs); the
+// workgroup runs the peel program for each graph of its chunk with all n trait
+// positions of the interval folded together.  Within a level every op is owned
+// by one wavefront (host-built schedule, longest first onto the least loaded
+// wave), so op, consumed-table and child descriptors are wave-uniform and read
+// with scalar loads through the constant address space, the op type never
+// diverges, and the lanes of the wave span (position, cell) of that op's
+// table.  Levels are separated by one barrier; tables live in an LDS arena
+// laid out by the host compiler.
 //
 // Arithmetic follows TraitRfunction (trait_rfunction.cc:9-137) and
 // Rfunction::evaluate_partner_peel (rfunction.cc:116-142) operation for
 // operation, so the FP64 variant reproduces the reference's table values.
-// The FP32 variant keeps per-(table, position) power-of-two exponents: a
-// producer records the bit pattern of its largest cell with an LDS atomicMax
-// and every consumer multiplies what it reads by 2^-e, so no table drifts out
-// of float range; the exponents are summed back in FP64 at the end.
+// The FP32 variant keeps a power-of-two exponent per table (all positions of
+// an interval share it: they differ only in the flanking thetas): the owning
+// wave reduces the table's largest cell in registers and publishes its bit
+// pattern; consumers multiply what they read by 2^-e, and the exponents are
+// summed back in FP64 at the end.
 //
 // After the last level, lane `pos` forms ln L - R - T (peeler.cc:97-99) in
 // FP64 and log-sums it over the chunk's graphs (lod_score.h:74-80, with a
@@ -30,12 +34,41 @@ namespace sl {
 
 namespace {
 
+typedef const __attribute__((address_space(4))) int32_t* cint_p;
+typedef const __attribute__((address_space(4))) uint32_t* cuint_p;
+typedef const __attribute__((address_space(4))) float* cflt_p;
+typedef const __attribute__((address_space(4))) double* cdbl_p;
+
+// wave-uniform descriptor reads: scalar loads
+__device__ __forceinline__ OpDesc load_op(const OpDesc* base, int i) {
+    cint_p q = (cint_p)(base + i);
+    OpDesc d;
+    d.type = q[0]; d.k = q[1]; d.peel = q[2]; d.n_prev = q[3];
+    d.tab_off = q[4]; d.size = q[5]; d.mat_digit = q[6]; d.idx_base = q[7];
+    d.n_child = q[8]; d.child_off = q[9]; d.prev_off = q[10]; d.task_base = q[11];
+    return d;
+}
+
+__device__ __forceinline__ ChildDesc load_child(const ChildDesc* base, int i) {
+    cint_p q = (cint_p)(base + i);
+    ChildDesc c;
+    c.person = q[0]; c.child_digit = q[1]; c.other_digit = q[2]; c.peel_is_mother = q[3];
+    return c;
+}
+
+template <typename T>
+__device__ __forceinline__ T load_const(const T* p, int i);
+template <>
+__device__ __forceinline__ float load_const<float>(const float* p, int i) { return ((cflt_p)p)[i]; }
+template <>
+__device__ __forceinline__ double load_const<double>(const double* p, int i) { return ((cdbl_p)p)[i]; }
+
 // affected-allele bits per phased trait: UU 00, AA 11, AU 01 (maternal), UA 10
 // (rfunction.h:81-107); kid trait from (maternal affected, paternal affected)
 // = UU, UA, AU, AA (rfunction.cc:94-113).
 __device__ __forceinline__ uint32_t kid_trait(uint32_t mat, uint32_t pat, int i, int j) {
-    const uint32_t aff = 0x9Cu;   // pt -> 2 bits: bit0 maternal allele affected, bit1 paternal
-    const uint32_t kt = 0x6Cu;    // (ma<<1|pa) -> UU=0, UA=3, AU=2, AA=1
+    const uint32_t aff = 0x9Cu;
+    const uint32_t kt = 0x6Cu;
     uint32_t ma = (aff >> (2 * mat + i)) & 1u;
     uint32_t pa = (aff >> (2 * pat + j)) & 1u;
     return (kt >> (2 * ((ma << 1) | pa))) & 3u;
@@ -50,23 +83,6 @@ __device__ __forceinline__ float scale_from_bits(uint32_t bits) {
 __device__ __forceinline__ int exp_from_bits(uint32_t bits) {
     uint32_t ef = (bits >> 23) & 0xFFu;
     return ef == 0u ? 0 : (int)ef - 126;
-}
-
-template <typename T, bool RESCALE>
-__device__ __forceinline__ T prev_value(const PrevDesc& pd, uint32_t ext, int pos, const T* arena,
-                                        const uint32_t* maxbits, int npos) {
-    int idx = 0;
-    const int nd = pd.n_digits;
-    for (int j = 0; j < nd; ++j) {
-        const uint32_t w = pd.digit[j];
-        const uint32_t src = w & 15u;
-        const uint32_t rb = (w >> 4) & 255u;
-        const uint32_t v = (ext >> (2 * src)) & 3u;
-        idx += (int)((rb >> (2 * v)) & 3u) * (int)(w >> 12);
-    }
-    T x = arena[pd.tab_off + pos * pd.size + idx];
-    if (RESCALE) x *= (T)scale_from_bits(maxbits[pd.op * npos + pos]);
-    return x;
 }
 
 template <typename T>
@@ -85,49 +101,61 @@ __device__ __forceinline__ T rho(uint32_t code, int i, int j, const Thetas<T>& t
     return r;
 }
 
-template <typename T, bool RESCALE, bool NODG>
-__device__ T eval_cell(const LodArgs& A, const OpDesc& d, uint32_t cell, int pos, const T* arena,
-                       const uint32_t* maxbits, const uint8_t* code, const Thetas<T>& th, const T* trait) {
-    const int k = d.k;
-    const PrevDesc* pv = A.prevs + d.prev_off;
+// Wave-uniform facts about the consumed tables of the op being evaluated.
+template <typename T>
+struct PrevSet {
+    int tab[kMaxPrev];
+    int size[kMaxPrev];
+    T scale[kMaxPrev];
+};
+
+template <typename T>
+__device__ __forceinline__ T select4(const T tr[4], uint32_t v) {
+    return v == 0u ? tr[0] : v == 1u ? tr[1] : v == 2u ? tr[2] : tr[3];
+}
+
+// One compact cell `c` of op `d` at position `pos`: the precompiled terms give
+// the summed node's trait value, which consumed-table cells to read, and for
+// PARENT ops which (i, j) transmit each child's trait.
+template <typename T, bool NODG>
+__device__ T eval_cell(const LodArgs& A, const OpDesc& d, const T tr[4], const PrevSet<T>& P, const int* kids,
+                       int c, int pos, const T* arena, const uint8_t* code, const Thetas<T>& th) {
     const int np = d.n_prev;
     const T quarter = (T)0.25;
+    const uint4 ti = *reinterpret_cast<const uint4*>(A.tinfo + (size_t)(d.task_base + c) * 4);
+    const uint32_t info[4] = {ti.x, ti.y, ti.z, ti.w};
+    const uint16_t* ix = A.idx + d.idx_base + (size_t)c * 4 * np;
     T total = (T)0;
     if (d.type == kChild) {
-        const uint32_t mat = (cell >> (2 * d.mat_digit)) & 3u;
-        const uint32_t pat = (cell >> (2 * d.pat_digit)) & 3u;
         const uint32_t cd = NODG ? 0u : (uint32_t)code[d.peel];
-        for (int ij = 0; ij < 4; ++ij) {
-            const int i = ij >> 1, j = ij & 1;
-            const uint32_t kt = kid_trait(mat, pat, i, j);
-            T tmp = trait[4 * d.peel + kt];
-            if (tmp == (T)0) continue;
-            const uint32_t ext = cell | (kt << (2 * k));
-            for (int q = 0; q < np; ++q) tmp *= prev_value<T, RESCALE>(pv[q], ext, pos, arena, maxbits, A.npos);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            if (!(info[t] & 4u)) continue;
+            T tmp = select4(tr, info[t] & 3u);
+#pragma unroll
+            for (int q = 0; q < kMaxPrev; ++q)
+                if (q < np) tmp *= arena[P.tab[q] + pos * P.size[q] + ix[t * np + q]] * P.scale[q];
             if (NODG) tmp *= quarter;
-            else tmp *= quarter * rho<T>(cd, i, j, th);
+            else tmp *= quarter * rho<T>(cd, t >> 1, t & 1, th);
             total += tmp;
         }
     } else if (d.type == kParent) {
-        const ChildDesc* cds = A.children + d.child_off;
-        for (uint32_t v = 0; v < 4; ++v) {
-            T tmp = trait[4 * d.peel + v];
-            if (tmp == (T)0) continue;
-            const uint32_t ext = cell | (v << (2 * k));
-            for (int q = 0; q < np; ++q) tmp *= prev_value<T, RESCALE>(pv[q], ext, pos, arena, maxbits, A.npos);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            if (!(info[t] & 4u)) continue;
+            T tmp = tr[t];
+#pragma unroll
+            for (int q = 0; q < kMaxPrev; ++q)
+                if (q < np) tmp *= arena[P.tab[q] + pos * P.size[q] + ix[t * np + q]] * P.scale[q];
             T child_prob = (T)1;
-            for (int c = 0; c < d.n_child; ++c) {
-                const ChildDesc cdsc = cds[c];
-                const uint32_t kid = (cell >> (2 * cdsc.child_digit)) & 3u;
-                const uint32_t other = (cell >> (2 * cdsc.other_digit)) & 3u;
-                const uint32_t mat = cdsc.peel_is_mother ? v : other;
-                const uint32_t pat = cdsc.peel_is_mother ? other : v;
-                const uint32_t cc = NODG ? 0u : (uint32_t)code[cdsc.person];
+            for (int ch = 0; ch < d.n_child; ++ch) {
+                const uint32_t m = (info[t] >> (4 + 4 * ch)) & 15u;
+                const uint32_t cc = NODG ? 0u : (uint32_t)code[kids[ch]];
                 T ctmp = (T)0;
+#pragma unroll
                 for (int ij = 0; ij < 4; ++ij) {
-                    const int i = ij >> 1, j = ij & 1;
-                    if (kid_trait(mat, pat, i, j) != kid) continue;
-                    ctmp += NODG ? quarter : quarter * rho<T>(cc, i, j, th);
+                    if (!((m >> ij) & 1u)) continue;
+                    ctmp += NODG ? quarter : quarter * rho<T>(cc, ij >> 1, ij & 1, th);
                 }
                 child_prob *= ctmp;
             }
@@ -135,11 +163,13 @@ __device__ T eval_cell(const LodArgs& A, const OpDesc& d, uint32_t cell, int pos
             total += tmp;
         }
     } else {
-        for (uint32_t v = 0; v < 4; ++v) {
-            T tmp = trait[4 * d.peel + v];
-            if (tmp == (T)0) continue;
-            const uint32_t ext = cell | (v << (2 * k));
-            for (int q = 0; q < np; ++q) tmp *= prev_value<T, RESCALE>(pv[q], ext, pos, arena, maxbits, A.npos);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            if (!(info[t] & 4u)) continue;
+            T tmp = tr[t];
+#pragma unroll
+            for (int q = 0; q < kMaxPrev; ++q)
+                if (q < np) tmp *= arena[P.tab[q] + pos * P.size[q] + ix[t * np + q]] * P.scale[q];
             total += tmp;
         }
     }
@@ -153,56 +183,19 @@ __device__ __forceinline__ double lse(double a, double b) {
     return m + log1p(exp(-fabs(a - b)));
 }
 
-// Evaluates every level of the program for one graph (or none when NODG).
-template <typename T, bool RESCALE, bool NODG>
-__device__ void run_levels(const LodArgs& A, T* arena, uint32_t* maxbits, const uint8_t* code, double ptheta,
-                           const T* trait) {
-    const int npos = A.npos;
-    for (int L = 0; L < A.n_levels; ++L) {
-        const int base = A.level_base[L];
-        const int nt = A.level_base[L + 1] - base;
-        const int total = nt * npos;
-        const float inv_nt = 1.0f / (float)nt;
-        for (int t = threadIdx.x; t < total; t += blockDim.x) {
-            int pos = (int)(((float)t + 0.5f) * inv_nt);
-            if (pos >= npos) pos = npos - 1;
-            int ti = t - pos * nt;
-            if (ti < 0) { --pos; ti += nt; }
-            else if (ti >= nt) { ++pos; ti -= nt; }
-            const uint32_t tw = A.tasks[base + ti];
-            const int op = (int)(tw >> 16);
-            const uint32_t cell = tw & 0xFFFFu;
-            const OpDesc d = A.ops[op];
-            Thetas<T> th;
-            if (!NODG) {
-                const double tl = ptheta * (double)(pos + 1);
-                const double tr = ptheta * (double)(npos - pos);
-                th.t1 = (T)tl;
-                th.a1 = (T)(1.0 - tl);
-                th.t2 = (T)tr;
-                th.a2 = (T)(1.0 - tr);
-            }
-            const T val = eval_cell<T, RESCALE, NODG>(A, d, cell, pos, arena, maxbits, code, th, trait);
-            arena[d.tab_off + pos * d.size + (base + ti - d.task_base)] = val;
-            if (RESCALE) atomicMax(&maxbits[op * npos + pos], __float_as_uint((float)val));
-        }
-        __syncthreads();
-    }
-}
-
-// log of the final likelihood at position `pos` including table exponents
-template <typename T, bool RESCALE>
-__device__ double final_log(const LodArgs& A, const T* arena, const uint32_t* maxbits, int pos) {
-    const OpDesc last = A.ops[A.last_op];
-    const double L = (double)arena[last.tab_off + pos * last.size];
-    double s = 0.0;
-    if (RESCALE) {
-        int e = 0;
-        for (int o = 0; o < A.n_ops; ++o)
-            if (o != A.last_op) e += exp_from_bits(maxbits[o * A.npos + pos]);
-        s = (double)e * 0.69314718055994530942;
-    }
-    return L > 0.0 ? log(L) + s : -INFINITY;
+// max over the wavefront of non-negative floats: DPP within each row of 16
+// lanes (quad swaps, half-row and row mirrors), then the four row maxima
+__device__ __forceinline__ float wave_max(float v) {
+    int x = __float_as_int(v);
+    x = __float_as_int(fmaxf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, true))));
+    x = __float_as_int(fmaxf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, true))));
+    x = __float_as_int(fmaxf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, true))));
+    x = __float_as_int(fmaxf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, true))));
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(x, 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(x, 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(x, 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(x, 48));
+    return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
 }
 
 template <typename T>
@@ -212,12 +205,91 @@ __device__ __forceinline__ const double* trait_ptr<double>(const LodArgs& A) { r
 template <>
 __device__ __forceinline__ const float* trait_ptr<float>(const LodArgs& A) { return A.trait_f; }
 
+// Evaluates every level of the program for one graph (or none when NODG).
+// `ths` holds the flanking thetas of each position (LDS).
+template <typename T, bool RESCALE, bool NODG>
+__device__ void run_levels(const LodArgs& A, T* arena, uint32_t* maxbits, const uint8_t* code, const T* ths) {
+    const int npos = A.npos;
+    const int W = A.waves;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63u);
+    const T* trait = trait_ptr<T>(A);
+    for (int L = 0; L < A.n_levels; ++L) {
+        const int s0 = ((cint_p)A.sched_off)[L * W + wave];
+        const int s1 = ((cint_p)A.sched_off)[L * W + wave + 1];
+        for (int s = s0; s < s1; ++s) {
+            const int op = ((cint_p)A.sched_ops)[s];
+            const OpDesc d = load_op(A.ops, op);
+            T tr[4];
+            for (int v = 0; v < 4; ++v) tr[v] = load_const<T>(trait, 4 * d.peel + v);
+            PrevSet<T> P;
+#pragma unroll
+            for (int q = 0; q < kMaxPrev; ++q) {
+                if (q < d.n_prev) {
+                    cint_p pd = (cint_p)(A.prevs + d.prev_off + q);
+                    P.tab[q] = pd[1];
+                    P.size[q] = pd[2];
+                    P.scale[q] = RESCALE ? (T)scale_from_bits(maxbits[pd[0]]) : (T)1;
+                } else {
+                    P.tab[q] = 0;
+                    P.size[q] = 0;
+                    P.scale[q] = (T)1;
+                }
+            }
+            int kids[7];
+#pragma unroll
+            for (int ch = 0; ch < 7; ++ch) kids[ch] = ch < d.n_child ? ((cint_p)(A.children + d.child_off + ch))[0] : 0;
+            const int items = d.size * npos;
+            const float inv = 1.0f / (float)d.size;
+            float mx = 0.0f;
+            for (int it = lane; it < items; it += 64) {
+                int pos = (int)(((float)it + 0.5f) * inv);
+                if (pos >= npos) pos = npos - 1;
+                int c = it - pos * d.size;
+                if (c < 0) { --pos; c += d.size; }
+                else if (c >= d.size) { ++pos; c -= d.size; }
+                Thetas<T> th;
+                if (!NODG) {
+                    th.t1 = ths[4 * pos + 0];
+                    th.a1 = ths[4 * pos + 1];
+                    th.t2 = ths[4 * pos + 2];
+                    th.a2 = ths[4 * pos + 3];
+                }
+                const T val = eval_cell<T, NODG>(A, d, tr, P, kids, c, pos, arena, code, th);
+                arena[d.tab_off + pos * d.size + c] = val;
+                if (RESCALE) mx = fmaxf(mx, (float)val);
+            }
+            if (RESCALE) {
+                const float m = wave_max(mx);
+                if (lane == 0) maxbits[op] = __float_as_uint(m);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// log of the final likelihood at position `pos` including table exponents
+template <typename T, bool RESCALE>
+__device__ double final_log(const LodArgs& A, const T* arena, const uint32_t* maxbits, int pos) {
+    const OpDesc last = load_op(A.ops, A.last_op);
+    const double L = (double)arena[last.tab_off + pos * last.size];
+    double s = 0.0;
+    if (RESCALE) {
+        int e = 0;
+        for (int o = 0; o < A.n_ops; ++o)
+            if (o != A.last_op) e += exp_from_bits(maxbits[o]);
+        s = (double)e * 0.69314718055994530942;
+    }
+    return L > 0.0 ? log(L) + s : -INFINITY;
+}
+
 template <typename T, bool RESCALE>
 __global__ void lod_kernel(LodArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     T* arena = reinterpret_cast<T*>(smem);
     uint32_t* maxbits = reinterpret_cast<uint32_t*>(smem + A.arena_bytes);
-    uint8_t* code = reinterpret_cast<uint8_t*>(smem + A.arena_bytes + A.maxbits_bytes);
+    T* ths = reinterpret_cast<T*>(smem + A.arena_bytes + A.maxbits_bytes);
+    uint8_t* code = reinterpret_cast<uint8_t*>(smem + A.arena_bytes + A.maxbits_bytes + 4 * kMaxPos * sizeof(T));
 
     const int l = blockIdx.x;
     const int chunk = blockIdx.y;
@@ -227,8 +299,17 @@ __global__ void lod_kernel(LodArgs A) {
     const int nm = 2 * (A.N - A.F);
     const double ptheta = A.partial_theta[l];
     const double lt = A.log_theta[l], la = A.log_anti[l];
-    const T* trait = trait_ptr<T>(A);
     double run = -INFINITY;
+    // set_thetas (trait_rfunction.h:50-56): linear partial thetas
+    if ((int)threadIdx.x < npos) {
+        const int pos = threadIdx.x;
+        const double tl = ptheta * (double)(pos + 1);
+        const double tr = ptheta * (double)(npos - pos);
+        ths[4 * pos + 0] = (T)tl;
+        ths[4 * pos + 1] = (T)(1.0 - tl);
+        ths[4 * pos + 2] = (T)tr;
+        ths[4 * pos + 3] = (T)(1.0 - tr);
+    }
 
     for (int g = g0; g < g1; ++g) {
         const uint32_t* c0 = A.graphs + ((size_t)g * A.M + l) * A.W;
@@ -241,10 +322,8 @@ __global__ void lod_kernel(LodArgs A) {
             const uint32_t p1 = (c1[(b + 1) >> 5] >> ((b + 1) & 31)) & 1u;
             code[p] = (uint8_t)(m0 | (m1 << 1) | (p0 << 2) | (p1 << 3));
         }
-        if (RESCALE)
-            for (int i = threadIdx.x; i < A.n_ops * npos; i += blockDim.x) maxbits[i] = 0u;
         __syncthreads();
-        run_levels<T, RESCALE, false>(A, arena, maxbits, code, ptheta, trait);
+        run_levels<T, RESCALE, false>(A, arena, maxbits, code, ths);
         if ((int)threadIdx.x < npos) {
             const int pos = threadIdx.x;
             int ncross = 0;
@@ -266,10 +345,7 @@ __global__ void trait_prob_kernel(LodArgs A, double* out) {
     extern __shared__ __align__(16) unsigned char smem[];
     T* arena = reinterpret_cast<T*>(smem);
     uint32_t* maxbits = reinterpret_cast<uint32_t*>(smem + A.arena_bytes);
-    if (RESCALE)
-        for (int i = threadIdx.x; i < A.n_ops * A.npos; i += blockDim.x) maxbits[i] = 0u;
-    __syncthreads();
-    run_levels<T, RESCALE, true>(A, arena, maxbits, nullptr, 0.0, trait_ptr<T>(A));
+    run_levels<T, RESCALE, true>(A, arena, maxbits, nullptr, nullptr);
     if (threadIdx.x == 0) out[0] = final_log<T, RESCALE>(A, arena, maxbits, 0);
 }
 
@@ -310,12 +386,14 @@ hipError_t configure_lod_lds(int precision, size_t lds) {
     if (lds <= 64 * 1024) return hipSuccess;
     hipError_t e;
     if (precision == 0) {
-        e = hipFuncSetAttribute((const void*)lod_kernel<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        e = hipFuncSetAttribute((const void*)lod_kernel<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void*)trait_prob_kernel<double, false>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     } else {
-        e = hipFuncSetAttribute((const void*)lod_kernel<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        e = hipFuncSetAttribute((const void*)lod_kernel<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
         if (e == hipSuccess)
             e = hipFuncSetAttribute((const void*)trait_prob_kernel<float, true>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

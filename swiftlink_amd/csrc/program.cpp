@@ -11,6 +11,16 @@ namespace {
 
 int popc4(unsigned m) { return __builtin_popcount(m & 15u); }
 
+// kid trait from the parents' phased traits and the transmitted alleles
+// (rfunction.cc:94-113, rfunction.h:81-107): allele 0 of a parent is its
+// maternal one; AU carries the affected allele maternally.
+uint32_t host_kid_trait(uint32_t mat, uint32_t pat, int i, int j) {
+    const uint32_t aff = 0x9Cu, kt = 0x6Cu;
+    uint32_t ma = (aff >> (2 * mat + i)) & 1u;
+    uint32_t pa = (aff >> (2 * pat + j)) & 1u;
+    return (kt >> (2 * ((ma << 1) | pa))) & 3u;
+}
+
 // rank byte: 2 bits per trait value v = number of legal values below v
 uint8_t rank_byte(unsigned mask) {
     uint8_t r = 0;
@@ -63,7 +73,8 @@ struct Arena {
 
 }  // namespace
 
-std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int npos, PeelProgram* out) {
+std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int npos, int waves,
+                                PeelProgram* out) {
     PeelProgram& P = *out;
     P = PeelProgram();
     const int N = ped.N;
@@ -138,7 +149,8 @@ std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int
             d.k = k;
             d.peel = pv.peel[o];
             d.size = size[o];
-            d.mat_digit = d.pat_digit = -1;
+            d.mat_digit = -1;
+            d.idx_base = 0;
             d.task_base = (int)P.tasks.size();
             // cells in ascending dense order whose digits are all legal
             int ncell = 0;
@@ -182,8 +194,7 @@ std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int
             double tau = 0.0;
             if (d.type == kChild) {
                 d.mat_digit = digit_of(ped.mother[d.peel]);
-                d.pat_digit = digit_of(ped.father[d.peel]);
-                if (ped.mother[d.peel] < 0 || d.mat_digit < 0 || d.pat_digit < 0)
+                if (ped.mother[d.peel] < 0 || d.mat_digit < 0 || digit_of(ped.father[d.peel]) < 0)
                     return fmt("op %d: CHILD peel without both parents in the cutset", o);
                 tau = 6.0;
             }
@@ -210,6 +221,52 @@ std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int
             }
             flops += 4.0 * d.size * (d.n_prev + tau + 1.0);
             lds += 4.0 * d.size * d.n_prev;
+            // precompiled terms of every cell
+            const int pat_digit = d.type == kChild ? digit_of(ped.father[d.peel]) : -1;
+            d.idx_base = (int)P.idx.size();
+            if (d.n_child > 7) return fmt("op %d: PARENT peel with more than 7 children in the cutset", o);
+            for (int c = 0; c < d.size; ++c) {
+                const uint32_t cell = P.tasks[d.task_base + c] & 0xFFFFu;
+                for (int t = 0; t < 4; ++t) {
+                    uint32_t val;
+                    if (d.type == kChild) {
+                        uint32_t mat = (cell >> (2 * d.mat_digit)) & 3u, pat = (cell >> (2 * pat_digit)) & 3u;
+                        val = host_kid_trait(mat, pat, t >> 1, t & 1);
+                    } else {
+                        val = (uint32_t)t;
+                    }
+                    uint32_t info = val;
+                    if (ped.disease_prob[4 * d.peel + val] != 0.0) info |= 4u;
+                    for (int ch = 0; ch < d.n_child; ++ch) {
+                        const ChildDesc& cd = P.children[d.child_off + ch];
+                        uint32_t kid = (cell >> (2 * cd.child_digit)) & 3u;
+                        uint32_t other = (cell >> (2 * cd.other_digit)) & 3u;
+                        uint32_t mat = cd.peel_is_mother ? val : other, pat = cd.peel_is_mother ? other : val;
+                        uint32_t m = 0;
+                        for (int ij = 0; ij < 4; ++ij)
+                            if (host_kid_trait(mat, pat, ij >> 1, ij & 1) == kid) m |= 1u << ij;
+                        info |= m << (4 + 4 * ch);
+                    }
+                    P.tinfo.push_back(info);
+                    const uint32_t ext = cell | (val << (2 * k));
+                    const bool valid = (info & 4u) != 0u;  // invalid terms are never read
+                    for (int q = 0; q < d.n_prev; ++q) {
+                        if (!valid) {
+                            P.idx.push_back(0);
+                            continue;
+                        }
+                        const PrevDesc& pd = P.prevs[d.prev_off + q];
+                        uint32_t ix = 0;
+                        for (int j = 0; j < pd.n_digits; ++j) {
+                            const uint32_t w = pd.digit[j];
+                            const uint32_t v = (ext >> (2 * (w & 15u))) & 3u;
+                            ix += ((((w >> 4) & 255u) >> (2 * v)) & 3u) * (w >> 12);
+                        }
+                        if (ix >= (uint32_t)pd.size) return fmt("op %d: consumed-table index out of range", o);
+                        P.idx.push_back((uint16_t)ix);
+                    }
+                }
+            }
         }
         // allocate this level's outputs, then release the tables it consumed
         for (int o : by_level[L]) {
@@ -221,6 +278,33 @@ std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int
     }
     P.level_base[P.n_levels] = (int)P.tasks.size();
     for (auto& pd : P.prevs) pd.tab_off = offs[pd.op];
+    // wave schedule: per level, ops sorted by estimated cost, each to the
+    // least loaded wave
+    if (waves < 1 || waves > 16) return fmt("workgroup of %d wavefronts outside 1..16", waves);
+    P.waves = waves;
+    P.sched_off.assign((size_t)P.n_levels * waves + 1, 0);
+    P.sched_ops.clear();
+    for (int L = 0; L < P.n_levels; ++L) {
+        std::vector<std::pair<double, int>> cost;
+        for (int o : by_level[L]) {
+            const OpDesc& d = P.ops[o];
+            double iters = (double)((d.size * npos + 63) / 64);
+            cost.push_back({iters * (2.0 + d.n_prev + 3.0 * d.n_child) + 1.0, o});
+        }
+        std::stable_sort(cost.begin(), cost.end(), [](auto& a, auto& b) { return a.first > b.first; });
+        std::vector<double> load(waves, 0.0);
+        std::vector<std::vector<int>> per(waves);
+        for (auto& c : cost) {
+            int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[w] += c.first;
+            per[w].push_back(c.second);
+        }
+        for (int w = 0; w < waves; ++w) {
+            P.sched_off[(size_t)L * waves + w] = (int)P.sched_ops.size();
+            P.sched_ops.insert(P.sched_ops.end(), per[w].begin(), per[w].end());
+        }
+    }
+    P.sched_off[(size_t)P.n_levels * waves] = (int)P.sched_ops.size();
     P.arena_elems = arena.peak;
     P.flops_per_eval = flops;
     P.lds_bytes_per_eval = lds;  // in elements; scaled by the element size by the caller

@@ -40,7 +40,7 @@ struct OpDesc {
     int32_t tab_off;      // arena offset (elements) of position 0
     int32_t size;         // compact cells per position
     int32_t mat_digit;    // CHILD: cutset digit of the mother
-    int32_t pat_digit;    // CHILD: cutset digit of the father
+    int32_t idx_base;     // into idx[]: (cell*4 + term)*n_prev + q
     int32_t n_child;      // PARENT
     int32_t child_off;    // into children[]
     int32_t prev_off;     // into prevs[]
@@ -72,8 +72,19 @@ struct PeelProgram {
     std::vector<OpDesc> ops;
     std::vector<PrevDesc> prevs;
     std::vector<ChildDesc> children;
-    std::vector<uint32_t> tasks;        // all levels, concatenated
+    std::vector<uint32_t> tasks;        // dense cell of every compact cell, per op at task_base
+    // Precompiled terms, 4 per compact cell (task_base*4 + cell*4 + t): the
+    // trait value of the summed node (CHILD: kid trait of (i, j) = (t>>1, t&1);
+    // otherwise the peel node's value t) in bits 0-1, bit 2 set when its
+    // disease probability is non-zero, and for PARENT ops bits 4.. hold per
+    // child a 4-bit mask of the (i, j) that give that child its trait.
+    std::vector<uint32_t> tinfo;
+    // compact index into each consumed table for every (cell, term)
+    std::vector<uint16_t> idx;
     std::vector<int32_t> level_base;    // [n_levels+1] into tasks
+    int waves = 1;                      // wavefronts per workgroup the schedule is built for
+    std::vector<int32_t> sched_off;     // [n_levels*waves+1] into sched_ops
+    std::vector<int32_t> sched_ops;     // ops of level L run by wave w: sched_off[L*waves+w] ..
     std::vector<uint8_t> lod_mask;      // [N] trait values with non-zero disease prob
     int arena_elems = 0;                // per position-set (all positions)
     double flops_per_eval = 0.0;        // algorithmic flops per (graph, interval, position)
@@ -99,8 +110,11 @@ struct PeelView {
     const int32_t* child;
 };
 
-// Compiles `peel` for `ped` with `npos` positions folded into each table slot.
-// Returns an empty string on success, else the reason the table is rejected.
-std::string compile_lod_program(const PedigreeView& ped, const PeelView& peel, int npos, PeelProgram* out);
+// Compiles `peel` for `ped` with `npos` positions folded into each table slot
+// and a per-level schedule of ops over `waves` wavefronts (longest first onto
+// the least loaded wave).  Returns an empty string on success, else the reason
+// the table is rejected.
+std::string compile_lod_program(const PedigreeView& ped, const PeelView& peel, int npos, int waves,
+                                PeelProgram* out);
 
 }  // namespace sl

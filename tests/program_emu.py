@@ -39,7 +39,71 @@ class Program:
         self.tasks = [x & 0xFFFFFFFF for x in w[p: p + nt]]
         p += nt
         self.level_base = w[p: p + self.n_levels + 1]
-        assert len(self.level_base) == self.n_levels + 1
+        p += self.n_levels + 1
+        self.waves = w[p]
+        p += 1
+        self.sched_off = w[p: p + self.n_levels * self.waves + 1]
+        p += self.n_levels * self.waves + 1
+        self.sched_ops = w[p: p + self.n_ops]
+        p += self.n_ops
+        nti = w[p]
+        self.tinfo = [x & 0xFFFFFFFF for x in w[p + 1: p + 1 + nti]]
+        p += 1 + nti
+        nix = w[p]
+        self.idx = w[p + 1: p + 1 + nix]
+        assert p + 1 + nix == len(w)
+
+    def check_terms(self, case):
+        """The precompiled term words and table indices agree with what the
+        digit maps give (kid trait of (i, j), validity, child masks, indices)."""
+        for o, d in enumerate(self.ops):
+            typ, k, peel, n_prev, tab_off, size, md, idx_base, n_child, child_off, prev_off, task_base = d
+            prs = self.prevs[prev_off: prev_off + n_prev]
+            cds = self.children[child_off: child_off + n_child]
+            for c in range(size):
+                cell = self.tasks[task_base + c] & 0xFFFF
+                for t in range(4):
+                    if typ == 1:
+                        mother, father = int(case.mother[peel]), int(case.father[peel])
+                        cut = list(case.ops.cutset(o))
+                        mat = (cell >> (2 * cut.index(mother))) & 3
+                        pat = (cell >> (2 * cut.index(father))) & 3
+                        val = kid_trait(mat, pat, t >> 1, t & 1)
+                    else:
+                        val = t
+                    info = self.tinfo[(task_base + c) * 4 + t]
+                    assert info & 3 == val
+                    assert bool(info & 4) == (case.disease_prob[peel, val] != 0)
+                    for ch, (person, cdig, odig, pim) in enumerate(cds):
+                        kid = (cell >> (2 * cdig)) & 3
+                        other = (cell >> (2 * odig)) & 3
+                        mat, pat = (val, other) if pim else (other, val)
+                        m = sum(1 << ij for ij in range(4) if kid_trait(mat, pat, ij >> 1, ij & 1) == kid)
+                        assert (info >> (4 + 4 * ch)) & 15 == m
+                    ext = cell | (val << (2 * k))
+                    for q, pr in enumerate(prs):
+                        if not info & 4:
+                            assert self.idx[idx_base + (c * 4 + t) * n_prev + q] == 0
+                            continue
+                        ix = 0
+                        for j in range(pr[3]):
+                            wd = pr[4 + j] & 0xFFFFFFFF
+                            v = (ext >> (2 * (wd & 15))) & 3
+                            ix += ((((wd >> 4) & 255) >> (2 * v)) & 3) * (wd >> 12)
+                        assert self.idx[idx_base + (c * 4 + t) * n_prev + q] == ix
+
+    def check_schedule(self):
+        """Every op is run exactly once, by one wave, in its own level."""
+        seen = []
+        for L in range(self.n_levels):
+            level_ops = {self.tasks[t] >> 16 for t in range(self.level_base[L], self.level_base[L + 1])}
+            ran = []
+            for w in range(self.waves):
+                a, b = self.sched_off[L * self.waves + w], self.sched_off[L * self.waves + w + 1]
+                ran += self.sched_ops[a:b]
+            assert sorted(ran) == sorted(level_ops), L
+            seen += ran
+        assert sorted(seen) == list(range(self.n_ops))
 
     def check_bounds(self, npos):
         """Every table slot the kernel touches lies inside the arena."""
@@ -61,7 +125,7 @@ class Program:
         N, F = case.N, case.F
         ftype = np.float32 if fp32 else float
         arena = [ftype(0)] * self.arena
-        maxv = np.zeros((self.n_ops, npos), np.float32)
+        maxv = np.zeros(self.n_ops, np.float32)
         trait = case.disease_prob.astype(np.float32) if fp32 else case.disease_prob
         code = [0] * N
         if dg is not None:
@@ -71,10 +135,10 @@ class Program:
                 code[p] = int(col0[2 * p]) | (int(col1[2 * p]) << 1) | (int(col0[2 * p + 1]) << 2) | (int(col1[2 * p + 1]) << 3)
         pt = case.partial_theta[locus] if dg is not None else 0.0
 
-        def scale(op, pos):
-            if not fp32 or maxv[op, pos] == 0:
+        def scale(op, pos):  # one exponent per table, shared by all positions
+            if not fp32 or maxv[op] == 0:
                 return 1.0, 0
-            e = math.frexp(float(maxv[op, pos]))[1]  # max = m * 2^e, m in [0.5,1)
+            e = math.frexp(float(maxv[op]))[1]  # max = m * 2^e, m in [0.5,1)
             return ftype(2.0 ** -e), e
 
         def prev_value(pr, ext, pos):
@@ -106,61 +170,41 @@ class Program:
                 th = (ftype(tl), ftype(1.0 - tl), ftype(tr), ftype(1.0 - tr))
                 for t in range(base, end):
                     op = self.tasks[t] >> 16
-                    cell = self.tasks[t] & 0xFFFF
-                    typ, k, peel, n_prev, tab_off, size, md, pd, n_child, child_off, prev_off, task_base = self.ops[op]
+                    typ, k, peel, n_prev, tab_off, size, md, idx_base, n_child, child_off, prev_off, task_base = self.ops[op]
+                    c = t - task_base
                     prs = self.prevs[prev_off: prev_off + n_prev]
+                    kids = [cd[0] for cd in self.children[child_off: child_off + n_child]]
                     total = ftype(0.0)
-                    if typ == 1:
-                        mat, pat = (cell >> (2 * md)) & 3, (cell >> (2 * pd)) & 3
-                        for ij in range(4):
-                            i, j = ij >> 1, ij & 1
-                            kt = kid_trait(mat, pat, i, j)
-                            tmp = ftype(trait[peel, kt])
-                            if tmp == 0:
-                                continue
-                            ext = cell | (kt << (2 * k))
-                            for pr in prs:
-                                tmp = ftype(tmp * prev_value(pr, ext, pos))
+                    for term in range(4):
+                        info = self.tinfo[t * 4 + term]
+                        if not info & 4:
+                            continue
+                        tmp = ftype(trait[peel, info & 3])
+                        for qq, pr in enumerate(prs):
+                            ix = self.idx[idx_base + (c * 4 + term) * n_prev + qq]
+                            x = arena[pr[1] + pos * pr[2] + ix]
+                            if fp32:
+                                x = ftype(x * scale(pr[0], pos)[0])
+                            tmp = ftype(tmp * x)
+                        if typ == 1:
                             if dg is None:
                                 tmp = ftype(tmp * q)
                             else:
-                                tmp = ftype(tmp * ftype(q * rho(code[peel], i, j, th)))
-                            total = ftype(total + tmp)
-                    elif typ == 2:
-                        cds = self.children[child_off: child_off + n_child]
-                        for v in range(4):
-                            tmp = ftype(trait[peel, v])
-                            if tmp == 0:
-                                continue
-                            ext = cell | (v << (2 * k))
-                            for pr in prs:
-                                tmp = ftype(tmp * prev_value(pr, ext, pos))
+                                tmp = ftype(tmp * ftype(q * rho(code[peel], term >> 1, term & 1, th)))
+                        elif typ == 2:
                             cp = ftype(1.0)
-                            for person, cdig, odig, pim in cds:
-                                kid = (cell >> (2 * cdig)) & 3
-                                other = (cell >> (2 * odig)) & 3
-                                mat, pat = (v, other) if pim else (other, v)
+                            for ch, person in enumerate(kids):
+                                m = (info >> (4 + 4 * ch)) & 15
                                 ct = ftype(0.0)
                                 for ij in range(4):
-                                    i, j = ij >> 1, ij & 1
-                                    if kid_trait(mat, pat, i, j) != kid:
-                                        continue
-                                    ct = ftype(ct + (q if dg is None else ftype(q * rho(code[person], i, j, th))))
+                                    if (m >> ij) & 1:
+                                        ct = ftype(ct + (q if dg is None else ftype(q * rho(code[person], ij >> 1, ij & 1, th))))
                                 cp = ftype(cp * ct)
                             tmp = ftype(tmp * cp)
-                            total = ftype(total + tmp)
-                    else:
-                        for v in range(4):
-                            tmp = ftype(trait[peel, v])
-                            if tmp == 0:
-                                continue
-                            ext = cell | (v << (2 * k))
-                            for pr in prs:
-                                tmp = ftype(tmp * prev_value(pr, ext, pos))
-                            total = ftype(total + tmp)
-                    arena[tab_off + pos * size + (t - task_base)] = total
+                        total = ftype(total + tmp)
+                    arena[tab_off + pos * size + c] = total
                     if fp32:
-                        maxv[op, pos] = max(maxv[op, pos], np.float32(total))
+                        maxv[op] = max(maxv[op], np.float32(total))
         out = []
         last = self.ops[self.last_op]
         for pos in range(npos):

@@ -32,6 +32,8 @@ def test_program_bounds_and_fp64_bit_exact(name):
     case, fx = load_golden(name)
     P = Program(native.program_export(case))
     P.check_bounds(case.nlod)
+    P.check_schedule()
+    P.check_terms(case)
     o = Oracle(case)
     assert P.run(case, None, 0)[0] == o.trait_prob()
     g = fx["graphs"][0]
@@ -73,3 +75,19 @@ def test_pack_graphs_layout():
         for l in (0, 57):
             for b in range(2 * (case.N - case.F)):
                 assert ((p[gi, l, b // 32] >> (b % 32)) & 1) == g[gi, l, 2 * case.F + b]
+
+
+def test_program_ped150_terms_and_bounds(tmp_path):
+    """150 people, 3 loops: the compiled program's schedule, precompiled terms
+    and arena bounds, and two intervals executed against the oracle."""
+    from swiftlink_amd import synth
+    case, data = synth.make_case("ped150_snp1000", str(tmp_path))
+    P = Program(native.program_export(case))
+    P.check_schedule()
+    P.check_bounds(case.nlod)
+    P.check_terms(case)
+    g = synth.true_graph(data, case)
+    o = Oracle(case)
+    for l in (0, 600):
+        ref = [np.log(o.trait_likelihood(g, l, off)) for off in range(1, case.nlod + 1)]
+        assert np.array_equal(P.run(case, g, l), ref)
