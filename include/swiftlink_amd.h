@@ -145,6 +145,30 @@ int sl_profile_read(sl_ctx* ctx, int32_t kernel, double* ms, int64_t* launches);
  * position) from the compiled program: flops and LDS bytes. */
 int sl_lod_work(sl_ctx* ctx, double* flops_per_eval, double* lds_bytes_per_eval);
 
+/* ---- MCMC chains on the device (the intended GPUMarkovChain,
+ * gpu_markov_chain.h:76-125) ---- */
+
+/* Uploads n_chains descent graphs (host, int32 [C][M][2N]) as the chain
+ * states, packed on the device; clears the per-chain status.  Counterpart of
+ * GPUMarkovChain's copy_to_gpu (gpu_markov_chain.h:110). */
+int sl_mcmc_init(sl_ctx* ctx, int32_t n_chains, const int32_t* graphs);
+
+/* Copies the chain states back in the reference layout int32 [C][M][2N]
+ * (founder slots 0).  Counterpart of copy_from_gpu (gpu_markov_chain.h:111). */
+int sl_mcmc_get_graphs(sl_ctx* ctx, int32_t* graphs);
+
+/* Per-chain status: 0, or SL_E_ZERO_LIKELIHOOD / SL_E_ILLEGAL_GRAPH when a
+ * step met a zero likelihood (locus_sampler2.cc:137-142,
+ * meiosis_sampler.cc:33-36); the failing step leaves the graph unchanged. */
+int sl_mcmc_status(sl_ctx* ctx, int32_t* status);
+
+/* One L-sweep on every chain: LocusSampler::step (locus_sampler2.cc:128-159)
+ * at every locus, loci of parity first_parity ascending then the other parity
+ * (non-adjacent loci run concurrently, as run_gpu_lsampler_kernel launched
+ * with window 2 at gpu_markov_chain.cc:1060-1126).  Locus l of chain c draws
+ * the uniforms (draw, step=l, sweep, chain=c) of the Philox stream. */
+int sl_lsweep(sl_ctx* ctx, uint64_t seed, uint32_t sweep, int32_t first_parity);
+
 /* Compiles a peel table into the level-scheduled program the LOD kernel
  * runs and serialises it into buf (int32 words; *needed gets the length).
  * Host only, no device work: lets the program's indexing be checked on a

@@ -11,6 +11,8 @@
 #include "../../include/swiftlink_amd.h"
 #include "kernels.h"
 #include "program.h"
+#include "sampler_kernels.h"
+#include "sampler_program.h"
 
 using sl::PeelProgram;
 
@@ -59,6 +61,30 @@ struct sl_ctx {
     int gpw = 4;
     size_t lds = 0;
     int arena_bytes = 0, maxbits_bytes = 0;
+    // sampler program and chain state
+    sl::SampProgram sprog;
+    int samp_block = 128;
+    int cells_cap = 0;
+    size_t samp_lds = 0;
+    sl::SampOp* d_sops = nullptr;
+    sl::SampPrev* d_sprevs = nullptr;
+    sl::ChildDesc* d_schildren = nullptr;
+    int32_t* d_ssched_off = nullptr;
+    int32_t* d_ssched_ops = nullptr;
+    int32_t* d_rlevel_off = nullptr;
+    int32_t* d_rlevel_ops = nullptr;
+    int32_t* d_mother = nullptr;
+    int32_t* d_father = nullptr;
+    uint8_t* d_legal_t = nullptr;
+    double* d_gprob_d = nullptr;
+    float* d_gprob_f = nullptr;
+    double* d_theta = nullptr;
+    uint32_t* d_chains = nullptr;   // [chains][M][W]
+    int32_t* d_status = nullptr;    // [chains]
+    int n_chains = 0;
+    bool samp_ready = false;
+    std::string samp_why;           // why the sampler program was rejected, if it was
+    std::vector<int32_t> mother_h, father_h;
     // profiling
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -173,7 +199,10 @@ int check_desc(const sl_pedigree_desc* p, const sl_map_desc* m, const sl_peel_de
 }
 
 int release_all(sl_ctx* c) {
-    void* ptrs[] = {c->d_ops, c->d_prevs, c->d_children, c->d_tasks, c->d_tinfo, c->d_idx, c->d_level_base,
+    void* ptrs[] = {c->d_sops, c->d_sprevs, c->d_schildren, c->d_ssched_off, c->d_ssched_ops, c->d_rlevel_off,
+                    c->d_rlevel_ops, c->d_mother, c->d_father, c->d_legal_t, c->d_gprob_d, c->d_gprob_f,
+                    c->d_theta, c->d_chains, c->d_status,
+                    c->d_ops, c->d_prevs, c->d_children, c->d_tasks, c->d_tinfo, c->d_idx, c->d_level_base,
                     c->d_sched_off,
                     c->d_sched_ops, c->d_trait_d,
                     c->d_trait_f, c->d_ptheta, c->d_logt, c->d_loga, c->d_acc, c->d_scalar,
@@ -285,6 +314,51 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
         release_all(c);
         delete c;
         return fail(nullptr, rc, m);
+    }
+    // sampler program (the L-sampler needs the per-locus legal masks to bound its LDS)
+    {
+        int64_t max_cells = 0;
+        c->samp_why = sl::compile_sampler_program(pv, kv, ped->legal, c->M, c->samp_block / 64, &c->sprog, &max_cells);
+        if (c->samp_why.empty()) {
+            c->cells_cap = (int)((max_cells + 7) / 8 * 8);
+            c->samp_lds = sl::lsampler_lds_bytes(c->precision, c->cells_cap, c->sprog.n_ops, c->N, c->F, c->samp_block);
+            if (c->samp_lds > 160 * 1024) c->samp_why = "sampler tables need more than 160 KiB of LDS at some locus";
+        }
+        if (c->samp_why.empty()) {
+            std::vector<uint8_t> lt((size_t)c->M * c->N);
+            for (int p = 0; p < c->N; ++p)
+                for (int l = 0; l < c->M; ++l) lt[(size_t)l * c->N + p] = ped->legal[(size_t)p * c->M + l];
+            std::vector<float> gf((size_t)c->N * c->M * 4);
+            for (size_t i = 0; i < gf.size(); ++i) gf[i] = (float)ped->geno_prob[i];
+            c->mother_h.assign(ped->mother, ped->mother + c->N);
+            c->father_h.assign(ped->father, ped->father + c->N);
+            if ((rc = upload(c, &c->d_sops, c->sprog.ops.data(), c->sprog.ops.size())) ||
+                (rc = upload(c, &c->d_sprevs, c->sprog.prevs.data(), c->sprog.prevs.size())) ||
+                (rc = upload(c, &c->d_schildren, c->sprog.children.data(), c->sprog.children.size())) ||
+                (rc = upload(c, &c->d_ssched_off, c->sprog.sched_off.data(), c->sprog.sched_off.size())) ||
+                (rc = upload(c, &c->d_ssched_ops, c->sprog.sched_ops.data(), c->sprog.sched_ops.size())) ||
+                (rc = upload(c, &c->d_rlevel_off, c->sprog.rlevel_off.data(), c->sprog.rlevel_off.size())) ||
+                (rc = upload(c, &c->d_rlevel_ops, c->sprog.rlevel_ops.data(), c->sprog.rlevel_ops.size())) ||
+                (rc = upload(c, &c->d_mother, ped->mother, (size_t)c->N)) ||
+                (rc = upload(c, &c->d_father, ped->father, (size_t)c->N)) ||
+                (rc = upload(c, &c->d_legal_t, lt.data(), lt.size())) ||
+                (rc = upload(c, &c->d_gprob_d, ped->geno_prob, (size_t)c->N * c->M * 4)) ||
+                (rc = upload(c, &c->d_gprob_f, gf.data(), gf.size())) ||
+                (rc = upload(c, &c->d_theta, c->theta.data(), c->theta.size()))) {
+                std::string m = c->err;
+                release_all(c);
+                delete c;
+                return fail(nullptr, rc, m);
+            }
+            e = sl::configure_lsampler_lds(c->precision, c->samp_lds);
+            if (e != hipSuccess) {
+                std::string m = std::string("sampler LDS configuration failed: ") + hipGetErrorString(e);
+                release_all(c);
+                delete c;
+                return fail(nullptr, SL_E_DEVICE, m);
+            }
+            c->samp_ready = true;
+        }
     }
     e = sl::launch_fill(c->d_acc, -INFINITY, nacc, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -457,6 +531,112 @@ int sl_lod_work(sl_ctx* c, double* flops, double* lds_bytes) {
     if (!c) return SL_E_INVALID;
     if (flops) *flops = c->prog.flops_per_eval;
     if (lds_bytes) *lds_bytes = c->prog.lds_bytes_per_eval * (c->precision == SL_FP64 ? 8.0 : 4.0);
+    return SL_OK;
+}
+
+static int need_sampler(sl_ctx* c) {
+    if (!c->samp_ready) return fail(c, SL_E_UNSUPPORTED, "sampler unavailable: " + c->samp_why);
+    return SL_OK;
+}
+
+int sl_mcmc_init(sl_ctx* c, int32_t n_chains, const int32_t* graphs) {
+    if (!c || n_chains <= 0 || !graphs) return SL_E_INVALID;
+    int rc = need_sampler(c);
+    if (rc) return rc;
+    const size_t words = (size_t)n_chains * c->M * c->W;
+    if (n_chains != c->n_chains) {
+        if (c->d_chains) SL_HIP(c, hipFree(c->d_chains));
+        if (c->d_status) SL_HIP(c, hipFree(c->d_status));
+        c->d_chains = nullptr;
+        c->d_status = nullptr;
+        SL_HIP(c, hipMalloc((void**)&c->d_chains, words * sizeof(uint32_t)));
+        SL_HIP(c, hipMalloc((void**)&c->d_status, (size_t)n_chains * sizeof(int32_t)));
+        c->n_chains = n_chains;
+    }
+    rc = stage_graphs(c, graphs, n_chains);
+    if (rc) return rc;
+    SL_HIP(c, hipMemcpyAsync(c->d_chains, c->d_packed, words * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    SL_HIP(c, hipMemsetAsync(c->d_status, 0, (size_t)n_chains * sizeof(int32_t), c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    return SL_OK;
+}
+
+int sl_mcmc_get_graphs(sl_ctx* c, int32_t* out) {
+    if (!c || !out || c->n_chains <= 0) return SL_E_INVALID;
+    const size_t words = (size_t)c->n_chains * c->M * c->W;
+    std::vector<uint32_t> h(words);
+    SL_HIP(c, hipMemcpyAsync(h.data(), c->d_chains, words * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    const int nm = 2 * (c->N - c->F);
+    for (int ch = 0; ch < c->n_chains; ++ch)
+        for (int l = 0; l < c->M; ++l) {
+            int32_t* col = out + ((size_t)ch * c->M + l) * 2 * c->N;
+            const uint32_t* w = h.data() + ((size_t)ch * c->M + l) * c->W;
+            for (int i = 0; i < 2 * c->F; ++i) col[i] = 0;
+            for (int b = 0; b < nm; ++b) col[2 * c->F + b] = (int32_t)((w[b >> 5] >> (b & 31)) & 1u);
+        }
+    return SL_OK;
+}
+
+int sl_mcmc_status(sl_ctx* c, int32_t* status) {
+    if (!c || !status || c->n_chains <= 0) return SL_E_INVALID;
+    SL_HIP(c, hipMemcpyAsync(status, c->d_status, (size_t)c->n_chains * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    return SL_OK;
+}
+
+static sl::SampArgs samp_args(sl_ctx* c) {
+    sl::SampArgs A{};
+    A.ops = c->d_sops;
+    A.prevs = c->d_sprevs;
+    A.children = c->d_schildren;
+    A.sched_off = c->d_ssched_off;
+    A.sched_ops = c->d_ssched_ops;
+    A.rlevel_off = c->d_rlevel_off;
+    A.rlevel_ops = c->d_rlevel_ops;
+    A.n_ops = c->sprog.n_ops;
+    A.n_levels = c->sprog.n_levels;
+    A.n_rlevels = c->sprog.n_rlevels;
+    A.waves = c->sprog.waves;
+    A.N = c->N;
+    A.F = c->F;
+    A.M = c->M;
+    A.W = c->W;
+    A.mother = c->d_mother;
+    A.father = c->d_father;
+    A.legal_t = c->d_legal_t;
+    A.geno_prob_d = c->d_gprob_d;
+    A.geno_prob_f = c->d_gprob_f;
+    A.theta = c->d_theta;
+    A.graphs = c->d_chains;
+    A.status = c->d_status;
+    A.cells_cap = c->cells_cap;
+    return A;
+}
+
+int sl_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int32_t first_parity) {
+    if (!c || (first_parity != 0 && first_parity != 1)) return SL_E_INVALID;
+    int rc = need_sampler(c);
+    if (rc) return rc;
+    if (c->n_chains <= 0) return fail(c, SL_E_INVALID, "no chains: call sl_mcmc_init first");
+    sl::SampArgs A = samp_args(c);
+    A.seed = seed;
+    A.sweep = sweep;
+    for (int pass = 0; pass < 2; ++pass) {
+        A.parity = pass == 0 ? first_parity : 1 - first_parity;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (c->prof) {
+            e0 = take_event(c);
+            e1 = take_event(c);
+            if (e0 && e1) SL_HIP(c, hipEventRecord(e0, c->stream));
+        }
+        SL_HIP(c, sl::launch_lsampler(A, c->precision, c->samp_block, c->samp_lds, c->n_chains, c->stream));
+        if (c->prof && e0 && e1) {
+            SL_HIP(c, hipEventRecord(e1, c->stream));
+            c->ev_used.push_back({SL_KERNEL_LSAMPLER, {e0, e1}});
+        }
+    }
     return SL_OK;
 }
 

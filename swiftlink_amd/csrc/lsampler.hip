@@ -1,0 +1,473 @@
+// L-sampler kernel for gfx950: one workgroup per (locus, chain).
+//
+// One launch samples every locus of one parity (even or odd) for all chains;
+// loci of one parity never neighbour each other, so each workgroup reads the
+// fixed columns l-1 and l+1 and rewrites column l.  Two launches make an
+// L-sweep; that equals the reference's sequential LocusSampler::step over the
+// loci in the order (parity p0 ascending, then the other parity).
+//
+// Per workgroup (LocusSampler::step, locus_sampler2.cc:128-159):
+//  1. legal masks of every person at l (genotype elimination), compact table
+//     sizes and LDS offsets (prefix sum), per-child transmission
+//     distributions from the neighbouring indicators
+//     (get_recombination_distribution, sampler_rfunction.cc:102-157), and the
+//     op draws of the Philox stream;
+//  2. forward peel by levels, every op owned by one wavefront: presum cells
+//     (cell, v) and tables as in SamplerRfunction::evaluate_{child,parent}_peel
+//     and Rfunction::evaluate_partner_peel (sampler_rfunction.cc:190-287,
+//     rfunction.cc:116-142), same operation order;
+//  3. reverse sampling by reverse-dependency levels, one lane per op
+//     (SamplerRfunction::sample, sampler_rfunction.cc:159-188), op o draws
+//     uniform number n_ops-1-o as in the reference's reverse loop;
+//  4. meiosis indicators (sample_meiosis_indicators, locus_sampler2.cc:93-125):
+//     heterozygous parents fix the bit, homozygous parents draw in person
+//     order after the op draws; the whole column l is rewritten.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+
+#include "sampler_program.h"
+#include "sampler_kernels.h"
+#include "philox.h"
+
+namespace sl {
+
+namespace {
+
+typedef const __attribute__((address_space(4))) int32_t* cint_p;
+typedef const __attribute__((address_space(4))) double* cdbl_p;
+typedef const __attribute__((address_space(4))) float* cflt_p;
+
+enum { PT_UU = 0, PT_AA = 1, PT_AU = 2, PT_UA = 3 };
+
+// r-th set bit (r < popcount) of a 4-bit mask
+__device__ __forceinline__ uint32_t select_bit(uint32_t m, uint32_t r) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        if ((m >> b) & 1u) {
+            if (r == 0u) { v = (uint32_t)b; r = 0xFFu; }
+            else if (r != 0xFFu) --r;
+        }
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t rank_in(uint32_t m, uint32_t v) { return __popc(m & ((1u << v) - 1u)); }
+
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__device__ __forceinline__ float scale_from_bits(uint32_t bits) {
+    uint32_t ef = (bits >> 23) & 0xFFu;
+    if (ef == 0u) return 1.0f;
+    return __uint_as_float((253u - ef) << 23);
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+    int x = __float_as_int(v);
+    x = __float_as_int(fmaxf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, true))));
+    x = __float_as_int(fmaxf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, true))));
+    x = __float_as_int(fmaxf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, true))));
+    x = __float_as_int(fmaxf(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, true))));
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(x, 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(x, 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(x, 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(x, 48));
+    return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+}
+
+// exclusive prefix sum of a[0..n) in place (LDS), whole workgroup; returns total
+__device__ int block_exclusive_scan(int* a, int n, int* tmp) {
+    const int T = blockDim.x, t = threadIdx.x;
+    const int per = (n + T - 1) / T;
+    const int b0 = min(n, t * per), b1 = min(n, b0 + per);
+    int s = 0;
+    for (int i = b0; i < b1; ++i) s += a[i];
+    tmp[t] = s;
+    __syncthreads();
+    for (int off = 1; off < T; off <<= 1) {
+        int v = t >= off ? tmp[t - off] : 0;
+        __syncthreads();
+        tmp[t] += v;
+        __syncthreads();
+    }
+    int run = t > 0 ? tmp[t - 1] : 0;
+    const int total = tmp[T - 1];
+    for (int i = b0; i < b1; ++i) {
+        int x = a[i];
+        a[i] = run;
+        run += x;
+    }
+    __syncthreads();
+    return total;
+}
+
+template <typename T>
+__device__ __forceinline__ T trait_at(const SampArgs& A, int person, int locus, int v);
+template <>
+__device__ __forceinline__ double trait_at<double>(const SampArgs& A, int person, int locus, int v) {
+    return ((cdbl_p)A.geno_prob_d)[((size_t)person * A.M + locus) * 4 + v];
+}
+template <>
+__device__ __forceinline__ float trait_at<float>(const SampArgs& A, int person, int locus, int v) {
+    return ((cflt_p)A.geno_prob_f)[((size_t)person * A.M + locus) * 4 + v];
+}
+
+// transmission probability of kid trait `kid` from parent traits (m, p)
+// (transmission_matrix, sampler_rfunction.cc:350-419): mat_dist x pat_dist
+template <typename T>
+__device__ __forceinline__ T trans(const T* dist, uint32_t m, uint32_t p, uint32_t kid) {
+    // allele of the kid from each parent: maternal A for AA/AU, paternal A for AA/UA
+    const uint32_t am = (kid == PT_AA || kid == PT_AU) ? 1u : 0u;
+    const uint32_t ap = (kid == PT_AA || kid == PT_UA) ? 1u : 0u;
+    T md, pd;
+    if (m == PT_UU) md = am ? (T)0 : (T)1;
+    else if (m == PT_AA) md = am ? (T)1 : (T)0;
+    else md = dist[(m == PT_AU ? 0 : 2) + am];
+    if (p == PT_UU) pd = ap ? (T)0 : (T)1;
+    else if (p == PT_AA) pd = ap ? (T)1 : (T)0;
+    else pd = dist[4 + (p == PT_AU ? 0 : 2) + ap];
+    return md * pd;
+}
+
+template <typename T, bool RESCALE>
+__global__ void lsampler_kernel(SampArgs A) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int N = A.N, F = A.F, M = A.M, n_ops = A.n_ops;
+    const int NF = N - F;
+    // LDS carve-up (sizes fixed by the host: cells_cap, n_ops, N)
+    T* tab = reinterpret_cast<T*>(smem);
+    T* pre = tab + A.cells_cap;
+    T* dist = pre + 4 * (size_t)A.cells_cap;           // [NF][8]
+    double* uop = reinterpret_cast<double*>(dist + 8 * NF);  // [n_ops]
+    int* off = reinterpret_cast<int*>(uop + n_ops);     // [n_ops+1]
+    int* scan_tmp = off + n_ops + 1;                    // [blockDim]
+    int* mcount = scan_tmp + blockDim.x;                // [2NF] draw slots of meioses
+    uint32_t* maxbits = reinterpret_cast<uint32_t*>(mcount + 2 * NF);  // [n_ops]
+    uint8_t* mask = reinterpret_cast<uint8_t*>(maxbits + n_ops);       // [N]
+    uint8_t* pmk = mask + N;                            // [N]
+    uint8_t* bits = pmk + N;                            // [2NF]
+    __shared__ int s_status;
+
+    const int locus = A.parity + 2 * blockIdx.x;
+    const int chain = blockIdx.y;
+    if (locus >= M) return;  // uniform over the workgroup
+    uint32_t* g = A.graphs + (size_t)chain * M * A.W;
+    const uint32_t* colL = locus > 0 ? g + (size_t)(locus - 1) * A.W : nullptr;
+    const uint32_t* colR = locus < M - 1 ? g + (size_t)(locus + 1) * A.W : nullptr;
+    const double thL = locus > 0 ? A.theta[locus - 1] : 1.0;  // theta2 (l-1, l)
+    const double thR = locus < M - 1 ? A.theta[locus] : 1.0;  // theta (l, l+1)
+    const int tid = threadIdx.x;
+
+    // 1. masks, table sizes, transmission distributions, op draws
+    for (int p = tid; p < N; p += blockDim.x) mask[p] = A.legal_t[(size_t)locus * N + p];
+    if (tid == 0) s_status = 0;
+    __syncthreads();
+    for (int o = tid; o < n_ops; o += blockDim.x) {
+        cint_p d = (cint_p)(A.ops + o);
+        const int k = d[1];
+        int s = 1;
+        for (int q = 0; q < k; ++q) s *= __popc(mask[d[9 + q]]);
+        off[o] = s;
+    }
+    for (int b = tid; b < NF; b += blockDim.x) {
+        const int person = F + b;
+        for (int side = 0; side < 2; ++side) {
+            const int bit = 2 * b + side;
+            double tmp0 = 0.5, tmp1 = 0.5;
+            if (colL) {
+                const bool cross = ((colL[bit >> 5] >> (bit & 31)) & 1u) != 0u;
+                tmp0 *= cross ? thL : 1.0 - thL;
+                tmp1 *= cross ? 1.0 - thL : thL;
+            }
+            if (colR) {
+                const bool cross = ((colR[bit >> 5] >> (bit & 31)) & 1u) != 0u;
+                tmp0 *= cross ? thR : 1.0 - thR;
+                tmp1 *= cross ? 1.0 - thR : thR;
+            }
+            const double total = tmp0 + tmp1;
+            const double uAU = tmp1 / total, uUA = tmp0 / total;
+            T* dd = dist + 8 * (person - F) + 4 * side;
+            dd[0] = (T)uAU;
+            dd[1] = (T)(1.0 - uAU);
+            dd[2] = (T)uUA;
+            dd[3] = (T)(1.0 - uUA);
+        }
+    }
+    for (int o = tid; o < n_ops; o += blockDim.x)
+        uop[o] = philox_uniform(A.seed, (uint32_t)chain, A.sweep, (uint32_t)locus, (uint32_t)(n_ops - 1 - o));
+    __syncthreads();
+    const int total_cells = block_exclusive_scan(off, n_ops, scan_tmp);
+    if (tid == 0) off[n_ops] = total_cells;
+    __syncthreads();
+    if (total_cells > A.cells_cap) {  // cannot happen when the host bound holds
+        if (tid == 0) A.status[chain] = -6;
+        return;
+    }
+
+    // 2. forward peel by levels
+    const int W = A.waves;
+    const int wave = uni(tid >> 6);
+    const int lane = tid & 63;
+    for (int L = 0; L < A.n_levels; ++L) {
+        const int s0 = ((cint_p)A.sched_off)[L * W + wave];
+        const int s1 = ((cint_p)A.sched_off)[L * W + wave + 1];
+        for (int s = s0; s < s1; ++s) {
+            const int op = ((cint_p)A.sched_ops)[s];
+            cint_p d = (cint_p)(A.ops + op);
+            const int type = d[0], k = d[1], peel = d[2], n_prev = d[3], n_child = d[4], child_off = d[5];
+            const int prev_off = d[6], mat_digit = d[7], pat_digit = d[8];
+            // per-digit legal masks packed 4 bits per digit; digit k is the peel node
+            uint32_t cmw = 0;
+            int size = 1;
+            int stride[kMaxCut];
+#pragma unroll
+            for (int q = 0; q < kMaxCut; ++q) {
+                stride[q] = size;
+                if (q < k) {
+                    const uint32_t m = (uint32_t)uni(mask[d[9 + q]]);
+                    cmw |= m << (4 * q);
+                    size *= __popc(m);
+                }
+            }
+            const uint32_t pmask = (uint32_t)uni(mask[peel]);
+            cmw |= pmask << (4 * k);
+            T tr[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) tr[v] = trait_at<T>(A, peel, locus, v);
+            const int my_off = uni(off[op]);
+            // consumed tables: offset, per-digit source and stride at this locus
+            int p_off[kMaxPrev], p_nd[kMaxPrev];
+            T p_scale[kMaxPrev];
+            int p_src[kMaxPrev][kMaxCut], p_stride[kMaxPrev][kMaxCut];
+#pragma unroll
+            for (int q = 0; q < kMaxPrev; ++q) {
+                p_off[q] = 0;
+                p_nd[q] = 0;
+                p_scale[q] = (T)1;
+                int st = 1;
+#pragma unroll
+                for (int j = 0; j < kMaxCut; ++j) {
+                    p_src[q][j] = 0;
+                    p_stride[q][j] = 0;
+                }
+                if (q < n_prev) {
+                    cint_p pd = (cint_p)(A.prevs + prev_off + q);
+                    const int pop = pd[0];
+                    p_off[q] = uni(off[pop]);
+                    p_nd[q] = pd[1];
+                    if (RESCALE) p_scale[q] = (T)scale_from_bits(maxbits[pop]);
+#pragma unroll
+                    for (int j = 0; j < kMaxCut; ++j) {
+                        if (j < p_nd[q]) {
+                            const int src = pd[2 + j];
+                            p_src[q][j] = src;
+                            p_stride[q][j] = st;
+                            st *= __popc((cmw >> (4 * src)) & 15u);
+                        }
+                    }
+                }
+            }
+            int kids[7], kid_digit[7], other_digit[7], kid_pim[7];
+#pragma unroll
+            for (int c = 0; c < 7; ++c) {
+                cint_p cd = (cint_p)(A.children + child_off + (c < n_child ? c : 0));
+                kids[c] = c < n_child ? cd[0] : F;
+                kid_digit[c] = c < n_child ? cd[1] : 0;
+                other_digit[c] = c < n_child ? cd[2] : 0;
+                kid_pim[c] = c < n_child ? cd[3] : 0;
+            }
+            float mx = 0.0f;
+            for (int c = lane; c < size; c += 64) {
+                // mixed-radix decode of the compact cell into trait values (2 bits per digit)
+                uint32_t ext = 0;
+                int rem = c;
+#pragma unroll
+                for (int q = 0; q < kMaxCut; ++q) {
+                    if (q < k) {
+                        const uint32_t m = (cmw >> (4 * q)) & 15u;
+                        const int n = __popc(m);
+                        const int quo = (int)(((float)rem + 0.5f) / (float)n);
+                        const int r = rem - quo * n;
+                        rem = quo;
+                        ext |= select_bit(m, (uint32_t)r) << (2 * q);
+                    }
+                }
+                T total = (T)0;
+                T* pc = pre + 4 * (size_t)(my_off + c);
+                for (int v = 0; v < 4; ++v) {
+                    T tmp = tr[v];
+                    if (!((pmask >> v) & 1u) || tmp == (T)0) {
+                        pc[v] = (T)0;
+                        continue;
+                    }
+                    const uint32_t ev = ext | ((uint32_t)v << (2 * k));
+                    if (type == kChild)
+                        tmp *= trans<T>(dist + 8 * (peel - F), (ev >> (2 * mat_digit)) & 3u,
+                                        (ev >> (2 * pat_digit)) & 3u, (uint32_t)v);
+#pragma unroll
+                    for (int q = 0; q < kMaxPrev; ++q) {
+                        if (q < n_prev) {
+                            int idx = 0;
+#pragma unroll
+                            for (int j = 0; j < kMaxCut; ++j) {
+                                if (j < p_nd[q]) {
+                                    const int src = p_src[q][j];
+                                    const uint32_t x = (ev >> (2 * src)) & 3u;
+                                    const uint32_t mm = (cmw >> (4 * src)) & 15u;
+                                    idx += (int)rank_in(mm, x) * p_stride[q][j];
+                                }
+                            }
+                            tmp *= tab[p_off[q] + idx] * p_scale[q];
+                        }
+                    }
+                    if (type == kParent) {
+                        T child_prob = (T)1;
+#pragma unroll
+                        for (int ch = 0; ch < 7; ++ch) {
+                            if (ch < n_child) {
+                                const uint32_t kid = (ev >> (2 * kid_digit[ch])) & 3u;
+                                const uint32_t other = (ev >> (2 * other_digit[ch])) & 3u;
+                                const uint32_t mt = kid_pim[ch] ? (uint32_t)v : other;
+                                const uint32_t pt = kid_pim[ch] ? other : (uint32_t)v;
+                                child_prob *= trans<T>(dist + 8 * (kids[ch] - F), mt, pt, kid);
+                            }
+                        }
+                        tmp *= child_prob;
+                    }
+                    pc[v] = tmp;
+                    total += tmp;
+                }
+                tab[my_off + c] = total;
+                if (RESCALE) mx = fmaxf(mx, (float)total);
+            }
+            if (RESCALE) {
+                const float m = wave_max(mx);
+                if (lane == 0) maxbits[op] = __float_as_uint(m);
+            }
+        }
+        __syncthreads();
+    }
+    // likelihood of the locus (locus_sampler2.cc:137-142)
+    if (tid == 0 && !(tab[off[n_ops - 1]] > (T)0)) s_status = -4;
+    __syncthreads();
+    if (s_status != 0) {
+        if (tid == 0) A.status[chain] = s_status;
+        return;
+    }
+
+    // 3. reverse sampling by reverse levels
+    for (int R = 0; R < A.n_rlevels; ++R) {
+        const int r0 = ((cint_p)A.rlevel_off)[R], r1 = ((cint_p)A.rlevel_off)[R + 1];
+        for (int i = r0 + tid; i < r1; i += blockDim.x) {
+            const int op = A.rlevel_ops[i];
+            const SampOp* dp = A.ops + op;
+            const int k = dp->k, peel = dp->peel;
+            int base = 0, st = 1;
+            for (int q = 0; q < k; ++q) {
+                const int who = dp->cut[q];
+                const uint32_t m = mask[who];
+                base += (int)rank_in(m, pmk[who]) * st;
+                st *= __popc(m);
+            }
+            const T* pc = pre + 4 * (size_t)(off[op] + base);
+            T pdv[4] = {pc[0], pc[1], pc[2], pc[3]};
+            const T tot = pdv[0] + pdv[1] + pdv[2] + pdv[3];
+            if (tot != (T)0)
+                for (int v = 0; v < 4; ++v) pdv[v] /= tot;
+            const double u = uop[op];
+            T run = (T)0;
+            int last = 0, chosen = -1;
+            for (int v = 0; v < 4; ++v) {
+                run += pdv[v];
+                if (u < (double)run) { chosen = v; break; }
+                if (pdv[v] != (T)0) last = v;
+            }
+            pmk[peel] = (uint8_t)(chosen >= 0 ? chosen : last);
+        }
+        __syncthreads();
+    }
+
+    // 4. meiosis indicators
+    for (int b = tid; b < 2 * NF; b += blockDim.x) {
+        const int person = F + (b >> 1), side = b & 1;
+        const int parent = side ? A.father[person] : A.mother[person];
+        const uint32_t pt = pmk[parent];
+        mcount[b] = (pt == PT_UU || pt == PT_AA) ? 1 : 0;
+    }
+    __syncthreads();
+    block_exclusive_scan(mcount, 2 * NF, scan_tmp);
+    for (int b = tid; b < 2 * NF; b += blockDim.x) {
+        const int person = F + (b >> 1), side = b & 1;
+        const int parent = side ? A.father[person] : A.mother[person];
+        const uint32_t pt = pmk[parent];
+        const uint32_t t = pmk[person];
+        uint32_t bit;
+        if (pt == PT_AU || pt == PT_UA) {
+            // allele the kid got from this parent; heterozygous parent fixes the indicator
+            const bool allele_u = side ? (t == PT_UU || t == PT_AU) : (t == PT_UU || t == PT_UA);
+            bit = allele_u ? (pt == PT_UA ? 0u : 1u) : (pt == PT_UA ? 1u : 0u);
+        } else {
+            double p0 = 1.0, p1 = 1.0;
+            if (colL) {
+                const uint32_t nb = (colL[b >> 5] >> (b & 31)) & 1u;
+                p0 *= (nb == 0u) ? 1.0 - thL : thL;
+                p1 *= (nb == 1u) ? 1.0 - thL : thL;
+            }
+            if (colR) {
+                const uint32_t nb = (colR[b >> 5] >> (b & 31)) & 1u;
+                p0 *= (nb == 0u) ? 1.0 - thR : thR;
+                p1 *= (nb == 1u) ? 1.0 - thR : thR;
+            }
+            const double u = philox_uniform(A.seed, (uint32_t)chain, A.sweep, (uint32_t)locus,
+                                            (uint32_t)(n_ops + mcount[b]));
+            bit = (u < (p0 / (p0 + p1))) ? 0u : 1u;
+        }
+        bits[b] = (uint8_t)bit;
+    }
+    __syncthreads();
+    uint32_t* col = g + (size_t)locus * A.W;
+    for (int w = tid; w < A.W; w += blockDim.x) {
+        uint32_t word = 0;
+        for (int t = 0; t < 32; ++t) {
+            const int b = 32 * w + t;
+            if (b < 2 * NF) word |= (uint32_t)bits[b] << t;
+        }
+        col[w] = word;
+    }
+}
+
+}  // namespace
+
+size_t lsampler_lds_bytes(int precision, int cells_cap, int n_ops, int N, int F, int block) {
+    const size_t esz = precision == 0 ? 8 : 4;
+    const int NF = N - F;
+    size_t b = esz * (size_t)cells_cap * 5 + esz * 8 * (size_t)NF;
+    b = (b + 7) / 8 * 8;
+    b += 8 * (size_t)n_ops;                          // uop
+    b += 4 * ((size_t)n_ops + 1 + block + 2 * NF);   // off, scan, mcount
+    b += 4 * (size_t)n_ops;                          // maxbits
+    b += 2 * (size_t)N + 2 * (size_t)NF;             // mask, pmk, bits
+    return (b + 15) / 16 * 16;
+}
+
+hipError_t configure_lsampler_lds(int precision, size_t lds) {
+    if (lds <= 64 * 1024) return hipSuccess;
+    if (precision == 0)
+        return hipFuncSetAttribute((const void*)lsampler_kernel<double, false>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return hipFuncSetAttribute((const void*)lsampler_kernel<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds);
+}
+
+hipError_t launch_lsampler(const SampArgs& A, int precision, int block, size_t lds, int n_chains, hipStream_t s) {
+    const int nloci = (A.M - A.parity + 1) / 2;
+    dim3 grid(nloci, n_chains);
+    if (precision == 0)
+        hipLaunchKernelGGL((lsampler_kernel<double, false>), grid, dim3(block), lds, s, A);
+    else
+        hipLaunchKernelGGL((lsampler_kernel<float, true>), grid, dim3(block), lds, s, A);
+    return hipGetLastError();
+}
+
+}  // namespace sl

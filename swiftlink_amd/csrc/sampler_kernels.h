@@ -1,0 +1,39 @@
+// Argument blocks and launchers of the sampler kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "sampler_program.h"
+
+namespace sl {
+
+struct SampArgs {
+    const SampOp* ops;
+    const SampPrev* prevs;
+    const ChildDesc* children;
+    const int32_t* sched_off;
+    const int32_t* sched_ops;
+    const int32_t* rlevel_off;
+    const int32_t* rlevel_ops;
+    int n_ops, n_levels, n_rlevels, waves;
+    int N, F, M, W;
+    const int32_t* mother;       // [N]
+    const int32_t* father;       // [N]
+    const uint8_t* legal_t;      // [M][N] genotype-elimination masks
+    const double* geno_prob_d;   // [N][M][4] sampler trait probabilities
+    const float* geno_prob_f;
+    const double* theta;         // [M-1]
+    uint32_t* graphs;            // [chains][M][W]
+    int32_t* status;             // [chains]
+    uint64_t seed;
+    uint32_t sweep;
+    int parity;
+    int cells_cap;
+};
+
+size_t lsampler_lds_bytes(int precision, int cells_cap, int n_ops, int N, int F, int block);
+hipError_t configure_lsampler_lds(int precision, size_t lds);
+hipError_t launch_lsampler(const SampArgs& A, int precision, int block, size_t lds, int n_chains, hipStream_t s);
+
+}  // namespace sl

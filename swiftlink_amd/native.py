@@ -73,6 +73,10 @@ _SIGS = {
     "sl_lod_work": (C.c_int, [C.c_void_p, f64p, f64p]),
     "sl_program_export": (C.c_int, [C.POINTER(PedigreeDesc), C.POINTER(PeelDesc), C.c_int32, i32p, C.c_int64,
                                     i64p]),
+    "sl_mcmc_init": (C.c_int, [C.c_void_p, C.c_int32, i32p]),
+    "sl_mcmc_get_graphs": (C.c_int, [C.c_void_p, i32p]),
+    "sl_mcmc_status": (C.c_int, [C.c_void_p, i32p]),
+    "sl_lsweep": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32]),
 }
 
 
@@ -213,6 +217,25 @@ class Context:
         n = C.c_int64(0)
         self._check(lib().sl_profile_read(self.h, kernel, C.byref(ms), C.byref(n)), "sl_profile_read")
         return ms.value, n.value
+
+    # ---- chains ----
+    def mcmc_init(self, graphs: np.ndarray):
+        g = np.ascontiguousarray(graphs, np.int32).reshape(-1, self.case.M, 2 * self.case.N)
+        self.n_chains = g.shape[0]
+        self._check(lib().sl_mcmc_init(self.h, g.shape[0], _ptr(g, i32p)), "sl_mcmc_init")
+
+    def mcmc_graphs(self) -> np.ndarray:
+        out = np.zeros((self.n_chains, self.case.M, 2 * self.case.N), np.int32)
+        self._check(lib().sl_mcmc_get_graphs(self.h, _ptr(out, i32p)), "sl_mcmc_get_graphs")
+        return out
+
+    def mcmc_status(self) -> np.ndarray:
+        out = np.zeros(self.n_chains, np.int32)
+        self._check(lib().sl_mcmc_status(self.h, _ptr(out, i32p)), "sl_mcmc_status")
+        return out
+
+    def lsweep(self, seed: int, sweep: int, first_parity: int = 0):
+        self._check(lib().sl_lsweep(self.h, seed, sweep, first_parity), "sl_lsweep")
 
     def lod_work(self):
         f = C.c_double(0)
