@@ -169,6 +169,29 @@ int sl_mcmc_status(sl_ctx* ctx, int32_t* status);
  * the uniforms (draw, step=l, sweep, chain=c) of the Philox stream. */
 int sl_lsweep(sl_ctx* ctx, uint64_t seed, uint32_t sweep, int32_t first_parity);
 
+/* One M-sweep on every chain: MeiosisSampler::reset then step for each
+ * meiosis in turn (meiosis_sampler.cc:17-203; driven at markov_chain.cc:345-351).
+ * order == NULL: the non-ignorable meioses (markov_chain.cc:65-80,
+ * person.cc:208-222) shuffled per chain by Fisher-Yates with the draws
+ * (draw, step=0xFFFFFFFE, sweep, chain); otherwise order[0..n_order) (meiosis
+ * ids 2(p-F)+parent) is used for every chain.  Step j draws
+ * (draw, step=j, sweep, chain), one draw per locus whose two states are both
+ * possible, from the last locus down.  A chain whose graph is illegal at some
+ * locus gets status SL_E_ILLEGAL_GRAPH and is left unchanged. */
+int sl_msweep(sl_ctx* ctx, uint64_t seed, uint32_t sweep, const int32_t* order, int32_t n_order);
+
+/* Founder-allele-graph likelihood of every chain's current graph at every
+ * locus into per_locus [C][M] (FounderAlleleGraph4::reset + likelihood per
+ * locus, the legality check of DescentGraph::_sum_prior_prob,
+ * descent_graph.cc:243-265).  0 marks an illegal locus; such a chain also
+ * gets status SL_E_ILLEGAL_GRAPH. */
+int sl_mcmc_likelihood(sl_ctx* ctx, double* per_locus);
+
+/* Per chain: draws since sl_mcmc_init whose uniform lay within 1e-12 of the
+ * decision threshold (where a reassociated sum can decide differently from
+ * the reference's serial one).  No counterpart in the reference. */
+int sl_mcmc_near_edge(sl_ctx* ctx, int64_t* counts);
+
 /* Compiles a peel table into the level-scheduled program the LOD kernel
  * runs and serialises it into buf (int32 words; *needed gets the length).
  * Host only, no device work: lets the program's indexing be checked on a

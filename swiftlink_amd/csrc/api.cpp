@@ -10,6 +10,7 @@
 
 #include "../../include/swiftlink_amd.h"
 #include "kernels.h"
+#include "msampler_kernels.h"
 #include "program.h"
 #include "sampler_kernels.h"
 #include "sampler_program.h"
@@ -85,6 +86,30 @@ struct sl_ctx {
     bool samp_ready = false;
     std::string samp_why;           // why the sampler program was rejected, if it was
     std::vector<int32_t> mother_h, father_h;
+    // M-sampler
+    int m_regs = 0;
+    int m_block = sl::kMsampBlock;
+    int m_wave_bytes = 0;
+    size_t m_lds = 0;
+    bool m_ready = false;
+    std::string m_why;
+    std::vector<int32_t> meioses_h;
+    int n_gen = 0;
+    int32_t* d_gen_off = nullptr;
+    int32_t* d_gen_persons = nullptr;
+    uint8_t* d_geno_t = nullptr;
+    double* d_minor = nullptr;
+    int32_t* d_meioses = nullptr;
+    int32_t* d_order = nullptr;
+    size_t order_cap = 0;
+    int32_t* d_fixed_order = nullptr;
+    size_t fixed_cap = 0;
+    double* d_lcur = nullptr;
+    double* d_lflip = nullptr;
+    double* d_fbuf = nullptr;
+    uint8_t* d_maps = nullptr;
+    uint32_t* d_counter = nullptr;
+    uint32_t* d_near = nullptr;
     // profiling
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -202,6 +227,8 @@ int release_all(sl_ctx* c) {
     void* ptrs[] = {c->d_sops, c->d_sprevs, c->d_schildren, c->d_ssched_off, c->d_ssched_ops, c->d_rlevel_off,
                     c->d_rlevel_ops, c->d_mother, c->d_father, c->d_legal_t, c->d_gprob_d, c->d_gprob_f,
                     c->d_theta, c->d_chains, c->d_status,
+                    c->d_gen_off, c->d_gen_persons, c->d_geno_t, c->d_minor, c->d_meioses, c->d_order,
+                    c->d_fixed_order, c->d_lcur, c->d_lflip, c->d_fbuf, c->d_maps, c->d_counter, c->d_near,
                     c->d_ops, c->d_prevs, c->d_children, c->d_tasks, c->d_tinfo, c->d_idx, c->d_level_base,
                     c->d_sched_off,
                     c->d_sched_ops, c->d_trait_d,
@@ -215,6 +242,79 @@ int release_all(sl_ctx* c) {
         (void)hipEventDestroy(u.second.second);
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    return SL_OK;
+}
+
+// M-sampler tables: founder-allele label order, non-ignorable meioses,
+// locus-major genotypes.  Leaves m_ready false (with the reason) when the
+// pedigree is outside what the kernel handles.
+int setup_msampler(sl_ctx* c, const sl_pedigree_desc* ped) {
+    const int N = c->N, F = c->F, M = c->M;
+    c->m_regs = sl::msampler_regs(F);
+    if (!c->m_regs) {
+        c->m_why = "more than 512 founder alleles";
+        return SL_OK;
+    }
+    // generations: founders 0, others 1 + deeper parent (the label order of
+    // find_founderallelegraph_ordering, meiosis_sampler.cc:42-72, by levels)
+    std::vector<int> depth(N, -1);
+    for (int i = 0; i < F; ++i) depth[i] = 0;
+    int done = F, maxd = 0;
+    while (done < N) {
+        int progress = 0;
+        for (int i = F; i < N; ++i) {
+            if (depth[i] >= 0) continue;
+            const int dm = depth[c->mother_h[i]], df = depth[c->father_h[i]];
+            if (dm < 0 || df < 0) continue;
+            depth[i] = 1 + (dm > df ? dm : df);
+            maxd = depth[i] > maxd ? depth[i] : maxd;
+            ++progress;
+        }
+        if (!progress) return fail(c, SL_E_INVALID, "pedigree has a cycle of descent");
+        done += progress;
+    }
+    std::vector<int32_t> gen_off, gen_persons;
+    for (int d = 1; d <= maxd; ++d) {
+        gen_off.push_back((int32_t)gen_persons.size());
+        for (int i = F; i < N; ++i)
+            if (depth[i] == d) gen_persons.push_back(i);
+    }
+    gen_off.push_back((int32_t)gen_persons.size());
+    c->n_gen = maxd;
+    // non-ignorable meioses (markov_chain.cc:65-80): a meiosis from a founder
+    // parent with one child is skipped (safe_to_ignore_meiosis, person.cc:208-222)
+    std::vector<int> kids(N, 0);
+    for (int i = F; i < N; ++i) {
+        ++kids[c->mother_h[i]];
+        ++kids[c->father_h[i]];
+    }
+    c->meioses_h.clear();
+    for (int b = 0; b < 2 * (N - F); ++b) {
+        const int person = F + b / 2;
+        const int parent = b % 2 ? c->father_h[person] : c->mother_h[person];
+        if (!(parent < F && kids[parent] == 1)) c->meioses_h.push_back(b);
+    }
+    std::vector<uint8_t> gt((size_t)M * N, 0);
+    for (int i = 0; i < N; ++i)
+        if (ped->typed[i])
+            for (int l = 0; l < M; ++l) gt[(size_t)l * N + i] = ped->genotypes[(size_t)i * M + l];
+    c->m_wave_bytes = (4 * c->W + 4 * (N - F) + 15) / 16 * 16;
+    sl::MsampArgs A{};
+    A.wave_bytes = c->m_wave_bytes;
+    c->m_lds = sl::msampler_lds_bytes(A, c->m_block);
+    if (c->m_lds > 64 * 1024) {
+        c->m_why = "M-sampler workgroup needs more than 64 KiB of LDS";
+        return SL_OK;
+    }
+    int rc;
+    if ((rc = upload(c, &c->d_gen_off, gen_off.data(), gen_off.size())) ||
+        (rc = upload(c, &c->d_gen_persons, gen_persons.data(), gen_persons.size())) ||
+        (rc = upload(c, &c->d_geno_t, gt.data(), gt.size())) ||
+        (rc = upload(c, &c->d_minor, c->minor.data(), c->minor.size())) ||
+        (rc = upload(c, &c->d_meioses, c->meioses_h.data(), c->meioses_h.size())))
+        return rc;
+    SL_HIP(c, hipStreamSynchronize(c->stream));  // the host tables above go out of scope
+    c->m_ready = true;
     return SL_OK;
 }
 
@@ -315,6 +415,23 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
         delete c;
         return fail(nullptr, rc, m);
     }
+    // arrays both samplers read
+    c->mother_h.assign(ped->mother, ped->mother + c->N);
+    c->father_h.assign(ped->father, ped->father + c->N);
+    if ((rc = upload(c, &c->d_mother, ped->mother, (size_t)c->N)) ||
+        (rc = upload(c, &c->d_father, ped->father, (size_t)c->N)) ||
+        (rc = upload(c, &c->d_theta, c->theta.data(), c->theta.size()))) {
+        std::string m = c->err;
+        release_all(c);
+        delete c;
+        return fail(nullptr, rc, m);
+    }
+    if ((rc = setup_msampler(c, ped))) {
+        std::string m = c->err;
+        release_all(c);
+        delete c;
+        return fail(nullptr, rc, m);
+    }
     // sampler program (the L-sampler needs the per-locus legal masks to bound its LDS)
     {
         int64_t max_cells = 0;
@@ -330,8 +447,6 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
                 for (int l = 0; l < c->M; ++l) lt[(size_t)l * c->N + p] = ped->legal[(size_t)p * c->M + l];
             std::vector<float> gf((size_t)c->N * c->M * 4);
             for (size_t i = 0; i < gf.size(); ++i) gf[i] = (float)ped->geno_prob[i];
-            c->mother_h.assign(ped->mother, ped->mother + c->N);
-            c->father_h.assign(ped->father, ped->father + c->N);
             if ((rc = upload(c, &c->d_sops, c->sprog.ops.data(), c->sprog.ops.size())) ||
                 (rc = upload(c, &c->d_sprevs, c->sprog.prevs.data(), c->sprog.prevs.size())) ||
                 (rc = upload(c, &c->d_schildren, c->sprog.children.data(), c->sprog.children.size())) ||
@@ -339,12 +454,9 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
                 (rc = upload(c, &c->d_ssched_ops, c->sprog.sched_ops.data(), c->sprog.sched_ops.size())) ||
                 (rc = upload(c, &c->d_rlevel_off, c->sprog.rlevel_off.data(), c->sprog.rlevel_off.size())) ||
                 (rc = upload(c, &c->d_rlevel_ops, c->sprog.rlevel_ops.data(), c->sprog.rlevel_ops.size())) ||
-                (rc = upload(c, &c->d_mother, ped->mother, (size_t)c->N)) ||
-                (rc = upload(c, &c->d_father, ped->father, (size_t)c->N)) ||
                 (rc = upload(c, &c->d_legal_t, lt.data(), lt.size())) ||
                 (rc = upload(c, &c->d_gprob_d, ped->geno_prob, (size_t)c->N * c->M * 4)) ||
-                (rc = upload(c, &c->d_gprob_f, gf.data(), gf.size())) ||
-                (rc = upload(c, &c->d_theta, c->theta.data(), c->theta.size()))) {
+                (rc = upload(c, &c->d_gprob_f, gf.data(), gf.size()))) {
                 std::string m = c->err;
                 release_all(c);
                 delete c;
@@ -541,22 +653,35 @@ static int need_sampler(sl_ctx* c) {
 
 int sl_mcmc_init(sl_ctx* c, int32_t n_chains, const int32_t* graphs) {
     if (!c || n_chains <= 0 || !graphs) return SL_E_INVALID;
-    int rc = need_sampler(c);
-    if (rc) return rc;
+    if (!c->samp_ready && !c->m_ready)
+        return fail(c, SL_E_UNSUPPORTED, "samplers unavailable: " + c->samp_why + "; " + c->m_why);
+    int rc;
     const size_t words = (size_t)n_chains * c->M * c->W;
+    const size_t cm = (size_t)n_chains * c->M;
     if (n_chains != c->n_chains) {
-        if (c->d_chains) SL_HIP(c, hipFree(c->d_chains));
-        if (c->d_status) SL_HIP(c, hipFree(c->d_status));
-        c->d_chains = nullptr;
-        c->d_status = nullptr;
+        void** bufs[] = {(void**)&c->d_chains, (void**)&c->d_status, (void**)&c->d_lcur, (void**)&c->d_lflip,
+                         (void**)&c->d_fbuf, (void**)&c->d_maps, (void**)&c->d_counter, (void**)&c->d_near};
+        for (void** b : bufs) {
+            if (*b) SL_HIP(c, hipFree(*b));
+            *b = nullptr;
+        }
+        c->n_chains = 0;
         SL_HIP(c, hipMalloc((void**)&c->d_chains, words * sizeof(uint32_t)));
         SL_HIP(c, hipMalloc((void**)&c->d_status, (size_t)n_chains * sizeof(int32_t)));
+        SL_HIP(c, hipMalloc((void**)&c->d_lcur, cm * sizeof(double)));
+        SL_HIP(c, hipMalloc((void**)&c->d_lflip, cm * sizeof(double)));
+        SL_HIP(c, hipMalloc((void**)&c->d_fbuf, 2 * cm * sizeof(double)));
+        SL_HIP(c, hipMalloc((void**)&c->d_maps, cm));
+        SL_HIP(c, hipMalloc((void**)&c->d_counter, (size_t)n_chains * sizeof(uint32_t)));
+        SL_HIP(c, hipMalloc((void**)&c->d_near, (size_t)n_chains * sizeof(uint32_t)));
         c->n_chains = n_chains;
     }
     rc = stage_graphs(c, graphs, n_chains);
     if (rc) return rc;
     SL_HIP(c, hipMemcpyAsync(c->d_chains, c->d_packed, words * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
     SL_HIP(c, hipMemsetAsync(c->d_status, 0, (size_t)n_chains * sizeof(int32_t), c->stream));
+    SL_HIP(c, hipMemsetAsync(c->d_counter, 0, (size_t)n_chains * sizeof(uint32_t), c->stream));
+    SL_HIP(c, hipMemsetAsync(c->d_near, 0, (size_t)n_chains * sizeof(uint32_t), c->stream));
     SL_HIP(c, hipStreamSynchronize(c->stream));
     return SL_OK;
 }
@@ -637,6 +762,114 @@ int sl_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int32_t first_parity) {
             c->ev_used.push_back({SL_KERNEL_LSAMPLER, {e0, e1}});
         }
     }
+    return SL_OK;
+}
+
+static sl::MsampArgs msamp_args(sl_ctx* c) {
+    sl::MsampArgs A{};
+    A.N = c->N;
+    A.F = c->F;
+    A.M = c->M;
+    A.W = c->W;
+    A.mother = c->d_mother;
+    A.father = c->d_father;
+    A.gen_off = c->d_gen_off;
+    A.gen_persons = c->d_gen_persons;
+    A.n_gen = c->n_gen;
+    A.geno_t = c->d_geno_t;
+    A.minor = c->d_minor;
+    A.theta = c->d_theta;
+    A.meioses = c->d_meioses;
+    A.n_meioses = (int)c->meioses_h.size();
+    A.order = c->d_order;
+    A.graphs = c->d_chains;
+    A.lcur = c->d_lcur;
+    A.lflip = c->d_lflip;
+    A.fbuf = c->d_fbuf;
+    A.maps = c->d_maps;
+    A.counter = c->d_counter;
+    A.status = c->d_status;
+    A.near_edge = c->d_near;
+    A.wave_bytes = c->m_wave_bytes;
+    A.n_chains = c->n_chains;
+    return A;
+}
+
+int sl_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* order, int32_t n_order) {
+    if (!c) return SL_E_INVALID;
+    if (!c->m_ready) return fail(c, SL_E_UNSUPPORTED, "M-sampler unavailable: " + c->m_why);
+    if (c->n_chains <= 0) return fail(c, SL_E_INVALID, "no chains: call sl_mcmc_init first");
+    int nm = (int)c->meioses_h.size();
+    if (order) {
+        if (n_order < 0 || n_order > 2 * (c->N - c->F)) return fail(c, SL_E_INVALID, "bad meiosis order length");
+        for (int j = 0; j < n_order; ++j)
+            if (order[j] < 0 || order[j] >= 2 * (c->N - c->F)) return fail(c, SL_E_INVALID, "meiosis id out of range");
+        nm = n_order;
+        if (nm > 0) {
+            int rc = ensure(c, &c->d_fixed_order, &c->fixed_cap, (size_t)nm);
+            if (rc) return rc;
+            SL_HIP(c, hipMemcpyAsync(c->d_fixed_order, order, (size_t)nm * sizeof(int32_t), hipMemcpyHostToDevice,
+                                     c->stream));
+            SL_HIP(c, hipStreamSynchronize(c->stream));
+        }
+    }
+    if (nm == 0) return SL_OK;
+    int rc = ensure(c, &c->d_order, &c->order_cap, (size_t)nm * c->n_chains);
+    if (rc) return rc;
+    sl::MsampArgs A = msamp_args(c);
+    A.seed = seed;
+    A.sweep = sweep;
+    A.n_meioses = nm;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->prof) {
+        e0 = take_event(c);
+        e1 = take_event(c);
+        if (e0 && e1) SL_HIP(c, hipEventRecord(e0, c->stream));
+    }
+    SL_HIP(c, sl::launch_morder(A, order ? c->d_fixed_order : nullptr, c->stream));
+    A.mode = 0;
+    SL_HIP(c, sl::launch_mstep(A, c->m_regs, c->m_block, c->m_lds, c->stream));
+    A.mode = 1;
+    for (int j = 0; j < nm; ++j) {
+        A.step = j;
+        SL_HIP(c, sl::launch_mstep(A, c->m_regs, c->m_block, c->m_lds, c->stream));
+    }
+    if (c->prof && e0 && e1) {
+        SL_HIP(c, hipEventRecord(e1, c->stream));
+        c->ev_used.push_back({SL_KERNEL_MSAMPLER, {e0, e1}});
+    }
+    return SL_OK;
+}
+
+int sl_mcmc_likelihood(sl_ctx* c, double* per_locus) {
+    if (!c || !per_locus) return SL_E_INVALID;
+    if (!c->m_ready) return fail(c, SL_E_UNSUPPORTED, "M-sampler unavailable: " + c->m_why);
+    if (c->n_chains <= 0) return fail(c, SL_E_INVALID, "no chains: call sl_mcmc_init first");
+    sl::MsampArgs A = msamp_args(c);
+    A.mode = 0;
+    std::vector<int32_t> st((size_t)c->n_chains, 0);
+    // chains already in error are evaluated too
+    SL_HIP(c, hipMemcpyAsync(st.data(), c->d_status, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    SL_HIP(c, hipMemsetAsync(c->d_status, 0, st.size() * sizeof(int32_t), c->stream));
+    SL_HIP(c, sl::launch_mstep(A, c->m_regs, c->m_block, c->m_lds, c->stream));
+    const size_t n = (size_t)c->n_chains * c->M;
+    SL_HIP(c, hipMemcpyAsync(per_locus, c->d_lcur, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    std::vector<int32_t> now(st.size());
+    SL_HIP(c, hipMemcpyAsync(now.data(), c->d_status, now.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < st.size(); ++i)
+        if (st[i] != 0) now[i] = st[i];
+    SL_HIP(c, hipMemcpyAsync(c->d_status, now.data(), now.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    return SL_OK;
+}
+
+int sl_mcmc_near_edge(sl_ctx* c, int64_t* counts) {
+    if (!c || !counts || c->n_chains <= 0) return SL_E_INVALID;
+    std::vector<uint32_t> h((size_t)c->n_chains);
+    SL_HIP(c, hipMemcpyAsync(h.data(), c->d_near, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < h.size(); ++i) counts[i] = (int64_t)h[i];
     return SL_OK;
 }
 

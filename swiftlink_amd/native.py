@@ -20,6 +20,8 @@ LIBPATH = os.path.join(HERE, "libswiftlink_amd.so")
 SL_FP64 = 0
 SL_FP32_RESCALED = 1
 SL_KERNEL_LOD, SL_KERNEL_LSAMPLER, SL_KERNEL_MSAMPLER = 0, 1, 2
+SL_OK, SL_E_INVALID, SL_E_UNSUPPORTED, SL_E_DEVICE = 0, -1, -2, -3
+SL_E_ZERO_LIKELIHOOD, SL_E_ILLEGAL_GRAPH, SL_E_CAPACITY = -4, -5, -6
 
 i32p = C.POINTER(C.c_int32)
 u32p = C.POINTER(C.c_uint32)
@@ -77,6 +79,9 @@ _SIGS = {
     "sl_mcmc_get_graphs": (C.c_int, [C.c_void_p, i32p]),
     "sl_mcmc_status": (C.c_int, [C.c_void_p, i32p]),
     "sl_lsweep": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32]),
+    "sl_msweep": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, i32p, C.c_int32]),
+    "sl_mcmc_likelihood": (C.c_int, [C.c_void_p, f64p]),
+    "sl_mcmc_near_edge": (C.c_int, [C.c_void_p, i64p]),
 }
 
 
@@ -236,6 +241,24 @@ class Context:
 
     def lsweep(self, seed: int, sweep: int, first_parity: int = 0):
         self._check(lib().sl_lsweep(self.h, seed, sweep, first_parity), "sl_lsweep")
+
+    def msweep(self, seed: int, sweep: int, order=None):
+        """One M-sweep on every chain; order=None shuffles the non-ignorable meioses per chain."""
+        if order is None:
+            self._check(lib().sl_msweep(self.h, seed, sweep, None, 0), "sl_msweep")
+        else:
+            o = np.ascontiguousarray(order, np.int32)
+            self._check(lib().sl_msweep(self.h, seed, sweep, _ptr(o, i32p), o.size), "sl_msweep")
+
+    def mcmc_likelihood(self) -> np.ndarray:
+        out = np.zeros((self.n_chains, self.case.M))
+        self._check(lib().sl_mcmc_likelihood(self.h, _ptr(out, f64p)), "sl_mcmc_likelihood")
+        return out
+
+    def mcmc_near_edge(self) -> np.ndarray:
+        out = np.zeros(self.n_chains, np.int64)
+        self._check(lib().sl_mcmc_near_edge(self.h, _ptr(out, i64p)), "sl_mcmc_near_edge")
+        return out
 
     def lod_work(self):
         f = C.c_double(0)

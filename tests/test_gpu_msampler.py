@@ -1,0 +1,153 @@
+"""M-sampler on the GPU against the oracle and the compiled reference.
+
+The founder-allele-graph likelihoods are the reference's bit for bit in FP64
+(every locus of every chain).  The forward-backward pass reassociates the
+reference's serial products into block scans, so a draw can decide
+differently only if its uniform lies within rounding of its threshold; the
+kernel counts draws within 1e-12 of their threshold, and these tests require
+that count to be zero and the sampled graphs to be identical to the
+reference's (golden fixtures) and the oracle's under the same Philox stream.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden, GOLDEN_CASES
+from oracle_lib import Oracle, uniform
+from swiftlink_amd import native, synth
+
+pytestmark = pytest.mark.gpu
+
+STEP_SHUFFLE = 0xFFFFFFFE
+
+
+def shuffled_order(o, seed, chain, sweep):
+    """Fisher-Yates over the non-ignorable meioses with the chain's shuffle draws."""
+    mo = [int(x) for x in o.meioses()]
+    d = 0
+    for i in range(len(mo) - 1, 0, -1):
+        j = min(int(uniform(seed, chain, sweep, STEP_SHUFFLE, d) * (i + 1)), i)
+        d += 1
+        mo[i], mo[j] = mo[j], mo[i]
+    return np.array(mo, np.int32)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_msweep_fp64_matches_reference(name):
+    case, fx = load_golden(name)
+    seed, chain, sweep = (int(x) for x in fx["msweep_seed"])
+    assert chain == 0
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(fx["graph_init"][None])
+        ctx.msweep(seed, sweep, fx["msweep_order"])
+        out = ctx.mcmc_graphs()[0]
+        assert np.all(ctx.mcmc_status() == 0)
+        assert np.all(ctx.mcmc_near_edge() == 0)
+    assert np.array_equal(out, fx["msweep_out"])
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_msweep_shuffled_chains_match_oracle(name):
+    case, fx = load_golden(name)
+    o = Oracle(case)
+    starts = np.stack([fx["graph_init"], fx["graphs"][0], fx["graphs"][1]])
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(starts)
+        ctx.msweep(4321, 7)
+        ctx.msweep(4321, 8)
+        out = ctx.mcmc_graphs()
+        assert np.all(ctx.mcmc_status() == 0)
+        assert np.all(ctx.mcmc_near_edge() == 0)
+    for c in range(3):
+        g = o.msweep(starts[c], shuffled_order(o, 4321, c, 7), 4321, c, 7)
+        g = o.msweep(g, shuffled_order(o, 4321, c, 8), 4321, c, 8)
+        assert np.array_equal(out[c], g), c
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_locus_likelihoods_bit_exact(name):
+    case, fx = load_golden(name)
+    o = Oracle(case)
+    starts = np.stack([fx["graph_init"], fx["graphs"][0], fx["lsweep_out"], fx["msweep_out"]])
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(starts)
+        lik = ctx.mcmc_likelihood()
+        assert np.all(ctx.mcmc_status() == 0)
+    for c in range(len(starts)):
+        ref = np.array([o.fag(starts[c], l) for l in range(case.M)])
+        assert np.all(ref > 0)
+        assert np.array_equal(lik[c], ref), c
+
+
+def _illegal_graph(case, o, g):
+    """A copy of g with one meiosis flipped at one locus so that the column is illegal."""
+    for l in range(case.M):
+        for b in range(2 * case.F, 2 * case.N):
+            h = g.copy()
+            h[l, b] ^= 1
+            if o.fag(h, l) == 0.0:
+                return h, l
+    return None, None
+
+
+def test_illegal_graph_is_reported_and_left_unchanged():
+    case, fx = load_golden("loop")
+    o = Oracle(case)
+    bad, locus = _illegal_graph(case, o, fx["graph_init"])
+    assert bad is not None
+    starts = np.stack([fx["graph_init"], bad])
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(starts)
+        lik = ctx.mcmc_likelihood()
+        assert lik[1, locus] == 0.0 and np.all(lik[0] > 0)
+        st = ctx.mcmc_status()
+        assert st[0] == 0 and st[1] == native.SL_E_ILLEGAL_GRAPH
+        ctx.mcmc_init(starts)
+        ctx.msweep(5, 0)
+        out = ctx.mcmc_graphs()
+        st = ctx.mcmc_status()
+    assert st[1] == native.SL_E_ILLEGAL_GRAPH and st[0] == 0
+    assert np.array_equal(out[1], bad)
+    assert np.array_equal(out[0], o.msweep(starts[0], shuffled_order(o, 5, 0, 0), 5, 0, 0))
+
+
+def test_lsweep_then_msweep_keeps_graphs_legal():
+    case, fx = load_golden("ped30_snp100")
+    o = Oracle(case)
+    starts = np.stack([fx["graph_init"]] * 4)
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(starts)
+        for s in range(6):
+            if s % 2:
+                ctx.msweep(99, s)
+            else:
+                ctx.lsweep(99, s, s % 4 // 2)
+        out = ctx.mcmc_graphs()
+        lik = ctx.mcmc_likelihood()
+        assert np.all(ctx.mcmc_status() == 0)
+    for c in range(4):
+        ref = np.array([o.fag(out[c], l) for l in range(case.M)])
+        assert np.all(ref > 0)
+        assert np.array_equal(lik[c], ref)
+
+
+@pytest.fixture(scope="module")
+def ped150(tmp_path_factory):
+    d = tmp_path_factory.mktemp("ped150m")
+    case, data = synth.make_case("ped150_snp1000", str(d))
+    return case, synth.true_graph(data, case), Oracle(case)
+
+
+def test_msweep_fp64_ped150(ped150):
+    case, g0, o = ped150
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(g0[None])
+        lik = ctx.mcmc_likelihood()[0]
+        ctx.msweep(2024, 1)
+        out = ctx.mcmc_graphs()[0]
+        assert np.all(ctx.mcmc_status() == 0)
+        near = int(ctx.mcmc_near_edge()[0])
+    ref = np.array([o.fag(g0, l) for l in range(case.M)])
+    assert np.array_equal(lik, ref)
+    print(f"ped150 M-sweep: {near} draws within 1e-12 of their threshold")
+    assert near == 0
+    assert np.array_equal(out, o.msweep(g0, shuffled_order(o, 2024, 0, 1), 2024, 0, 1))
