@@ -180,6 +180,33 @@ int sl_lsweep(sl_ctx* ctx, uint64_t seed, uint32_t sweep, int32_t first_parity);
  * locus gets status SL_E_ILLEGAL_GRAPH and is left unchanged. */
 int sl_msweep(sl_ctx* ctx, uint64_t seed, uint32_t sweep, const int32_t* order, int32_t n_order);
 
+/* Sweep schedule of MarkovChain::run (markov_chain.cc:314-404) with the
+ * options -b (burnin), -i + -b (sweeps), -x (period) and -l (lprob). */
+typedef struct {
+    int32_t sweeps;       /* sweeps per chain, burn-in included */
+    int32_t burnin;       /* sweeps before the first scored graph */
+    int32_t period;       /* score every period-th sweep (>= 1) */
+    double lprob;         /* probability that a sweep is an L-sweep */
+    uint64_t seed;        /* Philox key */
+    uint32_t first_sweep; /* index of the first sweep (to continue a run) */
+} sl_schedule;
+
+typedef struct {
+    int64_t l_sweeps;     /* summed over chains */
+    int64_t m_sweeps;
+    int64_t scored;       /* graphs added to the LOD accumulators */
+} sl_mcmc_stats;
+
+/* Runs every chain through the schedule on the device.  Sweep s of chain c
+ * is an L-sweep when the uniform (0, step=0xFFFFFFFF, s, c) < lprob, with
+ * loci of parity (1, 0xFFFFFFFF, s, c) < 0.5 ? 0 : 1 first, else an M-sweep
+ * (sl_lsweep, sl_msweep).  After sweep s, when s >= burnin and s % period ==
+ * 0, every chain's graph is scored into the LOD accumulators
+ * (markov_chain.cc:366-389; several chains merge as -R does).  Returns
+ * SL_E_ZERO_LIKELIHOOD or SL_E_ILLEGAL_GRAPH when a chain failed (its graph
+ * stops changing at the failing step; sl_mcmc_status says which). */
+int sl_mcmc_run(sl_ctx* ctx, const sl_schedule* schedule, sl_mcmc_stats* stats);
+
 /* Founder-allele-graph likelihood of every chain's current graph at every
  * locus into per_locus [C][M] (FounderAlleleGraph4::reset + likelihood per
  * locus, the legality check of DescentGraph::_sum_prior_prob,

@@ -11,6 +11,7 @@
 #include "../../include/swiftlink_amd.h"
 #include "kernels.h"
 #include "msampler_kernels.h"
+#include "philox.h"
 #include "program.h"
 #include "sampler_kernels.h"
 #include "sampler_program.h"
@@ -740,16 +741,17 @@ static sl::SampArgs samp_args(sl_ctx* c) {
     return A;
 }
 
-int sl_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int32_t first_parity) {
-    if (!c || (first_parity != 0 && first_parity != 1)) return SL_E_INVALID;
-    int rc = need_sampler(c);
-    if (rc) return rc;
-    if (c->n_chains <= 0) return fail(c, SL_E_INVALID, "no chains: call sl_mcmc_init first");
+// Both parity launches of one L-sweep.  check_kind: every chain takes its
+// kind and first parity from its schedule draws (lprob), first_parity unused.
+static int run_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int first_parity, int check_kind, double lprob) {
     sl::SampArgs A = samp_args(c);
     A.seed = seed;
     A.sweep = sweep;
+    A.check_kind = check_kind;
+    A.lprob = lprob;
     for (int pass = 0; pass < 2; ++pass) {
         A.parity = pass == 0 ? first_parity : 1 - first_parity;
+        A.pass = pass;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (c->prof) {
             e0 = take_event(c);
@@ -763,6 +765,14 @@ int sl_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int32_t first_parity) {
         }
     }
     return SL_OK;
+}
+
+int sl_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int32_t first_parity) {
+    if (!c || (first_parity != 0 && first_parity != 1)) return SL_E_INVALID;
+    int rc = need_sampler(c);
+    if (rc) return rc;
+    if (c->n_chains <= 0) return fail(c, SL_E_INVALID, "no chains: call sl_mcmc_init first");
+    return run_lsweep(c, seed, sweep, first_parity, 0, 0.0);
 }
 
 static sl::MsampArgs msamp_args(sl_ctx* c) {
@@ -795,24 +805,9 @@ static sl::MsampArgs msamp_args(sl_ctx* c) {
     return A;
 }
 
-int sl_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* order, int32_t n_order) {
-    if (!c) return SL_E_INVALID;
-    if (!c->m_ready) return fail(c, SL_E_UNSUPPORTED, "M-sampler unavailable: " + c->m_why);
-    if (c->n_chains <= 0) return fail(c, SL_E_INVALID, "no chains: call sl_mcmc_init first");
-    int nm = (int)c->meioses_h.size();
-    if (order) {
-        if (n_order < 0 || n_order > 2 * (c->N - c->F)) return fail(c, SL_E_INVALID, "bad meiosis order length");
-        for (int j = 0; j < n_order; ++j)
-            if (order[j] < 0 || order[j] >= 2 * (c->N - c->F)) return fail(c, SL_E_INVALID, "meiosis id out of range");
-        nm = n_order;
-        if (nm > 0) {
-            int rc = ensure(c, &c->d_fixed_order, &c->fixed_cap, (size_t)nm);
-            if (rc) return rc;
-            SL_HIP(c, hipMemcpyAsync(c->d_fixed_order, order, (size_t)nm * sizeof(int32_t), hipMemcpyHostToDevice,
-                                     c->stream));
-            SL_HIP(c, hipStreamSynchronize(c->stream));
-        }
-    }
+// One M-sweep: order, current-column likelihoods, one launch per meiosis step.
+static int run_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* dev_order, int nm, int check_kind,
+                      double lprob) {
     if (nm == 0) return SL_OK;
     int rc = ensure(c, &c->d_order, &c->order_cap, (size_t)nm * c->n_chains);
     if (rc) return rc;
@@ -820,13 +815,15 @@ int sl_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* order, in
     A.seed = seed;
     A.sweep = sweep;
     A.n_meioses = nm;
+    A.check_kind = check_kind;
+    A.lprob = lprob;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->prof) {
         e0 = take_event(c);
         e1 = take_event(c);
         if (e0 && e1) SL_HIP(c, hipEventRecord(e0, c->stream));
     }
-    SL_HIP(c, sl::launch_morder(A, order ? c->d_fixed_order : nullptr, c->stream));
+    SL_HIP(c, sl::launch_morder(A, dev_order, c->stream));
     A.mode = 0;
     SL_HIP(c, sl::launch_mstep(A, c->m_regs, c->m_block, c->m_lds, c->stream));
     A.mode = 1;
@@ -841,10 +838,84 @@ int sl_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* order, in
     return SL_OK;
 }
 
-int sl_mcmc_likelihood(sl_ctx* c, double* per_locus) {
-    if (!c || !per_locus) return SL_E_INVALID;
+static int need_msampler(sl_ctx* c) {
     if (!c->m_ready) return fail(c, SL_E_UNSUPPORTED, "M-sampler unavailable: " + c->m_why);
     if (c->n_chains <= 0) return fail(c, SL_E_INVALID, "no chains: call sl_mcmc_init first");
+    return SL_OK;
+}
+
+int sl_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* order, int32_t n_order) {
+    if (!c) return SL_E_INVALID;
+    int rc = need_msampler(c);
+    if (rc) return rc;
+    int nm = (int)c->meioses_h.size();
+    if (order) {
+        if (n_order < 0 || n_order > 2 * (c->N - c->F)) return fail(c, SL_E_INVALID, "bad meiosis order length");
+        for (int j = 0; j < n_order; ++j)
+            if (order[j] < 0 || order[j] >= 2 * (c->N - c->F)) return fail(c, SL_E_INVALID, "meiosis id out of range");
+        nm = n_order;
+        if (nm > 0) {
+            rc = ensure(c, &c->d_fixed_order, &c->fixed_cap, (size_t)nm);
+            if (rc) return rc;
+            SL_HIP(c, hipMemcpyAsync(c->d_fixed_order, order, (size_t)nm * sizeof(int32_t), hipMemcpyHostToDevice,
+                                     c->stream));
+            SL_HIP(c, hipStreamSynchronize(c->stream));
+        }
+    }
+    return run_msweep(c, seed, sweep, order ? c->d_fixed_order : nullptr, nm, 0, 0.0);
+}
+
+int sl_mcmc_run(sl_ctx* c, const sl_schedule* sch, sl_mcmc_stats* stats) {
+    if (!c || !sch) return SL_E_INVALID;
+    if (sch->sweeps < 0 || sch->burnin < 0 || sch->period < 1 || !(sch->lprob >= 0.0 && sch->lprob <= 1.0))
+        return fail(c, SL_E_INVALID, "bad schedule");
+    if (c->n_chains <= 0) return fail(c, SL_E_INVALID, "no chains: call sl_mcmc_init first");
+    const int C = c->n_chains;
+    sl_mcmc_stats st{0, 0, 0};
+    for (int k = 0; k < sch->sweeps; ++k) {
+        const uint32_t s = sch->first_sweep + (uint32_t)k;
+        int nl = 0, p0 = 0;
+        for (int ch = 0; ch < C; ++ch)
+            if (sl::philox_uniform(sch->seed, (uint32_t)ch, s, sl::kStepSched, 0) < sch->lprob) {
+                ++nl;
+                p0 = sl::philox_uniform(sch->seed, (uint32_t)ch, s, sl::kStepSched, 1) < 0.5 ? 0 : 1;
+            }
+        const int nm = C - nl;
+        int rc = SL_OK;
+        if (nl && !c->samp_ready) return fail(c, SL_E_UNSUPPORTED, "L-sampler unavailable: " + c->samp_why);
+        if (nm && !c->m_ready) return fail(c, SL_E_UNSUPPORTED, "M-sampler unavailable: " + c->m_why);
+        if (C == 1) {
+            // one chain: launch only the sweep it takes
+            rc = nl ? run_lsweep(c, sch->seed, s, p0, 0, 0.0)
+                    : run_msweep(c, sch->seed, s, nullptr, (int)c->meioses_h.size(), 0, 0.0);
+        } else {
+            if (nl) rc = run_lsweep(c, sch->seed, s, 0, 1, sch->lprob);
+            if (!rc && nm) rc = run_msweep(c, sch->seed, s, nullptr, (int)c->meioses_h.size(), 1, sch->lprob);
+        }
+        if (rc) return rc;
+        st.l_sweeps += nl;
+        st.m_sweeps += nm;
+        if ((int64_t)s >= sch->burnin && s % (uint32_t)sch->period == 0) {
+            rc = lod_run(c, c->d_chains, C, nullptr);
+            if (rc) return rc;
+            c->count += C;
+            st.scored += C;
+        }
+    }
+    if (stats) *stats = st;
+    std::vector<int32_t> status((size_t)C);
+    SL_HIP(c, hipMemcpyAsync(status.data(), c->d_status, status.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    for (int32_t v : status)
+        if (v != 0) return fail(c, v, "a chain met a zero likelihood or an illegal graph (see sl_mcmc_status)");
+    return SL_OK;
+}
+
+int sl_mcmc_likelihood(sl_ctx* c, double* per_locus) {
+    if (!c || !per_locus) return SL_E_INVALID;
+    int rc = need_msampler(c);
+    if (rc) return rc;
     sl::MsampArgs A = msamp_args(c);
     A.mode = 0;
     std::vector<int32_t> st((size_t)c->n_chains, 0);

@@ -149,9 +149,16 @@ __global__ void lsampler_kernel(SampArgs A) {
     uint8_t* bits = pmk + N;                            // [2NF]
     __shared__ int s_status;
 
-    const int locus = A.parity + 2 * blockIdx.x;
     const int chain = blockIdx.y;
-    if (locus >= M) return;  // uniform over the workgroup
+    int parity = A.parity;
+    if (A.check_kind) {
+        // the chain's sweep kind and first parity (draws 0 and 1 at step kStepSched)
+        if (!(philox_uniform(A.seed, (uint32_t)chain, A.sweep, kStepSched, 0) < A.lprob)) return;
+        const int p0 = philox_uniform(A.seed, (uint32_t)chain, A.sweep, kStepSched, 1) < 0.5 ? 0 : 1;
+        parity = A.pass == 0 ? p0 : 1 - p0;
+    }
+    const int locus = parity + 2 * blockIdx.x;
+    if (locus >= M || A.status[chain] != 0) return;  // uniform over the workgroup
     uint32_t* g = A.graphs + (size_t)chain * M * A.W;
     const uint32_t* colL = locus > 0 ? g + (size_t)(locus - 1) * A.W : nullptr;
     const uint32_t* colR = locus < M - 1 ? g + (size_t)(locus + 1) * A.W : nullptr;
@@ -461,7 +468,7 @@ hipError_t configure_lsampler_lds(int precision, size_t lds) {
 }
 
 hipError_t launch_lsampler(const SampArgs& A, int precision, int block, size_t lds, int n_chains, hipStream_t s) {
-    const int nloci = (A.M - A.parity + 1) / 2;
+    const int nloci = A.check_kind ? (A.M + 1) / 2 : (A.M - A.parity + 1) / 2;
     dim3 grid(nloci, n_chains);
     if (precision == 0)
         hipLaunchKernelGGL((lsampler_kernel<double, false>), grid, dim3(block), lds, s, A);

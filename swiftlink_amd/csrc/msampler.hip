@@ -40,9 +40,7 @@ namespace sl {
 
 namespace {
 
-constexpr int G_HETERO = 1, G_HOMOZ_A = 2, G_HOMOZ_B = 3;
-constexpr uint32_t kStepSched = 0xFFFFFFFFu;
-constexpr uint32_t kStepShuffle = 0xFFFFFFFEu;
+constexpr int G_HETERO = 1, G_HOMOZ_B = 3;  // genotype codes (genotype.h)
 constexpr int kIllegalGraph = -5;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
@@ -55,6 +53,13 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ uint32_t rdlane(uint32_t v, int lane) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+__device__ __forceinline__ double rdlane_d(double v, int lane) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint64_t lo = rdlane((uint32_t)b, lane);
+    const uint64_t hi = rdlane((uint32_t)(b >> 32), lane);
+    return __longlong_as_double((long long)((hi << 32) | lo));
 }
 
 // Compile-time loop: fn(std::integral_constant<int, I>) for I in [0, N).
@@ -93,16 +98,17 @@ struct RegArr {
 
 template <int R>
 struct Regs : RegArr<uint32_t, R> {
+    // register selected per lane with the uniform index, then one lane read
     __device__ __forceinline__ uint32_t get(int idx) const {
-        uint32_t out = 0;
-        static_for<0, R>([&](auto rc) {
+        uint32_t sel = this->template at<0>();
+        static_for<1, R>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
-            if ((idx >> 6) == r) out = rdlane(this->template at<r>(), idx & 63);
+            sel = (idx >> 6) == r ? this->template at<r>() : sel;
         });
-        return out;
+        return rdlane(sel, idx & 63);
     }
-    __device__ __forceinline__ void set(int idx, uint32_t x) {
-        const bool me = lane_id() == (idx & 63);
+    __device__ __forceinline__ void set(int idx, uint32_t x, bool en = true) {
+        const bool me = en && lane_id() == (idx & 63);
         static_for<0, R>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
             uint32_t& v = this->template at<r>();
@@ -111,327 +117,278 @@ struct Regs : RegArr<uint32_t, R> {
     }
 };
 
+// The two probability products of each component, kept as one pair per
+// element and always updated together.
+struct PPair {
+    double p0, p1;
+};
+
 template <int R>
-struct RegsD : RegArr<double, R> {
-    __device__ __forceinline__ double get(int idx) const {
-        double out = 0.0;
-        static_for<0, R>([&](auto rc) {
+struct RegsP : RegArr<PPair, R> {
+    __device__ __forceinline__ PPair get(int idx) const {
+        PPair sel = this->template at<0>();
+        static_for<1, R>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
-            if ((idx >> 6) == r) {
-                const uint64_t b = (uint64_t)__double_as_longlong(this->template at<r>());
-                const uint64_t lo = rdlane((uint32_t)b, idx & 63);
-                const uint64_t hi = rdlane((uint32_t)(b >> 32), idx & 63);
-                out = __longlong_as_double((long long)((hi << 32) | lo));
-            }
+            const PPair& o = this->template at<r>();
+            sel.p0 = (idx >> 6) == r ? o.p0 : sel.p0;
+            sel.p1 = (idx >> 6) == r ? o.p1 : sel.p1;
         });
-        return out;
+        return PPair{rdlane_d(sel.p0, idx & 63), rdlane_d(sel.p1, idx & 63)};
     }
-    // en is wave-uniform
-    __device__ __forceinline__ void set(int idx, double x, bool en = true) {
-        const bool me = lane_id() == (idx & 63);
+    // element idx becomes set ? (x0, x1) : (p0 * x0, p1 * x1) (en and set wave-uniform)
+    __device__ __forceinline__ void update(int idx, double x0, double x1, bool set, bool en) {
+        const bool me = en && lane_id() == (idx & 63);
         static_for<0, R>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
-            double& v = this->template at<r>();
-            v = (en && (idx >> 6) == r && me) ? x : v;
-        });
-    }
-    __device__ __forceinline__ void mul(int idx, double x, bool en = true) {
-        const bool me = lane_id() == (idx & 63);
-        static_for<0, R>([&](auto rc) {
-            constexpr int r = decltype(rc)::value;
-            double& v = this->template at<r>();
-            v = (en && (idx >> 6) == r && me) ? v * x : v;
+            PPair& v = this->template at<r>();
+            const bool hit = (idx >> 6) == r && me;
+            const double n0 = set ? x0 : v.p0 * x0;
+            const double n1 = set ? x1 : v.p1 * x1;
+            v.p0 = hit ? n0 : v.p0;
+            v.p1 = hit ? n1 : v.p1;
         });
     }
 };
 
-__device__ __forceinline__ bool fa_legal(int obs, int a1, int a2) {  // founder_allele_graph4.cc:451-466
-    if (obs == G_HETERO) return a1 != a2;
-    if (obs == G_HOMOZ_A) return a1 == G_HOMOZ_A && a2 == G_HOMOZ_A;
-    return a1 == G_HOMOZ_B && a2 == G_HOMOZ_B;
-}
-
-__device__ __forceinline__ int fa_other(int obs, int a1) {  // :468-483 (0 = impossible)
-    if (obs == G_HETERO) return a1 == G_HOMOZ_A ? G_HOMOZ_B : G_HOMOZ_A;
-    if (obs == G_HOMOZ_A) return a1 == G_HOMOZ_A ? G_HOMOZ_A : 0;
-    return a1 == G_HOMOZ_B ? G_HOMOZ_B : 0;
-}
-
-// Founder-allele graph of one column.  node word: member+1 (bits 0-15, 0 =
-// none), assignment 0 (bits 16-17), assignment 1 (bits 18-19); component
-// word: fixed+1 (bits 0-1), active (bit 2).
+// Founder-allele graph of one column.
+//
+// The reference keeps two alternative allele assignments per component
+// (founder_allele_graph4.cc:34-442).  Within a component that is not fixed
+// they are always complementary (a new heterozygous pair starts them so, and
+// every extension and merge preserves it), and of a fixed component only the
+// chosen one is ever read.  So a founder allele here carries one bit b, its
+// allele under assignment 0 (0 = A, 1 = B); under assignment k its allele is
+// b ^ k, and swapping a component's assignments on merge (combine_components'
+// flip) is b ^= 1 over its members.  node word: member+1 (bits 0-15, 0 =
+// none), b (bit 16); component word: fixed+1 (bits 0-1, 0 = not fixed),
+// active (bit 2).
+//
+// Each edge decides with wave-uniform scalar logic what the reference's case
+// analysis does, as a fixed set of actions: up to two node words, an optional
+// merge, up to two component words and one probability update
+// P_k[c] = set ? v_k : P_k[c] * v_k, applied with masked lane selects.  A
+// probability the reference sets to zero is multiplied by 0.0 and an
+// untouched one by 1.0 (probabilities are finite and non-negative, so both
+// are exact), and a merge multiplies in the merged component's zeroed
+// probabilities, so every product is the reference's bit for bit.
 template <int R>
 struct Fag {
     Regs<R> node, comp;
-    RegsD<R> p0, p1;
+    RegsP<R> pp;
     int gi;
-    double major, minor;
 
-    __device__ __forceinline__ void init(double q) {
+    __device__ __forceinline__ void init() {
         node = Regs<R>{};
         comp = Regs<R>{};
-        p0 = RegsD<R>{};
-        p1 = RegsD<R>{};
+        pp = RegsP<R>{};
         gi = 0;
-        minor = q;
-        major = 1.0 - q;
     }
-    __device__ __forceinline__ double freq(int g) const { return g == G_HOMOZ_A ? major : minor; }
-    __device__ __forceinline__ static int member_of(uint32_t w) { return (int)(w & 0xFFFFu) - 1; }
-    __device__ __forceinline__ static int assign_of(uint32_t w, int k) { return (int)((w >> (16 + 2 * k)) & 3u); }
-    __device__ __forceinline__ int fixed(int c) const { return (int)(comp.get(c) & 3u) - 1; }
-    __device__ __forceinline__ void set_fixed(int c, int f) { comp.set(c, (comp.get(c) & ~3u) | (uint32_t)(f + 1)); }
-    __device__ __forceinline__ void set_member(int x, int c) {
-        node.set(x, (node.get(x) & ~0xFFFFu) | (uint32_t)(c + 1));
+    __device__ __forceinline__ static uint32_t node_word(int c, int bit) {
+        return (uint32_t)(c + 1) | ((uint32_t)bit << 16);
     }
-    __device__ __forceinline__ void set_assign(int k, int x, int g) {
-        const int sh = 16 + 2 * k;
-        node.set(x, (node.get(x) & ~(3u << sh)) | ((uint32_t)g << sh));
-    }
-    // Every probability update writes both arrays unconditionally with
-    // selected values (a factor 1.0 is exact), so the compiler never merges
-    // updates of p0 and p1 into a store through a selected array address.
-    __device__ __forceinline__ void pset(int k, int c, double x) {
-        const double a = p0.get(c), b = p1.get(c);
-        pset2(c, k == 0 ? x : a, k != 0 ? x : b);
-    }
-    __device__ __forceinline__ void pmul(int k, int c, double x) { pmul2(c, k == 0 ? x : 1.0, k != 0 ? x : 1.0); }
-    __device__ __forceinline__ void pset2(int c, double x0, double x1) {
-        p0.set(c, x0);
-        p1.set(c, x1);
-    }
-    __device__ __forceinline__ void pmul2(int c, double x0, double x1) {
-        p0.mul(c, x0);
-        p1.mul(c, x1);
-    }
+    __device__ __forceinline__ static uint32_t comp_word(int fixed) { return (uint32_t)(fixed + 1) | 4u; }
 
-    // combine_components (founder_allele_graph4.cc:501-541): merge c2 into c1
-    __device__ __forceinline__ void combine(int c1, int c2, bool flip) {
-        const uint32_t m1 = (uint32_t)(c1 + 1), m2 = (uint32_t)(c2 + 1);
-        static_for<0, R>([&](auto rc) {
-            constexpr int r = decltype(rc)::value;
-            uint32_t w = node.template at<r>();
-            if ((w & 0xFFFFu) == m2) {
-                w = (w & ~0xFFFFu) | m1;
-                if (flip) w = (w & ~(0xFu << 16)) | (((w >> 18) & 3u) << 16) | (((w >> 16) & 3u) << 18);
-            }
-            node.template at<r>() = w;
-        });
-        const double q0 = p0.get(c2), q1 = p1.get(c2);
-        pmul2(c1, flip ? q1 : q0, flip ? q0 : q1);
-        comp.set(c2, comp.get(c2) & ~4u);
-    }
+    // one typed person with genotype g and founder alleles (x, y)
+    // (FounderAlleleGraph4::likelihood, :34-424); false when illegal.  The
+    // allele frequencies are arguments, not members: selecting between two
+    // members would make the compiler address the struct in memory.
+    __device__ __forceinline__ bool edge(int g, int x, int y, double major, double minor) {
+        const auto fv = [major, minor](int bit) { return bit ? minor : major; };
+        const bool het = g == G_HETERO;
+        const int v = g == G_HOMOZ_B ? 1 : 0;  // homozygous allele
+        const uint32_t wx = node.get(x);
+        const uint32_t wy = x == y ? wx : node.get(y);
+        const int c1 = (int)(wx & 0xFFFFu) - 1, c2 = (int)(wy & 0xFFFFu) - 1;
+        const int bx = (int)(wx >> 16) & 1, by = (int)(wy >> 16) & 1;
+        const int f1 = (int)(comp.get(c1 < 0 ? 0 : c1) & 3u) - 1;
+        const int f2 = (int)(comp.get(c2 < 0 ? 0 : c2) & 3u) - 1;
 
-    // one typed person with genotype g and founder alleles (mf, pf)
-    // (FounderAlleleGraph4::likelihood, :34-424); false when illegal
-    __device__ __forceinline__ bool edge(int g, int mf, int pf) {
-        if (mf == pf) {  // autozygous (:77-124)
-            if (g == G_HETERO) return false;
-            const uint32_t wm = node.get(mf);
-            const int g1 = member_of(wm);
-            if (g1 != -1) {
-                const int f1 = fixed(g1);
-                if (f1 != -1) {
-                    if (g != assign_of(wm, f1)) return false;
-                } else if (assign_of(wm, 0) == g) {
-                    set_fixed(g1, 0);
-                    pset(1, g1, 0.0);
-                } else if (assign_of(wm, 1) == g) {
-                    set_fixed(g1, 1);
-                    pset(0, g1, 0.0);
-                } else {
-                    return false;
+        bool legal = true;
+        uint32_t nwx = 0, nwy = 0;
+        bool wr_x = false, wr_y = false;
+        int ca = 0, cb = 0;
+        uint32_t cwa = 0, cwb = 0;
+        bool wr_a = false, wr_b = false;
+        bool merge = false;
+        int mflip = 0;
+        int cp = 0;
+        bool pu = false, pset = false;
+        double v0 = 1.0, v1 = 1.0;
+        bool newc = false;
+
+        if (x == y) {  // autozygous (:77-124)
+            if (het) {
+                legal = false;
+            } else if (c1 >= 0) {
+                if (f1 >= 0) {
+                    legal = (bx ^ f1) == v;
+                } else {  // becomes fixed to the assignment giving allele v
+                    const int f = bx ^ v;
+                    ca = c1;
+                    cwa = comp_word(f);
+                    wr_a = true;
+                    cp = c1;
+                    pu = true;
+                    v0 = f == 0 ? 1.0 : 0.0;
+                    v1 = f == 0 ? 0.0 : 1.0;
                 }
             } else {
-                node.set(mf, (wm & ~(0xFFFFu | (3u << 16))) | (uint32_t)(gi + 1) | ((uint32_t)g << 16));
-                comp.set(gi, 1u | 4u);  // fixed 0, active
-                pset2(gi, freq(g), 0.0);
-                ++gi;
+                nwx = node_word(gi, v);
+                wr_x = true;
+                ca = gi;
+                cwa = comp_word(0);
+                wr_a = true;
+                cp = gi;
+                pu = pset = true;
+                v0 = fv(v);
+                v1 = 0.0;
+                newc = true;
             }
-            return true;
-        }
-        const uint32_t wm = node.get(mf), wp = node.get(pf);
-        const int g1 = member_of(wm), g2 = member_of(wp);
-        if (g1 != -1) {
-            int f1 = fixed(g1);
-            if (g2 != -1) {
-                if (g1 == g2) {  // same component (:135-166)
-                    if (f1 != -1) {
-                        if (!fa_legal(g, assign_of(wm, f1), assign_of(wp, f1))) return false;
-                    } else {
-                        const bool l0 = fa_legal(g, assign_of(wm, 0), assign_of(wp, 0));
-                        const bool l1 = fa_legal(g, assign_of(wm, 1), assign_of(wp, 1));
-                        if (l0) {
-                            if (!l1) {
-                                set_fixed(g1, 0);
-                                pset(1, g1, 0.0);
-                            }
-                        } else if (l1) {
-                            set_fixed(g1, 1);
-                            pset(0, g1, 0.0);
-                        } else {
-                            return false;
-                        }
-                    }
-                    return true;
-                }
-                // two components (:167-278)
-                int f2 = fixed(g2);
-                if (f1 != -1) {
-                    if (f2 != -1) {
-                        if (!fa_legal(g, assign_of(wm, f1), assign_of(wp, f2))) return false;
-                    } else {
-                        const bool l0 = fa_legal(g, assign_of(wm, f1), assign_of(wp, 0));
-                        const bool l1 = fa_legal(g, assign_of(wm, f1), assign_of(wp, 1));
-                        if (l0) f2 = 0;
-                        else if (l1) f2 = 1;
-                        else return false;
-                    }
-                } else if (f2 != -1) {
-                    const bool l0 = fa_legal(g, assign_of(wm, 0), assign_of(wp, f2));
-                    const bool l1 = fa_legal(g, assign_of(wm, 1), assign_of(wp, f2));
-                    if (l0) f1 = 0;
-                    else if (l1) f1 = 1;
-                    else return false;
-                } else {
-                    const bool l0 = fa_legal(g, assign_of(wm, 0), assign_of(wp, 0));
-                    const bool l1 = fa_legal(g, assign_of(wm, 1), assign_of(wp, 0));
-                    const bool l2 = fa_legal(g, assign_of(wm, 0), assign_of(wp, 1));
-                    const bool l3 = fa_legal(g, assign_of(wm, 1), assign_of(wp, 1));
-                    if (!(l0 || l1 || l2 || l3)) return false;
-                    if (l0 && !(l1 || l2 || l3)) {
-                        f1 = f2 = 0;
-                    } else if (l1 && !(l0 || l2 || l3)) {
-                        f1 = 1;
-                        f2 = 0;
-                    } else if (l2 && !(l0 || l1 || l3)) {
-                        f1 = 0;
-                        f2 = 1;
-                    } else if (l3 && !(l0 || l1 || l2)) {
-                        f1 = f2 = 1;
-                    } else if (l0 && l3 && !(l1 || l2)) {
-                        f1 = f2 = -1;
-                    } else if (l1 && l2 && !(l0 || l3)) {
-                        f1 = f2 = -2;
-                    } else {
-                        f1 = f2 = -1;
-                    }
-                }
-                if (f1 != f2) {
-                    set_fixed(g1, f1);
-                    pset(1 - f1, g1, 0.0);
-                    pset(1 - f2, g2, 0.0);
-                    combine(g1, g2, true);
-                } else if (f1 == -2) {
-                    set_fixed(g1, -1);
-                    combine(g1, g2, true);
-                } else {
-                    set_fixed(g1, f1);
-                    if (f1 != -1) {
-                        pset(1 - f1, g1, 0.0);
-                        pset(1 - f2, g2, 0.0);
-                    }
-                    combine(g1, g2, false);
-                }
-                return true;
-            }
-            // only the maternal allele placed (:280-327)
-            if (f1 != -1) {
-                const int t0 = fa_other(g, assign_of(wm, f1));
-                if (t0 == 0) return false;
-                set_assign(f1, pf, t0);
-                pmul(f1, g1, freq(t0));
+        } else if (c1 >= 0 && c1 == c2) {  // same component (:135-166)
+            if (f1 >= 0) {
+                legal = het ? bx != by : ((bx ^ f1) == v && (by ^ f1) == v);
+            } else if (het) {
+                legal = bx != by;
             } else {
-                const int t0 = fa_other(g, assign_of(wm, 0));
-                const int t1 = fa_other(g, assign_of(wm, 1));
-                if (t0 != 0) {
-                    if (t1 != 0) {
-                        set_assign(0, pf, t0);
-                        set_assign(1, pf, t1);
-                        pmul2(g1, freq(t0), freq(t1));
-                    } else {
-                        set_assign(0, pf, t0);
-                        pmul(0, g1, freq(t0));
-                        pset(1, g1, 0.0);
-                        set_fixed(g1, 0);
-                    }
-                } else if (t1 != 0) {
-                    set_assign(1, pf, t1);
-                    pmul(1, g1, freq(t1));
-                    pset(0, g1, 0.0);
-                    set_fixed(g1, 1);
-                } else {
-                    return false;
-                }
+                legal = bx == by;
+                const int f = bx ^ v;
+                ca = c1;
+                cwa = comp_word(f);
+                wr_a = true;
+                cp = c1;
+                pu = true;
+                v0 = f == 0 ? 1.0 : 0.0;
+                v1 = f == 0 ? 0.0 : 1.0;
             }
-            set_member(pf, g1);
-            return true;
-        }
-        if (g2 != -1) {  // only the paternal allele placed (:329-379)
-            const int f2 = fixed(g2);
-            if (f2 != -1) {
-                const int t0 = fa_other(g, assign_of(wp, f2));
-                if (t0 == 0) return false;
-                set_assign(f2, mf, t0);
-                pmul(f2, g2, freq(t0));
+        } else if (c1 >= 0 && c2 >= 0) {  // two components (:167-278), then combine (:501-541)
+            int n1, n2, flip;
+            if (het) {
+                const int r = 1 ^ bx ^ by;  // required assignment1 ^ assignment2
+                flip = r;
+                if (f1 >= 0 && f2 >= 0) {
+                    legal = (f1 ^ f2) == r;
+                    n1 = f1;
+                    n2 = f2;
+                } else if (f1 >= 0) {
+                    n1 = f1;
+                    n2 = f1 ^ r;
+                } else if (f2 >= 0) {
+                    n1 = f2 ^ r;
+                    n2 = f2;
+                } else {
+                    n1 = n2 = -1;
+                }
             } else {
-                const int t0 = fa_other(g, assign_of(wp, 0));
-                const int t1 = fa_other(g, assign_of(wp, 1));
-                if (t0 != 0) {
-                    if (t1 != 0) {
-                        set_assign(0, mf, t0);
-                        set_assign(1, mf, t1);
-                        pmul2(g2, freq(t0), freq(t1));
-                    } else {
-                        set_assign(0, mf, t0);
-                        pmul(0, g2, freq(t0));
-                        pset(1, g2, 0.0);
-                        set_fixed(g2, 0);
-                    }
-                } else if (t1 != 0) {
-                    set_assign(1, mf, t1);
-                    pmul(1, g2, freq(t1));
-                    pset(0, g2, 0.0);
-                    set_fixed(g2, 1);
+                n1 = bx ^ v;
+                n2 = by ^ v;
+                legal = (f1 < 0 || f1 == n1) && (f2 < 0 || f2 == n2);
+                flip = n1 ^ n2;
+            }
+            const PPair q = pp.get(c2);
+            const double q0 = q.p0, q1 = q.p1;
+            const double z1_0 = n1 == 1 ? 0.0 : 1.0, z1_1 = n1 == 0 ? 0.0 : 1.0;
+            const double d0 = n2 == 1 ? 0.0 : q0, d1 = n2 == 0 ? 0.0 : q1;
+            merge = true;
+            mflip = flip;
+            ca = c1;
+            cwa = comp_word(n1);
+            wr_a = true;
+            cb = c2;
+            cwb = comp.get(c2) & ~4u;
+            wr_b = true;
+            cp = c1;
+            pu = true;
+            v0 = z1_0 * (flip ? d1 : d0);
+            v1 = z1_1 * (flip ? d0 : d1);
+        } else if (c1 >= 0 || c2 >= 0) {  // one allele placed (:280-379): the other joins
+            const bool xin = c1 >= 0;
+            const int c = xin ? c1 : c2, f = xin ? f1 : f2, bin = xin ? bx : by;
+            int nb, nf = f;
+            if (het) {
+                nb = bin ^ 1;
+                if (f >= 0) {
+                    v0 = f == 0 ? fv(bin ^ 1) : 1.0;
+                    v1 = f == 1 ? fv(bin) : 1.0;
                 } else {
-                    return false;
+                    v0 = fv(bin ^ 1);
+                    v1 = fv(bin);
+                }
+            } else {
+                const int k = bin ^ v;  // the only assignment giving the placed allele v
+                legal = f < 0 || f == k;
+                nb = v ^ k;
+                nf = k;
+                v0 = k == 0 ? fv(v) : 0.0;
+                v1 = k == 1 ? fv(v) : 0.0;
+                if (f >= 0) {
+                    v0 = k == 0 ? fv(v) : 1.0;
+                    v1 = k == 1 ? fv(v) : 1.0;
                 }
             }
-            set_member(mf, g2);
-            return true;
+            const uint32_t w = node_word(c, nb);
+            nwx = w;
+            nwy = w;
+            wr_x = !xin;
+            wr_y = xin;
+            ca = c;
+            cwa = comp_word(nf);
+            wr_a = true;
+            cp = c;
+            pu = true;
+        } else {  // new component of two alleles (:380-402)
+            nwx = node_word(gi, het ? 0 : v);
+            nwy = node_word(gi, het ? 1 : v);
+            wr_x = wr_y = true;
+            ca = gi;
+            cwa = comp_word(het ? -1 : 0);
+            wr_a = true;
+            cp = gi;
+            pu = pset = true;
+            v0 = het ? major * minor : fv(v) * fv(v);
+            v1 = het ? major * minor : 0.0;
+            newc = true;
         }
-        // new component of two alleles (:380-402)
-        const uint32_t mem = (uint32_t)(gi + 1);
-        if (g == G_HETERO) {
-            node.set(mf, (wm & ~(0xFFFFu | (0xFu << 16))) | mem | ((uint32_t)G_HOMOZ_A << 16) |
-                             ((uint32_t)G_HOMOZ_B << 18));
-            node.set(pf, (wp & ~(0xFFFFu | (0xFu << 16))) | mem | ((uint32_t)G_HOMOZ_B << 16) |
-                             ((uint32_t)G_HOMOZ_A << 18));
-            pset2(gi, major * minor, major * minor);
-            comp.set(gi, 4u);  // unfixed, active
-        } else {
-            node.set(mf, (wm & ~(0xFFFFu | (3u << 16))) | mem | ((uint32_t)g << 16));
-            node.set(pf, (wp & ~(0xFFFFu | (3u << 16))) | mem | ((uint32_t)g << 16));
-            pset2(gi, freq(g) * freq(g), 0.0);
-            comp.set(gi, 1u | 4u);  // fixed 0, active
+        if (!legal) return false;
+
+        node.set(x, nwx, wr_x);
+        node.set(y, nwy, wr_y);
+        if (merge) {  // members of c2 join c1, their bit flipped with the assignments
+            const uint32_t m2 = (uint32_t)(c2 + 1);
+            const uint32_t nw = (uint32_t)(c1 + 1);
+            const uint32_t fl = (uint32_t)mflip << 16;
+            static_for<0, R>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                const uint32_t w = node.template at<r>();
+                node.template at<r>() = (w & 0xFFFFu) == m2 ? (nw | ((w & 0x10000u) ^ fl)) : w;
+            });
         }
-        ++gi;
+        comp.set(ca, cwa, wr_a);
+        comp.set(cb, cwb, wr_b);
+        pp.update(cp, v0, v1, pset, pu);
+        gi += newc ? 1 : 0;
         return true;
     }
 
     // product over components in creation order (:426-441)
     // (an inactive component contributes an exact factor 1.0)
     __device__ __forceinline__ double likelihood() const {
-        RegsD<R> val;
+        RegArr<double, R> val;
         static_for<0, R>([&](auto rc) {
             constexpr int r = decltype(rc)::value;
             const uint32_t w = comp.template at<r>();
             const uint32_t f = w & 3u;
-            const double a = p0.template at<r>(), b = p1.template at<r>();
-            val.template at<r>() = !(w & 4u) ? 1.0 : f == 1u ? a : f == 2u ? b : a + b;
+            const PPair& q = pp.template at<r>();
+            val.template at<r>() = !(w & 4u) ? 1.0 : f == 1u ? q.p0 : f == 2u ? q.p1 : q.p0 + q.p1;
         });
         double ret = 1.0;
-        for (int c = 0; c < gi; ++c) ret *= val.get(c);
+        for (int c = 0; c < gi; ++c) {
+            double sel = val.template at<0>();
+            static_for<1, R>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                sel = (c >> 6) == r ? val.template at<r>() : sel;
+            });
+            ret *= rdlane_d(sel, c & 63);
+        }
         return ret;
     }
 };
@@ -455,7 +412,8 @@ __device__ __forceinline__ double fag_column(const MsampArgs& A, const uint32_t*
         wave_sync();
     }
     Fag<R> S;
-    S.init(A.minor[l]);
+    S.init();
+    const double minor = A.minor[l], major = 1.0 - minor;
     const uint8_t* gcol = A.geno_t + (size_t)l * A.N;
     for (int i0 = 0; i0 < A.N; i0 += 64) {
         const int i = i0 + lane;
@@ -474,7 +432,7 @@ __device__ __forceinline__ double fag_column(const MsampArgs& A, const uint32_t*
             mask &= mask - 1;
             const int ge = (int)rdlane((uint32_t)g, k);
             const uint32_t w = rdlane(lw, k);
-            if (!S.edge(ge, (int)(w & 0xFFFFu), (int)(w >> 16))) return 0.0;
+            if (!S.edge(ge, (int)(w & 0xFFFFu), (int)(w >> 16), major, minor)) return 0.0;
         }
     }
     return S.likelihood();

@@ -9,6 +9,12 @@
 
 namespace sl {
 
+// Philox step indices of the draws that are not per-locus or per-meiosis-step:
+// a sweep's kind (draw 0) and first L-sweep parity (draw 1), and the M-sweep's
+// meiosis-order shuffle.
+constexpr uint32_t kStepSched = 0xFFFFFFFFu;
+constexpr uint32_t kStepShuffle = 0xFFFFFFFEu;
+
 __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {

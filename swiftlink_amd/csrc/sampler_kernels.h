@@ -28,8 +28,11 @@ struct SampArgs {
     int32_t* status;             // [chains]
     uint64_t seed;
     uint32_t sweep;
-    int parity;
+    int parity;                  // loci of this parity (check_kind == 0)
     int cells_cap;
+    int check_kind;              // 1: act only on chains whose sweep is an L-sweep, and take
+    int pass;                    //    the parity of pass 0/1 from the chain's schedule draws
+    double lprob;                // L-sweep probability of the sweep schedule
 };
 
 size_t lsampler_lds_bytes(int precision, int cells_cap, int n_ops, int N, int F, int block);

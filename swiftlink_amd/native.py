@@ -51,6 +51,15 @@ class Options(C.Structure):
                 ("graphs_per_block", C.c_int32)]
 
 
+class Schedule(C.Structure):
+    _fields_ = [("sweeps", C.c_int32), ("burnin", C.c_int32), ("period", C.c_int32), ("lprob", C.c_double),
+                ("seed", C.c_uint64), ("first_sweep", C.c_uint32)]
+
+
+class McmcStats(C.Structure):
+    _fields_ = [("l_sweeps", C.c_int64), ("m_sweeps", C.c_int64), ("scored", C.c_int64)]
+
+
 class SlError(RuntimeError):
     pass
 
@@ -80,6 +89,7 @@ _SIGS = {
     "sl_mcmc_status": (C.c_int, [C.c_void_p, i32p]),
     "sl_lsweep": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32]),
     "sl_msweep": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, i32p, C.c_int32]),
+    "sl_mcmc_run": (C.c_int, [C.c_void_p, C.POINTER(Schedule), C.POINTER(McmcStats)]),
     "sl_mcmc_likelihood": (C.c_int, [C.c_void_p, f64p]),
     "sl_mcmc_near_edge": (C.c_int, [C.c_void_p, i64p]),
 }
@@ -249,6 +259,13 @@ class Context:
         else:
             o = np.ascontiguousarray(order, np.int32)
             self._check(lib().sl_msweep(self.h, seed, sweep, _ptr(o, i32p), o.size), "sl_msweep")
+
+    def mcmc_run(self, sweeps: int, burnin: int, period: int, lprob: float, seed: int, first_sweep: int = 0):
+        """Runs every chain through the sweep schedule, scoring into the LOD accumulators."""
+        sch = Schedule(sweeps, burnin, period, lprob, seed, first_sweep)
+        st = McmcStats()
+        self._check(lib().sl_mcmc_run(self.h, C.byref(sch), C.byref(st)), "sl_mcmc_run")
+        return {"l_sweeps": st.l_sweeps, "m_sweeps": st.m_sweeps, "scored": st.scored}
 
     def mcmc_likelihood(self) -> np.ndarray:
         out = np.zeros((self.n_chains, self.case.M))

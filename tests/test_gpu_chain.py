@@ -1,0 +1,118 @@
+"""Whole chains on the GPU (sl_mcmc_run) against the reference and the oracle.
+
+FP64: after the schedule, every chain's graph is identical to the reference's
+MarkovChain run fed the same Philox stream (golden fixture for chain 0) and to
+the oracle's (other chains), the scored-graph count matches, and the LOD
+accumulators agree within 1e-9 (the device merges graphs with a max-shifted
+log-sum-exp instead of the serial log_sum, so only rounding differs).  Several
+chains merge as the reference's -R replicates do: log-sum over chains.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden, GOLDEN_CASES
+from oracle_lib import Oracle
+from swiftlink_amd import native
+
+pytestmark = pytest.mark.gpu
+
+ACC_TOL = 1e-9
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_chain_fp64_matches_reference(name):
+    case, fx = load_golden(name)
+    seed, chain, sweeps, burnin, period = (int(x) for x in fx["chain_params"])
+    lprob = float(fx["chain_lprob"][0])
+    assert chain == 0
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(fx["graph_init"][None])
+        ctx.lod_reset()
+        st = ctx.mcmc_run(sweeps, burnin, period, lprob, seed)
+        out = ctx.mcmc_graphs()[0]
+        acc, cnt, _ = ctx.lod_results()
+        assert np.all(ctx.mcmc_near_edge() == 0)
+    kinds = fx["chain_kinds"]
+    assert st["l_sweeps"] == int((kinds == 0).sum()) and st["m_sweeps"] == int((kinds == 1).sum())
+    assert np.array_equal(out, fx["chain_out"])
+    assert cnt == int(fx["chain_count"][0]) == st["scored"]
+    np.testing.assert_allclose(acc, fx["chain_acc"], rtol=0, atol=ACC_TOL)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_chains_merge_like_replicates(name):
+    case, fx = load_golden(name)
+    o = Oracle(case)
+    starts = np.stack([fx["graph_init"], fx["graphs"][0], fx["graphs"][1], fx["graph_init"]])
+    sweeps, burnin, period, lprob, seed = 24, 6, 3, 0.5, 31
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(starts)
+        ctx.lod_reset()
+        st = ctx.mcmc_run(sweeps, burnin, period, lprob, seed)
+        out = ctx.mcmc_graphs()
+        acc, cnt, _ = ctx.lod_results()
+        assert np.all(ctx.mcmc_near_edge() == 0)
+    accs, total, kinds_all = [], 0, []
+    for c in range(len(starts)):
+        g, a, n, kinds = o.chain(starts[c], seed, c, sweeps, burnin, period, lprob)
+        assert np.array_equal(out[c], g), c
+        accs.append(a)
+        total += n
+        kinds_all.append(kinds)
+    kinds_all = np.array(kinds_all)
+    assert st["l_sweeps"] == int((kinds_all == 0).sum()) and st["m_sweeps"] == int((kinds_all == 1).sum())
+    assert cnt == total == st["scored"]
+    np.testing.assert_allclose(acc, np.logaddexp.reduce(np.array(accs), axis=0), rtol=0, atol=ACC_TOL)
+
+
+def test_run_continues_from_first_sweep():
+    case, fx = load_golden("ped30_snp100")
+    starts = np.stack([fx["graph_init"], fx["graphs"][0]])
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(starts)
+        ctx.lod_reset()
+        ctx.mcmc_run(30, 5, 5, 0.5, 8)
+        whole = ctx.mcmc_graphs()
+        acc_whole, n_whole, _ = ctx.lod_results()
+        ctx.mcmc_init(starts)
+        ctx.lod_reset()
+        ctx.mcmc_run(13, 5, 5, 0.5, 8)
+        ctx.mcmc_run(17, 5, 5, 0.5, 8, first_sweep=13)
+        parts = ctx.mcmc_graphs()
+        acc_parts, n_parts, _ = ctx.lod_results()
+    assert np.array_equal(whole, parts)
+    assert n_whole == n_parts
+    np.testing.assert_allclose(acc_parts, acc_whole, rtol=0, atol=ACC_TOL)
+
+
+def test_pure_l_and_pure_m_schedules():
+    case, fx = load_golden("east")
+    o = Oracle(case)
+    with native.Context(case, native.SL_FP64) as ctx:
+        for lprob, key in ((1.0, "l_sweeps"), (0.0, "m_sweeps")):
+            ctx.mcmc_init(fx["graph_init"][None])
+            st = ctx.mcmc_run(5, 100, 1, lprob, 3)
+            assert st[key] == 5 and st["scored"] == 0
+            g, _, _, kinds = o.chain(fx["graph_init"], 3, 0, 5, 100, 1, lprob)
+            assert np.all(kinds == (0 if lprob == 1.0 else 1))
+            assert np.array_equal(ctx.mcmc_graphs()[0], g)
+
+
+def test_chain_fp32_stays_legal_and_scores():
+    case, fx = load_golden("ped30_snp100")
+    o = Oracle(case)
+    starts = np.stack([fx["graph_init"]] * 3)
+    with native.Context(case, native.SL_FP32_RESCALED) as ctx:
+        ctx.mcmc_init(starts)
+        ctx.lod_reset()
+        st = ctx.mcmc_run(40, 10, 5, 0.5, 12)
+        out = ctx.mcmc_graphs()
+        acc, cnt, lod = ctx.lod_results()
+        lik = ctx.mcmc_likelihood()
+        assert np.all(ctx.mcmc_status() == 0)
+    assert cnt == st["scored"] == 3 * 6  # sweeps 10, 15, ..., 35
+    assert np.all(np.isfinite(acc)) and np.all(np.isfinite(lod))
+    for c in range(3):
+        ref = np.array([o.fag(out[c], l) for l in range(case.M)])
+        assert np.all(ref > 0)
+        assert np.array_equal(lik[c], ref)
