@@ -1,22 +1,26 @@
-"""Benchmark: LOD trait peel throughput on one MI355X.
+"""Benchmark: SwiftLink's MCMC linkage hot path on one MI355X.
 
-Workload (150 people over 6 generations with 3 consanguineous loops, 1000
-SNPs at 0.25 cM, 10% missing genotypes, 5 trait positions per interval):
-each step accumulates the LOD scores of a batch of descent graphs already
-resident in HBM, i.e. the trait peel at every (graph, interval, position).
-The metric's LOD evals/s is reported for the FP32-rescaled kernel; the plain
-FP64 kernel is timed on the same graphs, and |dLOD| between the two is stated.
+Workload ped150_snp1000: 150 people over 6 generations with 3 consanguineous
+loops, 1000 SNPs at 0.25 cM, 10% missing genotypes, 5 trait positions per
+interval; one chain, L-sampler:M-sampler 1:1, a scored graph every 10th sweep.
 
-Prints one JSON line (see the measurement contract in the design notes):
-value = LOD evals/s of the FP32-rescaled path, roofline of the peel kernel
-(HIP events on the library's stream), and a CPU baseline measured on this
-host with the reference's own CPU path (oracle/_ref/refharness) when present.
+A step is one sweep of the chain on the device (an L-sweep or an M-sweep as
+the schedule draw says), with the LOD scoring of the chain's graph on every
+10th sweep, so `value` is whole-job MCMC sweeps/s (FP32-rescaled samplers and
+peel, FP64 LOD accumulation).  Reported beside it: the FP64 chain's sweeps/s,
+LOD evals/s of the trait peel over a batch of 64 device-resident graphs in
+FP32-rescaled and FP64 with the largest |dLOD| between the two, the roofline
+of the dominant kernel (the meiosis-step kernel of the M-sampler, timed with
+HIP events on the library's stream) and of the peel kernel, and a CPU
+baseline: the reference's own CPU path (oracle/_ref/refharness) on this
+host's cores, or the oracle restatement when that binary is absent.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import subprocess
 import sys
 import tempfile
@@ -29,11 +33,24 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MCMC sweeps/s + LOD evals/s, FP32-rescaled vs FP64 at |ΔLOD|≤1e-4"
 FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+LPROB, PERIOD, SEED = 0.5, 10, 20250101
+
+
+def profile_traffic(kernel: str, workload: str):
+    """HBM bytes per launch from the committed rocprofv3 counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(tmpdir: str, base: str, case, threads: int):
-    """Times the reference CPU path (Peeler::process over every interval,
-    one Peeler per OpenMP thread) on a bounded sample of the same workload."""
+    """Times the reference CPU path on a bounded sample of the same schedule:
+    L-sweeps (LocusSampler::step over both parities), M-sweeps (MeiosisSampler
+    reset + steps, OpenMP over loci) and Peeler::process over every interval,
+    combined as sweeps/s = 1 / (p tL + (1-p) tM + tLOD / period)."""
     harness = os.path.join(ROOT, "oracle", "_ref", "refharness")
     p, m, d = base + ".ped", base + ".map", base + ".dat"
     if os.path.exists(harness):
@@ -41,43 +58,41 @@ def cpu_baseline(tmpdir: str, base: str, case, threads: int):
         os.makedirs(out, exist_ok=True)
         subprocess.run([harness, "graphs", p, m, d, out, "5", "1", "1"], check=True, stdout=subprocess.DEVNULL,
                        stderr=subprocess.DEVNULL)
-        n_graphs = 4 * threads
-        subprocess.run([harness, "bench", p, m, d, out, os.path.join(out, "graphs.i32"), str(threads), "0", "0",
-                        str(n_graphs)], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        nl, nm, ng = 10, 10, 2 * threads
+        subprocess.run([harness, "bench", p, m, d, out, os.path.join(out, "graphs.i32"), str(threads), str(nl),
+                        str(nm), str(ng)], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
         b = json.load(open(os.path.join(out, "bench.json")))
-        return {"value": b["lod_evals"] / b["lod_seconds"], "unit": "LOD evals/s", "cores": threads,
-                "kind": "reference",
-                "sample": f"{n_graphs} graphs x {case.M - 1} intervals x {case.nlod} positions "
-                          f"({b['lod_evals']} evals, {b['lod_seconds']:.1f} s wall) with the reference's "
-                          f"Peeler::process, FP64, {threads} OpenMP threads"}
-    # the reference binary is absent: time the oracle restatement, one core
+        tl, tm, tg = b["l_seconds"] / nl, b["m_seconds"] / nm, b["lod_seconds"] / ng
+        sweep = LPROB * tl + (1 - LPROB) * tm + tg / PERIOD
+        return {"value": 1.0 / sweep, "unit": "sweeps/s", "cores": threads, "kind": "reference",
+                "lod_evals_per_s": b["lod_evals"] / b["lod_seconds"],
+                "l_sweep_ms": tl * 1e3, "m_sweep_ms": tm * 1e3, "lod_graph_ms": tg * 1e3,
+                "sample": f"{nl} L-sweeps, {nm} M-sweeps and {ng} graphs x {case.M - 1} intervals x {case.nlod} "
+                          f"positions with the reference's LocusSampler, MeiosisSampler and Peeler, FP64, "
+                          f"{threads} OpenMP threads ({b['l_seconds'] + b['m_seconds'] + b['lod_seconds']:.1f} s "
+                          f"wall)"}
+    # the reference binary is absent: the oracle restatement, one core
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
-    from swiftlink_amd import synth  # noqa
     o = Oracle(case)
     g = np.load(os.path.join(tmpdir, "g.npy"))
+    sweeps = 10
     t0 = time.perf_counter()
-    n = 0
-    while time.perf_counter() - t0 < 10.0:
-        for l in range(0, case.M - 1, 7):
-            o.lod_terms(g, l)
-            n += case.nlod
+    o.chain(g, SEED, 0, sweeps, 0, PERIOD, LPROB)
     dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "LOD evals/s", "cores": 1, "kind": "port",
-            "sample": f"{n} evals over every 7th interval of one graph, FP64 oracle, 1 core, {dt:.1f} s"}
+    return {"value": sweeps / dt, "unit": "sweeps/s", "cores": 1, "kind": "port",
+            "sample": f"{sweeps} sweeps of the oracle chain (same schedule, scored every {PERIOD}th), FP64, "
+                      f"1 core, {dt:.1f} s"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=64, help="descent graphs per step")
+    ap.add_argument("--steps", type=int, default=200, help="timed sweeps")
+    ap.add_argument("--warmup", type=int, default=20, help="untimed sweeps")
+    ap.add_argument("--batch", type=int, default=64, help="descent graphs per LOD-throughput launch")
     ap.add_argument("--config", default="ped150_snp1000")
-    ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--gpw", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--fp32-only", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -103,44 +118,75 @@ def main():
     case, data = synth.make_case(args.config, tmp)
     g0 = synth.true_graph(data, case)
     np.save(os.path.join(tmp, "g.npy"), g0)
-    B = args.batch
-    graphs = np.repeat(g0[None], B, axis=0)
-    packed = torch.from_numpy(pack_graphs(case, graphs).reshape(-1).view(np.int32)).cuda()
-    torch.cuda.synchronize()
-    evals_per_step = B * (case.M - 1) * case.nlod
 
-    def run(precision, steps, warmup):
-        ctx = native.Context(case, precision, block=args.block, gpw=args.gpw)
-        for _ in range(warmup):
-            ctx.lod_accumulate_packed(packed.data_ptr(), B)
+    # ---- MCMC: one chain through the schedule, scoring on the device ----
+    def chain(precision, steps, warmup):
+        ctx = native.Context(case, precision)
+        ctx.mcmc_init(g0[None])
+        ctx.lod_reset()
+        ctx.mcmc_run(warmup, 0, PERIOD, LPROB, SEED)
         ctx.sync()
         ctx.profile(True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(steps):
-            ctx.lod_accumulate_packed(packed.data_ptr(), B)
+        st = ctx.mcmc_run(steps, 0, PERIOD, LPROB, SEED, first_sweep=warmup)
         ctx.sync()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        prof = {k: ctx.profile_read(k) for k in (native.SL_KERNEL_LOD, native.SL_KERNEL_LSAMPLER,
+                                                 native.SL_KERNEL_MSAMPLER)}
+        near = int(ctx.mcmc_near_edge().sum())
+        ctx.close()
+        return dt, st, prof, near
+
+    dt32, st32, prof32, near32 = chain(native.SL_FP32_RESCALED, args.steps, args.warmup)
+    n64 = max(20, args.steps // 4)
+    dt64, st64, _, _ = chain(native.SL_FP64, n64, args.warmup)
+    value = args.steps / dt32
+    fp64_sweeps = n64 / dt64
+
+    # ---- LOD throughput on a batch of resident graphs ----
+    B = args.batch
+    packed = torch.from_numpy(pack_graphs(case, np.repeat(g0[None], B, axis=0)).reshape(-1).view(np.int32)).cuda()
+    torch.cuda.synchronize()
+    evals_per_launch = B * (case.M - 1) * case.nlod
+
+    def lod(precision, launches):
+        ctx = native.Context(case, precision)
+        for _ in range(3):
+            ctx.lod_accumulate_packed(packed.data_ptr(), B)
+        ctx.sync()
+        ctx.lod_reset()
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(launches):
+            ctx.lod_accumulate_packed(packed.data_ptr(), B)
+        ctx.sync()
+        dt = time.perf_counter() - t0
         kms, nl = ctx.profile_read(native.SL_KERNEL_LOD)
-        acc, count, lod = ctx.lod_results()
+        _, _, lodv = ctx.lod_results()
         flops, _ = ctx.lod_work()
         ctx.close()
-        return dt, kms / max(nl, 1), lod, flops
+        return launches * evals_per_launch / dt, kms / max(nl, 1), lodv, flops
 
-    dt32, k32, lod32, flops = run(native.SL_FP32_RESCALED, args.steps, args.warmup)
-    if args.fp32_only:
-        dt64, k64, lod64 = float("nan"), float("nan"), lod32
-    else:
-        dt64, k64, lod64, _ = run(native.SL_FP64, max(2, args.steps // 4), args.warmup)
-    # same graphs, same count of accumulations -> comparable LOD
-    value = args.steps * evals_per_step / dt32
-    fp64_value = max(2, args.steps // 4) * evals_per_step / dt64
-    achieved = flops * evals_per_step / (k32 * 1e-3) / 1e12
+    lod32, k32, lodv32, flops = lod(native.SL_FP32_RESCALED, 20)
+    lod64, _, lodv64, _ = lod(native.SL_FP64, 8)
+
+    # ---- roofline of the meiosis-step kernel: algorithmic bytes per launch
+    # = per (chain, meiosis, locus) ceil(2N/8) descent-graph bytes + ceil(2
+    # N_typed/8) genotype bytes, times M loci and one chain ----
+    ms_ms, ms_launches = prof32[native.SL_KERNEL_MSAMPLER]
+    n_typed = int(np.count_nonzero(case.typed))
+    bytes_per_unit = -(-2 * case.N // 8) + -(-2 * n_typed // 8)
+    bytes_per_launch = case.M * bytes_per_unit
+    ms_avg = ms_ms / max(ms_launches, 1)
+    achieved = bytes_per_launch / (ms_avg * 1e-3) / 1e9 if ms_launches else 0.0
+    lod_achieved = flops * evals_per_launch / (k32 * 1e-3) / 1e12
+
     res = {
         "metric": METRIC,
         "value": value,
-        "unit": "LOD evals/s",
+        "unit": "sweeps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -149,17 +195,29 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded pedigree, gene-drop genotypes; the gene-drop descent graph replicated per batch)",
+        "data": "synthetic (seeded pedigree and gene-drop genotypes; the chain starts from the gene-drop graph)",
         "config": {"workload": f"{args.config}: {case.N} people ({case.F} founders), {case.M} SNPs, "
-                               f"{case.nlod} positions/interval, {B} graphs per step",
-                   "graphs_per_step": B, "evals_per_step": evals_per_step},
-        "fp64_lod_evals_per_s": fp64_value,
-        "fp32_vs_fp64_speedup": value / fp64_value,
-        "max_abs_dlod_fp32_vs_fp64": None if args.fp32_only else float(np.abs(lod32 - lod64).max()),
-        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no MFMA: tables of 4^k cells)",
-                     "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel_ms": k32, "flops_per_eval": flops, "kernel": "lod_kernel<float,true>"},
+                               f"{case.nlod} positions/interval; 1 chain, L:M 1:1, scored every {PERIOD}th sweep",
+                   "chains": 1, "lprob": LPROB, "period": PERIOD},
+        "sweeps": {"l_sweeps": st32["l_sweeps"], "m_sweeps": st32["m_sweeps"], "scored": st32["scored"],
+                   "near_edge_draws": near32},
+        "fp64_sweeps_per_s": fp64_sweeps,
+        "lod_evals_per_s": lod32,
+        "fp64_lod_evals_per_s": lod64,
+        "max_abs_dlod_fp32_vs_fp64": float(np.abs(lodv32 - lodv64).max()),
+        "kernel_ms": {"msampler_step": ms_avg,
+                      "lsampler_parity": prof32[native.SL_KERNEL_LSAMPLER][0] /
+                      max(prof32[native.SL_KERNEL_LSAMPLER][1], 1),
+                      "lod_scoring": prof32[native.SL_KERNEL_LOD][0] / max(prof32[native.SL_KERNEL_LOD][1], 1)},
+        "roofline": {"bound": "hbm", "kernel": "mstep_kernel (M-sampler meiosis step)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profile_traffic("mstep_kernel", args.config),
+                     "bytes_per_launch": bytes_per_launch, "launch_ms": ms_avg},
+        "lod_roofline": {"bound": "mfma", "compute": "fp32 VALU (no MFMA: tables of 4^k cells)",
+                         "kernel": "lod_kernel<float,true>", "achieved": lod_achieved, "peak": FP32_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": lod_achieved / FP32_PEAK_TFLOPS,
+                         "traffic": profile_traffic("lod_kernel", args.config),
+                         "flops_per_eval": flops, "launch_ms": k32},
     }
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)

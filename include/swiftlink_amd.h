@@ -132,8 +132,11 @@ int sl_lod_results(sl_ctx* ctx, double* acc_log, int64_t* count, double* lod);
 int sl_lod_terms(sl_ctx* ctx, const int32_t* graphs, int32_t n_graphs, double* out);
 
 /* Kernel timing with HIP events on the context's stream (enable != 0 starts
- * recording and clears the totals).  *ms and *launches cover the LOD peel
- * kernel since the last enable. */
+ * recording and clears the totals).  For kernel SL_KERNEL_LOD (peel kernel),
+ * SL_KERNEL_LSAMPLER (one parity launch) or SL_KERNEL_MSAMPLER (meiosis-step
+ * kernel, timed back to back over each M-sweep), *ms is the time summed since
+ * the last enable and *launches the number of launches it covers.  No
+ * counterpart in the reference. */
 int sl_profile(sl_ctx* ctx, int32_t enable);
 int sl_profile_read(sl_ctx* ctx, int32_t kernel, double* ms, int64_t* launches);
 

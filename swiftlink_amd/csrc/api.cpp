@@ -114,7 +114,12 @@ struct sl_ctx {
     // profiling
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+    struct Timed {
+        int kind;
+        hipEvent_t e0, e1;
+        int launches;  // kernel launches between the two events
+    };
+    std::vector<Timed> ev_used;
     double prof_ms[3] = {0, 0, 0};
     int64_t prof_n[3] = {0, 0, 0};
 };
@@ -239,8 +244,8 @@ int release_all(sl_ctx* c) {
         if (p) (void)hipFree(p);
     for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto& u : c->ev_used) {
-        (void)hipEventDestroy(u.second.first);
-        (void)hipEventDestroy(u.second.second);
+        (void)hipEventDestroy(u.e0);
+        (void)hipEventDestroy(u.e1);
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     return SL_OK;
@@ -537,7 +542,7 @@ static int lod_run(sl_ctx* c, const uint32_t* dev_graphs, int G, double* terms_d
     SL_HIP(c, sl::launch_lod(A, c->precision, c->block, c->lds, nchunks, c->stream));
     if (c->prof && e0 && e1) {
         SL_HIP(c, hipEventRecord(e1, c->stream));
-        c->ev_used.push_back({SL_KERNEL_LOD, {e0, e1}});
+        c->ev_used.push_back({SL_KERNEL_LOD, e0, e1, 1});
     }
     if (!terms_dev) SL_HIP(c, sl::launch_lod_merge(c->d_acc, c->d_partial, nchunks, (int)nacc, c->stream));
     return SL_OK;
@@ -611,8 +616,8 @@ int sl_profile(sl_ctx* c, int32_t enable) {
     if (!c) return SL_E_INVALID;
     SL_HIP(c, hipStreamSynchronize(c->stream));
     for (auto& u : c->ev_used) {
-        c->ev_pool.push_back(u.second.first);
-        c->ev_pool.push_back(u.second.second);
+        c->ev_pool.push_back(u.e0);
+        c->ev_pool.push_back(u.e1);
     }
     c->ev_used.clear();
     for (int i = 0; i < 3; ++i) {
@@ -628,11 +633,11 @@ int sl_profile_read(sl_ctx* c, int32_t kernel, double* ms, int64_t* launches) {
     SL_HIP(c, hipStreamSynchronize(c->stream));
     for (auto& u : c->ev_used) {
         float t = 0.0f;
-        SL_HIP(c, hipEventElapsedTime(&t, u.second.first, u.second.second));
-        c->prof_ms[u.first] += t;
-        c->prof_n[u.first] += 1;
-        c->ev_pool.push_back(u.second.first);
-        c->ev_pool.push_back(u.second.second);
+        SL_HIP(c, hipEventElapsedTime(&t, u.e0, u.e1));
+        c->prof_ms[u.kind] += t;
+        c->prof_n[u.kind] += u.launches;
+        c->ev_pool.push_back(u.e0);
+        c->ev_pool.push_back(u.e1);
     }
     c->ev_used.clear();
     if (ms) *ms = c->prof_ms[kernel];
@@ -761,7 +766,7 @@ static int run_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int first_parity
         SL_HIP(c, sl::launch_lsampler(A, c->precision, c->samp_block, c->samp_lds, c->n_chains, c->stream));
         if (c->prof && e0 && e1) {
             SL_HIP(c, hipEventRecord(e1, c->stream));
-            c->ev_used.push_back({SL_KERNEL_LSAMPLER, {e0, e1}});
+            c->ev_used.push_back({SL_KERNEL_LSAMPLER, e0, e1, 1});
         }
     }
     return SL_OK;
@@ -817,13 +822,13 @@ static int run_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* d
     A.n_meioses = nm;
     A.check_kind = check_kind;
     A.lprob = lprob;
+    SL_HIP(c, sl::launch_morder(A, dev_order, c->stream));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->prof) {
         e0 = take_event(c);
         e1 = take_event(c);
         if (e0 && e1) SL_HIP(c, hipEventRecord(e0, c->stream));
     }
-    SL_HIP(c, sl::launch_morder(A, dev_order, c->stream));
     A.mode = 0;
     SL_HIP(c, sl::launch_mstep(A, c->m_regs, c->m_block, c->m_lds, c->stream));
     A.mode = 1;
@@ -833,7 +838,7 @@ static int run_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* d
     }
     if (c->prof && e0 && e1) {
         SL_HIP(c, hipEventRecord(e1, c->stream));
-        c->ev_used.push_back({SL_KERNEL_MSAMPLER, {e0, e1}});
+        c->ev_used.push_back({SL_KERNEL_MSAMPLER, e0, e1, nm + 1});
     }
     return SL_OK;
 }
