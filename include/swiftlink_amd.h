@@ -133,9 +133,10 @@ int sl_lod_terms(sl_ctx* ctx, const int32_t* graphs, int32_t n_graphs, double* o
 
 /* Kernel timing with HIP events on the context's stream (enable != 0 starts
  * recording and clears the totals).  For kernel SL_KERNEL_LOD (peel kernel),
- * SL_KERNEL_LSAMPLER (one parity launch) or SL_KERNEL_MSAMPLER (meiosis-step
- * kernel, timed back to back over each M-sweep), *ms is the time summed since
- * the last enable and *launches the number of launches it covers.  No
+ * SL_KERNEL_LSAMPLER (one parity launch), SL_KERNEL_MSAMPLER (the
+ * founder-allele likelihoods of one meiosis step) or SL_KERNEL_MSAMPLER_FB
+ * (the forward-backward pass of one meiosis step), *ms is the time summed
+ * since the last enable and *launches the number of launches it covers.  No
  * counterpart in the reference. */
 int sl_profile(sl_ctx* ctx, int32_t enable);
 int sl_profile_read(sl_ctx* ctx, int32_t kernel, double* ms, int64_t* launches);
@@ -143,6 +144,7 @@ int sl_profile_read(sl_ctx* ctx, int32_t kernel, double* ms, int64_t* launches);
 #define SL_KERNEL_LOD 0
 #define SL_KERNEL_LSAMPLER 1
 #define SL_KERNEL_MSAMPLER 2
+#define SL_KERNEL_MSAMPLER_FB 3
 
 /* Algorithmic work of one LOD evaluation (one graph, one interval, one
  * position) from the compiled program: flops and LDS bytes. */

@@ -88,7 +88,6 @@ struct sl_ctx {
     std::string samp_why;           // why the sampler program was rejected, if it was
     std::vector<int32_t> mother_h, father_h;
     // M-sampler
-    int m_regs = 0;
     int m_block = sl::kMsampBlock;
     int m_wave_bytes = 0;
     size_t m_lds = 0;
@@ -98,6 +97,8 @@ struct sl_ctx {
     int n_gen = 0;
     int32_t* d_gen_off = nullptr;
     int32_t* d_gen_persons = nullptr;
+    uint2* d_ptab = nullptr;
+    int m_ptab_bytes = 0;
     uint8_t* d_geno_t = nullptr;
     double* d_minor = nullptr;
     int32_t* d_meioses = nullptr;
@@ -109,7 +110,6 @@ struct sl_ctx {
     double* d_lflip = nullptr;
     double* d_fbuf = nullptr;
     uint8_t* d_maps = nullptr;
-    uint32_t* d_counter = nullptr;
     uint32_t* d_near = nullptr;
     // profiling
     bool prof = false;
@@ -120,8 +120,8 @@ struct sl_ctx {
         int launches;  // kernel launches between the two events
     };
     std::vector<Timed> ev_used;
-    double prof_ms[3] = {0, 0, 0};
-    int64_t prof_n[3] = {0, 0, 0};
+    double prof_ms[4] = {0, 0, 0, 0};
+    int64_t prof_n[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -233,8 +233,8 @@ int release_all(sl_ctx* c) {
     void* ptrs[] = {c->d_sops, c->d_sprevs, c->d_schildren, c->d_ssched_off, c->d_ssched_ops, c->d_rlevel_off,
                     c->d_rlevel_ops, c->d_mother, c->d_father, c->d_legal_t, c->d_gprob_d, c->d_gprob_f,
                     c->d_theta, c->d_chains, c->d_status,
-                    c->d_gen_off, c->d_gen_persons, c->d_geno_t, c->d_minor, c->d_meioses, c->d_order,
-                    c->d_fixed_order, c->d_lcur, c->d_lflip, c->d_fbuf, c->d_maps, c->d_counter, c->d_near,
+                    c->d_gen_off, c->d_gen_persons, c->d_ptab, c->d_geno_t, c->d_minor, c->d_meioses, c->d_order,
+                    c->d_fixed_order, c->d_lcur, c->d_lflip, c->d_fbuf, c->d_maps, c->d_near,
                     c->d_ops, c->d_prevs, c->d_children, c->d_tasks, c->d_tinfo, c->d_idx, c->d_level_base,
                     c->d_sched_off,
                     c->d_sched_ops, c->d_trait_d,
@@ -256,9 +256,8 @@ int release_all(sl_ctx* c) {
 // pedigree is outside what the kernel handles.
 int setup_msampler(sl_ctx* c, const sl_pedigree_desc* ped) {
     const int N = c->N, F = c->F, M = c->M;
-    c->m_regs = sl::msampler_regs(F);
-    if (!c->m_regs) {
-        c->m_why = "more than 512 founder alleles";
+    if (!sl::msampler_supported(N, F)) {
+        c->m_why = "more than 1022 founder alleles or 65535 people";
         return SL_OK;
     }
     // generations: founders 0, others 1 + deeper parent (the label order of
@@ -287,6 +286,9 @@ int setup_msampler(sl_ctx* c, const sl_pedigree_desc* ped) {
     }
     gen_off.push_back((int32_t)gen_persons.size());
     c->n_gen = maxd;
+    std::vector<uint2> ptab;
+    for (int i : gen_persons)
+        ptab.push_back(make_uint2((uint32_t)i, (uint32_t)c->mother_h[i] | ((uint32_t)c->father_h[i] << 16)));
     // non-ignorable meioses (markov_chain.cc:65-80): a meiosis from a founder
     // parent with one child is skipped (safe_to_ignore_meiosis, person.cc:208-222)
     std::vector<int> kids(N, 0);
@@ -304,23 +306,48 @@ int setup_msampler(sl_ctx* c, const sl_pedigree_desc* ped) {
     for (int i = 0; i < N; ++i)
         if (ped->typed[i])
             for (int l = 0; l < M; ++l) gt[(size_t)l * N + i] = ped->genotypes[(size_t)i * M + l];
-    c->m_wave_bytes = (4 * c->W + 4 * (N - F) + 15) / 16 * 16;
-    sl::MsampArgs A{};
-    A.wave_bytes = c->m_wave_bytes;
-    c->m_lds = sl::msampler_lds_bytes(A, c->m_block);
-    if (c->m_lds > 64 * 1024) {
-        c->m_why = "M-sampler workgroup needs more than 64 KiB of LDS";
+    if (c->meioses_h.size() > 16384) {
+        c->m_why = "more than 16384 meioses";
         return SL_OK;
     }
+    c->m_wave_bytes = (int)sl::msampler_wave_bytes(N, F, c->W);
+    c->m_ptab_bytes = (int)sl::msampler_ptab_bytes(N, F);
+    sl::MsampArgs A{};
+    A.wave_bytes = c->m_wave_bytes;
+    A.ptab_bytes = c->m_ptab_bytes;
+    c->m_lds = sl::msampler_lds_bytes(A, c->m_block);
+    if (c->m_lds > 160 * 1024) {
+        c->m_why = "M-sampler workgroup needs more than 160 KiB of LDS";
+        return SL_OK;
+    }
+    SL_HIP(c, sl::configure_msampler_lds(c->m_lds));
     int rc;
     if ((rc = upload(c, &c->d_gen_off, gen_off.data(), gen_off.size())) ||
         (rc = upload(c, &c->d_gen_persons, gen_persons.data(), gen_persons.size())) ||
+        (rc = upload(c, &c->d_ptab, ptab.data(), ptab.size())) ||
         (rc = upload(c, &c->d_geno_t, gt.data(), gt.size())) ||
         (rc = upload(c, &c->d_minor, c->minor.data(), c->minor.size())) ||
         (rc = upload(c, &c->d_meioses, c->meioses_h.data(), c->meioses_h.size())))
         return rc;
     SL_HIP(c, hipStreamSynchronize(c->stream));  // the host tables above go out of scope
     c->m_ready = true;
+    return SL_OK;
+}
+
+// Runs launch(); with profiling on, brackets it with events on the stream.
+template <typename Fn>
+int timed_launch(sl_ctx* c, int kind, Fn launch) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->prof) {
+        e0 = take_event(c);
+        e1 = take_event(c);
+        if (e0 && e1) SL_HIP(c, hipEventRecord(e0, c->stream));
+    }
+    SL_HIP(c, launch());
+    if (c->prof && e0 && e1) {
+        SL_HIP(c, hipEventRecord(e1, c->stream));
+        c->ev_used.push_back({kind, e0, e1, 1});
+    }
     return SL_OK;
 }
 
@@ -620,7 +647,7 @@ int sl_profile(sl_ctx* c, int32_t enable) {
         c->ev_pool.push_back(u.e1);
     }
     c->ev_used.clear();
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 4; ++i) {
         c->prof_ms[i] = 0.0;
         c->prof_n[i] = 0;
     }
@@ -629,7 +656,7 @@ int sl_profile(sl_ctx* c, int32_t enable) {
 }
 
 int sl_profile_read(sl_ctx* c, int32_t kernel, double* ms, int64_t* launches) {
-    if (!c || kernel < 0 || kernel > 2) return SL_E_INVALID;
+    if (!c || kernel < 0 || kernel > 3) return SL_E_INVALID;
     SL_HIP(c, hipStreamSynchronize(c->stream));
     for (auto& u : c->ev_used) {
         float t = 0.0f;
@@ -666,7 +693,7 @@ int sl_mcmc_init(sl_ctx* c, int32_t n_chains, const int32_t* graphs) {
     const size_t cm = (size_t)n_chains * c->M;
     if (n_chains != c->n_chains) {
         void** bufs[] = {(void**)&c->d_chains, (void**)&c->d_status, (void**)&c->d_lcur, (void**)&c->d_lflip,
-                         (void**)&c->d_fbuf, (void**)&c->d_maps, (void**)&c->d_counter, (void**)&c->d_near};
+                         (void**)&c->d_fbuf, (void**)&c->d_maps, (void**)&c->d_near};
         for (void** b : bufs) {
             if (*b) SL_HIP(c, hipFree(*b));
             *b = nullptr;
@@ -678,7 +705,6 @@ int sl_mcmc_init(sl_ctx* c, int32_t n_chains, const int32_t* graphs) {
         SL_HIP(c, hipMalloc((void**)&c->d_lflip, cm * sizeof(double)));
         SL_HIP(c, hipMalloc((void**)&c->d_fbuf, 2 * cm * sizeof(double)));
         SL_HIP(c, hipMalloc((void**)&c->d_maps, cm));
-        SL_HIP(c, hipMalloc((void**)&c->d_counter, (size_t)n_chains * sizeof(uint32_t)));
         SL_HIP(c, hipMalloc((void**)&c->d_near, (size_t)n_chains * sizeof(uint32_t)));
         c->n_chains = n_chains;
     }
@@ -686,7 +712,6 @@ int sl_mcmc_init(sl_ctx* c, int32_t n_chains, const int32_t* graphs) {
     if (rc) return rc;
     SL_HIP(c, hipMemcpyAsync(c->d_chains, c->d_packed, words * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
     SL_HIP(c, hipMemsetAsync(c->d_status, 0, (size_t)n_chains * sizeof(int32_t), c->stream));
-    SL_HIP(c, hipMemsetAsync(c->d_counter, 0, (size_t)n_chains * sizeof(uint32_t), c->stream));
     SL_HIP(c, hipMemsetAsync(c->d_near, 0, (size_t)n_chains * sizeof(uint32_t), c->stream));
     SL_HIP(c, hipStreamSynchronize(c->stream));
     return SL_OK;
@@ -802,10 +827,11 @@ static sl::MsampArgs msamp_args(sl_ctx* c) {
     A.lflip = c->d_lflip;
     A.fbuf = c->d_fbuf;
     A.maps = c->d_maps;
-    A.counter = c->d_counter;
     A.status = c->d_status;
     A.near_edge = c->d_near;
     A.wave_bytes = c->m_wave_bytes;
+    A.ptab = c->d_ptab;
+    A.ptab_bytes = c->m_ptab_bytes;
     A.n_chains = c->n_chains;
     return A;
 }
@@ -823,22 +849,14 @@ static int run_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* d
     A.check_kind = check_kind;
     A.lprob = lprob;
     SL_HIP(c, sl::launch_morder(A, dev_order, c->stream));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->prof) {
-        e0 = take_event(c);
-        e1 = take_event(c);
-        if (e0 && e1) SL_HIP(c, hipEventRecord(e0, c->stream));
-    }
     A.mode = 0;
-    SL_HIP(c, sl::launch_mstep(A, c->m_regs, c->m_block, c->m_lds, c->stream));
+    SL_HIP(c, sl::launch_mstep(A, c->m_block, c->m_lds, c->stream));
     A.mode = 1;
     for (int j = 0; j < nm; ++j) {
         A.step = j;
-        SL_HIP(c, sl::launch_mstep(A, c->m_regs, c->m_block, c->m_lds, c->stream));
-    }
-    if (c->prof && e0 && e1) {
-        SL_HIP(c, hipEventRecord(e1, c->stream));
-        c->ev_used.push_back({SL_KERNEL_MSAMPLER, e0, e1, nm + 1});
+        rc = timed_launch(c, SL_KERNEL_MSAMPLER, [&] { return sl::launch_mstep(A, c->m_block, c->m_lds, c->stream); });
+        if (!rc) rc = timed_launch(c, SL_KERNEL_MSAMPLER_FB, [&] { return sl::launch_mfb(A, c->stream); });
+        if (rc) return rc;
     }
     return SL_OK;
 }
@@ -927,12 +945,13 @@ int sl_mcmc_likelihood(sl_ctx* c, double* per_locus) {
     // chains already in error are evaluated too
     SL_HIP(c, hipMemcpyAsync(st.data(), c->d_status, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     SL_HIP(c, hipMemsetAsync(c->d_status, 0, st.size() * sizeof(int32_t), c->stream));
-    SL_HIP(c, sl::launch_mstep(A, c->m_regs, c->m_block, c->m_lds, c->stream));
+    SL_HIP(c, sl::launch_mstep(A, c->m_block, c->m_lds, c->stream));
     const size_t n = (size_t)c->n_chains * c->M;
     SL_HIP(c, hipMemcpyAsync(per_locus, c->d_lcur, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     std::vector<int32_t> now(st.size());
     SL_HIP(c, hipMemcpyAsync(now.data(), c->d_status, now.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     SL_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; ++i) per_locus[i] = std::exp(per_locus[i]);  // the kernel keeps logs
     for (size_t i = 0; i < st.size(); ++i)
         if (st[i] != 0) now[i] = st[i];
     SL_HIP(c, hipMemcpyAsync(c->d_status, now.data(), now.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));

@@ -5,25 +5,32 @@
 // step j (meiosis m) is one launch of mstep_kernel over (locus, chain):
 //  * each wavefront evaluates the founder-allele-graph likelihood of its
 //    locus's column with bit m flipped (FounderAlleleGraph4::likelihood,
-//    founder_allele_graph4.cc:34-442).  Founder-allele labels of non-founders
-//    are propagated generation by generation in LDS; the graph itself -
-//    component of every founder allele, its two alternative allele
-//    assignments, each component's fixed state and its two probability
-//    products - lives in the wave's VGPRs (lane x%64 of register x/64 holds
-//    node or component x), so every edge is a handful of lane reads and
-//    writes under wave-uniform control flow.  Edges are taken in person order
-//    and products are formed in the reference's order, so the FP64 likelihood
-//    is the reference's bit for bit;
-//  * the last workgroup of the chain to finish (counter, no spinning) runs the
-//    forward-backward pass (:105-203) with its threads: a block scan of the
+//    founder_allele_graph4.cc:34-442).  The reference adds typed persons one
+//    by one to components that carry two complementary allele assignments;
+//    its result is the sum, over the allele assignments of the founder
+//    alleles that satisfy every genotype, of the product of allele
+//    frequencies, or 0 when none does.  Here the wave's lanes build the same
+//    components in parallel: founder-allele labels are propagated generation
+//    by generation, every typed person becomes an edge between its two
+//    founder alleles (heterozygous: the alleles differ, homozygous: both
+//    equal the genotype's allele), and a union-find with a parity bit per
+//    node (is the node's allele the opposite of its parent's?) is built by
+//    rounds of hooking roots onto smaller roots and path compression until
+//    no edge joins two components.  Then every edge is checked against the
+//    parities, homozygous edges fix their root's allele, and with n0/n1 the
+//    numbers of a root's nodes of parity 0/1 its component contributes
+//    maj^n0 min^n1 (root allele A), maj^n1 min^n0 (root allele B) or their
+//    sum, taken in log space.  Legality is exactly the reference's; the
+//    likelihood agrees with its products to rounding;
+//  * a second kernel, one workgroup per chain, runs the forward-backward pass
+//    (:105-203) with its threads: a block scan of the
 //    2x2 matrices T(theta_{l-1}) diag(e_l) gives the forward probabilities;
 //    a locus draws a uniform only when both its states are possible, so its
 //    draw index is the number of drawing loci above it (reverse scan); each
 //    locus becomes a 2-bit map from the next locus's indicator to its own,
 //    and a reverse scan of map compositions gives every new indicator.  The
-//    scans reassociate the reference's serial products, so thresholds differ
-//    from the reference's only by rounding: draws within 1e-12 of their
-//    threshold are counted per chain;
+//    likelihood rounding and the reassociated products move thresholds only
+//    by rounding: draws within 1e-12 of their threshold are counted per chain;
 //  * changed indicators are written back and the current-column likelihood
 //    carried over (raw_matrix, :116-134).
 // mode 0 of the same kernel computes the current-column likelihoods at the
@@ -31,7 +38,6 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
-#include <type_traits>
 
 #include "msampler_kernels.h"
 #include "philox.h"
@@ -51,391 +57,159 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ uint32_t rdlane(uint32_t v, int lane) {
-    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+// A wave's LDS: column words, union-find words, per-node accumulators, edges
+// and founder-allele labels.
+struct WaveArea {
+    uint32_t* colw;   // [W]
+    uint32_t* uf;     // [2F] parent (bits 0-15) | parity to parent (bit 16)
+    uint32_t* acc;    // [2F] root allele A needed (bit 0), B needed (bit 1), touched (bit 2),
+                      //      n0 (bits 3-13), n1 (bits 14-24)
+    uint32_t* edges;  // [N] x (bits 0-9) | y (bits 10-19) | heterozygous (bit 20) | allele B (bit 21)
+    uint16_t* lab;    // [2(N-F)] founder allele of (person, parent)
+};
+
+__device__ __forceinline__ WaveArea wave_area(const MsampArgs& A, unsigned char* base) {
+    WaveArea w;
+    w.colw = reinterpret_cast<uint32_t*>(base);
+    w.uf = w.colw + A.W;
+    w.acc = w.uf + 2 * A.F;
+    w.edges = w.acc + 2 * A.F;
+    w.lab = reinterpret_cast<uint16_t*>(w.edges + A.N);
+    return w;
 }
 
-__device__ __forceinline__ double rdlane_d(double v, int lane) {
-    const uint64_t b = (uint64_t)__double_as_longlong(v);
-    const uint64_t lo = rdlane((uint32_t)b, lane);
-    const uint64_t hi = rdlane((uint32_t)(b >> 32), lane);
-    return __longlong_as_double((long long)((hi << 32) | lo));
+// root of x and the parity of x relative to it; parents only ever point at
+// smaller nodes, so the walk ends
+__device__ __forceinline__ uint32_t uf_find(const uint32_t* uf, uint32_t x, uint32_t* par) {
+    uint32_t p = 0, w = uf[x];
+    while ((w & 0xFFFFu) != x) {
+        p ^= (w >> 16) & 1u;
+        x = w & 0xFFFFu;
+        w = uf[x];
+    }
+    *par = p;
+    return x;
 }
 
-// Compile-time loop: fn(std::integral_constant<int, I>) for I in [0, N).
-template <int I, int N, typename Fn>
-__device__ __forceinline__ void static_for(Fn&& fn) {
-    if constexpr (I < N) {
-        fn(std::integral_constant<int, I>{});
-        static_for<I + 1, N>(fn);
-    }
-}
-
-// One field per node/component spread over R registers (element x in lane
-// x%64 of register x/64).  The registers are named members reached only
-// through compile-time indices, so they never become an indexed stack array;
-// idx arguments are wave-uniform.
-template <typename T, int R>
-struct RegArr {
-    T v0{}, v1{}, v2{}, v3{}, v4{}, v5{}, v6{}, v7{};
-    static_assert(R >= 1 && R <= 8, "1..8 registers");
-    template <int r>
-    __device__ __forceinline__ T& at() {
-        if constexpr (r == 0) return v0;
-        else if constexpr (r == 1) return v1;
-        else if constexpr (r == 2) return v2;
-        else if constexpr (r == 3) return v3;
-        else if constexpr (r == 4) return v4;
-        else if constexpr (r == 5) return v5;
-        else if constexpr (r == 6) return v6;
-        else return v7;
-    }
-    template <int r>
-    __device__ __forceinline__ const T& at() const {
-        return const_cast<RegArr*>(this)->template at<r>();
-    }
-};
-
-template <int R>
-struct Regs : RegArr<uint32_t, R> {
-    // register selected per lane with the uniform index, then one lane read
-    __device__ __forceinline__ uint32_t get(int idx) const {
-        uint32_t sel = this->template at<0>();
-        static_for<1, R>([&](auto rc) {
-            constexpr int r = decltype(rc)::value;
-            sel = (idx >> 6) == r ? this->template at<r>() : sel;
-        });
-        return rdlane(sel, idx & 63);
-    }
-    __device__ __forceinline__ void set(int idx, uint32_t x, bool en = true) {
-        const bool me = en && lane_id() == (idx & 63);
-        static_for<0, R>([&](auto rc) {
-            constexpr int r = decltype(rc)::value;
-            uint32_t& v = this->template at<r>();
-            v = ((idx >> 6) == r && me) ? x : v;
-        });
-    }
-};
-
-// The two probability products of each component, kept as one pair per
-// element and always updated together.
-struct PPair {
-    double p0, p1;
-};
-
-template <int R>
-struct RegsP : RegArr<PPair, R> {
-    __device__ __forceinline__ PPair get(int idx) const {
-        PPair sel = this->template at<0>();
-        static_for<1, R>([&](auto rc) {
-            constexpr int r = decltype(rc)::value;
-            const PPair& o = this->template at<r>();
-            sel.p0 = (idx >> 6) == r ? o.p0 : sel.p0;
-            sel.p1 = (idx >> 6) == r ? o.p1 : sel.p1;
-        });
-        return PPair{rdlane_d(sel.p0, idx & 63), rdlane_d(sel.p1, idx & 63)};
-    }
-    // element idx becomes set ? (x0, x1) : (p0 * x0, p1 * x1) (en and set wave-uniform)
-    __device__ __forceinline__ void update(int idx, double x0, double x1, bool set, bool en) {
-        const bool me = en && lane_id() == (idx & 63);
-        static_for<0, R>([&](auto rc) {
-            constexpr int r = decltype(rc)::value;
-            PPair& v = this->template at<r>();
-            const bool hit = (idx >> 6) == r && me;
-            const double n0 = set ? x0 : v.p0 * x0;
-            const double n1 = set ? x1 : v.p1 * x1;
-            v.p0 = hit ? n0 : v.p0;
-            v.p1 = hit ? n1 : v.p1;
-        });
-    }
-};
-
-// Founder-allele graph of one column.
-//
-// The reference keeps two alternative allele assignments per component
-// (founder_allele_graph4.cc:34-442).  Within a component that is not fixed
-// they are always complementary (a new heterozygous pair starts them so, and
-// every extension and merge preserves it), and of a fixed component only the
-// chosen one is ever read.  So a founder allele here carries one bit b, its
-// allele under assignment 0 (0 = A, 1 = B); under assignment k its allele is
-// b ^ k, and swapping a component's assignments on merge (combine_components'
-// flip) is b ^= 1 over its members.  node word: member+1 (bits 0-15, 0 =
-// none), b (bit 16); component word: fixed+1 (bits 0-1, 0 = not fixed),
-// active (bit 2).
-//
-// Each edge decides with wave-uniform scalar logic what the reference's case
-// analysis does, as a fixed set of actions: up to two node words, an optional
-// merge, up to two component words and one probability update
-// P_k[c] = set ? v_k : P_k[c] * v_k, applied with masked lane selects.  A
-// probability the reference sets to zero is multiplied by 0.0 and an
-// untouched one by 1.0 (probabilities are finite and non-negative, so both
-// are exact), and a merge multiplies in the merged component's zeroed
-// probabilities, so every product is the reference's bit for bit.
-template <int R>
-struct Fag {
-    Regs<R> node, comp;
-    RegsP<R> pp;
-    int gi;
-
-    __device__ __forceinline__ void init() {
-        node = Regs<R>{};
-        comp = Regs<R>{};
-        pp = RegsP<R>{};
-        gi = 0;
-    }
-    __device__ __forceinline__ static uint32_t node_word(int c, int bit) {
-        return (uint32_t)(c + 1) | ((uint32_t)bit << 16);
-    }
-    __device__ __forceinline__ static uint32_t comp_word(int fixed) { return (uint32_t)(fixed + 1) | 4u; }
-
-    // one typed person with genotype g and founder alleles (x, y)
-    // (FounderAlleleGraph4::likelihood, :34-424); false when illegal.  The
-    // allele frequencies are arguments, not members: selecting between two
-    // members would make the compiler address the struct in memory.
-    __device__ __forceinline__ bool edge(int g, int x, int y, double major, double minor) {
-        const auto fv = [major, minor](int bit) { return bit ? minor : major; };
-        const bool het = g == G_HETERO;
-        const int v = g == G_HOMOZ_B ? 1 : 0;  // homozygous allele
-        const uint32_t wx = node.get(x);
-        const uint32_t wy = x == y ? wx : node.get(y);
-        const int c1 = (int)(wx & 0xFFFFu) - 1, c2 = (int)(wy & 0xFFFFu) - 1;
-        const int bx = (int)(wx >> 16) & 1, by = (int)(wy >> 16) & 1;
-        const int f1 = (int)(comp.get(c1 < 0 ? 0 : c1) & 3u) - 1;
-        const int f2 = (int)(comp.get(c2 < 0 ? 0 : c2) & 3u) - 1;
-
-        bool legal = true;
-        uint32_t nwx = 0, nwy = 0;
-        bool wr_x = false, wr_y = false;
-        int ca = 0, cb = 0;
-        uint32_t cwa = 0, cwb = 0;
-        bool wr_a = false, wr_b = false;
-        bool merge = false;
-        int mflip = 0;
-        int cp = 0;
-        bool pu = false, pset = false;
-        double v0 = 1.0, v1 = 1.0;
-        bool newc = false;
-
-        if (x == y) {  // autozygous (:77-124)
-            if (het) {
-                legal = false;
-            } else if (c1 >= 0) {
-                if (f1 >= 0) {
-                    legal = (bx ^ f1) == v;
-                } else {  // becomes fixed to the assignment giving allele v
-                    const int f = bx ^ v;
-                    ca = c1;
-                    cwa = comp_word(f);
-                    wr_a = true;
-                    cp = c1;
-                    pu = true;
-                    v0 = f == 0 ? 1.0 : 0.0;
-                    v1 = f == 0 ? 0.0 : 1.0;
-                }
-            } else {
-                nwx = node_word(gi, v);
-                wr_x = true;
-                ca = gi;
-                cwa = comp_word(0);
-                wr_a = true;
-                cp = gi;
-                pu = pset = true;
-                v0 = fv(v);
-                v1 = 0.0;
-                newc = true;
-            }
-        } else if (c1 >= 0 && c1 == c2) {  // same component (:135-166)
-            if (f1 >= 0) {
-                legal = het ? bx != by : ((bx ^ f1) == v && (by ^ f1) == v);
-            } else if (het) {
-                legal = bx != by;
-            } else {
-                legal = bx == by;
-                const int f = bx ^ v;
-                ca = c1;
-                cwa = comp_word(f);
-                wr_a = true;
-                cp = c1;
-                pu = true;
-                v0 = f == 0 ? 1.0 : 0.0;
-                v1 = f == 0 ? 0.0 : 1.0;
-            }
-        } else if (c1 >= 0 && c2 >= 0) {  // two components (:167-278), then combine (:501-541)
-            int n1, n2, flip;
-            if (het) {
-                const int r = 1 ^ bx ^ by;  // required assignment1 ^ assignment2
-                flip = r;
-                if (f1 >= 0 && f2 >= 0) {
-                    legal = (f1 ^ f2) == r;
-                    n1 = f1;
-                    n2 = f2;
-                } else if (f1 >= 0) {
-                    n1 = f1;
-                    n2 = f1 ^ r;
-                } else if (f2 >= 0) {
-                    n1 = f2 ^ r;
-                    n2 = f2;
-                } else {
-                    n1 = n2 = -1;
-                }
-            } else {
-                n1 = bx ^ v;
-                n2 = by ^ v;
-                legal = (f1 < 0 || f1 == n1) && (f2 < 0 || f2 == n2);
-                flip = n1 ^ n2;
-            }
-            const PPair q = pp.get(c2);
-            const double q0 = q.p0, q1 = q.p1;
-            const double z1_0 = n1 == 1 ? 0.0 : 1.0, z1_1 = n1 == 0 ? 0.0 : 1.0;
-            const double d0 = n2 == 1 ? 0.0 : q0, d1 = n2 == 0 ? 0.0 : q1;
-            merge = true;
-            mflip = flip;
-            ca = c1;
-            cwa = comp_word(n1);
-            wr_a = true;
-            cb = c2;
-            cwb = comp.get(c2) & ~4u;
-            wr_b = true;
-            cp = c1;
-            pu = true;
-            v0 = z1_0 * (flip ? d1 : d0);
-            v1 = z1_1 * (flip ? d0 : d1);
-        } else if (c1 >= 0 || c2 >= 0) {  // one allele placed (:280-379): the other joins
-            const bool xin = c1 >= 0;
-            const int c = xin ? c1 : c2, f = xin ? f1 : f2, bin = xin ? bx : by;
-            int nb, nf = f;
-            if (het) {
-                nb = bin ^ 1;
-                if (f >= 0) {
-                    v0 = f == 0 ? fv(bin ^ 1) : 1.0;
-                    v1 = f == 1 ? fv(bin) : 1.0;
-                } else {
-                    v0 = fv(bin ^ 1);
-                    v1 = fv(bin);
-                }
-            } else {
-                const int k = bin ^ v;  // the only assignment giving the placed allele v
-                legal = f < 0 || f == k;
-                nb = v ^ k;
-                nf = k;
-                v0 = k == 0 ? fv(v) : 0.0;
-                v1 = k == 1 ? fv(v) : 0.0;
-                if (f >= 0) {
-                    v0 = k == 0 ? fv(v) : 1.0;
-                    v1 = k == 1 ? fv(v) : 1.0;
-                }
-            }
-            const uint32_t w = node_word(c, nb);
-            nwx = w;
-            nwy = w;
-            wr_x = !xin;
-            wr_y = xin;
-            ca = c;
-            cwa = comp_word(nf);
-            wr_a = true;
-            cp = c;
-            pu = true;
-        } else {  // new component of two alleles (:380-402)
-            nwx = node_word(gi, het ? 0 : v);
-            nwy = node_word(gi, het ? 1 : v);
-            wr_x = wr_y = true;
-            ca = gi;
-            cwa = comp_word(het ? -1 : 0);
-            wr_a = true;
-            cp = gi;
-            pu = pset = true;
-            v0 = het ? major * minor : fv(v) * fv(v);
-            v1 = het ? major * minor : 0.0;
-            newc = true;
-        }
-        if (!legal) return false;
-
-        node.set(x, nwx, wr_x);
-        node.set(y, nwy, wr_y);
-        if (merge) {  // members of c2 join c1, their bit flipped with the assignments
-            const uint32_t m2 = (uint32_t)(c2 + 1);
-            const uint32_t nw = (uint32_t)(c1 + 1);
-            const uint32_t fl = (uint32_t)mflip << 16;
-            static_for<0, R>([&](auto rc) {
-                constexpr int r = decltype(rc)::value;
-                const uint32_t w = node.template at<r>();
-                node.template at<r>() = (w & 0xFFFFu) == m2 ? (nw | ((w & 0x10000u) ^ fl)) : w;
-            });
-        }
-        comp.set(ca, cwa, wr_a);
-        comp.set(cb, cwb, wr_b);
-        pp.update(cp, v0, v1, pset, pu);
-        gi += newc ? 1 : 0;
-        return true;
-    }
-
-    // product over components in creation order (:426-441)
-    // (an inactive component contributes an exact factor 1.0)
-    __device__ __forceinline__ double likelihood() const {
-        RegArr<double, R> val;
-        static_for<0, R>([&](auto rc) {
-            constexpr int r = decltype(rc)::value;
-            const uint32_t w = comp.template at<r>();
-            const uint32_t f = w & 3u;
-            const PPair& q = pp.template at<r>();
-            val.template at<r>() = !(w & 4u) ? 1.0 : f == 1u ? q.p0 : f == 2u ? q.p1 : q.p0 + q.p1;
-        });
-        double ret = 1.0;
-        for (int c = 0; c < gi; ++c) {
-            double sel = val.template at<0>();
-            static_for<1, R>([&](auto rc) {
-                constexpr int r = decltype(rc)::value;
-                sel = (c >> 6) == r ? val.template at<r>() : sel;
-            });
-            ret *= rdlane_d(sel, c & 63);
-        }
-        return ret;
-    }
-};
-
-// likelihood of the column in colw (LDS, this wave's copy) at locus l
-template <int R>
-__device__ __forceinline__ double fag_column(const MsampArgs& A, const uint32_t* colw, uint16_t* lab, int l) {
+// log of the founder-allele-graph likelihood of the column in w.colw at locus
+// l (-inf when illegal), computed by one wavefront
+__device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, const WaveArea& w, int l) {
     const int lane = lane_id();
-    // founder-allele labels of non-founders (reset, founder_allele_graph4.cc:543-569)
+    const int F = A.F, nodes = 2 * A.F;
+    // founder-allele labels of non-founders, parents first (reset, :543-569)
     for (int gen = 0; gen < A.n_gen; ++gen) {
         const int k1 = A.gen_off[gen + 1];
         for (int k = A.gen_off[gen] + lane; k < k1; k += 64) {
-            const int i = A.gen_persons[k];
-            const int b = 2 * (i - A.F);
-            const uint32_t bm = (colw[b >> 5] >> (b & 31)) & 1u;
-            const uint32_t bp = (colw[(b + 1) >> 5] >> ((b + 1) & 31)) & 1u;
-            const int mo = A.mother[i], fa = A.father[i];
-            lab[b] = mo < A.F ? (uint16_t)(2 * mo + (int)bm) : lab[2 * (mo - A.F) + (int)bm];
-            lab[b + 1] = fa < A.F ? (uint16_t)(2 * fa + (int)bp) : lab[2 * (fa - A.F) + (int)bp];
+            const uint2 e = ptab[k];
+            const int i = (int)e.x, mo = (int)(e.y & 0xFFFFu), fa = (int)(e.y >> 16);
+            const int b = 2 * (i - F);
+            const uint32_t bm = (w.colw[b >> 5] >> (b & 31)) & 1u;
+            const uint32_t bp = (w.colw[(b + 1) >> 5] >> ((b + 1) & 31)) & 1u;
+            w.lab[b] = mo < F ? (uint16_t)(2 * mo + (int)bm) : w.lab[2 * (mo - F) + (int)bm];
+            w.lab[b + 1] = fa < F ? (uint16_t)(2 * fa + (int)bp) : w.lab[2 * (fa - F) + (int)bp];
         }
         wave_sync();
     }
-    Fag<R> S;
-    S.init();
-    const double minor = A.minor[l], major = 1.0 - minor;
+    for (int x = lane; x < nodes; x += 64) {
+        w.uf[x] = (uint32_t)x;
+        w.acc[x] = 0u;
+    }
+    // edges: the typed persons with a genotype at this locus
     const uint8_t* gcol = A.geno_t + (size_t)l * A.N;
+    const uint64_t below = (1ull << lane) - 1ull;
+    int E = 0;
+    bool bad = false;
     for (int i0 = 0; i0 < A.N; i0 += 64) {
         const int i = i0 + lane;
-        int g = 0;
-        uint32_t lw = 0;
-        if (i < A.N) {
-            g = gcol[i];
-            if (g) {
-                lw = i < A.F ? ((uint32_t)(2 * i) | ((uint32_t)(2 * i + 1) << 16))
-                             : ((uint32_t)lab[2 * (i - A.F)] | ((uint32_t)lab[2 * (i - A.F) + 1] << 16));
-            }
+        const int g = i < A.N ? gcol[i] : 0;
+        uint32_t e = 0;
+        if (g) {
+            const uint32_t x = i < F ? (uint32_t)(2 * i) : w.lab[2 * (i - F)];
+            const uint32_t y = i < F ? (uint32_t)(2 * i + 1) : w.lab[2 * (i - F) + 1];
+            bad |= g == G_HETERO && x == y;  // an autozygous heterozygote (:77-81)
+            e = x | (y << 10) | ((g == G_HETERO ? 1u : 0u) << 20) | ((g == G_HOMOZ_B ? 1u : 0u) << 21);
         }
-        uint64_t mask = __ballot(g != 0);
-        while (mask) {
-            const int k = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            const int ge = (int)rdlane((uint32_t)g, k);
-            const uint32_t w = rdlane(lw, k);
-            if (!S.edge(ge, (int)(w & 0xFFFFu), (int)(w >> 16), major, minor)) return 0.0;
+        const uint64_t mask = __ballot(g != 0);
+        if (g) w.edges[E + __popcll(mask & below)] = e;
+        E += __popcll(mask);
+    }
+    if (__ballot(bad)) return -INFINITY;
+    wave_sync();
+    // components: hook the larger root under the smaller one with the parity
+    // the edge implies, compress paths, until no edge joins two components
+    // (every productive round removes a component, so at most 2F rounds)
+    for (int round = 0; round <= nodes; ++round) {
+        bool changed = false;
+        for (int e0 = 0; e0 < E; e0 += 64) {
+            const int e = e0 + lane;
+            if (e < E) {
+                const uint32_t ed = w.edges[e];
+                uint32_t px, py;
+                const uint32_t rx = uf_find(w.uf, ed & 0x3FFu, &px);
+                const uint32_t ry = uf_find(w.uf, (ed >> 10) & 0x3FFu, &py);
+                if (rx != ry) {
+                    w.uf[rx > ry ? rx : ry] = (rx < ry ? rx : ry) | ((px ^ py ^ ((ed >> 20) & 1u)) << 16);
+                    changed = true;
+                }
+            }
+            wave_sync();
+        }
+        for (int x0 = 0; x0 < nodes; x0 += 64) {
+            const int x = x0 + lane;
+            uint32_t r = 0, p = 0;
+            if (x < nodes) r = uf_find(w.uf, (uint32_t)x, &p);
+            wave_sync();
+            if (x < nodes) w.uf[x] = r | (p << 16);
+            wave_sync();
+        }
+        if (!__ballot(changed)) break;
+    }
+    // consistency: every edge's parity; homozygous edges fix their root's
+    // allele; mark the founder alleles that take part
+    bool illegal = false;
+    for (int e0 = 0; e0 < E; e0 += 64) {
+        const int e = e0 + lane;
+        if (e < E) {
+            const uint32_t ed = w.edges[e];
+            const uint32_t x = ed & 0x3FFu, y = (ed >> 10) & 0x3FFu;
+            const uint32_t het = (ed >> 20) & 1u, allele = (ed >> 21) & 1u;
+            uint32_t px, py;
+            const uint32_t rx = uf_find(w.uf, x, &px), ry = uf_find(w.uf, y, &py);
+            illegal |= rx != ry || (px ^ py) != het;
+            if (!het) atomicOr(&w.acc[rx], 1u << (allele ^ px));
+            atomicOr(&w.acc[x], 4u);
+            atomicOr(&w.acc[y], 4u);
         }
     }
-    return S.likelihood();
+    wave_sync();
+    if (__ballot(illegal)) return -INFINITY;
+    for (int x = lane; x < nodes; x += 64) {
+        if (w.acc[x] & 4u) {
+            uint32_t p;
+            const uint32_t r = uf_find(w.uf, (uint32_t)x, &p);
+            atomicAdd(&w.acc[r], p ? (1u << 14) : (1u << 3));
+        }
+    }
+    wave_sync();
+    const double minor = A.minor[l];
+    const double lmaj = log(1.0 - minor), lmin = log(minor);
+    double ll = 0.0;
+    bool conflict = false;
+    for (int x = lane; x < nodes; x += 64) {
+        const uint32_t a = w.acc[x];
+        if (!(a & 4u) || (w.uf[x] & 0xFFFFu) != (uint32_t)x) continue;  // untouched, or not a root
+        const double n0 = (double)((a >> 3) & 0x7FFu), n1 = (double)((a >> 14) & 0x7FFu);
+        const double l0 = n0 * lmaj + n1 * lmin;  // root allele A
+        const double l1 = n1 * lmaj + n0 * lmin;  // root allele B
+        const uint32_t need = a & 3u;
+        if (need == 3u) conflict = true;
+        else if (need == 1u) ll += l0;
+        else if (need == 2u) ll += l1;
+        else ll += fmax(l0, l1) + log1p(exp(-fabs(l0 - l1)));
+    }
+    if (__ballot(conflict)) return -INFINITY;
+    for (int off = 32; off > 0; off >>= 1) ll += __shfl_xor(ll, off);
+    return ll;
 }
 
 struct Mat {
@@ -458,16 +232,14 @@ __device__ __forceinline__ Mat mmul(const Mat& x, const Mat& y) {
 }
 
 // T(theta_{l-1}) diag(e_l), A_0 = diag(e_0); e_l = the two column likelihoods
-// of locus l scaled by a power of two
+// of locus l divided by the larger one
 __device__ __forceinline__ Mat locus_matrix(const MsampArgs& A, const double* lc, const double* lf,
                                             const uint32_t* g, int word, uint32_t bm, int l) {
     const bool cur = (g[(size_t)l * A.W + word] & bm) != 0;
-    double e0 = cur ? lf[l] : lc[l];
-    double e1 = cur ? lc[l] : lf[l];
-    int e;
-    (void)frexp(fmax(e0, e1), &e);
-    e0 = ldexp(e0, -e);
-    e1 = ldexp(e1, -e);
+    const double a0 = cur ? lf[l] : lc[l];  // log-likelihoods; the current column's is finite
+    const double a1 = cur ? lc[l] : lf[l];
+    const double mx = fmax(a0, a1);
+    const double e0 = exp(a0 - mx), e1 = exp(a1 - mx);  // exp(-inf) = 0 for an illegal flip
     if (l == 0) return Mat{e0, 0.0, 0.0, e1};
     const double th = A.theta[l - 1], an = 1.0 - A.theta[l - 1];
     return Mat{an * e0, th * e1, th * e0, an * e1};
@@ -478,9 +250,64 @@ __device__ __forceinline__ uint32_t compose(uint32_t f, uint32_t h) {
     return ((f >> (h & 1u)) & 1u) | (((f >> ((h >> 1) & 1u)) & 1u) << 1);
 }
 
-// forward-backward pass of one meiosis step (meiosis_sampler.cc:136-203)
-__device__ void forward_backward(const MsampArgs& A, int chain, int m, unsigned char* smem) {
-    const int B = blockDim.x, tid = threadIdx.x, M = A.M;
+__device__ __forceinline__ Mat shfl_up(const Mat& x, int d) {
+    return Mat{__shfl_up(x.a, d), __shfl_up(x.b, d), __shfl_up(x.c, d), __shfl_up(x.d, d)};
+}
+
+__device__ __forceinline__ bool is_msweep(const MsampArgs& A, int chain) {
+    return !A.check_kind || !(philox_uniform(A.seed, (uint32_t)chain, A.sweep, kStepSched, 0) < A.lprob);
+}
+
+// Founder-allele-graph likelihoods of one meiosis step (mode 1: the column
+// with meiosis m flipped, into lflip) or of the current columns (mode 0, into
+// lcur; a zero marks the chain's graph illegal).  Grid (loci / waves, chains).
+__global__ void __launch_bounds__(kMsampBlock) mstep_kernel(MsampArgs A) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_skip;
+    const int chain = blockIdx.y;
+    if (threadIdx.x == 0) s_skip = A.status[chain] != 0 || !is_msweep(A, chain);
+    __syncthreads();
+    if (s_skip) return;
+    const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = lane_id();
+    // person table of the label pass, staged once per workgroup
+    uint2* ptab = reinterpret_cast<uint2*>(smem);
+    for (int k = threadIdx.x; k < A.N - A.F; k += blockDim.x) ptab[k] = A.ptab[k];
+    __syncthreads();
+    const int l = blockIdx.x * nw + wave;
+    if (l >= A.M) return;
+    const int m = A.mode == 0 ? -1 : A.order[(size_t)chain * A.n_meioses + A.step];
+    WaveArea w = wave_area(A, smem + A.ptab_bytes + (size_t)wave * A.wave_bytes);
+    const uint32_t* src = A.graphs + ((size_t)chain * A.M + l) * A.W;
+    for (int k = lane; k < A.W; k += 64) {
+        uint32_t x = src[k];
+        if (m >= 0 && (m >> 5) == k) x ^= 1u << (m & 31);
+        w.colw[k] = x;
+    }
+    wave_sync();
+    const double L = fag_log_likelihood(A, ptab, w, l);
+    if (lane == 0) {
+        if (A.mode == 0) {
+            A.lcur[(size_t)chain * A.M + l] = L;
+            if (L == -INFINITY) A.status[chain] = kIllegalGraph;  // meiosis_sampler.cc:33-36
+        } else {
+            A.lflip[(size_t)chain * A.M + l] = L;
+        }
+    }
+}
+
+// Forward-backward pass of one meiosis step (meiosis_sampler.cc:136-203), one
+// workgroup per chain; thread t owns a contiguous segment of loci.  Scans run
+// within wavefronts by lane shuffles and across them through LDS.
+__global__ void __launch_bounds__(kFbBlock) mfb_kernel(MsampArgs A) {
+    __shared__ Mat wtot[kFbBlock / 64];
+    __shared__ uint32_t wcnt[kFbBlock / 64], wmap[kFbBlock / 64];
+    __shared__ int s_skip;
+    const int chain = blockIdx.x;
+    if (threadIdx.x == 0) s_skip = A.status[chain] != 0 || !is_msweep(A, chain);
+    __syncthreads();
+    if (s_skip) return;
+    const int B = blockDim.x, tid = threadIdx.x, lane = lane_id(), wave = tid >> 6, nw = B >> 6, M = A.M;
+    const int m = A.order[(size_t)chain * A.n_meioses + A.step];
     const int word = m >> 5;
     const uint32_t bm = 1u << (m & 31);
     uint32_t* g = A.graphs + (size_t)chain * M * A.W;
@@ -488,46 +315,43 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int m, unsigned 
     const double* lf = A.lflip + (size_t)chain * M;
     double* fb = A.fbuf + (size_t)chain * M * 2;
     uint8_t* mp = A.maps + (size_t)chain * M;
-    Mat* ms = reinterpret_cast<Mat*>(smem);
-    uint32_t* is = reinterpret_cast<uint32_t*>(ms + B);
     const int S = (M + B - 1) / B;
     const int a0 = min(M, tid * S), a1 = min(M, a0 + S);
+    const Mat I{1.0, 0.0, 0.0, 1.0};
 
-    // forward probabilities: exclusive block scan of segment products
-    Mat P{1.0, 0.0, 0.0, 1.0};
+    // forward probabilities: exclusive scan of the segment products
+    Mat P = I;
     for (int l = a0; l < a1; ++l) P = mmul(P, locus_matrix(A, lc, lf, g, word, bm, l));
-    ms[tid] = P;
-    __syncthreads();
-    for (int off = 1; off < B; off <<= 1) {
-        Mat v = ms[tid];
-        if (tid >= off) v = mmul(ms[tid - off], v);
-        __syncthreads();
-        ms[tid] = v;
-        __syncthreads();
+    for (int d = 1; d < 64; d <<= 1) {
+        const Mat o = shfl_up(P, d);
+        if (lane >= d) P = mmul(o, P);
     }
-    Mat Rm = tid > 0 ? ms[tid - 1] : Mat{1.0, 0.0, 0.0, 1.0};
+    if (lane == 63) wtot[wave] = P;
     __syncthreads();
+    Mat R = I;
+    for (int k = 0; k < wave; ++k) R = mmul(R, wtot[k]);
+    const Mat below = shfl_up(P, 1);
+    if (lane > 0) R = mmul(R, below);
     uint32_t draws = 0, near = 0;
     for (int l = a0; l < a1; ++l) {
-        Rm = mmul(Rm, locus_matrix(A, lc, lf, g, word, bm, l));
-        const double f0 = Rm.a + Rm.c, f1 = Rm.b + Rm.d;
+        R = mmul(R, locus_matrix(A, lc, lf, g, word, bm, l));
+        const double f0 = R.a + R.c, f1 = R.b + R.d;
         fb[2 * l] = f0;
         fb[2 * l + 1] = f1;
         // a draw happens when both weighted states are non-zero (:193-203)
         const double t = l < M - 1 ? A.theta[l] : 1.0, a = l < M - 1 ? 1.0 - A.theta[l] : 1.0;
         draws += (f0 * a != 0.0 && f1 * t != 0.0) ? 1u : 0u;
     }
-    // draw index of each segment's top locus: drawing loci above it
-    is[tid] = draws;
-    __syncthreads();
-    for (int off = 1; off < B; off <<= 1) {
-        const uint32_t v = tid + off < B ? is[tid + off] : 0u;
-        __syncthreads();
-        is[tid] += v;
-        __syncthreads();
+    // draw index of each segment's top locus: the drawing loci above it
+    uint32_t inc = draws;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_down(inc, d);
+        if (lane + d < 64) inc += o;
     }
-    uint32_t cnt = tid + 1 < B ? is[tid + 1] : 0u;
+    if (lane == 0) wcnt[wave] = inc;
     __syncthreads();
+    uint32_t cnt = inc - draws;
+    for (int k = wave + 1; k < nw; ++k) cnt += wcnt[k];
     // per-locus decision maps, top of the segment downwards
     uint32_t comp = 2u;  // identity
     for (int l = a1 - 1; l >= a0; --l) {
@@ -567,18 +391,20 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int m, unsigned 
         mp[l] = (uint8_t)map;
         comp = compose(map & 3u, comp);
     }
-    // indicator entering each segment from above: reverse scan of compositions
-    is[tid] = comp;
-    __syncthreads();
-    for (int off = 1; off < B; off <<= 1) {
-        const uint32_t mine = is[tid];
-        const uint32_t up = tid + off < B ? is[tid + off] : 2u;
-        __syncthreads();
-        is[tid] = compose(mine, up);
-        __syncthreads();
+    // indicator entering each segment from above: suffix composition of maps
+    uint32_t sc = comp;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_down(sc, d);
+        if (lane + d < 64) sc = compose(sc, o);
     }
+    if (lane == 0) wmap[wave] = sc;
+    __syncthreads();
+    uint32_t up = 2u;
+    for (int k = nw - 1; k > wave; --k) up = compose(wmap[k], up);
+    uint32_t nxt = __shfl_down(sc, 1);
+    if (lane == 63) nxt = 2u;
     // the map of locus M-1 is constant, so the composite above is too
-    uint32_t next = tid + 1 < B ? (is[tid + 1] & 1u) : 0u;
+    uint32_t next = compose(nxt, up) & 1u;
     for (int l = a1 - 1; l >= a0; --l) {
         const uint32_t map = mp[l];
         const uint32_t nb = (map >> next) & 1u;
@@ -594,103 +420,77 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int m, unsigned 
     if (near) atomicAdd(&A.near_edge[chain], near);
 }
 
-__device__ __forceinline__ bool is_msweep(const MsampArgs& A, int chain) {
-    return !A.check_kind || !(philox_uniform(A.seed, (uint32_t)chain, A.sweep, kStepSched, 0) < A.lprob);
-}
-
-template <int R>
-__global__ void __launch_bounds__(256) mstep_kernel(MsampArgs A) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int chain = blockIdx.y;
-    if (A.status[chain] != 0 || !is_msweep(A, chain)) return;
-    const int wave = threadIdx.x >> 6, lane = lane_id();
-    const int l = blockIdx.x * (blockDim.x >> 6) + wave;
-    const int m = A.mode == 0 ? -1 : A.order[(size_t)chain * A.n_meioses + A.step];
-    if (l < A.M) {
-        uint32_t* colw = reinterpret_cast<uint32_t*>(smem + (size_t)wave * A.wave_bytes);
-        uint16_t* lab = reinterpret_cast<uint16_t*>(colw + A.W);
-        const uint32_t* src = A.graphs + ((size_t)chain * A.M + l) * A.W;
-        for (int w = lane; w < A.W; w += 64) {
-            uint32_t x = src[w];
-            if (m >= 0 && (m >> 5) == w) x ^= 1u << (m & 31);
-            colw[w] = x;
-        }
-        wave_sync();
-        const double L = fag_column<R>(A, colw, lab, l);
-        if (lane == 0) {
-            if (A.mode == 0) {
-                A.lcur[(size_t)chain * A.M + l] = L;
-                if (L == 0.0) A.status[chain] = kIllegalGraph;  // meiosis_sampler.cc:33-36
-            } else {
-                A.lflip[(size_t)chain * A.M + l] = L;
-            }
-        }
-    }
-    if (A.mode == 0) return;
-    // the last workgroup of this chain to finish runs the forward-backward pass
-    __shared__ int s_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(&A.counter[chain], 1u) == gridDim.x - 1 ? 1 : 0;
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    if (threadIdx.x == 0) A.counter[chain] = 0;
-    forward_backward(A, chain, m, smem);
-}
-
 // this sweep's meiosis order: the given order, or a Fisher-Yates shuffle of the
 // non-ignorable meioses (random_shuffle, markov_chain.cc:346) driven by the
-// chain's draws at step kStepShuffle
+// chain's draws at step kStepShuffle.  Position i (from the top) swaps with
+// floor(u_d (i+1)), d = n-1-i; the uniforms are drawn in parallel, the swaps
+// done in order.  One workgroup per chain; dynamic LDS holds n swap targets.
 __global__ void morder_kernel(MsampArgs A, const int32_t* fixed_order) {
-    const int chain = blockIdx.x * blockDim.x + threadIdx.x;
-    if (chain >= A.n_chains || A.status[chain] != 0 || !is_msweep(A, chain)) return;
-    int32_t* o = A.order + (size_t)chain * A.n_meioses;
+    extern __shared__ int32_t jdx[];
+    __shared__ int s_skip;
+    const int chain = blockIdx.x;
+    if (threadIdx.x == 0) s_skip = A.status[chain] != 0 || !is_msweep(A, chain);
+    __syncthreads();
+    if (s_skip) return;
+    const int n = A.n_meioses;
+    int32_t* o = A.order + (size_t)chain * n;
     if (fixed_order) {
-        for (int i = 0; i < A.n_meioses; ++i) o[i] = fixed_order[i];
+        for (int i = threadIdx.x; i < n; i += blockDim.x) o[i] = fixed_order[i];
         return;
     }
-    for (int i = 0; i < A.n_meioses; ++i) o[i] = A.meioses[i];
-    uint32_t d = 0;
-    for (int i = A.n_meioses - 1; i > 0; --i) {
-        const double u = philox_uniform(A.seed, (uint32_t)chain, A.sweep, kStepShuffle, d++);
-        int j = (int)(u * (double)(i + 1));
-        if (j > i) j = i;
-        const int32_t t = o[i];
-        o[i] = o[j];
-        o[j] = t;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        o[i] = A.meioses[i];
+        if (i > 0) {
+            const double u = philox_uniform(A.seed, (uint32_t)chain, A.sweep, kStepShuffle, (uint32_t)(n - 1 - i));
+            const int j = (int)(u * (double)(i + 1));
+            jdx[i] = j > i ? i : j;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = n - 1; i > 0; --i) {
+            const int j = jdx[i];
+            const int32_t t = o[i];
+            o[i] = o[j];
+            o[j] = t;
+        }
     }
 }
 
 }  // namespace
 
-int msampler_regs(int F) {
-    const int r = (2 * F + 63) / 64;
-    if (r < 1) return 1;
-    return r <= kMaxFagRegs ? r : 0;
+bool msampler_supported(int N, int F) { return F >= 1 && 2 * F <= kMaxFagNodes && N <= 65535; }
+
+size_t msampler_wave_bytes(int N, int F, int W) {
+    const size_t b = 4 * (size_t)W + 2 * 4 * (size_t)(2 * F) + 4 * (size_t)N + 2 * 2 * (size_t)(N - F);
+    return (b + 15) / 16 * 16;
 }
 
+size_t msampler_ptab_bytes(int N, int F) { return ((size_t)(N - F) * 8 + 15) / 16 * 16; }
+
 size_t msampler_lds_bytes(const MsampArgs& A, int block) {
-    const size_t waves = (size_t)(block / 64) * (size_t)A.wave_bytes;
-    const size_t fb = (size_t)block * (sizeof(Mat) + sizeof(uint32_t));
-    return waves > fb ? waves : fb;
+    return (size_t)A.ptab_bytes + (size_t)(block / 64) * (size_t)A.wave_bytes;
+}
+
+hipError_t configure_msampler_lds(size_t lds) {
+    if (lds <= 64 * 1024) return hipSuccess;
+    return hipFuncSetAttribute((const void*)mstep_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 hipError_t launch_morder(const MsampArgs& A, const int32_t* fixed_order, hipStream_t s) {
-    const int b = 64;
-    hipLaunchKernelGGL(morder_kernel, dim3((A.n_chains + b - 1) / b), dim3(b), 0, s, A, fixed_order);
+    hipLaunchKernelGGL(morder_kernel, dim3(A.n_chains), dim3(256), sizeof(int32_t) * (size_t)A.n_meioses, s, A,
+                       fixed_order);
     return hipGetLastError();
 }
 
-hipError_t launch_mstep(const MsampArgs& A, int regs, int block, size_t lds, hipStream_t s) {
+hipError_t launch_mstep(const MsampArgs& A, int block, size_t lds, hipStream_t s) {
     const dim3 grid((A.M + block / 64 - 1) / (block / 64), A.n_chains);
-    switch (regs) {
-#define SL_MSTEP(R) \
-    case R: hipLaunchKernelGGL(mstep_kernel<R>, grid, dim3(block), lds, s, A); break;
-        SL_MSTEP(1) SL_MSTEP(2) SL_MSTEP(3) SL_MSTEP(4) SL_MSTEP(5) SL_MSTEP(6) SL_MSTEP(7) SL_MSTEP(8)
-#undef SL_MSTEP
-        default: return hipErrorInvalidValue;
-    }
+    hipLaunchKernelGGL(mstep_kernel, grid, dim3(block), lds, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_mfb(const MsampArgs& A, hipStream_t s) {
+    hipLaunchKernelGGL(mfb_kernel, dim3(A.n_chains), dim3(kFbBlock), 0, s, A);
     return hipGetLastError();
 }
 

@@ -8,16 +8,18 @@
 
 namespace sl {
 
-// The founder-allele graph of one locus lives in the VGPRs of one wavefront:
-// node x (founder allele) and component c sit in lane x%64 of register x/64,
-// so a pedigree needs ceil(2F/64) registers per field.
-constexpr int kMaxFagRegs = 8;  // 2F <= 512
-constexpr int kMsampBlock = 256;  // threads per workgroup, the kernel's launch bound
+// One wavefront builds the founder-allele graph of one locus in LDS; founder
+// alleles are 10-bit node ids in its edge words.
+constexpr int kMaxFagNodes = 1022;  // 2F
+constexpr int kMsampBlock = 256;    // threads per workgroup of the likelihood kernel (its launch bound)
+constexpr int kFbBlock = 1024;      // threads per workgroup of the forward-backward kernel
 
 struct MsampArgs {
     int N, F, M, W;
     const int32_t* mother;        // [N]
     const int32_t* father;        // [N]
+    const uint2* ptab;            // [N-F] label pass in generation order: (person, mother | father << 16)
+    int ptab_bytes;               // LDS bytes of the staged ptab
     const int32_t* gen_off;       // [n_gen+1] non-founders grouped by generation
     const int32_t* gen_persons;
     int n_gen;
@@ -32,7 +34,6 @@ struct MsampArgs {
     double* lflip;                // [chains][M] likelihood with the step's meiosis flipped
     double* fbuf;                 // [chains][M][2] forward probabilities
     uint8_t* maps;                // [chains][M] per-locus backward decision maps
-    uint32_t* counter;            // [chains] finished-workgroup counters (zero between launches)
     int32_t* status;              // [chains]
     uint32_t* near_edge;          // [chains] draws within 1e-12 of their threshold
     uint64_t seed;
@@ -45,9 +46,13 @@ struct MsampArgs {
     int n_chains;
 };
 
-int msampler_regs(int F);  // registers per field, 0 when unsupported
+bool msampler_supported(int N, int F);
+size_t msampler_wave_bytes(int N, int F, int W);
+size_t msampler_ptab_bytes(int N, int F);
 size_t msampler_lds_bytes(const MsampArgs& A, int block);
+hipError_t configure_msampler_lds(size_t lds);
 hipError_t launch_morder(const MsampArgs& A, const int32_t* fixed_order, hipStream_t s);
-hipError_t launch_mstep(const MsampArgs& A, int regs, int block, size_t lds, hipStream_t s);
+hipError_t launch_mstep(const MsampArgs& A, int block, size_t lds, hipStream_t s);
+hipError_t launch_mfb(const MsampArgs& A, hipStream_t s);
 
 }  // namespace sl
