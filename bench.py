@@ -8,7 +8,8 @@ A step is one sweep of the chain on the device (an L-sweep or an M-sweep as
 the schedule draw says), with the LOD scoring of the chain's graph on every
 10th sweep, so `value` is whole-job MCMC sweeps/s (FP32-rescaled samplers and
 peel, FP64 LOD accumulation).  Reported beside it: the FP64 chain's sweeps/s,
-LOD evals/s of the trait peel over a batch of 64 device-resident graphs in
+the sweeps/s of 64 replicate chains run together in one context, LOD evals/s
+of the trait peel over a batch of 64 device-resident graphs in
 FP32-rescaled and FP64 with the largest |dLOD| between the two, the roofline
 of the dominant kernel (the meiosis-step kernel of the M-sampler, timed with
 HIP events on the library's stream) and of the peel kernel, and a CPU
@@ -119,31 +120,36 @@ def main():
     g0 = synth.true_graph(data, case)
     np.save(os.path.join(tmp, "g.npy"), g0)
 
-    # ---- MCMC: one chain through the schedule, scoring on the device ----
-    def chain(precision, steps, warmup):
+    # ---- MCMC: chains through the schedule, scoring on the device ----
+    def chain(precision, chains, steps, warmup):
         ctx = native.Context(case, precision)
-        ctx.mcmc_init(g0[None])
+        ctx.mcmc_init(np.repeat(g0[None], chains, axis=0))
         ctx.lod_reset()
         ctx.mcmc_run(warmup, 0, PERIOD, LPROB, SEED)
         ctx.sync()
-        ctx.profile(True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         st = ctx.mcmc_run(steps, 0, PERIOD, LPROB, SEED, first_sweep=warmup)
         ctx.sync()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        prof = {k: ctx.profile_read(k) for k in (native.SL_KERNEL_LOD, native.SL_KERNEL_LSAMPLER,
-                                                 native.SL_KERNEL_MSAMPLER)}
         near = int(ctx.mcmc_near_edge().sum())
+        # kernel averages from a separate, shorter pass with events around every launch
+        ctx.profile(True)
+        ctx.mcmc_run(max(10, steps // 10), 0, PERIOD, LPROB, SEED, first_sweep=warmup + steps)
+        prof = {k: ctx.profile_read(k) for k in (native.SL_KERNEL_LOD, native.SL_KERNEL_LSAMPLER,
+                                                 native.SL_KERNEL_MSAMPLER, native.SL_KERNEL_MSAMPLER_FB)}
         ctx.close()
         return dt, st, prof, near
 
-    dt32, st32, prof32, near32 = chain(native.SL_FP32_RESCALED, args.steps, args.warmup)
+    dt32, st32, prof32, near32 = chain(native.SL_FP32_RESCALED, 1, args.steps, args.warmup)
     n64 = max(20, args.steps // 4)
-    dt64, st64, _, _ = chain(native.SL_FP64, n64, args.warmup)
+    dt64, st64, _, _ = chain(native.SL_FP64, 1, n64, args.warmup)
     value = args.steps / dt32
     fp64_sweeps = n64 / dt64
+    CB, nb = 64, max(10, args.steps // 10)
+    dtb, stb, _, nearb = chain(native.SL_FP32_RESCALED, CB, nb, 2)
+    batch_sweeps = CB * nb / dtb
 
     # ---- LOD throughput on a batch of resident graphs ----
     B = args.batch
@@ -176,6 +182,7 @@ def main():
     # = per (chain, meiosis, locus) ceil(2N/8) descent-graph bytes + ceil(2
     # N_typed/8) genotype bytes, times M loci and one chain ----
     ms_ms, ms_launches = prof32[native.SL_KERNEL_MSAMPLER]
+    avg = lambda k: prof32[k][0] / max(prof32[k][1], 1)
     n_typed = int(np.count_nonzero(case.typed))
     bytes_per_unit = -(-2 * case.N // 8) + -(-2 * n_typed // 8)
     bytes_per_launch = case.M * bytes_per_unit
@@ -202,14 +209,18 @@ def main():
         "sweeps": {"l_sweeps": st32["l_sweeps"], "m_sweeps": st32["m_sweeps"], "scored": st32["scored"],
                    "near_edge_draws": near32},
         "fp64_sweeps_per_s": fp64_sweeps,
+        "batched": {"chains": CB, "sweeps_per_chain": nb, "sweeps_per_s": batch_sweeps,
+                    "l_sweeps": stb["l_sweeps"], "m_sweeps": stb["m_sweeps"], "scored": stb["scored"],
+                    "near_edge_draws": nearb,
+                    "note": "64 replicate chains of the same pedigree in one context, same schedule"},
         "lod_evals_per_s": lod32,
         "fp64_lod_evals_per_s": lod64,
         "max_abs_dlod_fp32_vs_fp64": float(np.abs(lodv32 - lodv64).max()),
-        "kernel_ms": {"msampler_step": ms_avg,
-                      "lsampler_parity": prof32[native.SL_KERNEL_LSAMPLER][0] /
-                      max(prof32[native.SL_KERNEL_LSAMPLER][1], 1),
-                      "lod_scoring": prof32[native.SL_KERNEL_LOD][0] / max(prof32[native.SL_KERNEL_LOD][1], 1)},
-        "roofline": {"bound": "hbm", "kernel": "mstep_kernel (M-sampler meiosis step)",
+        "kernel_ms": {"msampler_likelihoods": ms_avg,
+                      "msampler_forward_backward": avg(native.SL_KERNEL_MSAMPLER_FB),
+                      "lsampler_parity": avg(native.SL_KERNEL_LSAMPLER),
+                      "lod_scoring": avg(native.SL_KERNEL_LOD)},
+        "roofline": {"bound": "hbm", "kernel": "mstep_kernel (founder-allele likelihoods of one meiosis step)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profile_traffic("mstep_kernel", args.config),
                      "bytes_per_launch": bytes_per_launch, "launch_ms": ms_avg},
