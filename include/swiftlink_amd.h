@@ -78,7 +78,16 @@ typedef struct {
     int32_t device;               /* HIP device ordinal */
     int32_t block_threads;        /* LOD kernel workgroup size; 0 = autotuned */
     int32_t graphs_per_block;     /* graphs folded into one LOD workgroup; 0 = default */
+    int32_t msampler_mode;        /* SL_MSAMPLER_*: how an M-sweep's steps are launched */
 } sl_options;
+
+/* M-sweep launch modes (results are identical): one launch per meiosis step,
+ * computing the next step's likelihoods for both values of the current
+ * meiosis's bit while its forward-backward pass runs; or two launches per
+ * step; or the first while the doubled likelihood work is small. */
+#define SL_MSAMPLER_AUTO 0
+#define SL_MSAMPLER_SPECULATIVE 1
+#define SL_MSAMPLER_TWO_LAUNCH 2
 
 /* Replaces GPULodscores::GPULodscores + gpu_init (gpu_lodscores.h:63-80,
  * gpu_lodscores.cc:119-161): validates and copies the pedigree, map and peel

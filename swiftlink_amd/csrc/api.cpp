@@ -20,6 +20,9 @@ using sl::PeelProgram;
 
 namespace {
 thread_local std::string g_create_error;
+// speculative M-steps double the likelihood wavefronts; used while they fit
+// in about two wavefronts per SIMD of the device
+constexpr int64_t kSpecWaves = 8192;
 }
 
 struct sl_ctx {
@@ -89,6 +92,7 @@ struct sl_ctx {
     std::vector<int32_t> mother_h, father_h;
     // M-sampler
     int m_block = sl::kMsampBlock;
+    int m_mode = SL_MSAMPLER_AUTO;
     int m_wave_bytes = 0;
     size_t m_lds = 0;
     bool m_ready = false;
@@ -383,6 +387,11 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
         return fail(nullptr, SL_E_INVALID, "block_threads must be a multiple of 64 in 64..1024");
     }
     if (opts && opts->graphs_per_block > 0) c->gpw = opts->graphs_per_block;
+    if (opts) c->m_mode = opts->msampler_mode;
+    if (c->m_mode < SL_MSAMPLER_AUTO || c->m_mode > SL_MSAMPLER_TWO_LAUNCH) {
+        delete c;
+        return fail(nullptr, SL_E_INVALID, "unknown msampler_mode");
+    }
 
     sl::PedigreeView pv{c->N, c->F, ped->mother, ped->father, ped->disease_prob};
     sl::PeelView kv{peel->n_ops, peel->type, peel->peel_node, peel->cut_offsets, peel->cutset,
@@ -702,7 +711,7 @@ int sl_mcmc_init(sl_ctx* c, int32_t n_chains, const int32_t* graphs) {
         SL_HIP(c, hipMalloc((void**)&c->d_chains, words * sizeof(uint32_t)));
         SL_HIP(c, hipMalloc((void**)&c->d_status, (size_t)n_chains * sizeof(int32_t)));
         SL_HIP(c, hipMalloc((void**)&c->d_lcur, cm * sizeof(double)));
-        SL_HIP(c, hipMalloc((void**)&c->d_lflip, cm * sizeof(double)));
+        SL_HIP(c, hipMalloc((void**)&c->d_lflip, 4 * cm * sizeof(double)));
         SL_HIP(c, hipMalloc((void**)&c->d_fbuf, 2 * cm * sizeof(double)));
         SL_HIP(c, hipMalloc((void**)&c->d_maps, cm));
         SL_HIP(c, hipMalloc((void**)&c->d_near, (size_t)n_chains * sizeof(uint32_t)));
@@ -849,13 +858,30 @@ static int run_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* d
     A.check_kind = check_kind;
     A.lprob = lprob;
     SL_HIP(c, sl::launch_morder(A, dev_order, c->stream));
-    A.mode = 0;
-    SL_HIP(c, sl::launch_mstep(A, c->m_block, c->m_lds, c->stream));
-    A.mode = 1;
-    for (int j = 0; j < nm; ++j) {
-        A.step = j;
-        rc = timed_launch(c, SL_KERNEL_MSAMPLER, [&] { return sl::launch_mstep(A, c->m_block, c->m_lds, c->stream); });
-        if (!rc) rc = timed_launch(c, SL_KERNEL_MSAMPLER_FB, [&] { return sl::launch_mfb(A, c->stream); });
+    // one launch per step when the doubled likelihood work still fits the
+    // device in one wave of workgroups (few chains), two launches otherwise
+    const bool spec = c->m_mode == SL_MSAMPLER_SPECULATIVE ||
+                      (c->m_mode == SL_MSAMPLER_AUTO && (int64_t)c->n_chains * c->M * 2 <= kSpecWaves);
+    A.spec = spec ? 1 : 0;
+    // start of the sweep: the current columns and step 0's flipped columns
+    A.with_lcur = 1;
+    A.step = 0;
+    A.n_jobs = 2 * c->M;
+    SL_HIP(c, sl::launch_mstep(A, c->m_lds, c->stream));
+    A.with_lcur = 0;
+    for (int k = 0; k < nm; ++k) {
+        A.fb_step = k;
+        A.step = k + 1;
+        const int jobs = k + 1 < nm ? (spec ? 2 : 1) * c->M : 0;
+        if (spec) {
+            A.n_jobs = jobs;
+            rc = timed_launch(c, SL_KERNEL_MSAMPLER, [&] { return sl::launch_mspec(A, c->m_lds, c->stream); });
+        } else {
+            rc = timed_launch(c, SL_KERNEL_MSAMPLER_FB, [&] { return sl::launch_mfb(A, c->stream); });
+            A.n_jobs = jobs;
+            if (!rc && jobs)
+                rc = timed_launch(c, SL_KERNEL_MSAMPLER, [&] { return sl::launch_mstep(A, c->m_lds, c->stream); });
+        }
         if (rc) return rc;
     }
     return SL_OK;
@@ -940,12 +966,13 @@ int sl_mcmc_likelihood(sl_ctx* c, double* per_locus) {
     int rc = need_msampler(c);
     if (rc) return rc;
     sl::MsampArgs A = msamp_args(c);
-    A.mode = 0;
+    A.with_lcur = 1;
+    A.n_jobs = c->M;
     std::vector<int32_t> st((size_t)c->n_chains, 0);
     // chains already in error are evaluated too
     SL_HIP(c, hipMemcpyAsync(st.data(), c->d_status, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     SL_HIP(c, hipMemsetAsync(c->d_status, 0, st.size() * sizeof(int32_t), c->stream));
-    SL_HIP(c, sl::launch_mstep(A, c->m_block, c->m_lds, c->stream));
+    SL_HIP(c, sl::launch_mstep(A, c->m_lds, c->stream));
     const size_t n = (size_t)c->n_chains * c->M;
     SL_HIP(c, hipMemcpyAsync(per_locus, c->d_lcur, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     std::vector<int32_t> now(st.size());

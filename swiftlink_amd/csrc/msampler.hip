@@ -1,40 +1,46 @@
 // M-sampler kernels for gfx950.
 //
 // MeiosisSampler (meiosis_sampler.cc:17-203) resamples one meiosis at a time,
-// in a shuffled order, at every locus jointly given all other indicators.  One
-// step j (meiosis m) is one launch of mstep_kernel over (locus, chain):
-//  * each wavefront evaluates the founder-allele-graph likelihood of its
-//    locus's column with bit m flipped (FounderAlleleGraph4::likelihood,
-//    founder_allele_graph4.cc:34-442).  The reference adds typed persons one
-//    by one to components that carry two complementary allele assignments;
-//    its result is the sum, over the allele assignments of the founder
-//    alleles that satisfy every genotype, of the product of allele
-//    frequencies, or 0 when none does.  Here the wave's lanes build the same
-//    components in parallel: founder-allele labels are propagated generation
-//    by generation, every typed person becomes an edge between its two
-//    founder alleles (heterozygous: the alleles differ, homozygous: both
-//    equal the genotype's allele), and a union-find with a parity bit per
-//    node (is the node's allele the opposite of its parent's?) is built by
-//    rounds of hooking roots onto smaller roots and path compression until
-//    no edge joins two components.  Then every edge is checked against the
-//    parities, homozygous edges fix their root's allele, and with n0/n1 the
-//    numbers of a root's nodes of parity 0/1 its component contributes
-//    maj^n0 min^n1 (root allele A), maj^n1 min^n0 (root allele B) or their
-//    sum, taken in log space.  Legality is exactly the reference's; the
-//    likelihood agrees with its products to rounding;
-//  * a second kernel, one workgroup per chain, runs the forward-backward pass
-//    (:105-203) with its threads: a block scan of the
-//    2x2 matrices T(theta_{l-1}) diag(e_l) gives the forward probabilities;
-//    a locus draws a uniform only when both its states are possible, so its
-//    draw index is the number of drawing loci above it (reverse scan); each
-//    locus becomes a 2-bit map from the next locus's indicator to its own,
-//    and a reverse scan of map compositions gives every new indicator.  The
-//    likelihood rounding and the reassociated products move thresholds only
-//    by rounding: draws within 1e-12 of their threshold are counted per chain;
-//  * changed indicators are written back and the current-column likelihood
-//    carried over (raw_matrix, :116-134).
-// mode 0 of the same kernel computes the current-column likelihoods at the
-// start of a sweep (MeiosisSampler::reset, :17-40).
+// in a shuffled order, at every locus jointly given all other indicators.
+// Step k (meiosis m_k) needs, at every locus, the founder-allele-graph
+// likelihood of the column with m_k flipped, then a forward-backward pass
+// along the loci.
+//  * Likelihoods (fag_job): one wavefront per (locus, chain) evaluates
+//    FounderAlleleGraph4::likelihood (founder_allele_graph4.cc:34-442).  The
+//    reference adds typed persons one by one to components that carry two
+//    complementary allele assignments; its result is the sum, over the allele
+//    assignments of the founder alleles that satisfy every genotype, of the
+//    product of allele frequencies, or 0 when none does.  Here the wave's
+//    lanes build the same components in parallel: founder-allele labels are
+//    propagated generation by generation, every typed person becomes an edge
+//    between its two founder alleles (heterozygous: the alleles differ,
+//    homozygous: both equal the genotype's allele), and a union-find with a
+//    parity bit per node (is the node's allele the opposite of its
+//    parent's?) is built by rounds of hooking roots onto smaller roots and
+//    path compression until no edge joins two components.  Then every edge is
+//    checked against the parities, homozygous edges fix their root's allele,
+//    and with n0/n1 the numbers of a root's nodes of parity 0/1 its component
+//    contributes maj^n0 min^n1 (root allele A), maj^n1 min^n0 (root allele B)
+//    or their sum, taken in log space.  Legality is exactly the reference's;
+//    the likelihood agrees with its products to rounding.
+//  * Forward-backward (forward_backward): one workgroup per chain; a block
+//    scan of the 2x2 matrices T(theta_{l-1}) diag(e_l) gives the forward
+//    probabilities; a locus draws a uniform only when both its states are
+//    possible, so its draw index is the number of drawing loci above it
+//    (reverse scan); each locus becomes a 2-bit map from the next locus's
+//    indicator to its own, and a reverse scan of map compositions gives every
+//    new indicator; changed indicators are written back and the current
+//    column's likelihood carried over (raw_matrix, :116-134).  The likelihood
+//    rounding and the reassociated products move thresholds only by rounding:
+//    draws within 1e-12 of their threshold are counted per chain.
+// Step k+1's likelihoods depend on step k's outcome at a locus only through
+// the one bit of m_k there.  With few chains (mspec_kernel) one launch runs
+// step k's forward-backward pass in one workgroup per chain and, in the other
+// workgroups, step k+1's likelihoods for both values of that bit, so a step
+// costs one launch; with many chains (mstep_kernel, mfb_kernel) each step is
+// a likelihood launch then a forward-backward launch.  The start of a sweep
+// also computes the current columns' likelihoods (MeiosisSampler::reset,
+// :17-40).
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
@@ -48,6 +54,7 @@ namespace {
 
 constexpr int G_HETERO = 1, G_HOMOZ_B = 3;  // genotype codes (genotype.h)
 constexpr int kIllegalGraph = -5;
+constexpr int kGenoPrefetch = 4;  // genotype chunks of 64 persons loaded ahead
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
 
@@ -96,6 +103,15 @@ __device__ __forceinline__ uint32_t uf_find(const uint32_t* uf, uint32_t x, uint
 __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, const WaveArea& w, int l) {
     const int lane = lane_id();
     const int F = A.F, nodes = 2 * A.F;
+    // the first genotype chunks are loaded before the label pass so that the
+    // global-load latency overlaps it
+    const uint8_t* gcol = A.geno_t + (size_t)l * A.N;
+    int gpre[kGenoPrefetch];
+#pragma unroll
+    for (int c = 0; c < kGenoPrefetch; ++c) {
+        const int i = 64 * c + lane;
+        gpre[c] = i < A.N ? gcol[i] : 0;
+    }
     // founder-allele labels of non-founders, parents first (reset, :543-569)
     for (int gen = 0; gen < A.n_gen; ++gen) {
         const int k1 = A.gen_off[gen + 1];
@@ -114,20 +130,27 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
         w.uf[x] = (uint32_t)x;
         w.acc[x] = 0u;
     }
-    // edges: the typed persons with a genotype at this locus
-    const uint8_t* gcol = A.geno_t + (size_t)l * A.N;
+    wave_sync();
+    // edges: the typed persons with a genotype at this locus; the founder
+    // alleles they touch are marked
     const uint64_t below = (1ull << lane) - 1ull;
     int E = 0;
     bool bad = false;
-    for (int i0 = 0; i0 < A.N; i0 += 64) {
+    for (int i0 = 0, c = 0; i0 < A.N; i0 += 64, ++c) {
         const int i = i0 + lane;
-        const int g = i < A.N ? gcol[i] : 0;
+        int g = 0;
+#pragma unroll
+        for (int k = 0; k < kGenoPrefetch; ++k)
+            if (c == k) g = gpre[k];
+        if (c >= kGenoPrefetch) g = i < A.N ? gcol[i] : 0;
         uint32_t e = 0;
         if (g) {
             const uint32_t x = i < F ? (uint32_t)(2 * i) : w.lab[2 * (i - F)];
             const uint32_t y = i < F ? (uint32_t)(2 * i + 1) : w.lab[2 * (i - F) + 1];
             bad |= g == G_HETERO && x == y;  // an autozygous heterozygote (:77-81)
             e = x | (y << 10) | ((g == G_HETERO ? 1u : 0u) << 20) | ((g == G_HOMOZ_B ? 1u : 0u) << 21);
+            atomicOr(&w.acc[x], 4u);
+            atomicOr(&w.acc[y], 4u);
         }
         const uint64_t mask = __ballot(g != 0);
         if (g) w.edges[E + __popcll(mask & below)] = e;
@@ -138,7 +161,8 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
     // components: hook the larger root under the smaller one with the parity
     // the edge implies, compress paths, until no edge joins two components
     // (every productive round removes a component, so at most 2F rounds)
-    for (int round = 0; round <= nodes; ++round) {
+    bool converged = false;
+    for (int round = 0; round <= nodes && !converged; ++round) {
         bool changed = false;
         for (int e0 = 0; e0 < E; e0 += 64) {
             const int e = e0 + lane;
@@ -162,32 +186,28 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
             if (x < nodes) w.uf[x] = r | (p << 16);
             wave_sync();
         }
-        if (!__ballot(changed)) break;
+        converged = !__ballot(changed);
     }
-    // consistency: every edge's parity; homozygous edges fix their root's
-    // allele; mark the founder alleles that take part
-    bool illegal = false;
+    // converged and compressed: every node's word holds its root and parity
+    // consistency: every edge's parity; homozygous edges fix their root's allele
+    bool illegal = !converged;
     for (int e0 = 0; e0 < E; e0 += 64) {
         const int e = e0 + lane;
         if (e < E) {
             const uint32_t ed = w.edges[e];
-            const uint32_t x = ed & 0x3FFu, y = (ed >> 10) & 0x3FFu;
+            const uint32_t ux = w.uf[ed & 0x3FFu], uy = w.uf[(ed >> 10) & 0x3FFu];
+            const uint32_t rx = ux & 0xFFFFu, px = (ux >> 16) & 1u;
             const uint32_t het = (ed >> 20) & 1u, allele = (ed >> 21) & 1u;
-            uint32_t px, py;
-            const uint32_t rx = uf_find(w.uf, x, &px), ry = uf_find(w.uf, y, &py);
-            illegal |= rx != ry || (px ^ py) != het;
+            illegal |= rx != (uy & 0xFFFFu) || (px ^ ((uy >> 16) & 1u)) != het;
             if (!het) atomicOr(&w.acc[rx], 1u << (allele ^ px));
-            atomicOr(&w.acc[x], 4u);
-            atomicOr(&w.acc[y], 4u);
         }
     }
     wave_sync();
     if (__ballot(illegal)) return -INFINITY;
     for (int x = lane; x < nodes; x += 64) {
         if (w.acc[x] & 4u) {
-            uint32_t p;
-            const uint32_t r = uf_find(w.uf, (uint32_t)x, &p);
-            atomicAdd(&w.acc[r], p ? (1u << 14) : (1u << 3));
+            const uint32_t u = w.uf[x];
+            atomicAdd(&w.acc[u & 0xFFFFu], ((u >> 16) & 1u) ? (1u << 14) : (1u << 3));
         }
     }
     wave_sync();
@@ -231,20 +251,6 @@ __device__ __forceinline__ Mat mmul(const Mat& x, const Mat& y) {
     return r;
 }
 
-// T(theta_{l-1}) diag(e_l), A_0 = diag(e_0); e_l = the two column likelihoods
-// of locus l divided by the larger one
-__device__ __forceinline__ Mat locus_matrix(const MsampArgs& A, const double* lc, const double* lf,
-                                            const uint32_t* g, int word, uint32_t bm, int l) {
-    const bool cur = (g[(size_t)l * A.W + word] & bm) != 0;
-    const double a0 = cur ? lf[l] : lc[l];  // log-likelihoods; the current column's is finite
-    const double a1 = cur ? lc[l] : lf[l];
-    const double mx = fmax(a0, a1);
-    const double e0 = exp(a0 - mx), e1 = exp(a1 - mx);  // exp(-inf) = 0 for an illegal flip
-    if (l == 0) return Mat{e0, 0.0, 0.0, e1};
-    const double th = A.theta[l - 1], an = 1.0 - A.theta[l - 1];
-    return Mat{an * e0, th * e1, th * e0, an * e1};
-}
-
 __device__ __forceinline__ uint32_t compose(uint32_t f, uint32_t h) {
     // 2-bit maps: bit x = image of x; (f o h)(x) = f(h(x))
     return ((f >> (h & 1u)) & 1u) | (((f >> ((h >> 1) & 1u)) & 1u) << 1);
@@ -258,140 +264,171 @@ __device__ __forceinline__ bool is_msweep(const MsampArgs& A, int chain) {
     return !A.check_kind || !(philox_uniform(A.seed, (uint32_t)chain, A.sweep, kStepSched, 0) < A.lprob);
 }
 
-// Founder-allele-graph likelihoods of one meiosis step (mode 1: the column
-// with meiosis m flipped, into lflip) or of the current columns (mode 0, into
-// lcur; a zero marks the chain's graph illegal).  Grid (loci / waves, chains).
-__global__ void __launch_bounds__(kMsampBlock) mstep_kernel(MsampArgs A) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int s_skip;
-    const int chain = blockIdx.y;
-    if (threadIdx.x == 0) s_skip = A.status[chain] != 0 || !is_msweep(A, chain);
-    __syncthreads();
-    if (s_skip) return;
-    const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = lane_id();
-    // person table of the label pass, staged once per workgroup
-    uint2* ptab = reinterpret_cast<uint2*>(smem);
-    for (int k = threadIdx.x; k < A.N - A.F; k += blockDim.x) ptab[k] = A.ptab[k];
-    __syncthreads();
-    const int l = blockIdx.x * nw + wave;
-    if (l >= A.M) return;
-    const int m = A.mode == 0 ? -1 : A.order[(size_t)chain * A.n_meioses + A.step];
-    WaveArea w = wave_area(A, smem + A.ptab_bytes + (size_t)wave * A.wave_bytes);
-    const uint32_t* src = A.graphs + ((size_t)chain * A.M + l) * A.W;
+// flipped-column log-likelihoods: [step parity][variant][chain][locus]
+__device__ __forceinline__ size_t lflip_index(const MsampArgs& A, int k, int v, int chain, int l) {
+    return (((size_t)(k & 1) * 2 + (size_t)v) * (size_t)A.n_chains + (size_t)chain) * (size_t)A.M + (size_t)l;
+}
+
+__device__ __forceinline__ int order_at(const MsampArgs& A, int chain, int k) {
+    return A.order[(size_t)chain * A.n_meioses + k];
+}
+
+// One founder-allele likelihood job of a wavefront, job q of this launch:
+//  * with A.with_lcur, jobs [0, M) are the current columns (into lcur; a zero
+//    marks the chain's graph illegal, meiosis_sampler.cc:33-36);
+//  * the other jobs are step A.step's columns with meiosis m_k flipped (into
+//    lflip, MeiosisSampler::step, :116-134).  With A.spec and k > 0, variant
+//    v first forces meiosis m_{k-1}'s bit to v: the forward-backward pass of
+//    step k-1 may be deciding that bit in the same launch, and it is the only
+//    bit it changes.
+__device__ void fag_job(const MsampArgs& A, int chain, int q, const uint2* ptab, unsigned char* wbase) {
+    const int lane = lane_id(), M = A.M;
+    int l, v = 0, flip = -1, force = -1;
+    bool current = false;
+    if (A.with_lcur && q < M) {
+        l = q;
+        current = true;
+    } else {
+        const int r = A.with_lcur ? q - M : q;
+        v = r / M;
+        l = r - v * M;
+        flip = order_at(A, chain, A.step);
+        if (A.spec && A.step > 0) force = order_at(A, chain, A.step - 1);
+    }
+    WaveArea w = wave_area(A, wbase);
+    const uint32_t* src = A.graphs + ((size_t)chain * M + l) * A.W;
     for (int k = lane; k < A.W; k += 64) {
         uint32_t x = src[k];
-        if (m >= 0 && (m >> 5) == k) x ^= 1u << (m & 31);
+        if (force >= 0 && (force >> 5) == k) x = (x & ~(1u << (force & 31))) | ((uint32_t)v << (force & 31));
+        if (flip >= 0 && (flip >> 5) == k) x ^= 1u << (flip & 31);
         w.colw[k] = x;
     }
     wave_sync();
     const double L = fag_log_likelihood(A, ptab, w, l);
     if (lane == 0) {
-        if (A.mode == 0) {
-            A.lcur[(size_t)chain * A.M + l] = L;
-            if (L == -INFINITY) A.status[chain] = kIllegalGraph;  // meiosis_sampler.cc:33-36
+        if (current) {
+            A.lcur[(size_t)chain * M + l] = L;
+            if (L == -INFINITY) A.status[chain] = kIllegalGraph;
         } else {
-            A.lflip[(size_t)chain * A.M + l] = L;
+            A.lflip[lflip_index(A, A.step, v, chain, l)] = L;
         }
     }
 }
 
-// Forward-backward pass of one meiosis step (meiosis_sampler.cc:136-203), one
-// workgroup per chain; thread t owns a contiguous segment of loci.  Scans run
-// within wavefronts by lane shuffles and across them through LDS.
-__global__ void __launch_bounds__(kFbBlock) mfb_kernel(MsampArgs A) {
-    __shared__ Mat wtot[kFbBlock / 64];
-    __shared__ uint32_t wcnt[kFbBlock / 64], wmap[kFbBlock / 64];
-    __shared__ int s_skip;
-    const int chain = blockIdx.x;
-    if (threadIdx.x == 0) s_skip = A.status[chain] != 0 || !is_msweep(A, chain);
-    __syncthreads();
-    if (s_skip) return;
-    const int B = blockDim.x, tid = threadIdx.x, lane = lane_id(), wave = tid >> 6, nw = B >> 6, M = A.M;
-    const int m = A.order[(size_t)chain * A.n_meioses + A.step];
-    const int word = m >> 5;
-    const uint32_t bm = 1u << (m & 31);
-    uint32_t* g = A.graphs + (size_t)chain * M * A.W;
-    double* lc = A.lcur + (size_t)chain * M;
-    const double* lf = A.lflip + (size_t)chain * M;
-    double* fb = A.fbuf + (size_t)chain * M * 2;
-    uint8_t* mp = A.maps + (size_t)chain * M;
-    const int S = (M + B - 1) / B;
-    const int a0 = min(M, tid * S), a1 = min(M, a0 + S);
-    const Mat I{1.0, 0.0, 0.0, 1.0};
+// likelihood buffers of step k at locus l: the flipped column's variant is the
+// bit of meiosis m_{k-1} there (already decided when step k's pass runs)
+struct StepView {
+    const MsampArgs* A;
+    int chain, k, word, pword;
+    uint32_t bm, pbm;
+    bool spec;
+    __device__ __forceinline__ double lf(const uint32_t* g, int l) const {
+        const int v = spec ? ((g[(size_t)l * A->W + pword] & pbm) ? 1 : 0) : 0;
+        return A->lflip[lflip_index(*A, k, v, chain, l)];
+    }
+};
 
-    // forward probabilities: exclusive scan of the segment products
-    Mat P = I;
-    for (int l = a0; l < a1; ++l) P = mmul(P, locus_matrix(A, lc, lf, g, word, bm, l));
+// Pieces of the forward-backward pass (meiosis_sampler.cc:136-203).
+
+// locus matrix from the two states' likelihoods scaled by the larger one:
+// T(theta_{l-1}) diag(e_l), A_0 = diag(e_0)
+__device__ __forceinline__ Mat emission_matrix(const MsampArgs& A, double e0, double e1, int l) {
+    if (l == 0) return Mat{e0, 0.0, 0.0, e1};
+    const double th = A.theta[l - 1], an = 1.0 - A.theta[l - 1];
+    return Mat{an * e0, th * e1, th * e0, an * e1};
+}
+
+// e0, e1 of locus l from the current and flipped columns' log-likelihoods
+// (the current column's is finite; exp(-inf) = 0 for an illegal flip)
+__device__ __forceinline__ void emissions(bool cur, double lcur, double lflip, double* e0, double* e1) {
+    const double x0 = cur ? lflip : lcur, x1 = cur ? lcur : lflip;
+    const double mx = fmax(x0, x1);
+    *e0 = exp(x0 - mx);
+    *e1 = exp(x1 - mx);
+}
+
+// 1 when locus l draws a uniform: both weighted states non-zero (:193-203)
+__device__ __forceinline__ uint32_t locus_draws(const MsampArgs& A, double f0, double f1, int l) {
+    const double t = l < A.M - 1 ? A.theta[l] : 1.0, a = l < A.M - 1 ? 1.0 - A.theta[l] : 1.0;
+    return (f0 * a != 0.0 && f1 * t != 0.0) ? 1u : 0u;
+}
+
+// 4-bit decision map of locus l: bit bn = new indicator when the next locus's
+// is bn; bit 2+bn = that decision's uniform lay within 1e-12 of its threshold.
+// Consumes the draw index cnt when the locus draws.
+__device__ __forceinline__ uint32_t locus_map(const MsampArgs& A, int chain, int k, double f0, double f1, int l,
+                                              uint32_t* cnt, uint32_t* near) {
+    double w0[2], w1[2];
+    if (l < A.M - 1) {
+        const double th = A.theta[l], an = 1.0 - A.theta[l];
+        w0[0] = f0 * an;
+        w1[0] = f1 * th;
+        w0[1] = f0 * th;
+        w1[1] = f1 * an;
+    } else {
+        w0[0] = w0[1] = f0;
+        w1[0] = w1[1] = f1;
+    }
+    // with theta in (0, 1) whether a locus draws does not depend on the next
+    // indicator; a difference (underflow in one branch only) is counted
+    const bool draw = w0[0] != 0.0 && w1[0] != 0.0;
+    if (draw != (w0[1] != 0.0 && w1[1] != 0.0)) ++*near;
+    const double u = draw ? philox_uniform(A.seed, (uint32_t)chain, A.sweep, (uint32_t)k, *cnt) : 0.0;
+    if (draw) ++*cnt;
+    uint32_t map = 0;
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn) {
+        uint32_t b, edge = 0;
+        if (w0[bn] == 0.0) {
+            b = 1u;
+        } else if (w1[bn] == 0.0) {
+            b = 0u;
+        } else {
+            const double thr = w0[bn] / (w0[bn] + w1[bn]);
+            b = u < thr ? 0u : 1u;
+            edge = fabs(u - thr) < 1e-12 ? 1u : 0u;
+        }
+        map |= (b << bn) | (edge << (2 + bn));
+    }
+    return map;
+}
+
+// exclusive left-to-right product of the threads' segment products
+__device__ __forceinline__ Mat scan_products(Mat P, Mat* wtot) {
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
     for (int d = 1; d < 64; d <<= 1) {
         const Mat o = shfl_up(P, d);
         if (lane >= d) P = mmul(o, P);
     }
     if (lane == 63) wtot[wave] = P;
     __syncthreads();
-    Mat R = I;
+    Mat R{1.0, 0.0, 0.0, 1.0};
     for (int k = 0; k < wave; ++k) R = mmul(R, wtot[k]);
     const Mat below = shfl_up(P, 1);
     if (lane > 0) R = mmul(R, below);
-    uint32_t draws = 0, near = 0;
-    for (int l = a0; l < a1; ++l) {
-        R = mmul(R, locus_matrix(A, lc, lf, g, word, bm, l));
-        const double f0 = R.a + R.c, f1 = R.b + R.d;
-        fb[2 * l] = f0;
-        fb[2 * l + 1] = f1;
-        // a draw happens when both weighted states are non-zero (:193-203)
-        const double t = l < M - 1 ? A.theta[l] : 1.0, a = l < M - 1 ? 1.0 - A.theta[l] : 1.0;
-        draws += (f0 * a != 0.0 && f1 * t != 0.0) ? 1u : 0u;
-    }
-    // draw index of each segment's top locus: the drawing loci above it
-    uint32_t inc = draws;
+    return R;
+}
+
+// sum of v over the threads after this one
+__device__ __forceinline__ uint32_t sum_above(uint32_t v, uint32_t* wcnt) {
+    const int lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t inc = v;
     for (int d = 1; d < 64; d <<= 1) {
         const uint32_t o = __shfl_down(inc, d);
         if (lane + d < 64) inc += o;
     }
     if (lane == 0) wcnt[wave] = inc;
     __syncthreads();
-    uint32_t cnt = inc - draws;
-    for (int k = wave + 1; k < nw; ++k) cnt += wcnt[k];
-    // per-locus decision maps, top of the segment downwards
-    uint32_t comp = 2u;  // identity
-    for (int l = a1 - 1; l >= a0; --l) {
-        const double f0 = fb[2 * l], f1 = fb[2 * l + 1];
-        double w0[2], w1[2];
-        if (l < M - 1) {
-            const double th = A.theta[l], an = 1.0 - A.theta[l];
-            w0[0] = f0 * an;
-            w1[0] = f1 * th;
-            w0[1] = f0 * th;
-            w1[1] = f1 * an;
-        } else {
-            w0[0] = w0[1] = f0;
-            w1[0] = w1[1] = f1;
-        }
-        // with theta in (0, 1) whether a locus draws does not depend on the next
-        // indicator; a difference (underflow in one branch only) is counted
-        const bool draw = w0[0] != 0.0 && w1[0] != 0.0;
-        if (draw != (w0[1] != 0.0 && w1[1] != 0.0)) ++near;
-        const double u = draw ? philox_uniform(A.seed, (uint32_t)chain, A.sweep, (uint32_t)A.step, cnt) : 0.0;
-        if (draw) ++cnt;
-        uint32_t map = 0;
-#pragma unroll
-        for (int bn = 0; bn < 2; ++bn) {
-            uint32_t b, edge = 0;
-            if (w0[bn] == 0.0) {
-                b = 1u;
-            } else if (w1[bn] == 0.0) {
-                b = 0u;
-            } else {
-                const double thr = w0[bn] / (w0[bn] + w1[bn]);
-                b = u < thr ? 0u : 1u;
-                edge = fabs(u - thr) < 1e-12 ? 1u : 0u;
-            }
-            map |= (b << bn) | (edge << (2 + bn));
-        }
-        mp[l] = (uint8_t)map;
-        comp = compose(map & 3u, comp);
-    }
-    // indicator entering each segment from above: suffix composition of maps
+    uint32_t r = inc - v;
+    for (int k = wave + 1; k < nw; ++k) r += wcnt[k];
+    return r;
+}
+
+// indicator entering this thread's segment from above: the composition of the
+// segment maps of every later thread applied to anything (the map of locus M-1
+// is constant, so the composite is too)
+__device__ __forceinline__ uint32_t bit_from_above(uint32_t comp, uint32_t* wmap) {
+    const int lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     uint32_t sc = comp;
     for (int d = 1; d < 64; d <<= 1) {
         const uint32_t o = __shfl_down(sc, d);
@@ -403,21 +440,177 @@ __global__ void __launch_bounds__(kFbBlock) mfb_kernel(MsampArgs A) {
     for (int k = nw - 1; k > wave; --k) up = compose(wmap[k], up);
     uint32_t nxt = __shfl_down(sc, 1);
     if (lane == 63) nxt = 2u;
-    // the map of locus M-1 is constant, so the composite above is too
-    uint32_t next = compose(nxt, up) & 1u;
-    for (int l = a1 - 1; l >= a0; --l) {
-        const uint32_t map = mp[l];
-        const uint32_t nb = (map >> next) & 1u;
-        near += (map >> (2 + next)) & 1u;
-        uint32_t* wp = g + (size_t)l * A.W + word;
-        const uint32_t cur = (*wp & bm) ? 1u : 0u;
-        if (nb != cur) {
-            *wp ^= bm;
-            lc[l] = lf[l];
+    return compose(nxt, up) & 1u;
+}
+
+// Forward-backward pass of step k on one chain by the whole workgroup; thread
+// t owns a contiguous segment of loci.  Scans run within wavefronts by lane
+// shuffles and across them through LDS.  SM > 0: segments of at most SM loci
+// whose likelihoods, forward probabilities and maps stay in registers; SM = 0:
+// any length, through the chain's global scratch.
+template <int SM>
+__device__ void forward_backward(const MsampArgs& A, int chain, int k, Mat* wtot, uint32_t* wcnt, uint32_t* wmap) {
+    const int B = blockDim.x, tid = threadIdx.x, M = A.M;
+    const int m = order_at(A, chain, k);
+    StepView sv;
+    sv.A = &A;
+    sv.chain = chain;
+    sv.k = k;
+    sv.word = m >> 5;
+    sv.bm = 1u << (m & 31);
+    sv.spec = A.spec && k > 0;
+    const int pm = sv.spec ? order_at(A, chain, k - 1) : 0;
+    sv.pword = pm >> 5;
+    sv.pbm = 1u << (pm & 31);
+    const int word = sv.word;
+    const uint32_t bm = sv.bm;
+    uint32_t* g = A.graphs + (size_t)chain * M * A.W;
+    double* lc = A.lcur + (size_t)chain * M;
+    const int S = (M + B - 1) / B;
+    const int a0 = min(M, tid * S), a1 = min(M, a0 + S);
+    uint32_t near = 0;
+    if constexpr (SM > 0) {
+        double e0[SM], e1[SM], lfv[SM], f0[SM], f1[SM];
+        uint32_t curm = 0, maps = 0;
+#pragma unroll
+        for (int s = 0; s < SM; ++s) {
+            const int l = a0 + s;
+            e0[s] = e1[s] = lfv[s] = 0.0;
+            if (l < a1) {
+                const bool cur = (g[(size_t)l * A.W + word] & bm) != 0;
+                lfv[s] = sv.lf(g, l);
+                emissions(cur, lc[l], lfv[s], &e0[s], &e1[s]);
+                curm |= (cur ? 1u : 0u) << s;
+            }
         }
-        next = nb;
+        Mat P{1.0, 0.0, 0.0, 1.0};
+#pragma unroll
+        for (int s = 0; s < SM; ++s)
+            if (a0 + s < a1) P = mmul(P, emission_matrix(A, e0[s], e1[s], a0 + s));
+        Mat R = scan_products(P, wtot);
+        uint32_t draws = 0;
+#pragma unroll
+        for (int s = 0; s < SM; ++s) {
+            f0[s] = f1[s] = 0.0;
+            if (a0 + s < a1) {
+                R = mmul(R, emission_matrix(A, e0[s], e1[s], a0 + s));
+                f0[s] = R.a + R.c;
+                f1[s] = R.b + R.d;
+                draws += locus_draws(A, f0[s], f1[s], a0 + s);
+            }
+        }
+        uint32_t cnt = sum_above(draws, wcnt);
+        uint32_t comp = 2u;  // identity
+#pragma unroll
+        for (int s = SM - 1; s >= 0; --s) {
+            if (a0 + s < a1) {
+                const uint32_t map = locus_map(A, chain, k, f0[s], f1[s], a0 + s, &cnt, &near);
+                maps |= map << (4 * s);
+                comp = compose(map & 3u, comp);
+            }
+        }
+        uint32_t next = bit_from_above(comp, wmap);
+#pragma unroll
+        for (int s = SM - 1; s >= 0; --s) {
+            const int l = a0 + s;
+            if (l < a1) {
+                const uint32_t map = (maps >> (4 * s)) & 15u;
+                const uint32_t nb = (map >> next) & 1u;
+                near += (map >> (2 + next)) & 1u;
+                if (nb != ((curm >> s) & 1u)) {
+                    g[(size_t)l * A.W + word] ^= bm;
+                    lc[l] = lfv[s];
+                }
+                next = nb;
+            }
+        }
+    } else {
+        double* fb = A.fbuf + (size_t)chain * M * 2;
+        uint8_t* mp = A.maps + (size_t)chain * M;
+        auto locus = [&](int l) {
+            double e0, e1;
+            emissions((g[(size_t)l * A.W + word] & bm) != 0, lc[l], sv.lf(g, l), &e0, &e1);
+            return emission_matrix(A, e0, e1, l);
+        };
+        Mat P{1.0, 0.0, 0.0, 1.0};
+        for (int l = a0; l < a1; ++l) P = mmul(P, locus(l));
+        Mat R = scan_products(P, wtot);
+        uint32_t draws = 0;
+        for (int l = a0; l < a1; ++l) {
+            R = mmul(R, locus(l));
+            fb[2 * l] = R.a + R.c;
+            fb[2 * l + 1] = R.b + R.d;
+            draws += locus_draws(A, R.a + R.c, R.b + R.d, l);
+        }
+        uint32_t cnt = sum_above(draws, wcnt);
+        uint32_t comp = 2u;
+        for (int l = a1 - 1; l >= a0; --l) {
+            const uint32_t map = locus_map(A, chain, k, fb[2 * l], fb[2 * l + 1], l, &cnt, &near);
+            mp[l] = (uint8_t)map;
+            comp = compose(map & 3u, comp);
+        }
+        uint32_t next = bit_from_above(comp, wmap);
+        for (int l = a1 - 1; l >= a0; --l) {
+            const uint32_t map = mp[l];
+            const uint32_t nb = (map >> next) & 1u;
+            near += (map >> (2 + next)) & 1u;
+            uint32_t* wp = g + (size_t)l * A.W + word;
+            if (nb != ((*wp & bm) ? 1u : 0u)) {
+                lc[l] = sv.lf(g, l);
+                *wp ^= bm;
+            }
+            next = nb;
+        }
     }
     if (near) atomicAdd(&A.near_edge[chain], near);
+}
+
+// status and sweep kind, read once per workgroup
+__device__ __forceinline__ bool skip_chain(const MsampArgs& A, int chain, int* s_skip) {
+    if (threadIdx.x == 0) *s_skip = A.status[chain] != 0 || !is_msweep(A, chain);
+    __syncthreads();
+    return *s_skip != 0;
+}
+
+__device__ __forceinline__ void run_jobs(const MsampArgs& A, int chain, int block, unsigned char* smem) {
+    const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
+    // person table of the label pass, staged once per workgroup
+    uint2* ptab = reinterpret_cast<uint2*>(smem);
+    for (int k = threadIdx.x; k < A.N - A.F; k += blockDim.x) ptab[k] = A.ptab[k];
+    __syncthreads();
+    const int q = block * nw + wave;
+    if (q < A.n_jobs) fag_job(A, chain, q, ptab, smem + A.ptab_bytes + (size_t)wave * A.wave_bytes);
+}
+
+// founder-allele likelihood jobs only; grid (jobs / waves, chains)
+__global__ void __launch_bounds__(kMsampBlock) mstep_kernel(MsampArgs A) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_skip;
+    if (skip_chain(A, blockIdx.y, &s_skip)) return;
+    run_jobs(A, blockIdx.y, blockIdx.x, smem);
+}
+
+// forward-backward pass of step fb_step only; one workgroup per chain
+template <int SM>
+__global__ void __launch_bounds__(kFbBlock) mfb_kernel(MsampArgs A) {
+    __shared__ Mat wtot[kFbBlock / 64];
+    __shared__ uint32_t wcnt[kFbBlock / 64], wmap[kFbBlock / 64];
+    __shared__ int s_skip;
+    if (skip_chain(A, blockIdx.x, &s_skip)) return;
+    forward_backward<SM>(A, blockIdx.x, A.fb_step, wtot, wcnt, wmap);
+}
+
+// one speculative launch: workgroup 0 of each chain runs step fb_step's
+// forward-backward pass while the others run the jobs of step fb_step + 1
+template <int SM>
+__global__ void __launch_bounds__(kMsampBlock) mspec_kernel(MsampArgs A) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ Mat wtot[kMsampBlock / 64];
+    __shared__ uint32_t wcnt[kMsampBlock / 64], wmap[kMsampBlock / 64];
+    __shared__ int s_skip;
+    if (skip_chain(A, blockIdx.y, &s_skip)) return;
+    if (blockIdx.x == 0) forward_backward<SM>(A, blockIdx.y, A.fb_step, wtot, wcnt, wmap);
+    else run_jobs(A, blockIdx.y, blockIdx.x - 1, smem);
 }
 
 // this sweep's meiosis order: the given order, or a Fisher-Yates shuffle of the
@@ -474,7 +667,14 @@ size_t msampler_lds_bytes(const MsampArgs& A, int block) {
 
 hipError_t configure_msampler_lds(size_t lds) {
     if (lds <= 64 * 1024) return hipSuccess;
-    return hipFuncSetAttribute((const void*)mstep_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int b = (int)lds;
+    hipError_t e = hipFuncSetAttribute((const void*)mstep_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<4>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<8>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    return e;
 }
 
 hipError_t launch_morder(const MsampArgs& A, const int32_t* fixed_order, hipStream_t s) {
@@ -483,14 +683,33 @@ hipError_t launch_morder(const MsampArgs& A, const int32_t* fixed_order, hipStre
     return hipGetLastError();
 }
 
-hipError_t launch_mstep(const MsampArgs& A, int block, size_t lds, hipStream_t s) {
-    const dim3 grid((A.M + block / 64 - 1) / (block / 64), A.n_chains);
-    hipLaunchKernelGGL(mstep_kernel, grid, dim3(block), lds, s, A);
+hipError_t launch_mstep(const MsampArgs& A, size_t lds, hipStream_t s) {
+    const int nw = kMsampBlock / 64;
+    const dim3 grid((A.n_jobs + nw - 1) / nw, A.n_chains);
+    hipLaunchKernelGGL(mstep_kernel, grid, dim3(kMsampBlock), lds, s, A);
     return hipGetLastError();
 }
 
 hipError_t launch_mfb(const MsampArgs& A, hipStream_t s) {
-    hipLaunchKernelGGL(mfb_kernel, dim3(A.n_chains), dim3(kFbBlock), 0, s, A);
+    const int seg = (A.M + kFbBlock - 1) / kFbBlock;
+    const dim3 grid(A.n_chains), block(kFbBlock);
+    if (seg <= 1) hipLaunchKernelGGL(mfb_kernel<1>, grid, block, 0, s, A);
+    else if (seg <= 2) hipLaunchKernelGGL(mfb_kernel<2>, grid, block, 0, s, A);
+    else if (seg <= 4) hipLaunchKernelGGL(mfb_kernel<4>, grid, block, 0, s, A);
+    else if (seg <= 8) hipLaunchKernelGGL(mfb_kernel<8>, grid, block, 0, s, A);
+    else hipLaunchKernelGGL(mfb_kernel<0>, grid, block, 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_mspec(const MsampArgs& A, size_t lds, hipStream_t s) {
+    const int nw = kMsampBlock / 64;
+    const int seg = (A.M + kMsampBlock - 1) / kMsampBlock;
+    const dim3 grid(1 + (A.n_jobs + nw - 1) / nw, A.n_chains), block(kMsampBlock);
+    if (seg <= 1) hipLaunchKernelGGL(mspec_kernel<1>, grid, block, lds, s, A);
+    else if (seg <= 2) hipLaunchKernelGGL(mspec_kernel<2>, grid, block, lds, s, A);
+    else if (seg <= 4) hipLaunchKernelGGL(mspec_kernel<4>, grid, block, lds, s, A);
+    else if (seg <= 8) hipLaunchKernelGGL(mspec_kernel<8>, grid, block, lds, s, A);
+    else hipLaunchKernelGGL(mspec_kernel<0>, grid, block, lds, s, A);
     return hipGetLastError();
 }
 

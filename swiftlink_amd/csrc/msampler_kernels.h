@@ -30,16 +30,19 @@ struct MsampArgs {
     int n_meioses;                // meioses sampled by this sweep
     int32_t* order;               // [chains][n_meioses] this sweep's meiosis order
     uint32_t* graphs;             // [chains][M][W]
-    double* lcur;                 // [chains][M] likelihood of the current column
-    double* lflip;                // [chains][M] likelihood with the step's meiosis flipped
+    double* lcur;                 // [chains][M] log-likelihood of the current column
+    double* lflip;                // [2 step parities][2 variants][chains][M] flipped-column log-likelihoods
     double* fbuf;                 // [chains][M][2] forward probabilities
     uint8_t* maps;                // [chains][M] per-locus backward decision maps
     int32_t* status;              // [chains]
     uint32_t* near_edge;          // [chains] draws within 1e-12 of their threshold
     uint64_t seed;
     uint32_t sweep;
-    int step;                     // meiosis step j (mode 1)
-    int mode;                     // 0: likelihood of the current columns; 1: one meiosis step
+    int step;                     // meiosis step whose likelihood jobs this launch runs
+    int fb_step;                  // meiosis step whose forward-backward pass this launch runs
+    int n_jobs;                   // likelihood jobs (wavefronts) per chain in this launch
+    int with_lcur;                // 1: jobs [0, M) are the current columns
+    int spec;                     // 1: jobs come in two variants of the previous meiosis's bit
     int check_kind;               // 1: act only on chains whose sweep `sweep` is an M-sweep
     double lprob;                 // L-sweep probability of the sweep schedule
     int wave_bytes;               // LDS bytes per wavefront
@@ -52,7 +55,8 @@ size_t msampler_ptab_bytes(int N, int F);
 size_t msampler_lds_bytes(const MsampArgs& A, int block);
 hipError_t configure_msampler_lds(size_t lds);
 hipError_t launch_morder(const MsampArgs& A, const int32_t* fixed_order, hipStream_t s);
-hipError_t launch_mstep(const MsampArgs& A, int block, size_t lds, hipStream_t s);
+hipError_t launch_mstep(const MsampArgs& A, size_t lds, hipStream_t s);
 hipError_t launch_mfb(const MsampArgs& A, hipStream_t s);
+hipError_t launch_mspec(const MsampArgs& A, size_t lds, hipStream_t s);
 
 }  // namespace sl

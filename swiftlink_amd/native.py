@@ -19,6 +19,7 @@ LIBPATH = os.path.join(HERE, "libswiftlink_amd.so")
 
 SL_FP64 = 0
 SL_FP32_RESCALED = 1
+SL_MSAMPLER_AUTO, SL_MSAMPLER_SPECULATIVE, SL_MSAMPLER_TWO_LAUNCH = 0, 1, 2
 SL_KERNEL_LOD, SL_KERNEL_LSAMPLER, SL_KERNEL_MSAMPLER, SL_KERNEL_MSAMPLER_FB = 0, 1, 2, 3
 SL_OK, SL_E_INVALID, SL_E_UNSUPPORTED, SL_E_DEVICE = 0, -1, -2, -3
 SL_E_ZERO_LIKELIHOOD, SL_E_ILLEGAL_GRAPH, SL_E_CAPACITY = -4, -5, -6
@@ -48,7 +49,7 @@ class PeelDesc(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("device", C.c_int32), ("block_threads", C.c_int32),
-                ("graphs_per_block", C.c_int32)]
+                ("graphs_per_block", C.c_int32), ("msampler_mode", C.c_int32)]
 
 
 class Schedule(C.Structure):
@@ -158,10 +159,11 @@ def program_export(case: Case, npos: Optional[int] = None) -> np.ndarray:
 class Context:
     """One sl_ctx: a pedigree, its map and peel program on one GPU."""
 
-    def __init__(self, case: Case, precision: int = SL_FP64, device: int = 0, block: int = 0, gpw: int = 0):
+    def __init__(self, case: Case, precision: int = SL_FP64, device: int = 0, block: int = 0, gpw: int = 0,
+                 msampler_mode: int = SL_MSAMPLER_AUTO):
         self.case = case
         self._d = _Descs(case)
-        opts = Options(precision, device, block, gpw)
+        opts = Options(precision, device, block, gpw, msampler_mode)
         h = C.c_void_p()
         rc = lib().sl_create(C.byref(self._d.ped), C.byref(self._d.map), C.byref(self._d.peel), C.byref(opts),
                              C.byref(h))

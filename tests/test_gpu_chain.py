@@ -39,13 +39,14 @@ def test_chain_fp64_matches_reference(name):
     np.testing.assert_allclose(acc, fx["chain_acc"], rtol=0, atol=ACC_TOL)
 
 
+@pytest.mark.parametrize("mode", [native.SL_MSAMPLER_SPECULATIVE, native.SL_MSAMPLER_TWO_LAUNCH])
 @pytest.mark.parametrize("name", GOLDEN_CASES)
-def test_chains_merge_like_replicates(name):
+def test_chains_merge_like_replicates(name, mode):
     case, fx = load_golden(name)
     o = Oracle(case)
     starts = np.stack([fx["graph_init"], fx["graphs"][0], fx["graphs"][1], fx["graph_init"]])
     sweeps, burnin, period, lprob, seed = 24, 6, 3, 0.5, 31
-    with native.Context(case, native.SL_FP64) as ctx:
+    with native.Context(case, native.SL_FP64, msampler_mode=mode) as ctx:
         ctx.mcmc_init(starts)
         ctx.lod_reset()
         st = ctx.mcmc_run(sweeps, burnin, period, lprob, seed)

@@ -33,12 +33,16 @@ def shuffled_order(o, seed, chain, sweep):
     return np.array(mo, np.int32)
 
 
+MODES = [native.SL_MSAMPLER_SPECULATIVE, native.SL_MSAMPLER_TWO_LAUNCH]
+
+
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("name", GOLDEN_CASES)
-def test_msweep_fp64_matches_reference(name):
+def test_msweep_fp64_matches_reference(name, mode):
     case, fx = load_golden(name)
     seed, chain, sweep = (int(x) for x in fx["msweep_seed"])
     assert chain == 0
-    with native.Context(case, native.SL_FP64) as ctx:
+    with native.Context(case, native.SL_FP64, msampler_mode=mode) as ctx:
         ctx.mcmc_init(fx["graph_init"][None])
         ctx.msweep(seed, sweep, fx["msweep_order"])
         out = ctx.mcmc_graphs()[0]
@@ -47,12 +51,13 @@ def test_msweep_fp64_matches_reference(name):
     assert np.array_equal(out, fx["msweep_out"])
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("name", GOLDEN_CASES)
-def test_msweep_shuffled_chains_match_oracle(name):
+def test_msweep_shuffled_chains_match_oracle(name, mode):
     case, fx = load_golden(name)
     o = Oracle(case)
     starts = np.stack([fx["graph_init"], fx["graphs"][0], fx["graphs"][1]])
-    with native.Context(case, native.SL_FP64) as ctx:
+    with native.Context(case, native.SL_FP64, msampler_mode=mode) as ctx:
         ctx.mcmc_init(starts)
         ctx.msweep(4321, 7)
         ctx.msweep(4321, 8)
@@ -169,9 +174,10 @@ def ped150(tmp_path_factory):
     return case, synth.true_graph(data, case), Oracle(case)
 
 
-def test_msweep_fp64_ped150(ped150):
+@pytest.mark.parametrize("mode", MODES)
+def test_msweep_fp64_ped150(ped150, mode):
     case, g0, o = ped150
-    with native.Context(case, native.SL_FP64) as ctx:
+    with native.Context(case, native.SL_FP64, msampler_mode=mode) as ctx:
         ctx.mcmc_init(g0[None])
         lik = ctx.mcmc_likelihood()[0]
         ctx.msweep(2024, 1)
