@@ -68,7 +68,7 @@ struct sl_ctx {
     int arena_bytes = 0, maxbits_bytes = 0;
     // sampler program and chain state
     sl::SampProgram sprog;
-    int samp_block = 128;
+    int samp_block = 256;  // L-sampler workgroup: 4 wavefronts share each peel level
     int cells_cap = 0;
     size_t samp_lds = 0;
     sl::SampOp* d_sops = nullptr;

@@ -139,7 +139,8 @@ __global__ void lsampler_kernel(SampArgs A) {
     T* tab = reinterpret_cast<T*>(smem);
     T* pre = tab + A.cells_cap;
     T* dist = pre + 4 * (size_t)A.cells_cap;           // [NF][8]
-    double* uop = reinterpret_cast<double*>(dist + 8 * NF);  // [n_ops]
+    T* trt = dist + 8 * NF;                             // [N][4] trait probabilities at the locus
+    double* uop = reinterpret_cast<double*>(trt + 4 * N);  // [n_ops]
     int* off = reinterpret_cast<int*>(uop + n_ops);     // [n_ops+1]
     int* scan_tmp = off + n_ops + 1;                    // [blockDim]
     int* mcount = scan_tmp + blockDim.x;                // [2NF] draw slots of meioses
@@ -203,6 +204,7 @@ __global__ void lsampler_kernel(SampArgs A) {
     }
     for (int o = tid; o < n_ops; o += blockDim.x)
         uop[o] = philox_uniform(A.seed, (uint32_t)chain, A.sweep, (uint32_t)locus, (uint32_t)(n_ops - 1 - o));
+    for (int i = tid; i < 4 * N; i += blockDim.x) trt[i] = trait_at<T>(A, i >> 2, locus, i & 3);
     __syncthreads();
     const int total_cells = block_exclusive_scan(off, n_ops, scan_tmp);
     if (tid == 0) off[n_ops] = total_cells;
@@ -241,7 +243,7 @@ __global__ void lsampler_kernel(SampArgs A) {
             cmw |= pmask << (4 * k);
             T tr[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) tr[v] = trait_at<T>(A, peel, locus, v);
+            for (int v = 0; v < 4; ++v) tr[v] = trt[4 * peel + v];
             const int my_off = uni(off[op]);
             // consumed tables: offset, per-digit source and stride at this locus
             int p_off[kMaxPrev], p_nd[kMaxPrev];
@@ -278,11 +280,20 @@ __global__ void lsampler_kernel(SampArgs A) {
             int kids[7], kid_digit[7], other_digit[7], kid_pim[7];
 #pragma unroll
             for (int c = 0; c < 7; ++c) {
-                cint_p cd = (cint_p)(A.children + child_off + (c < n_child ? c : 0));
-                kids[c] = c < n_child ? cd[0] : F;
-                kid_digit[c] = c < n_child ? cd[1] : 0;
-                other_digit[c] = c < n_child ? cd[2] : 0;
-                kid_pim[c] = c < n_child ? cd[3] : 0;
+                kids[c] = F;
+                kid_digit[c] = other_digit[c] = kid_pim[c] = 0;
+            }
+            if (type == kParent) {  // child descriptors only where the op has children
+#pragma unroll
+                for (int c = 0; c < 7; ++c) {
+                    if (c < n_child) {
+                        cint_p cd = (cint_p)(A.children + child_off + c);
+                        kids[c] = cd[0];
+                        kid_digit[c] = cd[1];
+                        other_digit[c] = cd[2];
+                        kid_pim[c] = cd[3];
+                    }
+                }
             }
             float mx = 0.0f;
             for (int c = lane; c < size; c += 64) {
@@ -449,7 +460,7 @@ __global__ void lsampler_kernel(SampArgs A) {
 size_t lsampler_lds_bytes(int precision, int cells_cap, int n_ops, int N, int F, int block) {
     const size_t esz = precision == 0 ? 8 : 4;
     const int NF = N - F;
-    size_t b = esz * (size_t)cells_cap * 5 + esz * 8 * (size_t)NF;
+    size_t b = esz * (size_t)cells_cap * 5 + esz * 8 * (size_t)NF + esz * 4 * (size_t)N;
     b = (b + 7) / 8 * 8;
     b += 8 * (size_t)n_ops;                          // uop
     b += 4 * ((size_t)n_ops + 1 + block + 2 * NF);   // off, scan, mcount
