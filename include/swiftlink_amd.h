@@ -84,10 +84,14 @@ typedef struct {
 /* M-sweep launch modes (results are identical): one launch per meiosis step,
  * computing the next step's likelihoods for both values of the current
  * meiosis's bit while its forward-backward pass runs; or two launches per
- * step; or the first while the doubled likelihood work is small. */
+ * step; or the first while the doubled likelihood work is small.
+ * SL_MSAMPLER_LDS_GRAPHS may be or-ed in: every founder-allele graph is then
+ * built in LDS, as it always is for pedigrees of more than 512 people (smaller
+ * ones keep a wavefront's graph in registers); results are identical. */
 #define SL_MSAMPLER_AUTO 0
 #define SL_MSAMPLER_SPECULATIVE 1
 #define SL_MSAMPLER_TWO_LAUNCH 2
+#define SL_MSAMPLER_LDS_GRAPHS 4
 
 /* Replaces GPULodscores::GPULodscores + gpu_init (gpu_lodscores.h:63-80,
  * gpu_lodscores.cc:119-161): validates and copies the pedigree, map and peel

@@ -100,7 +100,6 @@ struct sl_ctx {
     std::vector<int32_t> meioses_h;
     int n_gen = 0;
     int32_t* d_gen_off = nullptr;
-    int32_t* d_gen_persons = nullptr;
     uint2* d_ptab = nullptr;
     int m_ptab_bytes = 0;
     uint8_t* d_geno_t = nullptr;
@@ -237,7 +236,7 @@ int release_all(sl_ctx* c) {
     void* ptrs[] = {c->d_sops, c->d_sprevs, c->d_schildren, c->d_ssched_off, c->d_ssched_ops, c->d_rlevel_off,
                     c->d_rlevel_ops, c->d_mother, c->d_father, c->d_legal_t, c->d_gprob_d, c->d_gprob_f,
                     c->d_theta, c->d_chains, c->d_status,
-                    c->d_gen_off, c->d_gen_persons, c->d_ptab, c->d_geno_t, c->d_minor, c->d_meioses, c->d_order,
+                    c->d_gen_off, c->d_ptab, c->d_geno_t, c->d_minor, c->d_meioses, c->d_order,
                     c->d_fixed_order, c->d_lcur, c->d_lflip, c->d_fbuf, c->d_maps, c->d_near,
                     c->d_ops, c->d_prevs, c->d_children, c->d_tasks, c->d_tinfo, c->d_idx, c->d_level_base,
                     c->d_sched_off,
@@ -292,7 +291,8 @@ int setup_msampler(sl_ctx* c, const sl_pedigree_desc* ped) {
     c->n_gen = maxd;
     std::vector<uint2> ptab;
     for (int i : gen_persons)
-        ptab.push_back(make_uint2((uint32_t)i, (uint32_t)c->mother_h[i] | ((uint32_t)c->father_h[i] << 16)));
+        ptab.push_back(make_uint2((uint32_t)i | ((uint32_t)(depth[i] - 1) << 16),
+                                  (uint32_t)c->mother_h[i] | ((uint32_t)c->father_h[i] << 16)));
     // non-ignorable meioses (markov_chain.cc:65-80): a meiosis from a founder
     // parent with one child is skipped (safe_to_ignore_meiosis, person.cc:208-222)
     std::vector<int> kids(N, 0);
@@ -327,7 +327,6 @@ int setup_msampler(sl_ctx* c, const sl_pedigree_desc* ped) {
     SL_HIP(c, sl::configure_msampler_lds(c->m_lds));
     int rc;
     if ((rc = upload(c, &c->d_gen_off, gen_off.data(), gen_off.size())) ||
-        (rc = upload(c, &c->d_gen_persons, gen_persons.data(), gen_persons.size())) ||
         (rc = upload(c, &c->d_ptab, ptab.data(), ptab.size())) ||
         (rc = upload(c, &c->d_geno_t, gt.data(), gt.size())) ||
         (rc = upload(c, &c->d_minor, c->minor.data(), c->minor.size())) ||
@@ -388,7 +387,7 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
     }
     if (opts && opts->graphs_per_block > 0) c->gpw = opts->graphs_per_block;
     if (opts) c->m_mode = opts->msampler_mode;
-    if (c->m_mode < SL_MSAMPLER_AUTO || c->m_mode > SL_MSAMPLER_TWO_LAUNCH) {
+    if ((c->m_mode & ~(3 | SL_MSAMPLER_LDS_GRAPHS)) != 0 || (c->m_mode & 3) > SL_MSAMPLER_TWO_LAUNCH) {
         delete c;
         return fail(nullptr, SL_E_INVALID, "unknown msampler_mode");
     }
@@ -823,7 +822,6 @@ static sl::MsampArgs msamp_args(sl_ctx* c) {
     A.mother = c->d_mother;
     A.father = c->d_father;
     A.gen_off = c->d_gen_off;
-    A.gen_persons = c->d_gen_persons;
     A.n_gen = c->n_gen;
     A.geno_t = c->d_geno_t;
     A.minor = c->d_minor;
@@ -839,6 +837,7 @@ static sl::MsampArgs msamp_args(sl_ctx* c) {
     A.status = c->d_status;
     A.near_edge = c->d_near;
     A.wave_bytes = c->m_wave_bytes;
+    A.people_chunks = (c->m_mode & SL_MSAMPLER_LDS_GRAPHS) ? 0 : sl::msampler_people_chunks(c->N);
     A.ptab = c->d_ptab;
     A.ptab_bytes = c->m_ptab_bytes;
     A.n_chains = c->n_chains;
@@ -860,8 +859,9 @@ static int run_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* d
     SL_HIP(c, sl::launch_morder(A, dev_order, c->stream));
     // one launch per step when the doubled likelihood work still fits the
     // device in one wave of workgroups (few chains), two launches otherwise
-    const bool spec = c->m_mode == SL_MSAMPLER_SPECULATIVE ||
-                      (c->m_mode == SL_MSAMPLER_AUTO && (int64_t)c->n_chains * c->M * 2 <= kSpecWaves);
+    const int launch_mode = c->m_mode & 3;
+    const bool spec = launch_mode == SL_MSAMPLER_SPECULATIVE ||
+                      (launch_mode == SL_MSAMPLER_AUTO && (int64_t)c->n_chains * c->M * 2 <= kSpecWaves);
     A.spec = spec ? 1 : 0;
     // start of the sweep: the current columns and step 0's flipped columns
     A.with_lcur = 1;

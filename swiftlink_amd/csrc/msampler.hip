@@ -117,7 +117,7 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
         const int k1 = A.gen_off[gen + 1];
         for (int k = A.gen_off[gen] + lane; k < k1; k += 64) {
             const uint2 e = ptab[k];
-            const int i = (int)e.x, mo = (int)(e.y & 0xFFFFu), fa = (int)(e.y >> 16);
+            const int i = (int)(e.x & 0xFFFFu), mo = (int)(e.y & 0xFFFFu), fa = (int)(e.y >> 16);
             const int b = 2 * (i - F);
             const uint32_t bm = (w.colw[b >> 5] >> (b & 31)) & 1u;
             const uint32_t bp = (w.colw[(b + 1) >> 5] >> ((b + 1) & 31)) & 1u;
@@ -232,6 +232,184 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
     return ll;
 }
 
+// The same likelihood for pedigrees of at most 64*KC people, with the wave's
+// working set in registers: lane j holds people j, j+64, ... (genotype, edge,
+// the union-find words of the edge's two nodes) and slots j, j+64, ... of the
+// generation-ordered person table.  A round reads the union-find words of
+// every edge before it writes any hook, so after a compression every find is
+// a single LDS read, and the first round that hooks nothing has read exactly
+// what the consistency check needs.  The per-root terms and their order of
+// summation are those of fag_log_likelihood, so the result is the same.
+template <int KC>
+__device__ double fag_log_likelihood_r(const MsampArgs& A, const uint2* ptab, const WaveArea& w, int l) {
+    constexpr int KN = 2 * KC;  // node chunks (2F <= 2N)
+    const int lane = lane_id();
+    const int N = A.N, F = A.F, NF = A.N - A.F, nodes = 2 * A.F;
+    const uint8_t* gcol = A.geno_t + (size_t)l * N;
+    int g[KC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+        const int i = 64 * c + lane;
+        g[c] = i < N ? gcol[i] : 0;
+    }
+    const double minor = A.minor[l];
+#pragma unroll
+    for (int n = 0; n < KN; ++n) {
+        const int x = 64 * n + lane;
+        if (x < nodes) {
+            w.uf[x] = (uint32_t)x;
+            w.acc[x] = 0u;
+        }
+    }
+    // label slots (reset, founder_allele_graph4.cc:543-569): the source of
+    // each of a slot's two labels, a founder allele (bit 31 set) or the label
+    // slot of the parent it came from
+    uint32_t src0[KC], src1[KC];
+    int sgen[KC], sb[KC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+        const int k = 64 * c + lane;
+        sgen[c] = -1;
+        sb[c] = 0;
+        src0[c] = src1[c] = 0u;
+        if (k < NF) {
+            const uint2 e = ptab[k];
+            const int i = (int)(e.x & 0xFFFFu), mo = (int)(e.y & 0xFFFFu), fa = (int)(e.y >> 16);
+            const int b = 2 * (i - F);
+            const uint32_t bm = (w.colw[b >> 5] >> (b & 31)) & 1u;
+            const uint32_t bp = (w.colw[(b + 1) >> 5] >> ((b + 1) & 31)) & 1u;
+            sgen[c] = (int)(e.x >> 16);
+            sb[c] = b;
+            src0[c] = (mo < F ? (0x80000000u | (uint32_t)(2 * mo)) : (uint32_t)(2 * (mo - F))) | bm;
+            src1[c] = (fa < F ? (0x80000000u | (uint32_t)(2 * fa)) : (uint32_t)(2 * (fa - F))) | bp;
+        }
+    }
+    for (int gen = 0; gen < A.n_gen; ++gen) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            if (sgen[c] == gen) {
+                const uint32_t l0 = (src0[c] >> 31) ? (src0[c] & 0xFFFFu) : (uint32_t)w.lab[src0[c]];
+                const uint32_t l1 = (src1[c] >> 31) ? (src1[c] & 0xFFFFu) : (uint32_t)w.lab[src1[c]];
+                *reinterpret_cast<uint32_t*>(w.lab + sb[c]) = l0 | (l1 << 16);
+            }
+        }
+        wave_sync();
+    }
+    // edges of the typed people; the founder alleles they touch are marked
+    uint32_t e[KC];
+    bool bad = false;
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+        const int i = 64 * c + lane;
+        e[c] = 0u;
+        if (g[c]) {
+            uint32_t x = (uint32_t)(2 * i), y = (uint32_t)(2 * i + 1);
+            if (i >= F) {
+                const uint32_t xy = *reinterpret_cast<const uint32_t*>(w.lab + 2 * (i - F));
+                x = xy & 0xFFFFu;
+                y = xy >> 16;
+            }
+            const uint32_t het = g[c] == G_HETERO ? 1u : 0u;
+            bad |= het && x == y;  // an autozygous heterozygote (:77-81)
+            e[c] = x | (y << 10) | (het << 20) | ((g[c] == G_HOMOZ_B ? 1u : 0u) << 21) | (1u << 22);
+            atomicOr(&w.acc[x], 4u);
+            atomicOr(&w.acc[y], 4u);
+        }
+    }
+    if (__ballot(bad)) return -INFINITY;
+    // rounds of hooking and compression until a round hooks nothing (each
+    // hooking round removes a component, so at most 2F rounds hook)
+    uint32_t ux[KC], uy[KC];
+    bool converged = false;
+    for (int round = 0; round <= nodes; ++round) {
+        wave_sync();
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            ux[c] = uy[c] = 0u;
+            if (e[c]) {
+                ux[c] = w.uf[e[c] & 0x3FFu];
+                uy[c] = w.uf[(e[c] >> 10) & 0x3FFu];
+            }
+        }
+        wave_sync();
+        bool hooked = false;
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            const uint32_t rx = ux[c] & 0xFFFFu, ry = uy[c] & 0xFFFFu;
+            if (e[c] && rx != ry) {
+                const uint32_t par = (((ux[c] ^ uy[c]) >> 16) ^ (e[c] >> 20)) & 1u;
+                w.uf[rx > ry ? rx : ry] = (rx < ry ? rx : ry) | (par << 16);
+                hooked = true;
+            }
+        }
+        if (!__ballot(hooked)) {
+            converged = true;
+            break;
+        }
+        wave_sync();
+        uint32_t r[KN];
+#pragma unroll
+        for (int n = 0; n < KN; ++n) {
+            const int x = 64 * n + lane;
+            r[n] = 0u;
+            if (x < nodes) {
+                uint32_t p;
+                r[n] = uf_find(w.uf, (uint32_t)x, &p) | (p << 16);
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int n = 0; n < KN; ++n) {
+            const int x = 64 * n + lane;
+            if (x < nodes) w.uf[x] = r[n];
+        }
+    }
+    // the last round's words are compressed: check every edge's parity;
+    // homozygous edges fix their root's allele
+    bool illegal = !converged;
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+        if (e[c]) {
+            const uint32_t het = (e[c] >> 20) & 1u, allele = (e[c] >> 21) & 1u, px = (ux[c] >> 16) & 1u;
+            illegal |= (((ux[c] ^ uy[c]) >> 16) & 1u) != het;
+            if (!het) atomicOr(&w.acc[ux[c] & 0xFFFFu], 1u << (allele ^ px));
+        }
+    }
+    if (__ballot(illegal)) return -INFINITY;
+    uint32_t u[KN];
+#pragma unroll
+    for (int n = 0; n < KN; ++n) {
+        const int x = 64 * n + lane;
+        u[n] = 0xFFFFFFFFu;
+        if (x < nodes) {
+            u[n] = w.uf[x];
+            if (w.acc[x] & 4u) atomicAdd(&w.acc[u[n] & 0xFFFFu], ((u[n] >> 16) & 1u) ? (1u << 14) : (1u << 3));
+        }
+    }
+    wave_sync();
+    const double lmaj = log(1.0 - minor), lmin = log(minor);
+    double ll = 0.0;
+    bool conflict = false;
+#pragma unroll
+    for (int n = 0; n < KN; ++n) {
+        const int x = 64 * n + lane;
+        if (x >= nodes || (u[n] & 0xFFFFu) != (uint32_t)x) continue;  // not a root
+        const uint32_t a = w.acc[x];
+        if (!(a & 4u)) continue;  // untouched
+        const double n0 = (double)((a >> 3) & 0x7FFu), n1 = (double)((a >> 14) & 0x7FFu);
+        const double l0 = n0 * lmaj + n1 * lmin;
+        const double l1 = n1 * lmaj + n0 * lmin;
+        const uint32_t need = a & 3u;
+        if (need == 3u) conflict = true;
+        else if (need == 1u) ll += l0;
+        else if (need == 2u) ll += l1;
+        else ll += fmax(l0, l1) + log1p(exp(-fabs(l0 - l1)));
+    }
+    if (__ballot(conflict)) return -INFINITY;
+    for (int off = 32; off > 0; off >>= 1) ll += __shfl_xor(ll, off);
+    return ll;
+}
+
 struct Mat {
     double a, b, c, d;  // [[a, b], [c, d]]
 };
@@ -281,6 +459,7 @@ __device__ __forceinline__ int order_at(const MsampArgs& A, int chain, int k) {
 //    v first forces meiosis m_{k-1}'s bit to v: the forward-backward pass of
 //    step k-1 may be deciding that bit in the same launch, and it is the only
 //    bit it changes.
+template <int KC>
 __device__ void fag_job(const MsampArgs& A, int chain, int q, const uint2* ptab, unsigned char* wbase) {
     const int lane = lane_id(), M = A.M;
     int l, v = 0, flip = -1, force = -1;
@@ -304,7 +483,9 @@ __device__ void fag_job(const MsampArgs& A, int chain, int q, const uint2* ptab,
         w.colw[k] = x;
     }
     wave_sync();
-    const double L = fag_log_likelihood(A, ptab, w, l);
+    double L;
+    if constexpr (KC > 0) L = fag_log_likelihood_r<KC>(A, ptab, w, l);
+    else L = fag_log_likelihood(A, ptab, w, l);
     if (lane == 0) {
         if (current) {
             A.lcur[(size_t)chain * M + l] = L;
@@ -572,6 +753,7 @@ __device__ __forceinline__ bool skip_chain(const MsampArgs& A, int chain, int* s
     return *s_skip != 0;
 }
 
+template <int KC>
 __device__ __forceinline__ void run_jobs(const MsampArgs& A, int chain, int block, unsigned char* smem) {
     const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
     // person table of the label pass, staged once per workgroup
@@ -579,15 +761,17 @@ __device__ __forceinline__ void run_jobs(const MsampArgs& A, int chain, int bloc
     for (int k = threadIdx.x; k < A.N - A.F; k += blockDim.x) ptab[k] = A.ptab[k];
     __syncthreads();
     const int q = block * nw + wave;
-    if (q < A.n_jobs) fag_job(A, chain, q, ptab, smem + A.ptab_bytes + (size_t)wave * A.wave_bytes);
+    if (q < A.n_jobs) fag_job<KC>(A, chain, q, ptab, smem + A.ptab_bytes + (size_t)wave * A.wave_bytes);
 }
 
-// founder-allele likelihood jobs only; grid (jobs / waves, chains)
+// founder-allele likelihood jobs only; grid (jobs / waves, chains).  KC > 0:
+// the register-resident likelihood for at most 64*KC people
+template <int KC>
 __global__ void __launch_bounds__(kMsampBlock) mstep_kernel(MsampArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_skip;
     if (skip_chain(A, blockIdx.y, &s_skip)) return;
-    run_jobs(A, blockIdx.y, blockIdx.x, smem);
+    run_jobs<KC>(A, blockIdx.y, blockIdx.x, smem);
 }
 
 // forward-backward pass of step fb_step only; one workgroup per chain
@@ -602,7 +786,7 @@ __global__ void __launch_bounds__(kFbBlock) mfb_kernel(MsampArgs A) {
 
 // one speculative launch: workgroup 0 of each chain runs step fb_step's
 // forward-backward pass while the others run the jobs of step fb_step + 1
-template <int SM>
+template <int SM, int KC>
 __global__ void __launch_bounds__(kMsampBlock) mspec_kernel(MsampArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ Mat wtot[kMsampBlock / 64];
@@ -610,7 +794,7 @@ __global__ void __launch_bounds__(kMsampBlock) mspec_kernel(MsampArgs A) {
     __shared__ int s_skip;
     if (skip_chain(A, blockIdx.y, &s_skip)) return;
     if (blockIdx.x == 0) forward_backward<SM>(A, blockIdx.y, A.fb_step, wtot, wcnt, wmap);
-    else run_jobs(A, blockIdx.y, blockIdx.x - 1, smem);
+    else run_jobs<KC>(A, blockIdx.y, blockIdx.x - 1, smem);
 }
 
 // this sweep's meiosis order: the given order, or a Fisher-Yates shuffle of the
@@ -654,6 +838,8 @@ __global__ void morder_kernel(MsampArgs A, const int32_t* fixed_order) {
 
 bool msampler_supported(int N, int F) { return F >= 1 && 2 * F <= kMaxFagNodes && N <= 65535; }
 
+int msampler_people_chunks(int N) { return N <= 128 ? 2 : N <= 256 ? 4 : N <= 512 ? 8 : 0; }
+
 size_t msampler_wave_bytes(int N, int F, int W) {
     const size_t b = 4 * (size_t)W + 2 * 4 * (size_t)(2 * F) + 4 * (size_t)N + 2 * 2 * (size_t)(N - F);
     return (b + 15) / 16 * 16;
@@ -665,15 +851,40 @@ size_t msampler_lds_bytes(const MsampArgs& A, int block) {
     return (size_t)A.ptab_bytes + (size_t)(block / 64) * (size_t)A.wave_bytes;
 }
 
+namespace {
+
+template <int KC>
+hipError_t set_lds_kc(int b) {
+    hipError_t e = hipFuncSetAttribute((const void*)mstep_kernel<KC>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<0, KC>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<1, KC>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<2, KC>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<4, KC>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<8, KC>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    return e;
+}
+
+template <int KC>
+void launch_mspec_kc(const MsampArgs& A, size_t lds, hipStream_t s) {
+    const int nw = kMsampBlock / 64;
+    const int seg = (A.M + kMsampBlock - 1) / kMsampBlock;
+    const dim3 grid(1 + (A.n_jobs + nw - 1) / nw, A.n_chains), block(kMsampBlock);
+    if (seg <= 1) hipLaunchKernelGGL((mspec_kernel<1, KC>), grid, block, lds, s, A);
+    else if (seg <= 2) hipLaunchKernelGGL((mspec_kernel<2, KC>), grid, block, lds, s, A);
+    else if (seg <= 4) hipLaunchKernelGGL((mspec_kernel<4, KC>), grid, block, lds, s, A);
+    else if (seg <= 8) hipLaunchKernelGGL((mspec_kernel<8, KC>), grid, block, lds, s, A);
+    else hipLaunchKernelGGL((mspec_kernel<0, KC>), grid, block, lds, s, A);
+}
+
+}  // namespace
+
 hipError_t configure_msampler_lds(size_t lds) {
     if (lds <= 64 * 1024) return hipSuccess;
     const int b = (int)lds;
-    hipError_t e = hipFuncSetAttribute((const void*)mstep_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<4>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<8>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+    hipError_t e = set_lds_kc<0>(b);
+    if (e == hipSuccess) e = set_lds_kc<2>(b);
+    if (e == hipSuccess) e = set_lds_kc<4>(b);
+    if (e == hipSuccess) e = set_lds_kc<8>(b);
     return e;
 }
 
@@ -686,7 +897,12 @@ hipError_t launch_morder(const MsampArgs& A, const int32_t* fixed_order, hipStre
 hipError_t launch_mstep(const MsampArgs& A, size_t lds, hipStream_t s) {
     const int nw = kMsampBlock / 64;
     const dim3 grid((A.n_jobs + nw - 1) / nw, A.n_chains);
-    hipLaunchKernelGGL(mstep_kernel, grid, dim3(kMsampBlock), lds, s, A);
+    switch (A.people_chunks) {
+        case 2: hipLaunchKernelGGL(mstep_kernel<2>, grid, dim3(kMsampBlock), lds, s, A); break;
+        case 4: hipLaunchKernelGGL(mstep_kernel<4>, grid, dim3(kMsampBlock), lds, s, A); break;
+        case 8: hipLaunchKernelGGL(mstep_kernel<8>, grid, dim3(kMsampBlock), lds, s, A); break;
+        default: hipLaunchKernelGGL(mstep_kernel<0>, grid, dim3(kMsampBlock), lds, s, A); break;
+    }
     return hipGetLastError();
 }
 
@@ -702,14 +918,12 @@ hipError_t launch_mfb(const MsampArgs& A, hipStream_t s) {
 }
 
 hipError_t launch_mspec(const MsampArgs& A, size_t lds, hipStream_t s) {
-    const int nw = kMsampBlock / 64;
-    const int seg = (A.M + kMsampBlock - 1) / kMsampBlock;
-    const dim3 grid(1 + (A.n_jobs + nw - 1) / nw, A.n_chains), block(kMsampBlock);
-    if (seg <= 1) hipLaunchKernelGGL(mspec_kernel<1>, grid, block, lds, s, A);
-    else if (seg <= 2) hipLaunchKernelGGL(mspec_kernel<2>, grid, block, lds, s, A);
-    else if (seg <= 4) hipLaunchKernelGGL(mspec_kernel<4>, grid, block, lds, s, A);
-    else if (seg <= 8) hipLaunchKernelGGL(mspec_kernel<8>, grid, block, lds, s, A);
-    else hipLaunchKernelGGL(mspec_kernel<0>, grid, block, lds, s, A);
+    switch (A.people_chunks) {
+        case 2: launch_mspec_kc<2>(A, lds, s); break;
+        case 4: launch_mspec_kc<4>(A, lds, s); break;
+        case 8: launch_mspec_kc<8>(A, lds, s); break;
+        default: launch_mspec_kc<0>(A, lds, s); break;
+    }
     return hipGetLastError();
 }
 

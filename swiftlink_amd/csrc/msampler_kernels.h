@@ -18,10 +18,10 @@ struct MsampArgs {
     int N, F, M, W;
     const int32_t* mother;        // [N]
     const int32_t* father;        // [N]
-    const uint2* ptab;            // [N-F] label pass in generation order: (person, mother | father << 16)
+    const uint2* ptab;            // [N-F] label pass in generation order:
+                                  //   (person | generation << 16, mother | father << 16)
     int ptab_bytes;               // LDS bytes of the staged ptab
     const int32_t* gen_off;       // [n_gen+1] non-founders grouped by generation
-    const int32_t* gen_persons;
     int n_gen;
     const uint8_t* geno_t;        // [M][N] marker genotype, 0 when untyped
     const double* minor;          // [M] minor allele frequency
@@ -46,10 +46,13 @@ struct MsampArgs {
     int check_kind;               // 1: act only on chains whose sweep `sweep` is an M-sweep
     double lprob;                 // L-sweep probability of the sweep schedule
     int wave_bytes;               // LDS bytes per wavefront
+    int people_chunks;            // 2, 4, 8: graphs built in registers for at most 64x that many
+                                  // people; 0: in LDS, any size
     int n_chains;
 };
 
 bool msampler_supported(int N, int F);
+int msampler_people_chunks(int N);  // register chunks for N people, 0 when too many
 size_t msampler_wave_bytes(int N, int F, int W);
 size_t msampler_ptab_bytes(int N, int F);
 size_t msampler_lds_bytes(const MsampArgs& A, int block);

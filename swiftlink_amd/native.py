@@ -20,6 +20,7 @@ LIBPATH = os.path.join(HERE, "libswiftlink_amd.so")
 SL_FP64 = 0
 SL_FP32_RESCALED = 1
 SL_MSAMPLER_AUTO, SL_MSAMPLER_SPECULATIVE, SL_MSAMPLER_TWO_LAUNCH = 0, 1, 2
+SL_MSAMPLER_LDS_GRAPHS = 4  # or-ed into a mode: founder-allele graphs built in LDS
 SL_KERNEL_LOD, SL_KERNEL_LSAMPLER, SL_KERNEL_MSAMPLER, SL_KERNEL_MSAMPLER_FB = 0, 1, 2, 3
 SL_OK, SL_E_INVALID, SL_E_UNSUPPORTED, SL_E_DEVICE = 0, -1, -2, -3
 SL_E_ZERO_LIKELIHOOD, SL_E_ILLEGAL_GRAPH, SL_E_CAPACITY = -4, -5, -6

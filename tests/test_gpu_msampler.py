@@ -33,7 +33,11 @@ def shuffled_order(o, seed, chain, sweep):
     return np.array(mo, np.int32)
 
 
-MODES = [native.SL_MSAMPLER_SPECULATIVE, native.SL_MSAMPLER_TWO_LAUNCH]
+# launch modes, each with the founder-allele graphs in registers (the default
+# up to 512 people) and in LDS (the path of larger pedigrees)
+MODES = [m | g for m in (native.SL_MSAMPLER_SPECULATIVE, native.SL_MSAMPLER_TWO_LAUNCH)
+         for g in (0, native.SL_MSAMPLER_LDS_GRAPHS)]
+GRAPHS = [native.SL_MSAMPLER_AUTO, native.SL_MSAMPLER_LDS_GRAPHS]
 
 
 @pytest.mark.parametrize("mode", MODES)
@@ -79,12 +83,13 @@ def assert_likelihoods(got, ref):
     assert np.all(np.abs(got[nz] / ref[nz] - 1.0) <= LIK_RTOL)
 
 
+@pytest.mark.parametrize("graphs", GRAPHS)
 @pytest.mark.parametrize("name", GOLDEN_CASES)
-def test_locus_likelihoods_match_reference(name):
+def test_locus_likelihoods_match_reference(name, graphs):
     case, fx = load_golden(name)
     o = Oracle(case)
     starts = np.stack([fx["graph_init"], fx["graphs"][0], fx["lsweep_out"], fx["msweep_out"]])
-    with native.Context(case, native.SL_FP64) as ctx:
+    with native.Context(case, native.SL_FP64, msampler_mode=graphs) as ctx:
         ctx.mcmc_init(starts)
         lik = ctx.mcmc_likelihood()
         assert np.all(ctx.mcmc_status() == 0)
@@ -94,25 +99,46 @@ def test_locus_likelihoods_match_reference(name):
         assert_likelihoods(lik[c], ref)
 
 
+@pytest.mark.parametrize("graphs", GRAPHS)
 @pytest.mark.parametrize("name", GOLDEN_CASES)
-def test_flipped_columns_legality_and_likelihood(name):
+def test_flipped_columns_legality_and_likelihood(name, graphs):
     """Graphs with random single flips: many columns become illegal."""
     case, fx = load_golden(name)
     o = Oracle(case)
     rng = np.random.default_rng(11)
-    graphs = []
+    gs = []
     for _ in range(6):
         g = fx["graph_init"].copy()
         for l in range(case.M):
             g[l, rng.integers(2 * case.F, 2 * case.N)] ^= 1
-        graphs.append(g)
-    graphs = np.stack(graphs)
-    with native.Context(case, native.SL_FP64) as ctx:
-        ctx.mcmc_init(graphs)
+        gs.append(g)
+    gs = np.stack(gs)
+    with native.Context(case, native.SL_FP64, msampler_mode=graphs) as ctx:
+        ctx.mcmc_init(gs)
         lik = ctx.mcmc_likelihood()
-    ref = np.array([[o.fag(g, l) for l in range(case.M)] for g in graphs])
+    ref = np.array([[o.fag(g, l) for l in range(case.M)] for g in gs])
     assert (ref == 0.0).any() and (ref > 0.0).any()
     assert_likelihoods(lik, ref)
+
+
+def test_register_and_lds_graphs_agree_bitwise(ped150):
+    """Both founder-allele graph builds sum the same per-component terms in the same order."""
+    case, g0, _ = ped150
+    rng = np.random.default_rng(5)
+    gs = [g0]
+    for _ in range(3):
+        g = g0.copy()
+        for l in range(case.M):
+            g[l, rng.integers(2 * case.F, 2 * case.N)] ^= 1
+        gs.append(g)
+    gs = np.stack(gs)
+    lik = []
+    for graphs in GRAPHS:
+        with native.Context(case, native.SL_FP64, msampler_mode=graphs) as ctx:
+            ctx.mcmc_init(gs)
+            lik.append(ctx.mcmc_likelihood())
+    assert (lik[0] == 0.0).any() and (lik[0] > 0.0).any()
+    assert np.array_equal(lik[0], lik[1])
 
 
 def _illegal_graph(case, o, g):
