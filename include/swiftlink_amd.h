@@ -84,7 +84,9 @@ typedef struct {
 /* M-sweep launch modes (results are identical): one launch per meiosis step,
  * computing the next step's likelihoods for both values of the current
  * meiosis's bit while its forward-backward pass runs; or two launches per
- * step; or the first while the doubled likelihood work is small.
+ * step; or the first while the doubled likelihood work is small.  Pedigrees
+ * whose speculative workgroup would need more than 160 KiB of LDS always take
+ * two launches.
  * SL_MSAMPLER_LDS_GRAPHS may be or-ed in: every founder-allele graph is then
  * built in LDS, as it always is for pedigrees of more than 512 people (smaller
  * ones keep a wavefront's graph in registers); results are identical. */

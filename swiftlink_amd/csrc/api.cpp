@@ -91,10 +91,10 @@ struct sl_ctx {
     std::string samp_why;           // why the sampler program was rejected, if it was
     std::vector<int32_t> mother_h, father_h;
     // M-sampler
-    int m_block = sl::kMsampBlock;
     int m_mode = SL_MSAMPLER_AUTO;
     int m_wave_bytes = 0;
-    size_t m_lds = 0;
+    size_t m_lds = 0;       // dynamic LDS of a likelihood-kernel workgroup
+    size_t m_spec_lds = 0;  // and of a speculative-kernel workgroup (0: too large, no speculation)
     bool m_ready = false;
     std::string m_why;
     std::vector<int32_t> meioses_h;
@@ -319,12 +319,14 @@ int setup_msampler(sl_ctx* c, const sl_pedigree_desc* ped) {
     sl::MsampArgs A{};
     A.wave_bytes = c->m_wave_bytes;
     A.ptab_bytes = c->m_ptab_bytes;
-    c->m_lds = sl::msampler_lds_bytes(A, c->m_block);
+    c->m_lds = sl::msampler_lds_bytes(A, sl::kMsampBlock);
+    c->m_spec_lds = sl::msampler_lds_bytes(A, sl::kSpecBlock);
     if (c->m_lds > 160 * 1024) {
         c->m_why = "M-sampler workgroup needs more than 160 KiB of LDS";
         return SL_OK;
     }
-    SL_HIP(c, sl::configure_msampler_lds(c->m_lds));
+    if (c->m_spec_lds > 160 * 1024) c->m_spec_lds = 0;  // steps then take two launches
+    SL_HIP(c, sl::configure_msampler_lds(c->m_lds, c->m_spec_lds));
     int rc;
     if ((rc = upload(c, &c->d_gen_off, gen_off.data(), gen_off.size())) ||
         (rc = upload(c, &c->d_ptab, ptab.data(), ptab.size())) ||
@@ -860,8 +862,9 @@ static int run_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* d
     // one launch per step when the doubled likelihood work still fits the
     // device in one wave of workgroups (few chains), two launches otherwise
     const int launch_mode = c->m_mode & 3;
-    const bool spec = launch_mode == SL_MSAMPLER_SPECULATIVE ||
-                      (launch_mode == SL_MSAMPLER_AUTO && (int64_t)c->n_chains * c->M * 2 <= kSpecWaves);
+    const bool spec = c->m_spec_lds != 0 &&
+                      (launch_mode == SL_MSAMPLER_SPECULATIVE ||
+                       (launch_mode == SL_MSAMPLER_AUTO && (int64_t)c->n_chains * c->M * 2 <= kSpecWaves));
     A.spec = spec ? 1 : 0;
     // start of the sweep: the current columns and step 0's flipped columns
     A.with_lcur = 1;
@@ -875,7 +878,7 @@ static int run_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* d
         const int jobs = k + 1 < nm ? (spec ? 2 : 1) * c->M : 0;
         if (spec) {
             A.n_jobs = jobs;
-            rc = timed_launch(c, SL_KERNEL_MSAMPLER, [&] { return sl::launch_mspec(A, c->m_lds, c->stream); });
+            rc = timed_launch(c, SL_KERNEL_MSAMPLER, [&] { return sl::launch_mspec(A, c->m_spec_lds, c->stream); });
         } else {
             rc = timed_launch(c, SL_KERNEL_MSAMPLER_FB, [&] { return sl::launch_mfb(A, c->stream); });
             A.n_jobs = jobs;

@@ -414,11 +414,20 @@ struct Mat {
     double a, b, c, d;  // [[a, b], [c, d]]
 };
 
-// product with an exact power-of-two rescale keeping the largest entry in [0.5, 1)
+// product with an exact power-of-two rescale keeping the largest entry in
+// [0.5, 1): for a normal largest entry m with biased exponent E the factor is
+// 2^(1022-E), built from its bits; a subnormal m takes frexp/ldexp
 __device__ __forceinline__ Mat mmul(const Mat& x, const Mat& y) {
     Mat r{x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d, x.c * y.a + x.d * y.c, x.c * y.b + x.d * y.d};
     const double m = fmax(fmax(r.a, r.b), fmax(r.c, r.d));
-    if (m > 0.0) {
+    const uint32_t E = (uint32_t)(__builtin_bit_cast(uint64_t, m) >> 52) & 0x7FFu;
+    if (E != 0u && E < 2045u) {
+        const double f = __builtin_bit_cast(double, (uint64_t)(2045u - E) << 52);
+        r.a *= f;
+        r.b *= f;
+        r.c *= f;
+        r.d *= f;
+    } else if (m > 0.0 && E == 0u) {
         int e;
         (void)frexp(m, &e);
         r.a = ldexp(r.a, -e);
@@ -434,8 +443,51 @@ __device__ __forceinline__ uint32_t compose(uint32_t f, uint32_t h) {
     return ((f >> (h & 1u)) & 1u) | (((f >> ((h >> 1) & 1u)) & 1u) << 1);
 }
 
-__device__ __forceinline__ Mat shfl_up(const Mat& x, int d) {
-    return Mat{__shfl_up(x.a, d), __shfl_up(x.b, d), __shfl_up(x.c, d), __shfl_up(x.d, d)};
+// DPP lane moves of GFX9: row_shr:n / row_shl:n within rows of 16 lanes,
+// wave_shr:1 / wave_shl:1 across the wavefront, row_bcast:15 / row_bcast:31
+// from the last lane of a row to the rows above it.  Lanes without a source,
+// and rows outside the row mask, keep `old`.
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+constexpr int dpp_row_shl(int n) { return 0x100 + n; }
+constexpr int dpp_row_shr(int n) { return 0x110 + n; }
+
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ uint32_t u32_dpp(uint32_t old, uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)x, CTRL, ROWS, 0xf, false);
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double f64_dpp(double old, double x) {
+    const uint64_t o = __builtin_bit_cast(uint64_t, old), u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = u32_dpp<CTRL, ROWS>((uint32_t)o, (uint32_t)u);
+    const uint32_t hi = u32_dpp<CTRL, ROWS>((uint32_t)(o >> 32), (uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ Mat mat_dpp(const Mat& x) {  // the identity where there is no source
+    return Mat{f64_dpp<CTRL, ROWS>(1.0, x.a), f64_dpp<CTRL, ROWS>(0.0, x.b), f64_dpp<CTRL, ROWS>(0.0, x.c),
+               f64_dpp<CTRL, ROWS>(1.0, x.d)};
+}
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t x, int lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, lane);
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = readlane_u32((uint32_t)u, lane), hi = readlane_u32((uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// inclusive left-to-right products across the wavefront
+__device__ __forceinline__ Mat wave_products(Mat P) {
+    P = mmul(mat_dpp<dpp_row_shr(1)>(P), P);
+    P = mmul(mat_dpp<dpp_row_shr(2)>(P), P);
+    P = mmul(mat_dpp<dpp_row_shr(4)>(P), P);
+    P = mmul(mat_dpp<dpp_row_shr(8)>(P), P);
+    P = mmul(mat_dpp<kDppRowBcast15, 0xa>(P), P);
+    return mmul(mat_dpp<kDppRowBcast31, 0xc>(P), P);
 }
 
 __device__ __forceinline__ bool is_msweep(const MsampArgs& A, int chain) {
@@ -536,9 +588,9 @@ __device__ __forceinline__ uint32_t locus_draws(const MsampArgs& A, double f0, d
 
 // 4-bit decision map of locus l: bit bn = new indicator when the next locus's
 // is bn; bit 2+bn = that decision's uniform lay within 1e-12 of its threshold.
-// Consumes the draw index cnt when the locus draws.
+// A drawing locus uses draw index ctr of step k.
 __device__ __forceinline__ uint32_t locus_map(const MsampArgs& A, int chain, int k, double f0, double f1, int l,
-                                              uint32_t* cnt, uint32_t* near) {
+                                              uint32_t ctr, uint32_t* near) {
     double w0[2], w1[2];
     if (l < A.M - 1) {
         const double th = A.theta[l], an = 1.0 - A.theta[l];
@@ -554,8 +606,7 @@ __device__ __forceinline__ uint32_t locus_map(const MsampArgs& A, int chain, int
     // indicator; a difference (underflow in one branch only) is counted
     const bool draw = w0[0] != 0.0 && w1[0] != 0.0;
     if (draw != (w0[1] != 0.0 && w1[1] != 0.0)) ++*near;
-    const double u = draw ? philox_uniform(A.seed, (uint32_t)chain, A.sweep, (uint32_t)k, *cnt) : 0.0;
-    if (draw) ++*cnt;
+    const double u = draw ? philox_uniform(A.seed, (uint32_t)chain, A.sweep, (uint32_t)k, ctr) : 0.0;
     uint32_t map = 0;
 #pragma unroll
     for (int bn = 0; bn < 2; ++bn) {
@@ -574,34 +625,46 @@ __device__ __forceinline__ uint32_t locus_map(const MsampArgs& A, int chain, int
     return map;
 }
 
-// exclusive left-to-right product of the threads' segment products
+// exclusive left-to-right product of the threads' segment products: within
+// wavefronts by DPP, across them through LDS (by one row of lanes when there
+// are more than four)
 __device__ __forceinline__ Mat scan_products(Mat P, Mat* wtot) {
-    const int lane = lane_id(), wave = threadIdx.x >> 6;
-    for (int d = 1; d < 64; d <<= 1) {
-        const Mat o = shfl_up(P, d);
-        if (lane >= d) P = mmul(o, P);
-    }
+    const int lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    P = wave_products(P);
     if (lane == 63) wtot[wave] = P;
     __syncthreads();
     Mat R{1.0, 0.0, 0.0, 1.0};
-    for (int k = 0; k < wave; ++k) R = mmul(R, wtot[k]);
-    const Mat below = shfl_up(P, 1);
-    if (lane > 0) R = mmul(R, below);
-    return R;
+    if (nw <= 4) {
+        for (int q = 0; q < wave; ++q) R = mmul(R, wtot[q]);
+    } else {
+        Mat T = lane < nw ? wtot[lane] : Mat{1.0, 0.0, 0.0, 1.0};
+        T = mmul(mat_dpp<dpp_row_shr(1)>(T), T);
+        T = mmul(mat_dpp<dpp_row_shr(2)>(T), T);
+        T = mmul(mat_dpp<dpp_row_shr(4)>(T), T);
+        T = mmul(mat_dpp<dpp_row_shr(8)>(T), T);
+        const int wu = __builtin_amdgcn_readfirstlane(wave);
+        if (wu > 0)
+            R = Mat{readlane_f64(T.a, wu - 1), readlane_f64(T.b, wu - 1), readlane_f64(T.c, wu - 1),
+                    readlane_f64(T.d, wu - 1)};
+    }
+    return mmul(R, mat_dpp<kDppWaveShr1>(P));
 }
 
 // sum of v over the threads after this one
 __device__ __forceinline__ uint32_t sum_above(uint32_t v, uint32_t* wcnt) {
     const int lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    uint32_t inc = v;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t o = __shfl_down(inc, d);
-        if (lane + d < 64) inc += o;
-    }
-    if (lane == 0) wcnt[wave] = inc;
+    uint32_t sc = v;  // inclusive suffix sums within rows, then across them
+    sc += u32_dpp<dpp_row_shl(1)>(0u, sc);
+    sc += u32_dpp<dpp_row_shl(2)>(0u, sc);
+    sc += u32_dpp<dpp_row_shl(4)>(0u, sc);
+    sc += u32_dpp<dpp_row_shl(8)>(0u, sc);
+    const uint32_t t1 = readlane_u32(sc, 16), t2 = readlane_u32(sc, 32), t3 = readlane_u32(sc, 48);
+    const int row = lane >> 4;
+    sc += (row < 1 ? t1 : 0u) + (row < 2 ? t2 : 0u) + (row < 3 ? t3 : 0u);
+    if (lane == 0) wcnt[wave] = sc;
     __syncthreads();
-    uint32_t r = inc - v;
-    for (int k = wave + 1; k < nw; ++k) r += wcnt[k];
+    uint32_t r = sc - v;
+    for (int q = wave + 1; q < nw; ++q) r += wcnt[q];
     return r;
 }
 
@@ -610,18 +673,21 @@ __device__ __forceinline__ uint32_t sum_above(uint32_t v, uint32_t* wcnt) {
 // is constant, so the composite is too)
 __device__ __forceinline__ uint32_t bit_from_above(uint32_t comp, uint32_t* wmap) {
     const int lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    uint32_t sc = comp;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t o = __shfl_down(sc, d);
-        if (lane + d < 64) sc = compose(sc, o);
-    }
+    uint32_t sc = comp;  // this map composed with every later one, within rows, then across them
+    sc = compose(sc, u32_dpp<dpp_row_shl(1)>(2u, sc));
+    sc = compose(sc, u32_dpp<dpp_row_shl(2)>(2u, sc));
+    sc = compose(sc, u32_dpp<dpp_row_shl(4)>(2u, sc));
+    sc = compose(sc, u32_dpp<dpp_row_shl(8)>(2u, sc));
+    const uint32_t t3 = readlane_u32(sc, 48);
+    const uint32_t t2 = compose(readlane_u32(sc, 32), t3);
+    const uint32_t t1 = compose(readlane_u32(sc, 16), t2);
+    const int row = lane >> 4;
+    sc = compose(sc, row == 0 ? t1 : row == 1 ? t2 : row == 2 ? t3 : 2u);
     if (lane == 0) wmap[wave] = sc;
     __syncthreads();
     uint32_t up = 2u;
-    for (int k = nw - 1; k > wave; --k) up = compose(wmap[k], up);
-    uint32_t nxt = __shfl_down(sc, 1);
-    if (lane == 63) nxt = 2u;
-    return compose(nxt, up) & 1u;
+    for (int q = nw - 1; q > wave; --q) up = compose(wmap[q], up);
+    return compose(u32_dpp<kDppWaveShl1>(2u, sc), up) & 1u;
 }
 
 // Forward-backward pass of step k on one chain by the whole workgroup; thread
@@ -651,25 +717,40 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int k, Mat* wtot
     const int a0 = min(M, tid * S), a1 = min(M, a0 + S);
     uint32_t near = 0;
     if constexpr (SM > 0) {
-        double e0[SM], e1[SM], lfv[SM], f0[SM], f1[SM];
-        uint32_t curm = 0, maps = 0;
+        // every load of the pass at once: the column words holding m_k and
+        // m_{k-1}, the current column's likelihood and both flipped variants
+        uint32_t gw[SM], pw[SM];
+        double lcv[SM], lf0[SM], lf1[SM];
 #pragma unroll
         for (int s = 0; s < SM; ++s) {
             const int l = a0 + s;
-            e0[s] = e1[s] = lfv[s] = 0.0;
+            gw[s] = pw[s] = 0u;
+            lcv[s] = lf0[s] = lf1[s] = 0.0;
             if (l < a1) {
-                const bool cur = (g[(size_t)l * A.W + word] & bm) != 0;
-                lfv[s] = sv.lf(g, l);
-                emissions(cur, lc[l], lfv[s], &e0[s], &e1[s]);
-                curm |= (cur ? 1u : 0u) << s;
+                gw[s] = g[(size_t)l * A.W + word];
+                lcv[s] = lc[l];
+                lf0[s] = A.lflip[lflip_index(A, k, 0, chain, l)];
+                if (sv.spec) {
+                    pw[s] = g[(size_t)l * A.W + sv.pword];
+                    lf1[s] = A.lflip[lflip_index(A, k, 1, chain, l)];
+                }
             }
+        }
+        double e0[SM], e1[SM], lfv[SM], f0[SM], f1[SM];
+        uint32_t curm = 0, drm = 0;
+#pragma unroll
+        for (int s = 0; s < SM; ++s) {
+            const bool cur = (gw[s] & bm) != 0;
+            lfv[s] = (pw[s] & sv.pbm) ? lf1[s] : lf0[s];
+            e0[s] = e1[s] = 0.0;
+            if (a0 + s < a1) emissions(cur, lcv[s], lfv[s], &e0[s], &e1[s]);
+            curm |= (cur ? 1u : 0u) << s;
         }
         Mat P{1.0, 0.0, 0.0, 1.0};
 #pragma unroll
         for (int s = 0; s < SM; ++s)
             if (a0 + s < a1) P = mmul(P, emission_matrix(A, e0[s], e1[s], a0 + s));
         Mat R = scan_products(P, wtot);
-        uint32_t draws = 0;
 #pragma unroll
         for (int s = 0; s < SM; ++s) {
             f0[s] = f1[s] = 0.0;
@@ -677,27 +758,31 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int k, Mat* wtot
                 R = mmul(R, emission_matrix(A, e0[s], e1[s], a0 + s));
                 f0[s] = R.a + R.c;
                 f1[s] = R.b + R.d;
-                draws += locus_draws(A, f0[s], f1[s], a0 + s);
+                drm |= locus_draws(A, f0[s], f1[s], a0 + s) << s;
             }
         }
-        uint32_t cnt = sum_above(draws, wcnt);
+        // the backward pass runs from the last locus down, so a drawing
+        // locus's draw index counts the draws of later threads and of the
+        // later loci of this segment
+        const uint32_t cnt = sum_above((uint32_t)__popc(drm), wcnt);
+        uint32_t mp[SM];
+#pragma unroll
+        for (int s = 0; s < SM; ++s) {
+            mp[s] = 0u;
+            if (a0 + s < a1)
+                mp[s] = locus_map(A, chain, k, f0[s], f1[s], a0 + s, cnt + (uint32_t)__popc(drm >> (s + 1)), &near);
+        }
         uint32_t comp = 2u;  // identity
 #pragma unroll
-        for (int s = SM - 1; s >= 0; --s) {
-            if (a0 + s < a1) {
-                const uint32_t map = locus_map(A, chain, k, f0[s], f1[s], a0 + s, &cnt, &near);
-                maps |= map << (4 * s);
-                comp = compose(map & 3u, comp);
-            }
-        }
+        for (int s = SM - 1; s >= 0; --s)
+            if (a0 + s < a1) comp = compose(mp[s] & 3u, comp);
         uint32_t next = bit_from_above(comp, wmap);
 #pragma unroll
         for (int s = SM - 1; s >= 0; --s) {
             const int l = a0 + s;
             if (l < a1) {
-                const uint32_t map = (maps >> (4 * s)) & 15u;
-                const uint32_t nb = (map >> next) & 1u;
-                near += (map >> (2 + next)) & 1u;
+                const uint32_t nb = (mp[s] >> next) & 1u;
+                near += (mp[s] >> (2 + next)) & 1u;
                 if (nb != ((curm >> s) & 1u)) {
                     g[(size_t)l * A.W + word] ^= bm;
                     lc[l] = lfv[s];
@@ -726,7 +811,8 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int k, Mat* wtot
         uint32_t cnt = sum_above(draws, wcnt);
         uint32_t comp = 2u;
         for (int l = a1 - 1; l >= a0; --l) {
-            const uint32_t map = locus_map(A, chain, k, fb[2 * l], fb[2 * l + 1], l, &cnt, &near);
+            const uint32_t map = locus_map(A, chain, k, fb[2 * l], fb[2 * l + 1], l, cnt, &near);
+            cnt += locus_draws(A, fb[2 * l], fb[2 * l + 1], l);
             mp[l] = (uint8_t)map;
             comp = compose(map & 3u, comp);
         }
@@ -787,10 +873,10 @@ __global__ void __launch_bounds__(kFbBlock) mfb_kernel(MsampArgs A) {
 // one speculative launch: workgroup 0 of each chain runs step fb_step's
 // forward-backward pass while the others run the jobs of step fb_step + 1
 template <int SM, int KC>
-__global__ void __launch_bounds__(kMsampBlock) mspec_kernel(MsampArgs A) {
+__global__ void __launch_bounds__(kSpecBlock) mspec_kernel(MsampArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ Mat wtot[kMsampBlock / 64];
-    __shared__ uint32_t wcnt[kMsampBlock / 64], wmap[kMsampBlock / 64];
+    __shared__ Mat wtot[kSpecBlock / 64];
+    __shared__ uint32_t wcnt[kSpecBlock / 64], wmap[kSpecBlock / 64];
     __shared__ int s_skip;
     if (skip_chain(A, blockIdx.y, &s_skip)) return;
     if (blockIdx.x == 0) forward_backward<SM>(A, blockIdx.y, A.fb_step, wtot, wcnt, wmap);
@@ -838,7 +924,9 @@ __global__ void morder_kernel(MsampArgs A, const int32_t* fixed_order) {
 
 bool msampler_supported(int N, int F) { return F >= 1 && 2 * F <= kMaxFagNodes && N <= 65535; }
 
-int msampler_people_chunks(int N) { return N <= 128 ? 2 : N <= 256 ? 4 : N <= 512 ? 8 : 0; }
+int msampler_people_chunks(int N) {
+    return N <= 128 ? 2 : N <= 192 ? 3 : N <= 256 ? 4 : N <= 384 ? 6 : N <= 512 ? 8 : 0;
+}
 
 size_t msampler_wave_bytes(int N, int F, int W) {
     const size_t b = 4 * (size_t)W + 2 * 4 * (size_t)(2 * F) + 4 * (size_t)N + 2 * 2 * (size_t)(N - F);
@@ -853,22 +941,28 @@ size_t msampler_lds_bytes(const MsampArgs& A, int block) {
 
 namespace {
 
+hipError_t set_lds(const void* kernel, size_t b) {
+    if (b <= 64 * 1024) return hipSuccess;
+    return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b);
+}
+
 template <int KC>
-hipError_t set_lds_kc(int b) {
-    hipError_t e = hipFuncSetAttribute((const void*)mstep_kernel<KC>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<0, KC>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<1, KC>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<2, KC>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<4, KC>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)mspec_kernel<8, KC>, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+hipError_t set_lds_kc(size_t job, size_t spec) {
+    hipError_t e = set_lds((const void*)mstep_kernel<KC>, job);
+    if (spec == 0) return e;
+    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<0, KC>, spec);
+    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<1, KC>, spec);
+    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<2, KC>, spec);
+    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<4, KC>, spec);
+    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<8, KC>, spec);
     return e;
 }
 
 template <int KC>
 void launch_mspec_kc(const MsampArgs& A, size_t lds, hipStream_t s) {
-    const int nw = kMsampBlock / 64;
-    const int seg = (A.M + kMsampBlock - 1) / kMsampBlock;
-    const dim3 grid(1 + (A.n_jobs + nw - 1) / nw, A.n_chains), block(kMsampBlock);
+    const int nw = kSpecBlock / 64;
+    const int seg = (A.M + kSpecBlock - 1) / kSpecBlock;
+    const dim3 grid(1 + (A.n_jobs + nw - 1) / nw, A.n_chains), block(kSpecBlock);
     if (seg <= 1) hipLaunchKernelGGL((mspec_kernel<1, KC>), grid, block, lds, s, A);
     else if (seg <= 2) hipLaunchKernelGGL((mspec_kernel<2, KC>), grid, block, lds, s, A);
     else if (seg <= 4) hipLaunchKernelGGL((mspec_kernel<4, KC>), grid, block, lds, s, A);
@@ -878,13 +972,13 @@ void launch_mspec_kc(const MsampArgs& A, size_t lds, hipStream_t s) {
 
 }  // namespace
 
-hipError_t configure_msampler_lds(size_t lds) {
-    if (lds <= 64 * 1024) return hipSuccess;
-    const int b = (int)lds;
-    hipError_t e = set_lds_kc<0>(b);
-    if (e == hipSuccess) e = set_lds_kc<2>(b);
-    if (e == hipSuccess) e = set_lds_kc<4>(b);
-    if (e == hipSuccess) e = set_lds_kc<8>(b);
+hipError_t configure_msampler_lds(size_t job_lds, size_t spec_lds) {
+    hipError_t e = set_lds_kc<0>(job_lds, spec_lds);
+    if (e == hipSuccess) e = set_lds_kc<2>(job_lds, spec_lds);
+    if (e == hipSuccess) e = set_lds_kc<3>(job_lds, spec_lds);
+    if (e == hipSuccess) e = set_lds_kc<4>(job_lds, spec_lds);
+    if (e == hipSuccess) e = set_lds_kc<6>(job_lds, spec_lds);
+    if (e == hipSuccess) e = set_lds_kc<8>(job_lds, spec_lds);
     return e;
 }
 
@@ -899,7 +993,9 @@ hipError_t launch_mstep(const MsampArgs& A, size_t lds, hipStream_t s) {
     const dim3 grid((A.n_jobs + nw - 1) / nw, A.n_chains);
     switch (A.people_chunks) {
         case 2: hipLaunchKernelGGL(mstep_kernel<2>, grid, dim3(kMsampBlock), lds, s, A); break;
+        case 3: hipLaunchKernelGGL(mstep_kernel<3>, grid, dim3(kMsampBlock), lds, s, A); break;
         case 4: hipLaunchKernelGGL(mstep_kernel<4>, grid, dim3(kMsampBlock), lds, s, A); break;
+        case 6: hipLaunchKernelGGL(mstep_kernel<6>, grid, dim3(kMsampBlock), lds, s, A); break;
         case 8: hipLaunchKernelGGL(mstep_kernel<8>, grid, dim3(kMsampBlock), lds, s, A); break;
         default: hipLaunchKernelGGL(mstep_kernel<0>, grid, dim3(kMsampBlock), lds, s, A); break;
     }
@@ -920,7 +1016,9 @@ hipError_t launch_mfb(const MsampArgs& A, hipStream_t s) {
 hipError_t launch_mspec(const MsampArgs& A, size_t lds, hipStream_t s) {
     switch (A.people_chunks) {
         case 2: launch_mspec_kc<2>(A, lds, s); break;
+        case 3: launch_mspec_kc<3>(A, lds, s); break;
         case 4: launch_mspec_kc<4>(A, lds, s); break;
+        case 6: launch_mspec_kc<6>(A, lds, s); break;
         case 8: launch_mspec_kc<8>(A, lds, s); break;
         default: launch_mspec_kc<0>(A, lds, s); break;
     }

@@ -12,6 +12,7 @@ namespace sl {
 // alleles are 10-bit node ids in its edge words.
 constexpr int kMaxFagNodes = 1022;  // 2F
 constexpr int kMsampBlock = 256;    // threads per workgroup of the likelihood kernel (its launch bound)
+constexpr int kSpecBlock = 512;     // threads per workgroup of the speculative kernel
 constexpr int kFbBlock = 1024;      // threads per workgroup of the forward-backward kernel
 
 struct MsampArgs {
@@ -46,8 +47,8 @@ struct MsampArgs {
     int check_kind;               // 1: act only on chains whose sweep `sweep` is an M-sweep
     double lprob;                 // L-sweep probability of the sweep schedule
     int wave_bytes;               // LDS bytes per wavefront
-    int people_chunks;            // 2, 4, 8: graphs built in registers for at most 64x that many
-                                  // people; 0: in LDS, any size
+    int people_chunks;            // 2, 3, 4, 6, 8: graphs built in registers for at most 64x that
+                                  // many people; 0: in LDS, any size
     int n_chains;
 };
 
@@ -56,7 +57,7 @@ int msampler_people_chunks(int N);  // register chunks for N people, 0 when too 
 size_t msampler_wave_bytes(int N, int F, int W);
 size_t msampler_ptab_bytes(int N, int F);
 size_t msampler_lds_bytes(const MsampArgs& A, int block);
-hipError_t configure_msampler_lds(size_t lds);
+hipError_t configure_msampler_lds(size_t job_lds, size_t spec_lds);  // spec_lds 0: no speculative launches
 hipError_t launch_morder(const MsampArgs& A, const int32_t* fixed_order, hipStream_t s);
 hipError_t launch_mstep(const MsampArgs& A, size_t lds, hipStream_t s);
 hipError_t launch_mfb(const MsampArgs& A, hipStream_t s);
