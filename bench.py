@@ -11,7 +11,7 @@ peel, FP64 LOD accumulation).  Reported beside it: the FP64 chain's sweeps/s,
 the sweeps/s of 64 replicate chains run together in one context, LOD evals/s
 of the trait peel over a batch of 64 device-resident graphs in
 FP32-rescaled and FP64 with the largest |dLOD| between the two, the roofline
-of the dominant kernel (the meiosis-step kernel of the M-sampler, timed with
+of the dominant kernel (the M-sampler's one-launch meiosis step, timed with
 HIP events on the library's stream) and of the peel kernel, and a CPU
 baseline: the reference's own CPU path (oracle/_ref/refharness) on this
 host's cores, or the oracle restatement when that binary is absent.
@@ -59,7 +59,7 @@ def cpu_baseline(tmpdir: str, base: str, case, threads: int):
         os.makedirs(out, exist_ok=True)
         subprocess.run([harness, "graphs", p, m, d, out, "5", "1", "1"], check=True, stdout=subprocess.DEVNULL,
                        stderr=subprocess.DEVNULL)
-        nl, nm, ng = 10, 10, 2 * threads
+        nl, nm, ng = 200, 200, 8 * threads
         subprocess.run([harness, "bench", p, m, d, out, os.path.join(out, "graphs.i32"), str(threads), str(nl),
                         str(nm), str(ng)], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
         b = json.load(open(os.path.join(out, "bench.json")))
@@ -178,14 +178,16 @@ def main():
     lod32, k32, lodv32, flops = lod(native.SL_FP32_RESCALED, 20)
     lod64, _, lodv64, _ = lod(native.SL_FP64, 8)
 
-    # ---- roofline of the meiosis-step kernel: algorithmic bytes per launch
-    # = per (chain, meiosis, locus) ceil(2N/8) descent-graph bytes + ceil(2
-    # N_typed/8) genotype bytes, times M loci and one chain ----
+    # ---- roofline of the meiosis-step kernel (one chain: mspec_kernel, the
+    # forward-backward pass of step k beside the founder-allele likelihoods of
+    # step k+1 for both values of meiosis k's bit): algorithmic bytes per
+    # launch = per (chain, meiosis, locus) ceil(2N/8) descent-graph bytes +
+    # ceil(2 N_typed/8) genotype bytes, times 2M likelihood jobs ----
     ms_ms, ms_launches = prof32[native.SL_KERNEL_MSAMPLER]
     avg = lambda k: prof32[k][0] / max(prof32[k][1], 1)
     n_typed = int(np.count_nonzero(case.typed))
     bytes_per_unit = -(-2 * case.N // 8) + -(-2 * n_typed // 8)
-    bytes_per_launch = case.M * bytes_per_unit
+    bytes_per_launch = 2 * case.M * bytes_per_unit
     ms_avg = ms_ms / max(ms_launches, 1)
     achieved = bytes_per_launch / (ms_avg * 1e-3) / 1e9 if ms_launches else 0.0
     lod_achieved = flops * evals_per_launch / (k32 * 1e-3) / 1e12
@@ -220,9 +222,10 @@ def main():
                       "msampler_forward_backward": avg(native.SL_KERNEL_MSAMPLER_FB),
                       "lsampler_parity": avg(native.SL_KERNEL_LSAMPLER),
                       "lod_scoring": avg(native.SL_KERNEL_LOD)},
-        "roofline": {"bound": "hbm", "kernel": "mstep_kernel (founder-allele likelihoods of one meiosis step)",
+        "roofline": {"bound": "hbm", "kernel": "mspec_kernel (one meiosis step: forward-backward pass of step k, "
+                                               "likelihoods of step k+1 for both values of meiosis k's bit)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profile_traffic("mstep_kernel", args.config),
+                     "traffic": profile_traffic("mspec_kernel", args.config),
                      "bytes_per_launch": bytes_per_launch, "launch_ms": ms_avg},
         "lod_roofline": {"bound": "mfma", "compute": "fp32 VALU (no MFMA: tables of 4^k cells)",
                          "kernel": "lod_kernel<float,true>", "achieved": lod_achieved, "peak": FP32_PEAK_TFLOPS,

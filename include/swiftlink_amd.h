@@ -148,11 +148,13 @@ int sl_lod_terms(sl_ctx* ctx, const int32_t* graphs, int32_t n_graphs, double* o
 
 /* Kernel timing with HIP events on the context's stream (enable != 0 starts
  * recording and clears the totals).  For kernel SL_KERNEL_LOD (peel kernel),
- * SL_KERNEL_LSAMPLER (one parity launch), SL_KERNEL_MSAMPLER (the
- * founder-allele likelihoods of one meiosis step) or SL_KERNEL_MSAMPLER_FB
- * (the forward-backward pass of one meiosis step), *ms is the time summed
- * since the last enable and *launches the number of launches it covers.  No
- * counterpart in the reference. */
+ * SL_KERNEL_LSAMPLER (one parity launch), SL_KERNEL_MSAMPLER (one meiosis
+ * step's launch: the founder-allele likelihoods, with the previous step's
+ * forward-backward pass in the one-launch mode) or SL_KERNEL_MSAMPLER_FB (the
+ * forward-backward pass of one meiosis step in the two-launch mode), *ms is
+ * the time summed since the last enable and *launches the number of launches
+ * it covers.  The one-launch mode's back-to-back step launches are timed as
+ * one span per M-sweep.  No counterpart in the reference. */
 int sl_profile(sl_ctx* ctx, int32_t enable);
 int sl_profile_read(sl_ctx* ctx, int32_t kernel, double* ms, int64_t* launches);
 

@@ -102,7 +102,7 @@ struct sl_ctx {
     int32_t* d_gen_off = nullptr;
     uint2* d_ptab = nullptr;
     int m_ptab_bytes = 0;
-    uint8_t* d_geno_t = nullptr;
+    uint32_t* d_geno2 = nullptr;
     double* d_minor = nullptr;
     int32_t* d_meioses = nullptr;
     int32_t* d_order = nullptr;
@@ -225,6 +225,9 @@ int check_desc(const sl_pedigree_desc* p, const sl_map_desc* m, const sl_peel_de
         if (!founder && (p->mother[i] < 0 || p->mother[i] >= N || p->father[i] < 0 || p->father[i] >= N ||
                          p->mother[i] == i || p->father[i] == i))
             return fail(nullptr, SL_E_INVALID, "non-founder with a missing or out-of-range parent");
+        for (int l = 0; l < p->n_markers; ++l)
+            if (p->genotypes[(size_t)i * p->n_markers + l] > 3)
+                return fail(nullptr, SL_E_INVALID, "genotype code outside 0..3");
     }
     for (int l = 0; l + 1 < p->n_markers; ++l)
         if (!(m->theta[l] > 0.0 && m->theta[l] < 0.5 + 1e-12))
@@ -236,7 +239,7 @@ int release_all(sl_ctx* c) {
     void* ptrs[] = {c->d_sops, c->d_sprevs, c->d_schildren, c->d_ssched_off, c->d_ssched_ops, c->d_rlevel_off,
                     c->d_rlevel_ops, c->d_mother, c->d_father, c->d_legal_t, c->d_gprob_d, c->d_gprob_f,
                     c->d_theta, c->d_chains, c->d_status,
-                    c->d_gen_off, c->d_ptab, c->d_geno_t, c->d_minor, c->d_meioses, c->d_order,
+                    c->d_gen_off, c->d_ptab, c->d_geno2, c->d_minor, c->d_meioses, c->d_order,
                     c->d_fixed_order, c->d_lcur, c->d_lflip, c->d_fbuf, c->d_maps, c->d_near,
                     c->d_ops, c->d_prevs, c->d_children, c->d_tasks, c->d_tinfo, c->d_idx, c->d_level_base,
                     c->d_sched_off,
@@ -306,10 +309,12 @@ int setup_msampler(sl_ctx* c, const sl_pedigree_desc* ped) {
         const int parent = b % 2 ? c->father_h[person] : c->mother_h[person];
         if (!(parent < F && kids[parent] == 1)) c->meioses_h.push_back(b);
     }
-    std::vector<uint8_t> gt((size_t)M * N, 0);
+    const int GW = (N + 15) / 16;
+    std::vector<uint32_t> gt((size_t)M * GW, 0u);
     for (int i = 0; i < N; ++i)
         if (ped->typed[i])
-            for (int l = 0; l < M; ++l) gt[(size_t)l * N + i] = ped->genotypes[(size_t)i * M + l];
+            for (int l = 0; l < M; ++l)
+                gt[(size_t)l * GW + i / 16] |= ((uint32_t)ped->genotypes[(size_t)i * M + l] & 3u) << (2 * (i % 16));
     if (c->meioses_h.size() > 16384) {
         c->m_why = "more than 16384 meioses";
         return SL_OK;
@@ -330,7 +335,7 @@ int setup_msampler(sl_ctx* c, const sl_pedigree_desc* ped) {
     int rc;
     if ((rc = upload(c, &c->d_gen_off, gen_off.data(), gen_off.size())) ||
         (rc = upload(c, &c->d_ptab, ptab.data(), ptab.size())) ||
-        (rc = upload(c, &c->d_geno_t, gt.data(), gt.size())) ||
+        (rc = upload(c, &c->d_geno2, gt.data(), gt.size())) ||
         (rc = upload(c, &c->d_minor, c->minor.data(), c->minor.size())) ||
         (rc = upload(c, &c->d_meioses, c->meioses_h.data(), c->meioses_h.size())))
         return rc;
@@ -352,6 +357,25 @@ int timed_launch(sl_ctx* c, int kind, Fn launch) {
     if (c->prof && e0 && e1) {
         SL_HIP(c, hipEventRecord(e1, c->stream));
         c->ev_used.push_back({kind, e0, e1, 1});
+    }
+    return SL_OK;
+}
+
+// Runs body(), which makes `launches` back-to-back launches of one kernel;
+// with profiling on, brackets them with one pair of events.
+template <typename Fn>
+int timed_span(sl_ctx* c, int kind, int launches, Fn body) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->prof) {
+        e0 = take_event(c);
+        e1 = take_event(c);
+        if (e0 && e1) SL_HIP(c, hipEventRecord(e0, c->stream));
+    }
+    const int rc = body();
+    if (rc) return rc;
+    if (c->prof && e0 && e1) {
+        SL_HIP(c, hipEventRecord(e1, c->stream));
+        c->ev_used.push_back({kind, e0, e1, launches});
     }
     return SL_OK;
 }
@@ -825,7 +849,8 @@ static sl::MsampArgs msamp_args(sl_ctx* c) {
     A.father = c->d_father;
     A.gen_off = c->d_gen_off;
     A.n_gen = c->n_gen;
-    A.geno_t = c->d_geno_t;
+    A.geno2 = c->d_geno2;
+    A.GW = (c->N + 15) / 16;
     A.minor = c->d_minor;
     A.theta = c->d_theta;
     A.meioses = c->d_meioses;
@@ -872,19 +897,23 @@ static int run_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* d
     A.n_jobs = 2 * c->M;
     SL_HIP(c, sl::launch_mstep(A, c->m_lds, c->stream));
     A.with_lcur = 0;
+    if (spec)
+        return timed_span(c, SL_KERNEL_MSAMPLER, nm, [&] {
+            for (int k = 0; k < nm; ++k) {
+                A.fb_step = k;
+                A.step = k + 1;
+                A.n_jobs = k + 1 < nm ? 2 * c->M : 0;
+                SL_HIP(c, sl::launch_mspec(A, c->m_spec_lds, c->stream));
+            }
+            return SL_OK;
+        });
     for (int k = 0; k < nm; ++k) {
         A.fb_step = k;
         A.step = k + 1;
-        const int jobs = k + 1 < nm ? (spec ? 2 : 1) * c->M : 0;
-        if (spec) {
-            A.n_jobs = jobs;
-            rc = timed_launch(c, SL_KERNEL_MSAMPLER, [&] { return sl::launch_mspec(A, c->m_spec_lds, c->stream); });
-        } else {
-            rc = timed_launch(c, SL_KERNEL_MSAMPLER_FB, [&] { return sl::launch_mfb(A, c->stream); });
-            A.n_jobs = jobs;
-            if (!rc && jobs)
-                rc = timed_launch(c, SL_KERNEL_MSAMPLER, [&] { return sl::launch_mstep(A, c->m_lds, c->stream); });
-        }
+        rc = timed_launch(c, SL_KERNEL_MSAMPLER_FB, [&] { return sl::launch_mfb(A, c->stream); });
+        A.n_jobs = k + 1 < nm ? c->M : 0;
+        if (!rc && A.n_jobs)
+            rc = timed_launch(c, SL_KERNEL_MSAMPLER, [&] { return sl::launch_mstep(A, c->m_lds, c->stream); });
         if (rc) return rc;
     }
     return SL_OK;

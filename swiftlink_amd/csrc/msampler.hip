@@ -98,6 +98,11 @@ __device__ __forceinline__ uint32_t uf_find(const uint32_t* uf, uint32_t x, uint
     return x;
 }
 
+// genotype of person i at locus l from the 2-bit packed table (0: untyped)
+__device__ __forceinline__ int genotype_at(const MsampArgs& A, int l, int i) {
+    return (int)((A.geno2[(size_t)l * A.GW + (i >> 4)] >> (2 * (i & 15))) & 3u);
+}
+
 // log of the founder-allele-graph likelihood of the column in w.colw at locus
 // l (-inf when illegal), computed by one wavefront
 __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, const WaveArea& w, int l) {
@@ -105,12 +110,11 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
     const int F = A.F, nodes = 2 * A.F;
     // the first genotype chunks are loaded before the label pass so that the
     // global-load latency overlaps it
-    const uint8_t* gcol = A.geno_t + (size_t)l * A.N;
     int gpre[kGenoPrefetch];
 #pragma unroll
     for (int c = 0; c < kGenoPrefetch; ++c) {
         const int i = 64 * c + lane;
-        gpre[c] = i < A.N ? gcol[i] : 0;
+        gpre[c] = i < A.N ? genotype_at(A, l, i) : 0;
     }
     // founder-allele labels of non-founders, parents first (reset, :543-569)
     for (int gen = 0; gen < A.n_gen; ++gen) {
@@ -142,7 +146,7 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
 #pragma unroll
         for (int k = 0; k < kGenoPrefetch; ++k)
             if (c == k) g = gpre[k];
-        if (c >= kGenoPrefetch) g = i < A.N ? gcol[i] : 0;
+        if (c >= kGenoPrefetch) g = i < A.N ? genotype_at(A, l, i) : 0;
         uint32_t e = 0;
         if (g) {
             const uint32_t x = i < F ? (uint32_t)(2 * i) : w.lab[2 * (i - F)];
@@ -245,12 +249,11 @@ __device__ double fag_log_likelihood_r(const MsampArgs& A, const uint2* ptab, co
     constexpr int KN = 2 * KC;  // node chunks (2F <= 2N)
     const int lane = lane_id();
     const int N = A.N, F = A.F, NF = A.N - A.F, nodes = 2 * A.F;
-    const uint8_t* gcol = A.geno_t + (size_t)l * N;
     int g[KC];
 #pragma unroll
     for (int c = 0; c < KC; ++c) {
         const int i = 64 * c + lane;
-        g[c] = i < N ? gcol[i] : 0;
+        g[c] = i < N ? genotype_at(A, l, i) : 0;
     }
     const double minor = A.minor[l];
 #pragma unroll
@@ -520,11 +523,14 @@ __device__ void fag_job(const MsampArgs& A, int chain, int q, const uint2* ptab,
         l = q;
         current = true;
     } else {
+        // with two variants (A.spec, k > 0), jobs 2l and 2l+1 share a
+        // workgroup, and so an XCD's L2, for locus l's column
         const int r = A.with_lcur ? q - M : q;
-        v = r / M;
-        l = r - v * M;
+        const bool two = A.spec && A.step > 0;
+        v = two ? (r & 1) : 0;
+        l = two ? (r >> 1) : r;
         flip = order_at(A, chain, A.step);
-        if (A.spec && A.step > 0) force = order_at(A, chain, A.step - 1);
+        if (two) force = order_at(A, chain, A.step - 1);
     }
     WaveArea w = wave_area(A, wbase);
     const uint32_t* src = A.graphs + ((size_t)chain * M + l) * A.W;

@@ -24,7 +24,9 @@ struct MsampArgs {
     int ptab_bytes;               // LDS bytes of the staged ptab
     const int32_t* gen_off;       // [n_gen+1] non-founders grouped by generation
     int n_gen;
-    const uint8_t* geno_t;        // [M][N] marker genotype, 0 when untyped
+    const uint32_t* geno2;        // [M][GW] marker genotypes, 2 bits per person (bits 2(i%16) of word
+                                  // i/16; 0 untyped, 1 heterozygous, 2/3 homozygous A/B)
+    int GW;                       // ceil(N/16)
     const double* minor;          // [M] minor allele frequency
     const double* theta;          // [M-1]
     const int32_t* meioses;       // [n_meioses] non-ignorable meioses (bit index 2(p-F)+parent)
