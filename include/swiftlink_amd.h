@@ -241,6 +241,61 @@ int sl_mcmc_likelihood(sl_ctx* ctx, double* per_locus);
  * the reference's serial one).  No counterpart in the reference. */
 int sl_mcmc_near_edge(sl_ctx* ctx, int64_t* counts);
 
+/* ---- The trait model and the adjoint of the trait peel ----
+ *
+ * The reference fixes the disease model at start-up (DiseaseModel,
+ * disease_model.cc; Person::init_probs, person.cc:85-119) and scores LOD
+ * only.  These entry points have no counterpart there: they vary the model
+ * and differentiate the trait peel, for MOD-score maximisation over sampled
+ * descent graphs (the graphs depend on the marker data only). */
+
+/* A disease model in the reference's terms: penetrance[k] = P(affected | k
+ * disease alleles), the disease allele frequency, and per person the
+ * affection status 0 unknown, 1 unaffected, 2 affected.  Autosomal. */
+typedef struct {
+    double penetrance[3];
+    double disease_freq;
+    const int8_t* affection;      /* [N] */
+} sl_trait_model;
+
+/* The model's disease probabilities disease_prob[N][4] (phased trait order
+ * UU, AA, AU, UA; Person::init_probs with DiseaseModel's penetrance and
+ * allele prior, disease_model.cc:146-187) and their derivatives
+ * jac[N][4][4] with respect to (penetrance[0..2], disease_freq).  Founders
+ * come first.  Host only; either output may be NULL. */
+int sl_trait_model_tables(const sl_trait_model* model, int32_t n_people, int32_t n_founders, double* disease_prob,
+                          double* jac);
+
+/* Replaces the context's disease probabilities with the model's, recompiles
+ * the LOD peel program for the new set of non-zero trait values and drops
+ * the cached P(T).  The LOD accumulators keep what was added under the old
+ * model (call sl_lod_reset). */
+int sl_set_trait_model(sl_ctx* ctx, const sl_trait_model* model);
+
+/* Adjoint of the trait peel.  For every graph g (host, reference layout),
+ * interval l and position o: out[((g*(M-1) + l)*n + o)*7 + 0] = ln L - R - T
+ * (as sl_lod_terms) and [1..6] = d ln L / d(penetrance[0..2], disease_freq,
+ * theta_L, theta_R), theta_L = (o+1) p_l and theta_R = (n-o) p_l being the
+ * position's flanking recombination fractions (linear partial theta,
+ * genetic_map.h:186-192).  One workgroup per (graph, interval, position)
+ * runs the peel forward with every table kept in LDS, then sweeps the ops
+ * in reverse; nothing is stored between calls.  Requires
+ * sl_set_trait_model; a zero likelihood gives ln L = -inf and zero
+ * derivatives. */
+int sl_trait_grad(sl_ctx* ctx, const int32_t* graphs, int32_t n_graphs, double* out);
+
+/* out[0] = ln P(T), out[1..4] = its derivatives with respect to
+ * (penetrance[0..2], disease_freq).  Requires sl_set_trait_model. */
+int sl_trait_prob_grad(sl_ctx* ctx, double* out);
+
+/* LOD at every position over graphs already on the device (packed,
+ * uint32 [G][M][W]), lod[l*n+o] = (ln sum_g e^(ln L_g - R - T) - ln G -
+ * ln P(T)) / ln 10, and its derivatives dlod[(l*n+o)*6 + k] with respect to
+ * the six parameters of sl_trait_grad: the mean of the graphs' derivatives
+ * weighted by e^(ln L_g - R - T), minus those of ln P(T), over ln 10.
+ * Requires sl_set_trait_model. */
+int sl_lod_grad(sl_ctx* ctx, const uint32_t* dev_graphs, int32_t n_graphs, double* lod, double* dlod);
+
 /* Compiles a peel table into the level-scheduled program the LOD kernel
  * runs and serialises it into buf (int32 words; *needed gets the length).
  * Host only, no device work: lets the program's indexing be checked on a

@@ -256,6 +256,138 @@ double orc_trait_prob(const orc_case* c) {
     return log(orc_trait_likelihood(c, NULL, 0, 1));
 }
 
+/* ---------------------------------------------------------------------------
+ * Gradient of the trait peel in forward mode: every table cell carries its
+ * value and its derivatives with respect to (penetrance 0..2, disease allele
+ * frequency, theta_L, theta_R).  Every cell is evaluated: the lod-cell
+ * pruning of trait_peel drops cells whose disease probability is zero, but
+ * not whose derivative is zero.  The formulas are trait_peel's
+ * (trait_rfunction.cc:9-137, rfunction.cc:116-142) differentiated by the
+ * product rule; this is independent of the library's reverse sweep.
+ * ------------------------------------------------------------------------- */
+#define ORC_ND 6
+typedef struct {
+    double v, d[ORC_ND];
+} dual_t;
+
+static dual_t dual_const(double v) {
+    dual_t r;
+    r.v = v;
+    for (int k = 0; k < ORC_ND; ++k) r.d[k] = 0.0;
+    return r;
+}
+
+static dual_t dual_mul(dual_t a, dual_t b) {
+    dual_t r;
+    r.v = a.v * b.v;
+    for (int k = 0; k < ORC_ND; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+
+static dual_t dual_add(dual_t a, dual_t b) {
+    dual_t r;
+    r.v = a.v + b.v;
+    for (int k = 0; k < ORC_ND; ++k) r.d[k] = a.d[k] + b.d[k];
+    return r;
+}
+
+typedef struct {
+    dual_t theta, antitheta, theta2, antitheta2;
+} dthetas_t;
+
+static dual_t trait_rho_dual(const orc_case* c, const int32_t* dg, int locus, const dthetas_t* t, int person, int i,
+                             int j) {
+    dual_t r = dual_const(1.0);
+    r = dual_mul(r, (dg_get(c, dg, person, locus, 0) == i) ? t->antitheta : t->theta);
+    r = dual_mul(r, (dg_get(c, dg, person, locus + 1, 0) == i) ? t->antitheta2 : t->theta2);
+    r = dual_mul(r, (dg_get(c, dg, person, locus, 1) == j) ? t->antitheta : t->theta);
+    r = dual_mul(r, (dg_get(c, dg, person, locus + 1, 1) == j) ? t->antitheta2 : t->theta2);
+    return r;
+}
+
+int orc_trait_grad(const orc_case* c, const int32_t* dg, int locus, int offset, const double* jac, double* out) {
+    dual_t** tab = (dual_t**)calloc((size_t)c->n_ops, sizeof(dual_t*));
+    dual_t* dp = (dual_t*)malloc(sizeof(dual_t) * 4 * (size_t)c->N);
+    int* asg = (int*)malloc(sizeof(int) * (size_t)c->N);
+    if (!tab || !dp || !asg) return -1;
+    for (int p = 0; p < c->N; ++p)
+        for (int t = 0; t < 4; ++t) {
+            dual_t x = dual_const(c->disease_prob[4 * p + t]);
+            for (int k = 0; k < 4; ++k) x.d[k] = jac[(4 * p + t) * 4 + k];
+            dp[4 * p + t] = x;
+        }
+    dthetas_t th;
+    th.theta = dual_const(0.0);
+    th.antitheta = dual_const(1.0);
+    th.theta2 = dual_const(0.0);
+    th.antitheta2 = dual_const(1.0);
+    if (dg) {
+        th.theta.v = c->partial_theta[locus] * offset;
+        th.theta.d[4] = 1.0;
+        th.antitheta.v = 1.0 - th.theta.v;
+        th.antitheta.d[4] = -1.0;
+        th.theta2.v = c->partial_theta[locus] * (c->nlod + 1 - offset);
+        th.theta2.d[5] = 1.0;
+        th.antitheta2.v = 1.0 - th.theta2.v;
+        th.antitheta2.d[5] = -1.0;
+    }
+    const dual_t quarter = dual_const(0.25);
+    for (int o = 0; o < c->n_ops; ++o) {
+        const int32_t* cut = c->cut + c->cut_off[o];
+        int k = c->cut_off[o + 1] - c->cut_off[o];
+        int size = 1 << (2 * k);
+        int peel = c->op_peel[o];
+        tab[o] = (dual_t*)malloc(sizeof(dual_t) * (size_t)size);
+        for (int cell = 0; cell < size; ++cell) {
+            for (int q = 0; q < k; ++q) asg[cut[q]] = (cell >> (2 * q)) & 3;
+            dual_t total = dual_const(0.0);
+            for (int a = 0; a < 4; ++a) {
+                /* CHILD: a = 2i + j, the peel node's trait follows; otherwise the node's value */
+                int i = a >> 1, j = a & 1, val = a;
+                if (c->op_type[o] == OP_CHILD) val = phased_trait(asg[c->mother[peel]], asg[c->father[peel]], i, j);
+                asg[peel] = val;
+                dual_t tmp = dp[4 * peel + val];
+                for (int q = c->prev_off[o]; q < c->prev_off[o + 1]; ++q) {
+                    int pr = c->prev[q];
+                    const int32_t* keys = c->cut + c->cut_off[pr];
+                    int np = c->cut_off[pr + 1] - c->cut_off[pr];
+                    tmp = dual_mul(tmp, tab[pr][table_index(keys, np, asg)]);
+                }
+                if (c->op_type[o] == OP_CHILD) {
+                    tmp = dual_mul(tmp, dg ? dual_mul(quarter, trait_rho_dual(c, dg, locus, &th, peel, i, j)) : quarter);
+                } else if (c->op_type[o] == OP_PARENT) {
+                    dual_t child_prob = dual_const(1.0);
+                    for (int q = 0; q < k; ++q) {
+                        int ch = cut[q];
+                        if (c->mother[ch] != peel && c->father[ch] != peel) continue;
+                        int mat = c->mother[ch] == peel ? val : asg[c->mother[ch]];
+                        int pat = c->father[ch] == peel ? val : asg[c->father[ch]];
+                        dual_t ctmp = dual_const(0.0);
+                        for (int ii = 0; ii < 2; ++ii)
+                            for (int jj = 0; jj < 2; ++jj) {
+                                if (phased_trait(mat, pat, ii, jj) != asg[ch]) continue;
+                                ctmp = dual_add(ctmp, dg ? dual_mul(quarter, trait_rho_dual(c, dg, locus, &th, ch, ii, jj))
+                                                         : quarter);
+                            }
+                        child_prob = dual_mul(child_prob, ctmp);
+                    }
+                    tmp = dual_mul(tmp, child_prob);
+                }
+                total = dual_add(total, tmp);
+            }
+            tab[o][cell] = total;
+        }
+    }
+    const dual_t L = tab[c->n_ops - 1][0];
+    out[0] = L.v;
+    for (int k = 0; k < ORC_ND; ++k) out[1 + k] = L.d[k];
+    for (int o = 0; o < c->n_ops; ++o) free(tab[o]);
+    free(tab);
+    free(dp);
+    free(asg);
+    return 0;
+}
+
 /* DescentGraph::get_recombination_prob(locus, false) (descent_graph.cc:212-242) */
 static double recomb_log(const orc_case* c, const int32_t* dg, int locus) {
     double tmp = 0.0;

@@ -60,6 +60,14 @@ double orc_trait_prob(const orc_case* c);
  * `offset` in 1..nlod (trait_rfunction.cc:28-137, rfunction.cc:116-198). */
 double orc_trait_likelihood(const orc_case* c, const int32_t* dg, int locus, int offset);
 
+/* Gradient of the trait peel likelihood (not log) at interval `locus`,
+ * sub-position `offset` in 1..nlod, or of P(T) when dg is NULL: out[0] = L,
+ * out[1..6] = dL/d(penetrance 0..2, disease allele frequency, theta_L,
+ * theta_R), given jac[N][4][4] = d disease_prob[p][t] / d(penetrance 0..2,
+ * frequency).  Forward mode over every cell (trait_rfunction.cc:9-137
+ * differentiated).  Returns 0, or -1 when out of memory. */
+int orc_trait_grad(const orc_case* c, const int32_t* dg, int locus, int offset, const double* jac, double* out);
+
 /* Peeler::process (peeler.cc:79-103): out[o] = ln L - R_locus - T, o = 0..nlod-1.
  * Returns 0, or -1 when a likelihood is <= 0. */
 int orc_lod_terms(const orc_case* c, const int32_t* dg, int locus, double* out);

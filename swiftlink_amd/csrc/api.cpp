@@ -2,6 +2,7 @@
 // kernel launches.  Every entry point returns a status code; nothing aborts.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -23,6 +24,8 @@ thread_local std::string g_create_error;
 // speculative M-steps double the likelihood wavefronts; used while they fit
 // in about two wavefronts per SIMD of the device
 constexpr int64_t kSpecWaves = 8192;
+// graphs per adjoint launch (its grid's z extent)
+constexpr int kMaxGridZ = 65535;
 }
 
 struct sl_ctx {
@@ -66,6 +69,29 @@ struct sl_ctx {
     int gpw = 4;
     size_t lds = 0;
     int arena_bytes = 0, maxbits_bytes = 0;
+    // the peel table and disease probabilities (host copies, for recompiles)
+    std::vector<int32_t> pk_type, pk_peel, pk_cut_off, pk_cut, pk_prev_off, pk_prev, pk_child_off, pk_child;
+    std::vector<double> dp_h;
+    bool has_model = false;
+    // adjoint of the trait peel: unpruned program, one position, no arena reuse
+    PeelProgram gprog;
+    bool g_ready = false;
+    size_t g_lds = 0;
+    sl::OpDesc* g_ops = nullptr;
+    sl::PrevDesc* g_prevs = nullptr;
+    sl::ChildDesc* g_children = nullptr;
+    uint32_t* g_tasks = nullptr;
+    uint32_t* g_tinfo = nullptr;
+    uint16_t* g_idx = nullptr;
+    int32_t* g_level_base = nullptr;
+    int32_t* g_sched_off = nullptr;
+    int32_t* g_sched_ops = nullptr;
+    double* d_gdp = nullptr;    // [N][4] disease probabilities of the model
+    double* d_jac = nullptr;    // [N][4][4] their derivatives
+    double* d_gterms = nullptr;
+    size_t gterms_cap = 0;
+    double* d_gstate = nullptr;
+    size_t gstate_cap = 0;
     // sampler program and chain state
     sl::SampProgram sprog;
     int samp_block = 256;  // L-sampler workgroup: 4 wavefronts share each peel level
@@ -245,7 +271,9 @@ int release_all(sl_ctx* c) {
                     c->d_sched_off,
                     c->d_sched_ops, c->d_trait_d,
                     c->d_trait_f, c->d_ptheta, c->d_logt, c->d_loga, c->d_acc, c->d_scalar,
-                    c->d_partial, c->d_packed, c->d_stage, c->d_terms};
+                    c->d_partial, c->d_packed, c->d_stage, c->d_terms,
+                    c->g_ops, c->g_prevs, c->g_children, c->g_tasks, c->g_tinfo, c->g_idx, c->g_level_base,
+                    c->g_sched_off, c->g_sched_ops, c->d_gdp, c->d_jac, c->d_gterms, c->d_gstate};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
@@ -418,6 +446,18 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
         return fail(nullptr, SL_E_INVALID, "unknown msampler_mode");
     }
 
+    {
+        const int n = peel->n_ops;
+        c->pk_type.assign(peel->type, peel->type + n);
+        c->pk_peel.assign(peel->peel_node, peel->peel_node + n);
+        c->pk_cut_off.assign(peel->cut_offsets, peel->cut_offsets + n + 1);
+        c->pk_cut.assign(peel->cutset, peel->cutset + peel->cut_offsets[n]);
+        c->pk_prev_off.assign(peel->prev_offsets, peel->prev_offsets + n + 1);
+        c->pk_prev.assign(peel->prev, peel->prev + peel->prev_offsets[n]);
+        c->pk_child_off.assign(peel->child_offsets, peel->child_offsets + n + 1);
+        c->pk_child.assign(peel->children, peel->children + peel->child_offsets[n]);
+        c->dp_h.assign(ped->disease_prob, ped->disease_prob + 4 * (size_t)c->N);
+    }
     sl::PedigreeView pv{c->N, c->F, ped->mother, ped->father, ped->disease_prob};
     sl::PeelView kv{peel->n_ops, peel->type, peel->peel_node, peel->cut_offsets, peel->cutset,
                     peel->prev_offsets, peel->prev, peel->child_offsets, peel->children};
@@ -1024,6 +1064,272 @@ int sl_mcmc_near_edge(sl_ctx* c, int64_t* counts) {
     SL_HIP(c, hipMemcpyAsync(h.data(), c->d_near, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     SL_HIP(c, hipStreamSynchronize(c->stream));
     for (size_t i = 0; i < h.size(); ++i) counts[i] = (int64_t)h[i];
+    return SL_OK;
+}
+
+// ---- trait model and the adjoint of the trait peel ----
+
+namespace {
+
+sl::PeelView peel_view(const sl_ctx* c) {
+    auto p = [](const std::vector<int32_t>& v) { return v.empty() ? nullptr : v.data(); };
+    return sl::PeelView{(int)c->pk_type.size(), p(c->pk_type), p(c->pk_peel), p(c->pk_cut_off), p(c->pk_cut),
+                        p(c->pk_prev_off), p(c->pk_prev), p(c->pk_child_off), p(c->pk_child)};
+}
+
+// Person::init_probs (person.cc:85-119) with DiseaseModel's penetrance and
+// allele prior (disease_model.cc:146-187), autosomal, and the derivatives.
+// Phased order UU, AA, AU, UA carries 0, 2, 1, 1 disease alleles.
+int model_tables(const sl_trait_model* m, int N, int F, double* dp, double* jac) {
+    if (!m || !m->affection || N < 1 || F < 1 || F > N) return SL_E_INVALID;
+    const double f = m->disease_freq;
+    if (!(f > 0.0 && f < 1.0)) return SL_E_INVALID;
+    for (int k = 0; k < 3; ++k)
+        if (!(m->penetrance[k] >= 0.0 && m->penetrance[k] <= 1.0)) return SL_E_INVALID;
+    const int alleles[4] = {0, 2, 1, 1};
+    for (int i = 0; i < N; ++i) {
+        const int aff = m->affection[i];
+        if (aff < 0 || aff > 2) return SL_E_INVALID;
+        for (int t = 0; t < 4; ++t) {
+            const int k = alleles[t];
+            const double pen = m->penetrance[k];
+            double v, dpen = 0.0, dfreq = 0.0;
+            if (i >= F) {
+                v = aff == 0 ? 0.25 : aff == 2 ? pen : 1.0 - pen;
+                dpen = aff == 0 ? 0.0 : aff == 2 ? 1.0 : -1.0;
+            } else {
+                const double prior = k == 2 ? f * f : k == 1 ? f * (1.0 - f) : (1.0 - f) * (1.0 - f);
+                const double dprior = k == 2 ? 2.0 * f : k == 1 ? 1.0 - 2.0 * f : -2.0 * (1.0 - f);
+                if (aff == 2) {
+                    v = prior * pen;
+                    dpen = prior;
+                    dfreq = dprior * pen;
+                } else if (aff == 1) {
+                    v = prior * (1.0 - pen);
+                    dpen = -prior;
+                    dfreq = dprior * (1.0 - pen);
+                } else {
+                    v = (prior * pen) + (prior * (1.0 - pen));
+                    dfreq = dprior;
+                }
+            }
+            if (dp) dp[4 * i + t] = v;
+            if (jac) {
+                double* J = jac + (size_t)16 * i + 4 * t;
+                J[0] = J[1] = J[2] = 0.0;
+                J[k] = dpen;
+                J[3] = dfreq;
+            }
+        }
+    }
+    return SL_OK;
+}
+
+// (Re)compiles the LOD program for c->dp_h and uploads it with the trait
+// arrays; on failure the context keeps its old program.
+int install_lod_program(sl_ctx* c) {
+    sl::PedigreeView pv{c->N, c->F, c->mother_h.data(), c->father_h.data(), c->dp_h.data()};
+    PeelProgram P;
+    std::string why = sl::compile_lod_program(pv, peel_view(c), c->npos, c->block / 64, &P);
+    if (!why.empty()) return fail(c, SL_E_UNSUPPORTED, why);
+    const size_t esz = c->precision == SL_FP64 ? 8 : 4;
+    const int arena_bytes = (int)((P.arena_elems * esz + 15) / 16 * 16);
+    const int maxbits_bytes = c->precision == SL_FP64 ? 0 : (int)((P.n_ops * 4 + 15) / 16 * 16);
+    const size_t lds = (size_t)arena_bytes + maxbits_bytes + 4 * sl::kMaxPos * esz + ((c->N + 15) / 16 * 16);
+    if (lds > 160 * 1024) return fail(c, SL_E_UNSUPPORTED, "peel program needs more than 160 KiB of LDS");
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    void** bufs[] = {(void**)&c->d_ops, (void**)&c->d_prevs, (void**)&c->d_children, (void**)&c->d_tasks,
+                     (void**)&c->d_tinfo, (void**)&c->d_idx, (void**)&c->d_level_base, (void**)&c->d_sched_off,
+                     (void**)&c->d_sched_ops, (void**)&c->d_trait_d, (void**)&c->d_trait_f};
+    for (void** b : bufs) {
+        if (*b) SL_HIP(c, hipFree(*b));
+        *b = nullptr;
+    }
+    c->prog = std::move(P);
+    c->arena_bytes = arena_bytes;
+    c->maxbits_bytes = maxbits_bytes;
+    c->lds = lds;
+    SL_HIP(c, sl::configure_lod_lds(c->precision, c->lds));
+    std::vector<float> tf(c->dp_h.size());
+    for (size_t i = 0; i < tf.size(); ++i) tf[i] = (float)c->dp_h[i];
+    int rc;
+    if ((rc = upload(c, &c->d_ops, c->prog.ops.data(), c->prog.ops.size())) ||
+        (rc = upload(c, &c->d_prevs, c->prog.prevs.data(), c->prog.prevs.size())) ||
+        (rc = upload(c, &c->d_children, c->prog.children.data(), c->prog.children.size())) ||
+        (rc = upload(c, &c->d_tasks, c->prog.tasks.data(), c->prog.tasks.size())) ||
+        (rc = upload(c, &c->d_tinfo, c->prog.tinfo.data(), c->prog.tinfo.size())) ||
+        (rc = upload(c, &c->d_idx, c->prog.idx.data(), c->prog.idx.size())) ||
+        (rc = upload(c, &c->d_level_base, c->prog.level_base.data(), c->prog.level_base.size())) ||
+        (rc = upload(c, &c->d_sched_off, c->prog.sched_off.data(), c->prog.sched_off.size())) ||
+        (rc = upload(c, &c->d_sched_ops, c->prog.sched_ops.data(), c->prog.sched_ops.size())) ||
+        (rc = upload(c, &c->d_trait_d, c->dp_h.data(), c->dp_h.size())) ||
+        (rc = upload(c, &c->d_trait_f, tf.data(), tf.size())))
+        return rc;
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    return SL_OK;
+}
+
+// The adjoint's program: every trait value legal (its cells are those of a
+// model with no zero disease probability), one position, no arena reuse.
+int ensure_grad_program(sl_ctx* c) {
+    if (!c->has_model) return fail(c, SL_E_INVALID, "no trait model: call sl_set_trait_model first");
+    if (c->g_ready) return SL_OK;
+    const std::vector<double> ones(4 * (size_t)c->N, 1.0);
+    sl::PedigreeView pv{c->N, c->F, c->mother_h.data(), c->father_h.data(), ones.data()};
+    std::string why = sl::compile_lod_program(pv, peel_view(c), 1, sl::kGradBlock / 64, &c->gprog, false);
+    if (!why.empty()) return fail(c, SL_E_UNSUPPORTED, why);
+    c->g_lds = 2 * (size_t)c->gprog.arena_elems * sizeof(double) + sizeof(double) * (4 + 6 * (sl::kGradBlock / 64)) +
+               ((c->N + 15) / 16 * 16);
+    if (c->g_lds > 160 * 1024)
+        return fail(c, SL_E_UNSUPPORTED, "the adjoint's peel tables need more than 160 KiB of LDS");
+    SL_HIP(c, sl::configure_grad_lds(c->g_lds));
+    const PeelProgram& P = c->gprog;
+    int rc;
+    if ((rc = upload(c, &c->g_ops, P.ops.data(), P.ops.size())) ||
+        (rc = upload(c, &c->g_prevs, P.prevs.data(), P.prevs.size())) ||
+        (rc = upload(c, &c->g_children, P.children.data(), P.children.size())) ||
+        (rc = upload(c, &c->g_tasks, P.tasks.data(), P.tasks.size())) ||
+        (rc = upload(c, &c->g_tinfo, P.tinfo.data(), P.tinfo.size())) ||
+        (rc = upload(c, &c->g_idx, P.idx.data(), P.idx.size())) ||
+        (rc = upload(c, &c->g_level_base, P.level_base.data(), P.level_base.size())) ||
+        (rc = upload(c, &c->g_sched_off, P.sched_off.data(), P.sched_off.size())) ||
+        (rc = upload(c, &c->g_sched_ops, P.sched_ops.data(), P.sched_ops.size())))
+        return rc;
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    c->g_ready = true;
+    return SL_OK;
+}
+
+sl::LodArgs grad_args(sl_ctx* c) {
+    sl::LodArgs A = lod_args(c);
+    const PeelProgram& P = c->gprog;
+    A.ops = c->g_ops;
+    A.prevs = c->g_prevs;
+    A.children = c->g_children;
+    A.tasks = c->g_tasks;
+    A.tinfo = c->g_tinfo;
+    A.idx = c->g_idx;
+    A.level_base = c->g_level_base;
+    A.sched_off = c->g_sched_off;
+    A.sched_ops = c->g_sched_ops;
+    A.waves = P.waves;
+    A.n_levels = P.n_levels;
+    A.last_op = P.last_op;
+    A.n_ops = P.n_ops;
+    A.npos = 1;
+    A.trait_d = c->d_gdp;
+    A.trait_f = nullptr;
+    A.arena_bytes = P.arena_elems * (int)sizeof(double);
+    A.maxbits_bytes = 0;
+    A.partial = nullptr;
+    A.terms = nullptr;
+    return A;
+}
+
+}  // namespace
+
+int sl_trait_model_tables(const sl_trait_model* model, int32_t N, int32_t F, double* dp, double* jac) {
+    return model_tables(model, N, F, dp, jac);
+}
+
+int sl_set_trait_model(sl_ctx* c, const sl_trait_model* model) {
+    if (!c || !model) return SL_E_INVALID;
+    std::vector<double> dp(4 * (size_t)c->N), jac(16 * (size_t)c->N);
+    if (model_tables(model, c->N, c->F, dp.data(), jac.data()) != SL_OK)
+        return fail(c, SL_E_INVALID, "trait model: penetrances in [0, 1], frequency in (0, 1), affection 0..2");
+    std::vector<double> old = c->dp_h;
+    c->dp_h = dp;
+    int rc = install_lod_program(c);
+    if (rc) {
+        c->dp_h = old;
+        return rc;
+    }
+    if (!c->d_gdp) SL_HIP(c, hipMalloc((void**)&c->d_gdp, dp.size() * sizeof(double)));
+    if (!c->d_jac) SL_HIP(c, hipMalloc((void**)&c->d_jac, jac.size() * sizeof(double)));
+    SL_HIP(c, hipMemcpyAsync(c->d_gdp, dp.data(), dp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SL_HIP(c, hipMemcpyAsync(c->d_jac, jac.data(), jac.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    c->has_model = true;
+    c->trait_prob_valid = false;
+    return SL_OK;
+}
+
+int sl_trait_grad(sl_ctx* c, const int32_t* graphs, int32_t G, double* out) {
+    if (!c || !graphs || !out || G <= 0) return SL_E_INVALID;
+    int rc = ensure_grad_program(c);
+    if (rc) return rc;
+    rc = stage_graphs(c, graphs, G);
+    if (rc) return rc;
+    const size_t n = (size_t)G * (c->M - 1) * c->npos * 7;
+    rc = ensure(c, &c->d_gterms, &c->gterms_cap, n);
+    if (rc) return rc;
+    sl::LodArgs A = grad_args(c);
+    const size_t per_graph = (size_t)(c->M - 1) * c->npos * 7;
+    for (int g0 = 0; g0 < G; g0 += kMaxGridZ) {  // the grid's z extent is the graph
+        A.graphs = c->d_packed + (size_t)g0 * c->M * c->W;
+        A.G = std::min(kMaxGridZ, G - g0);
+        sl::GradArgs Gd{c->d_jac, c->d_gterms + (size_t)g0 * per_graph, c->npos};
+        SL_HIP(c, sl::launch_grad(A, Gd, 0, c->g_lds, c->stream));
+    }
+    SL_HIP(c, hipMemcpyAsync(out, c->d_gterms, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    return SL_OK;
+}
+
+int sl_trait_prob_grad(sl_ctx* c, double* out) {
+    if (!c || !out) return SL_E_INVALID;
+    int rc = ensure_grad_program(c);
+    if (rc) return rc;
+    rc = ensure(c, &c->d_gterms, &c->gterms_cap, (size_t)7);
+    if (rc) return rc;
+    sl::LodArgs A = grad_args(c);
+    sl::GradArgs Gd{c->d_jac, c->d_gterms, c->npos};
+    SL_HIP(c, sl::launch_grad(A, Gd, 1, c->g_lds, c->stream));
+    double v[7];
+    SL_HIP(c, hipMemcpyAsync(v, c->d_gterms, sizeof v, hipMemcpyDeviceToHost, c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 5; ++k) out[k] = v[k];
+    return SL_OK;
+}
+
+int sl_lod_grad(sl_ctx* c, const uint32_t* dev_graphs, int32_t G, double* lod, double* dlod) {
+    if (!c || !dev_graphs || G <= 0 || !lod || !dlod) return SL_E_INVALID;
+    int rc = ensure_grad_program(c);
+    if (rc) return rc;
+    const int n = (c->M - 1) * c->npos;
+    // graphs per launch: at most 256 MiB of per-graph terms
+    const int chunk = (int)std::max<size_t>(
+        1, std::min<size_t>(std::min<size_t>((size_t)G, (size_t)kMaxGridZ), (256u << 20) / ((size_t)n * 7 * 8)));
+    if ((rc = ensure(c, &c->d_gterms, &c->gterms_cap, (size_t)chunk * n * 7)) ||
+        (rc = ensure(c, &c->d_gstate, &c->gstate_cap, (size_t)n * 7 + 7)))
+        return rc;
+    // state [n][7] starts at (ln 0, 0...)
+    std::vector<double> init((size_t)n * 7, 0.0);
+    for (int i = 0; i < n; ++i) init[(size_t)i * 7] = -INFINITY;
+    SL_HIP(c, hipMemcpyAsync(c->d_gstate, init.data(), init.size() * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    sl::LodArgs A = grad_args(c);
+    for (int g0 = 0; g0 < G; g0 += chunk) {
+        const int gc = std::min(chunk, G - g0);
+        A.graphs = dev_graphs + (size_t)g0 * c->M * c->W;
+        A.G = gc;
+        sl::GradArgs Gd{c->d_jac, c->d_gterms, c->npos};
+        SL_HIP(c, sl::launch_grad(A, Gd, 0, c->g_lds, c->stream));
+        SL_HIP(c, sl::launch_grad_fold(c->d_gstate, c->d_gterms, gc, n, c->stream));
+    }
+    // P(T) and its derivatives after the state
+    sl::GradArgs Gp{c->d_jac, c->d_gstate + (size_t)n * 7, c->npos};
+    SL_HIP(c, sl::launch_grad(A, Gp, 1, c->g_lds, c->stream));
+    std::vector<double> st((size_t)n * 7 + 7);
+    SL_HIP(c, hipMemcpyAsync(st.data(), c->d_gstate, st.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    const double* pt = st.data() + (size_t)n * 7;
+    const double ln10 = std::log(10.0), lnG = std::log((double)G);
+    for (int i = 0; i < n; ++i) {
+        const double* s = st.data() + (size_t)i * 7;
+        lod[i] = (s[0] - lnG - pt[0]) / ln10;
+        for (int k = 0; k < 6; ++k) dlod[(size_t)i * 6 + k] = (s[1 + k] - (k < 4 ? pt[1 + k] : 0.0)) / ln10;
+    }
     return SL_OK;
 }
 

@@ -34,7 +34,24 @@ struct LodArgs {
     int arena_bytes, maxbits_bytes;
 };
 
+// The adjoint of the trait peel (FP64): runs an unpruned program compiled
+// with one position per table and no arena reuse, one workgroup of
+// kGradBlock threads per (interval, position, graph).
+constexpr int kGradBlock = 256;
+
+struct GradArgs {
+    const double* jac;   // [N][4][4] d disease_prob / d(penetrance 0..2, disease freq)
+    double* out;         // [G][M-1][npos][7]: ln L - R - T, then 6 derivatives of ln L; [7] for P(T)
+    int npos;            // trait positions per interval (the program itself has one)
+};
+
 hipError_t launch_lod(const LodArgs& A, int precision, int block, size_t lds, int nchunks, hipStream_t s);
+hipError_t configure_grad_lds(size_t lds);
+// nodg: P(T) (grid of one); else grid (M-1, npos, A.G)
+hipError_t launch_grad(const LodArgs& A, const GradArgs& Gd, int nodg, size_t lds, hipStream_t s);
+// folds G graphs' terms [G][n][7] into the running state [n][7] (ln sum e^t,
+// then the e^(t - state)-weighted sums of the derivatives)
+hipError_t launch_grad_fold(double* state, const double* terms, int G, int n, hipStream_t s);
 hipError_t launch_trait_prob(const LodArgs& A, int precision, int block, size_t lds, double* out, hipStream_t s);
 hipError_t launch_lod_merge(double* acc, const double* partial, int nchunks, int n, hipStream_t s);
 hipError_t launch_fill(double* p, double v, int n, hipStream_t s);

@@ -349,6 +349,260 @@ __global__ void trait_prob_kernel(LodArgs A, double* out) {
     if (threadIdx.x == 0) out[0] = final_log<T, RESCALE>(A, arena, maxbits, 0);
 }
 
+// ---- Adjoint of the trait peel ----
+//
+// Forward: run_levels over a program compiled with every trait value legal
+// (a zero disease probability zeroes a term but not its derivative), one
+// position and a separate arena slot per table.  Reverse: adj holds d ln L /
+// d(table cell); the ops are visited level by level from the last, and each
+// (cell, term) of an op, term = f * prod_q P_q with f the term's non-table
+// factor, passes adj * f * prod_{r != q} P_r to its consumed cell q (LDS
+// atomics: cells of one op read the same consumed cells) and adds its share of
+// the parameter derivatives to per-thread sums.  Each table has one consumer,
+// so a table's adjoint is complete once the levels above it have run.
+
+// rho and its derivatives with respect to theta_L (t1) and theta_R (t2)
+__device__ __forceinline__ void rho_grad(uint32_t code, int i, int j, const Thetas<double>& th, double* r,
+                                         double* dl, double* dr) {
+    const bool e1 = (int)(code & 1u) == i, e2 = (int)((code >> 1) & 1u) == i;
+    const bool e3 = (int)((code >> 2) & 1u) == j, e4 = (int)((code >> 3) & 1u) == j;
+    const double f1 = e1 ? th.a1 : th.t1, f2 = e2 ? th.a2 : th.t2, f3 = e3 ? th.a1 : th.t1, f4 = e4 ? th.a2 : th.t2;
+    const double s1 = e1 ? -1.0 : 1.0, s2 = e2 ? -1.0 : 1.0, s3 = e3 ? -1.0 : 1.0, s4 = e4 ? -1.0 : 1.0;
+    double x = 1.0;
+    x *= f1;
+    x *= f2;
+    x *= f3;
+    x *= f4;
+    *r = x;
+    *dl = s1 * f2 * f3 * f4 + f1 * f2 * s3 * f4;
+    *dr = f1 * s2 * f3 * f4 + f1 * f2 * f3 * s4;
+}
+
+template <bool NODG>
+__device__ void adjoint_op(const LodArgs& A, const GradArgs& Gd, const OpDesc& d, const double tr[4],
+                           const PrevSet<double>& P, const int* kids, const double* arena, double* adj,
+                           const uint8_t* code, const Thetas<double>& th, double acc[6], int lane) {
+    const int np = d.n_prev;
+    const double* J = Gd.jac + (size_t)16 * d.peel;
+    for (int c = lane; c < d.size; c += 64) {
+        const double a = adj[d.tab_off + c];
+        if (a == 0.0) continue;
+        const uint4 ti = *reinterpret_cast<const uint4*>(A.tinfo + (size_t)(d.task_base + c) * 4);
+        const uint32_t info[4] = {ti.x, ti.y, ti.z, ti.w};
+        const uint16_t* ix = A.idx + d.idx_base + (size_t)c * 4 * np;
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t val = d.type == kChild ? (info[t] & 3u) : (uint32_t)t;
+            // consumed cells and prefix products
+            double pv[kMaxPrev], pre[kMaxPrev + 1];
+            pre[0] = 1.0;
+#pragma unroll
+            for (int q = 0; q < kMaxPrev; ++q) {
+                pv[q] = q < np ? arena[P.tab[q] + ix[t * np + q]] : 1.0;
+                pre[q + 1] = pre[q] * pv[q];
+            }
+            // the term's non-table factor without the trait probability, and
+            // its derivatives with respect to theta_L, theta_R
+            double fr = 1.0, fl = 0.0, fq = 0.0;
+            if (d.type == kChild) {
+                if (NODG) {
+                    fr = 0.25;
+                } else {
+                    double r, dl, dr;
+                    rho_grad((uint32_t)code[d.peel], t >> 1, t & 1, th, &r, &dl, &dr);
+                    fr = 0.25 * r;
+                    fl = 0.25 * dl;
+                    fq = 0.25 * dr;
+                }
+            } else if (d.type == kParent) {
+                // product over children of S_ch, with the derivative of the product
+                double S = 1.0, SL = 0.0, SR = 0.0;
+                for (int ch = 0; ch < d.n_child; ++ch) {
+                    const uint32_t m = (info[t] >> (4 + 4 * ch)) & 15u;
+                    double s = 0.0, sl = 0.0, sr = 0.0;
+                    for (int ij = 0; ij < 4; ++ij) {
+                        if (!((m >> ij) & 1u)) continue;
+                        if (NODG) {
+                            s += 0.25;
+                        } else {
+                            double r, dl, dr;
+                            rho_grad((uint32_t)code[kids[ch]], ij >> 1, ij & 1, th, &r, &dl, &dr);
+                            s += 0.25 * r;
+                            sl += 0.25 * dl;
+                            sr += 0.25 * dr;
+                        }
+                    }
+                    SL = SL * s + S * sl;
+                    SR = SR * s + S * sr;
+                    S *= s;
+                }
+                fr = S;
+                fl = SL;
+                fq = SR;
+            }
+            const double trv = tr[val];
+            const double prodP = pre[kMaxPrev];
+            // d / d disease_prob[peel][val], contracted with its Jacobian
+            const double gtr = a * fr * prodP;
+            if (gtr != 0.0) {
+                const double* Jv = J + 4 * val;
+                acc[0] += gtr * Jv[0];
+                acc[1] += gtr * Jv[1];
+                acc[2] += gtr * Jv[2];
+                acc[3] += gtr * Jv[3];
+            }
+            const double at = a * trv;
+            if (!NODG && at != 0.0) {
+                acc[4] += at * prodP * fl;
+                acc[5] += at * prodP * fq;
+            }
+            // consumed cells
+            if (at * fr != 0.0) {
+                double suf = 1.0;
+#pragma unroll
+                for (int q = kMaxPrev - 1; q >= 0; --q) {
+                    if (q < np) {
+                        atomicAdd(&adj[P.tab[q] + ix[t * np + q]], at * fr * pre[q] * suf);
+                        suf *= pv[q];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Reverse sweep over the levels, last first.
+template <bool NODG>
+__device__ void adjoint_levels(const LodArgs& A, const GradArgs& Gd, const double* arena, double* adj,
+                               const uint8_t* code, const double* ths, double acc[6]) {
+    const int W = A.waves;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63u);
+    Thetas<double> th;
+    if (!NODG) {
+        th.t1 = ths[0];
+        th.a1 = ths[1];
+        th.t2 = ths[2];
+        th.a2 = ths[3];
+    }
+    for (int L = A.n_levels - 1; L >= 0; --L) {
+        const int s0 = ((cint_p)A.sched_off)[L * W + wave];
+        const int s1 = ((cint_p)A.sched_off)[L * W + wave + 1];
+        for (int s = s0; s < s1; ++s) {
+            const int op = ((cint_p)A.sched_ops)[s];
+            const OpDesc d = load_op(A.ops, op);
+            double tr[4];
+            for (int v = 0; v < 4; ++v) tr[v] = load_const<double>(A.trait_d, 4 * d.peel + v);
+            PrevSet<double> P;
+#pragma unroll
+            for (int q = 0; q < kMaxPrev; ++q) {
+                if (q < d.n_prev) {
+                    cint_p pd = (cint_p)(A.prevs + d.prev_off + q);
+                    P.tab[q] = pd[1];
+                    P.size[q] = pd[2];
+                } else {
+                    P.tab[q] = 0;
+                    P.size[q] = 0;
+                }
+                P.scale[q] = 1.0;
+            }
+            int kids[7];
+#pragma unroll
+            for (int ch = 0; ch < 7; ++ch) kids[ch] = ch < d.n_child ? ((cint_p)(A.children + d.child_off + ch))[0] : 0;
+            adjoint_op<NODG>(A, Gd, d, tr, P, kids, arena, adj, code, th, acc, lane);
+        }
+        __syncthreads();
+    }
+}
+
+// grid (M-1, npos, G) or, NODG, one workgroup for P(T)
+template <bool NODG>
+__global__ void __launch_bounds__(kGradBlock) grad_kernel(LodArgs A, GradArgs Gd) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int E = A.arena_bytes / 8;
+    double* arena = reinterpret_cast<double*>(smem);
+    double* adj = arena + E;
+    double* ths = adj + E;
+    double* red = ths + 4;
+    uint8_t* code = reinterpret_cast<uint8_t*>(red + 6 * (kGradBlock / 64));
+    const int l = blockIdx.x, pos = blockIdx.y, g = blockIdx.z;
+    const int tid = threadIdx.x;
+    const uint32_t* c0 = nullptr;
+    const uint32_t* c1 = nullptr;
+    if (!NODG) {
+        c0 = A.graphs + ((size_t)g * A.M + l) * A.W;
+        c1 = c0 + A.W;
+        if (tid == 0) {
+            // set_thetas (trait_rfunction.h:50-56): linear partial thetas
+            const double ptheta = A.partial_theta[l];
+            const double tl = ptheta * (double)(pos + 1), tq = ptheta * (double)(Gd.npos - pos);
+            ths[0] = tl;
+            ths[1] = 1.0 - tl;
+            ths[2] = tq;
+            ths[3] = 1.0 - tq;
+        }
+        for (int p = A.F + tid; p < A.N; p += blockDim.x) {
+            const int b = 2 * (p - A.F);
+            const uint32_t m0 = (c0[b >> 5] >> (b & 31)) & 1u;
+            const uint32_t m1 = (c1[b >> 5] >> (b & 31)) & 1u;
+            const uint32_t p0 = (c0[(b + 1) >> 5] >> ((b + 1) & 31)) & 1u;
+            const uint32_t p1 = (c1[(b + 1) >> 5] >> ((b + 1) & 31)) & 1u;
+            code[p] = (uint8_t)(m0 | (m1 << 1) | (p0 << 2) | (p1 << 3));
+        }
+    }
+    for (int i = tid; i < E; i += blockDim.x) adj[i] = 0.0;
+    __syncthreads();
+    run_levels<double, false, NODG>(A, arena, nullptr, NODG ? nullptr : code, NODG ? nullptr : ths);
+    const OpDesc last = load_op(A.ops, A.last_op);
+    const double L = arena[last.tab_off];
+    if (tid == 0) adj[last.tab_off] = L > 0.0 ? 1.0 / L : 0.0;
+    __syncthreads();
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (L > 0.0) adjoint_levels<NODG>(A, Gd, arena, adj, NODG ? nullptr : code, ths, acc);
+    // workgroup sums of the six derivatives
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off);
+    if ((tid & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) red[6 * (tid >> 6) + k] = acc[k];
+    __syncthreads();
+    if (tid == 0) {
+        double* o = NODG ? Gd.out : Gd.out + (((size_t)g * (A.M - 1) + l) * Gd.npos + pos) * 7;
+        double lv = L > 0.0 ? log(L) : -INFINITY;
+        if (!NODG) {
+            int ncross = 0;
+            for (int w = 0; w < A.W; ++w) ncross += __popc(c0[w] ^ c1[w]);
+            const int nm = 2 * (A.N - A.F);
+            lv -= (double)ncross * A.log_theta[l] + (double)(nm - ncross) * A.log_anti[l] + A.trans;
+        }
+        o[0] = lv;
+        for (int k = 0; k < 6; ++k) {
+            double v = 0.0;
+            for (int w = 0; w < kGradBlock / 64; ++w) v += red[6 * w + k];
+            o[1 + k] = v;
+        }
+    }
+}
+
+// running state [n][7]: ln sum_g e^{t_g} and sum_g e^{t_g - state} dt_g
+__global__ void grad_fold_kernel(double* state, const double* terms, int G, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double S = state[(size_t)i * 7];
+    double V[6];
+    for (int k = 0; k < 6; ++k) V[k] = state[(size_t)i * 7 + 1 + k];
+    for (int g = 0; g < G; ++g) {
+        const double* t = terms + ((size_t)g * n + i) * 7;
+        if (t[0] == -INFINITY) continue;
+        const double S2 = lse(S, t[0]);
+        const double wo = S == -INFINITY ? 0.0 : exp(S - S2), wn = exp(t[0] - S2);
+        for (int k = 0; k < 6; ++k) V[k] = V[k] * wo + wn * t[1 + k];
+        S = S2;
+    }
+    state[(size_t)i * 7] = S;
+    for (int k = 0; k < 6; ++k) state[(size_t)i * 7 + 1 + k] = V[k];
+}
+
 __global__ void lod_merge_kernel(double* acc, const double* partial, int nchunks, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -407,6 +661,28 @@ hipError_t launch_lod(const LodArgs& A, int precision, int block, size_t lds, in
         hipLaunchKernelGGL((lod_kernel<double, false>), grid, dim3(block), lds, s, A);
     else
         hipLaunchKernelGGL((lod_kernel<float, true>), grid, dim3(block), lds, s, A);
+    return hipGetLastError();
+}
+
+hipError_t configure_grad_lds(size_t lds) {
+    if (lds <= 64 * 1024) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute((const void*)grad_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)grad_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return e;
+}
+
+hipError_t launch_grad(const LodArgs& A, const GradArgs& Gd, int nodg, size_t lds, hipStream_t s) {
+    if (nodg)
+        hipLaunchKernelGGL(grad_kernel<true>, dim3(1), dim3(kGradBlock), lds, s, A, Gd);
+    else
+        hipLaunchKernelGGL(grad_kernel<false>, dim3(A.M - 1, Gd.npos, A.G), dim3(kGradBlock), lds, s, A, Gd);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_fold(double* state, const double* terms, int G, int n, hipStream_t s) {
+    hipLaunchKernelGGL(grad_fold_kernel, dim3((n + 255) / 256), dim3(256), 0, s, state, terms, G, n);
     return hipGetLastError();
 }
 

@@ -74,7 +74,7 @@ struct Arena {
 }  // namespace
 
 std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int npos, int waves,
-                                PeelProgram* out) {
+                                PeelProgram* out, bool reuse) {
     PeelProgram& P = *out;
     P = PeelProgram();
     const int N = ped.N;
@@ -273,8 +273,10 @@ std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int
             offs[o] = arena.alloc(size[o] * npos);
             P.ops[o].tab_off = offs[o];
         }
-        for (int o : by_level[L])
-            for (int q = pv.prev_off[o]; q < pv.prev_off[o + 1]; ++q) arena.release(offs[pv.prev[q]], size[pv.prev[q]] * npos);
+        if (reuse)
+            for (int o : by_level[L])
+                for (int q = pv.prev_off[o]; q < pv.prev_off[o + 1]; ++q)
+                    arena.release(offs[pv.prev[q]], size[pv.prev[q]] * npos);
     }
     P.level_base[P.n_levels] = (int)P.tasks.size();
     for (auto& pd : P.prevs) pd.tab_off = offs[pd.op];

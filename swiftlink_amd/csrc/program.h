@@ -112,9 +112,10 @@ struct PeelView {
 
 // Compiles `peel` for `ped` with `npos` positions folded into each table slot
 // and a per-level schedule of ops over `waves` wavefronts (longest first onto
-// the least loaded wave).  Returns an empty string on success, else the reason
-// the table is rejected.
+// the least loaded wave).  With `reuse` false every table keeps its own arena
+// slot (the adjoint reads them all after the forward pass).  Returns an empty
+// string on success, else the reason the table is rejected.
 std::string compile_lod_program(const PedigreeView& ped, const PeelView& peel, int npos, int waves,
-                                PeelProgram* out);
+                                PeelProgram* out, bool reuse = true);
 
 }  // namespace sl

@@ -62,6 +62,10 @@ class McmcStats(C.Structure):
     _fields_ = [("l_sweeps", C.c_int64), ("m_sweeps", C.c_int64), ("scored", C.c_int64)]
 
 
+class TraitModel(C.Structure):
+    _fields_ = [("penetrance", C.c_double * 3), ("disease_freq", C.c_double), ("affection", C.POINTER(C.c_int8))]
+
+
 class SlError(RuntimeError):
     pass
 
@@ -94,6 +98,11 @@ _SIGS = {
     "sl_mcmc_run": (C.c_int, [C.c_void_p, C.POINTER(Schedule), C.POINTER(McmcStats)]),
     "sl_mcmc_likelihood": (C.c_int, [C.c_void_p, f64p]),
     "sl_mcmc_near_edge": (C.c_int, [C.c_void_p, i64p]),
+    "sl_trait_model_tables": (C.c_int, [C.POINTER(TraitModel), C.c_int32, C.c_int32, f64p, f64p]),
+    "sl_set_trait_model": (C.c_int, [C.c_void_p, C.POINTER(TraitModel)]),
+    "sl_trait_grad": (C.c_int, [C.c_void_p, i32p, C.c_int32, f64p]),
+    "sl_trait_prob_grad": (C.c_int, [C.c_void_p, f64p]),
+    "sl_lod_grad": (C.c_int, [C.c_void_p, u32p, C.c_int32, f64p, f64p]),
 }
 
 
@@ -113,6 +122,26 @@ def lib():
 
 def exported_symbols():
     return list(_SIGS)
+
+
+def _trait_model(penetrance, freq, affection):
+    aff = np.ascontiguousarray(affection, np.int8)
+    m = TraitModel((C.c_double * 3)(*[float(x) for x in penetrance]), float(freq),
+                   aff.ctypes.data_as(C.POINTER(C.c_int8)))
+    return m, aff
+
+
+def trait_model_tables(penetrance, freq, affection, n_founders: int):
+    """Disease probabilities [N][4] of a model and their derivatives [N][4][4]
+    with respect to (penetrance 0..2, disease frequency); host only."""
+    m, keep = _trait_model(penetrance, freq, affection)
+    N = keep.shape[0]
+    dp = np.zeros((N, 4))
+    jac = np.zeros((N, 4, 4))
+    rc = lib().sl_trait_model_tables(C.byref(m), N, n_founders, _ptr(dp, f64p), _ptr(jac, f64p))
+    if rc:
+        raise SlError(f"sl_trait_model_tables failed ({rc})")
+    return dp, jac
 
 
 def _ptr(a, t):
@@ -220,6 +249,34 @@ class Context:
         cnt = C.c_int64(0)
         self._check(lib().sl_lod_results(self.h, _ptr(acc, f64p), C.byref(cnt), _ptr(lod, f64p)), "sl_lod_results")
         return acc, cnt.value, lod
+
+    def set_trait_model(self, penetrance, freq, affection=None):
+        """Disease model (penetrance[3], disease allele frequency, per-person
+        affection 0/1/2, by default the case's)."""
+        aff = self.case.affection if affection is None else affection
+        m, keep = _trait_model(penetrance, freq, aff)
+        self._check(lib().sl_set_trait_model(self.h, C.byref(m)), "sl_set_trait_model")
+
+    def trait_grad(self, graphs: np.ndarray) -> np.ndarray:
+        """[G][M-1][n][7]: ln L - R - T, then d ln L / d(pen0, pen1, pen2, freq, theta_L, theta_R)."""
+        g = np.ascontiguousarray(graphs, np.int32).reshape(-1, self.case.M, 2 * self.case.N)
+        out = np.zeros((g.shape[0], self.case.M - 1, self.case.nlod, 7))
+        self._check(lib().sl_trait_grad(self.h, _ptr(g, i32p), g.shape[0], _ptr(out, f64p)), "sl_trait_grad")
+        return out
+
+    def trait_prob_grad(self) -> np.ndarray:
+        """[ln P(T), d/d pen0, pen1, pen2, freq]."""
+        out = np.zeros(5)
+        self._check(lib().sl_trait_prob_grad(self.h, _ptr(out, f64p)), "sl_trait_prob_grad")
+        return out
+
+    def lod_grad_packed(self, dev_ptr: int, n: int):
+        """LOD [M-1][n] and its derivatives [M-1][n][6] over n packed device-resident graphs."""
+        lod = np.zeros((self.case.M - 1, self.case.nlod))
+        dlod = np.zeros((self.case.M - 1, self.case.nlod, 6))
+        self._check(lib().sl_lod_grad(self.h, C.cast(C.c_void_p(dev_ptr), u32p), n, _ptr(lod, f64p),
+                                      _ptr(dlod, f64p)), "sl_lod_grad")
+        return lod, dlod
 
     def lod_terms(self, graphs: np.ndarray) -> np.ndarray:
         g = np.ascontiguousarray(graphs, np.int32).reshape(-1, self.case.M, 2 * self.case.N)

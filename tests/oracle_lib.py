@@ -42,6 +42,7 @@ def _load():
     lib.orc_trait_likelihood.argtypes = [P, i32p, C.c_int, C.c_int]
     lib.orc_trait_likelihood.restype = C.c_double
     lib.orc_lod_terms.argtypes = [P, i32p, C.c_int, f64p]
+    lib.orc_trait_grad.argtypes = [P, i32p, C.c_int, C.c_int, f64p, f64p]
     lib.orc_log_sum.argtypes = [C.c_double, C.c_double]
     lib.orc_log_sum.restype = C.c_double
     lib.orc_lod_accumulate.argtypes = [P, i32p, C.c_int, f64p]
@@ -97,6 +98,18 @@ class Oracle:
     def trait_likelihood(self, dg, locus, offset) -> float:
         d = None if dg is None else np.ascontiguousarray(dg, np.int32)
         return lib().orc_trait_likelihood(self.ref, None if d is None else _p(d, i32p), locus, offset)
+
+    def trait_grad(self, dg, locus, offset, jac) -> np.ndarray:
+        """[L, dL/d(pen0, pen1, pen2, freq, theta_L, theta_R)] by forward mode
+        (P(T) when dg is None); jac[N][4][4] = d disease_prob / d(pen, freq)."""
+        out = np.zeros(7)
+        j = np.ascontiguousarray(jac, np.float64)
+        d = None if dg is None else np.ascontiguousarray(dg, np.int32)
+        rc = lib().orc_trait_grad(self.ref, None if d is None else _p(d, i32p), locus, offset, _p(j, f64p),
+                                  _p(out, f64p))
+        if rc:
+            raise MemoryError("orc_trait_grad")
+        return out
 
     def lod_terms(self, dg, locus) -> np.ndarray:
         out = np.zeros(self.case.nlod)
