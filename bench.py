@@ -14,7 +14,12 @@ FP32-rescaled and FP64 with the largest |dLOD| between the two, the roofline
 of the dominant kernel (the M-sampler's one-launch meiosis step, timed with
 HIP events on the library's stream) and of the peel kernel, and a CPU
 baseline: the reference's own CPU path (oracle/_ref/refharness) on this
-host's cores, or the oracle restatement when that binary is absent.
+host's cores, or the oracle restatement when that binary is absent.  The
+`adjoint` object times the trait-peel adjoint (LOD and its gradient with
+respect to penetrances, disease allele frequency and flanking thetas at every
+position) over 64 resident graphs of ped200_snp2000 (200 people, 7
+generations, 4 loops, 2000 SNPs), beside the oracle's forward-mode gradient
+on one core.
 """
 from __future__ import annotations
 
@@ -36,6 +41,7 @@ METRIC = "MCMC sweeps/s + LOD evals/s, FP32-rescaled vs FP64 at |ΔLOD|≤1e-4"
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 LPROB, PERIOD, SEED = 0.5, 10, 20250101
+ADJ_CONFIG = "ped200_snp2000"  # the differentiation workload: 200 people, 7 generations, 4 loops, 2000 SNPs
 
 
 def profile_traffic(kernel: str, workload: str):
@@ -84,6 +90,31 @@ def cpu_baseline(tmpdir: str, base: str, case, threads: int):
     return {"value": sweeps / dt, "unit": "sweeps/s", "cores": 1, "kind": "port",
             "sample": f"{sweeps} sweeps of the oracle chain (same schedule, scored every {PERIOD}th), FP64, "
                       f"1 core, {dt:.1f} s"}
+
+
+MOD_MODEL = ((0.05, 0.3, 0.85), 0.12)  # trait model of the adjoint workload
+
+
+def cpu_adjoint_baseline(case, graph, jac, positions: int):
+    """The oracle's forward-mode FP64 gradient (the adjoint's reference) on one
+    core over the first `positions` (interval, position) pairs of one graph."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    o = Oracle(case)
+    t0 = time.perf_counter()
+    done = 0
+    for l in range(case.M - 1):
+        for p in range(case.nlod):
+            o.trait_grad(graph, l, p + 1, jac)
+            done += 1
+            if done >= positions:
+                break
+        if done >= positions:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "gradient evals/s", "cores": 1, "kind": "port",
+            "sample": f"{done} (graph, position) gradients of one graph with the oracle's forward-mode FP64 "
+                      f"gradient, 1 core, {dt:.1f} s"}
 
 
 def main():
@@ -178,6 +209,21 @@ def main():
     lod32, k32, lodv32, flops = lod(native.SL_FP32_RESCALED, 20)
     lod64, _, lodv64, _ = lod(native.SL_FP64, 8)
 
+    # ---- adjoint of the trait peel: LOD and its gradient over resident
+    # graphs of the differentiation workload (ped200_snp2000) ----
+    acase, adata = synth.make_case(ADJ_CONFIG, tempfile.mkdtemp(prefix="sladj_"))
+    ag0 = synth.true_graph(adata, acase)
+    AG = 64
+    apacked = torch.from_numpy(pack_graphs(acase, np.repeat(ag0[None], AG, axis=0)).reshape(-1).view(np.int32)).cuda()
+    torch.cuda.synchronize()
+    with native.Context(acase, native.SL_FP64) as actx:
+        actx.set_trait_model(*MOD_MODEL)
+        actx.lod_grad_packed(apacked.data_ptr(), 2)
+        t0 = time.perf_counter()
+        alod, adlod = actx.lod_grad_packed(apacked.data_ptr(), AG)
+        adt = time.perf_counter() - t0
+    apos = (acase.M - 1) * acase.nlod
+
     # ---- roofline of the meiosis-step kernel (one chain: mspec_kernel, the
     # forward-backward pass of step k beside the founder-allele likelihoods of
     # step k+1 for both values of meiosis k's bit): algorithmic bytes per
@@ -233,9 +279,23 @@ def main():
                          "traffic": profile_traffic("lod_kernel", args.config),
                          "flops_per_eval": flops, "launch_ms": k32},
     }
+    res["adjoint"] = {
+        "workload": f"{ADJ_CONFIG}: {acase.N} people, {acase.M} SNPs, {apos} positions; {AG} resident graphs "
+                    f"(copies of the gene-drop graph), model penetrance {MOD_MODEL[0]}, frequency {MOD_MODEL[1]}",
+        "graphs": AG, "positions": apos, "ms": adt * 1e3, "gradient_evals_per_s": AG * apos / adt,
+        "partials_per_eval": 6, "dtype": "f64", "max_lod": float(alod.max()),
+        "note": "sl_lod_grad: ln L and d ln L/d(3 penetrances, frequency, theta_L, theta_R) per (graph, position), "
+                "reduced on the device to the LOD and its gradient at every position"}
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(tmp, os.path.join(tmp, args.config), case, threads)
+        _, ajac = native.trait_model_tables(MOD_MODEL[0], MOD_MODEL[1], acase.affection, acase.F)
+        from swiftlink_amd.linkage import disease_probs
+        import dataclasses
+        dp = np.array([disease_probs(int(acase.affection[i]), i < acase.F, MOD_MODEL[1], MOD_MODEL[0])
+                       for i in range(acase.N)])
+        res["adjoint"]["cpu_baseline"] = cpu_adjoint_baseline(
+            dataclasses.replace(acase, disease_prob=dp.reshape(acase.disease_prob.shape)), ag0, ajac, 15000)
     print(json.dumps(res))
 
 
