@@ -241,6 +241,30 @@ int sl_mcmc_likelihood(sl_ctx* ctx, double* per_locus);
  * the reference's serial one).  No counterpart in the reference. */
 int sl_mcmc_near_edge(sl_ctx* ctx, int64_t* counts);
 
+/* ---- Batches: several pedigrees' chains, one captured sweep ----
+ *
+ * The reference runs one pedigree at a time (LinkageProgram::run,
+ * linkage_program.cc:41-71) and one chain per MarkovChain; -R replicates run
+ * one after another.  A batch groups contexts (one per pedigree, each holding
+ * its replicate chains from sl_mcmc_init) and captures one sweep of all of
+ * them into a single hipGraph: each context's branch runs on its own stream
+ * (its L-sweep and M-sweep launches for the chains that draw them, then the
+ * gated LOD scoring of its chains) and advances a device sweep counter the
+ * kernels read, so one graph replay per sweep drives every pedigree at once
+ * with no host work between sweeps.  Results equal sl_mcmc_run on each
+ * context with the same schedule, bit for bit.  All contexts must be on one
+ * device; a context belongs to at most one live batch. */
+typedef struct sl_batch sl_batch;
+
+int sl_batch_create(sl_ctx* const* ctxs, int32_t n_ctx, sl_batch** out);
+
+/* Runs `schedule` on every context of the batch; stats[i] receives context
+ * i's counts (may be NULL).  Returns the first context's failure, as
+ * sl_mcmc_run would. */
+int sl_batch_run(sl_batch* batch, const sl_schedule* schedule, sl_mcmc_stats* stats);
+
+void sl_batch_destroy(sl_batch* batch);
+
 /* ---- The trait model and the adjoint of the trait peel ----
  *
  * The reference fixes the disease model at start-up (DiseaseModel,

@@ -28,6 +28,16 @@ constexpr int64_t kSpecWaves = 8192;
 constexpr int kMaxGridZ = 65535;
 }
 
+struct sl_ctx;
+
+// contexts whose sweeps are captured together (sl_batch_*)
+struct sl_batch {
+    std::vector<sl_ctx*> ctx;
+    hipStream_t origin = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+};
+
 struct sl_ctx {
     std::string err;
     int device = 0;
@@ -92,6 +102,10 @@ struct sl_ctx {
     size_t gterms_cap = 0;
     double* d_gstate = nullptr;
     size_t gstate_cap = 0;
+    // captured sweeps: device sweep index the kernels read, and the batch
+    // the context belongs to
+    uint32_t* d_sweep = nullptr;
+    sl_batch* batch = nullptr;
     // sampler program and chain state
     sl::SampProgram sprog;
     int samp_block = 256;  // L-sampler workgroup: 4 wavefronts share each peel level
@@ -273,7 +287,7 @@ int release_all(sl_ctx* c) {
                     c->d_trait_f, c->d_ptheta, c->d_logt, c->d_loga, c->d_acc, c->d_scalar,
                     c->d_partial, c->d_packed, c->d_stage, c->d_terms,
                     c->g_ops, c->g_prevs, c->g_children, c->g_tasks, c->g_tinfo, c->g_idx, c->g_level_base,
-                    c->g_sched_off, c->g_sched_ops, c->d_gdp, c->d_jac, c->d_gterms, c->d_gstate};
+                    c->g_sched_off, c->g_sched_ops, c->d_gdp, c->d_jac, c->d_gterms, c->d_gstate, c->d_sweep};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
@@ -594,6 +608,10 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
 void sl_destroy(sl_ctx* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->batch) {  // leaves its batch
+        auto& v = ctx->batch->ctx;
+        v.erase(std::find(v.begin(), v.end(), ctx));
+    }
     release_all(ctx);
     delete ctx;
 }
@@ -624,7 +642,8 @@ int sl_lod_reset(sl_ctx* c) {
     return SL_OK;
 }
 
-static int lod_run(sl_ctx* c, const uint32_t* dev_graphs, int G, double* terms_dev) {
+static int lod_run(sl_ctx* c, const uint32_t* dev_graphs, int G, double* terms_dev, const uint32_t* gate = nullptr,
+                   int gate_burnin = 0, int gate_period = 1) {
     const int nchunks = (G + c->gpw - 1) / c->gpw;
     const size_t nacc = (size_t)(c->M - 1) * c->npos;
     int rc = ensure(c, &c->d_partial, &c->partial_cap, nacc * nchunks);
@@ -634,6 +653,9 @@ static int lod_run(sl_ctx* c, const uint32_t* dev_graphs, int G, double* terms_d
     A.G = G;
     A.partial = c->d_partial;
     A.terms = terms_dev;
+    A.gate_sweep = gate;
+    A.gate_burnin = gate_burnin;
+    A.gate_period = gate_period;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->prof) {
         e0 = take_event(c);
@@ -645,7 +667,9 @@ static int lod_run(sl_ctx* c, const uint32_t* dev_graphs, int G, double* terms_d
         SL_HIP(c, hipEventRecord(e1, c->stream));
         c->ev_used.push_back({SL_KERNEL_LOD, e0, e1, 1});
     }
-    if (!terms_dev) SL_HIP(c, sl::launch_lod_merge(c->d_acc, c->d_partial, nchunks, (int)nacc, c->stream));
+    if (!terms_dev)
+        SL_HIP(c, sl::launch_lod_merge(c->d_acc, c->d_partial, nchunks, (int)nacc, c->stream, gate, gate_burnin,
+                                       gate_period));
     return SL_OK;
 }
 
@@ -847,10 +871,12 @@ static sl::SampArgs samp_args(sl_ctx* c) {
 
 // Both parity launches of one L-sweep.  check_kind: every chain takes its
 // kind and first parity from its schedule draws (lprob), first_parity unused.
-static int run_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int first_parity, int check_kind, double lprob) {
+static int run_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int first_parity, int check_kind, double lprob,
+                      const uint32_t* sweep_dev = nullptr) {
     sl::SampArgs A = samp_args(c);
     A.seed = seed;
     A.sweep = sweep;
+    A.sweep_dev = sweep_dev;
     A.check_kind = check_kind;
     A.lprob = lprob;
     for (int pass = 0; pass < 2; ++pass) {
@@ -913,13 +939,14 @@ static sl::MsampArgs msamp_args(sl_ctx* c) {
 
 // One M-sweep: order, current-column likelihoods, one launch per meiosis step.
 static int run_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* dev_order, int nm, int check_kind,
-                      double lprob) {
+                      double lprob, const uint32_t* sweep_dev = nullptr) {
     if (nm == 0) return SL_OK;
     int rc = ensure(c, &c->d_order, &c->order_cap, (size_t)nm * c->n_chains);
     if (rc) return rc;
     sl::MsampArgs A = msamp_args(c);
     A.seed = seed;
     A.sweep = sweep;
+    A.sweep_dev = sweep_dev;
     A.n_meioses = nm;
     A.check_kind = check_kind;
     A.lprob = lprob;
@@ -1065,6 +1092,142 @@ int sl_mcmc_near_edge(sl_ctx* c, int64_t* counts) {
     SL_HIP(c, hipStreamSynchronize(c->stream));
     for (size_t i = 0; i < h.size(); ++i) counts[i] = (int64_t)h[i];
     return SL_OK;
+}
+
+// ---- batches: one captured sweep over several contexts ----
+
+
+int sl_batch_create(sl_ctx* const* ctxs, int32_t n, sl_batch** out) {
+    if (!ctxs || n < 1 || !out) return fail(nullptr, SL_E_INVALID, "batch: need contexts and an output pointer");
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) return fail(nullptr, SL_E_INVALID, "batch: null context");
+        if (ctxs[i]->batch) return fail(nullptr, SL_E_INVALID, "batch: a context already belongs to a batch");
+        if (ctxs[i]->device != ctxs[0]->device) return fail(nullptr, SL_E_INVALID, "batch: contexts on different devices");
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i]) return fail(nullptr, SL_E_INVALID, "batch: a context appears twice");
+    }
+    sl_batch* b = new sl_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    hipError_t e = hipSetDevice(ctxs[0]->device);
+    if (e == hipSuccess) e = hipStreamCreate(&b->origin);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    b->join.assign(n, nullptr);
+    for (int i = 0; i < n && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming);
+    for (int i = 0; i < n && e == hipSuccess; ++i)
+        if (!ctxs[i]->d_sweep) e = hipMalloc((void**)&ctxs[i]->d_sweep, sizeof(uint32_t));
+    if (e != hipSuccess) {
+        std::string m = std::string("batch setup failed: ") + hipGetErrorString(e);
+        sl_batch_destroy(b);
+        return fail(nullptr, SL_E_DEVICE, m);
+    }
+    for (sl_ctx* c : b->ctx) c->batch = b;
+    *out = b;
+    return SL_OK;
+}
+
+void sl_batch_destroy(sl_batch* b) {
+    if (!b) return;
+    if (b->origin) (void)hipStreamSynchronize(b->origin);
+    for (sl_ctx* c : b->ctx)
+        if (c && c->batch == b) c->batch = nullptr;
+    for (hipEvent_t e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->origin) (void)hipStreamDestroy(b->origin);
+    delete b;
+}
+
+namespace {
+
+// One sweep of context c with the sweep index read from c->d_sweep: the
+// L-sweep and M-sweep for the chains that draw them, the gated scoring of all
+// its chains, then the counter's advance.  Everything is a kernel launch on
+// c->stream (capturable).
+int enqueue_sweep(sl_ctx* c, const sl_schedule* sch) {
+    int rc = SL_OK;
+    if (c->samp_ready) rc = run_lsweep(c, sch->seed, 0u, 0, 1, sch->lprob, c->d_sweep);
+    if (!rc && c->m_ready)
+        rc = run_msweep(c, sch->seed, 0u, nullptr, (int)c->meioses_h.size(), 1, sch->lprob, c->d_sweep);
+    if (!rc) rc = lod_run(c, c->d_chains, c->n_chains, nullptr, c->d_sweep, sch->burnin, sch->period);
+    if (!rc) SL_HIP(c, sl::launch_advance(c->d_sweep, c->stream));
+    return rc;
+}
+
+}  // namespace
+
+int sl_batch_run(sl_batch* b, const sl_schedule* sch, sl_mcmc_stats* stats) {
+    if (!b || !sch || b->ctx.empty()) return SL_E_INVALID;
+    sl_ctx* c0 = b->ctx[0];
+    if (sch->sweeps < 0 || sch->burnin < 0 || sch->period < 1 || !(sch->lprob >= 0.0 && sch->lprob <= 1.0))
+        return fail(c0, SL_E_INVALID, "bad schedule");
+    const int n = (int)b->ctx.size();
+    std::vector<sl_mcmc_stats> st(n, sl_mcmc_stats{0, 0, 0});
+    // what sl_mcmc_run checks per sweep, and the counts it reports, from the
+    // same host draws; buffers the sweep uses are allocated before capture
+    for (int i = 0; i < n; ++i) {
+        sl_ctx* c = b->ctx[i];
+        if (c->n_chains <= 0) return fail(c, SL_E_INVALID, "no chains: call sl_mcmc_init first");
+        if (c->prof) return fail(c, SL_E_INVALID, "profiling is not available in a batch");
+        const int C = c->n_chains;
+        for (int k = 0; k < sch->sweeps; ++k) {
+            const uint32_t s = sch->first_sweep + (uint32_t)k;
+            int nl = 0;
+            for (int ch = 0; ch < C; ++ch)
+                if (sl::philox_uniform(sch->seed, (uint32_t)ch, s, sl::kStepSched, 0) < sch->lprob) ++nl;
+            if (nl && !c->samp_ready) return fail(c, SL_E_UNSUPPORTED, "L-sampler unavailable: " + c->samp_why);
+            if (C - nl && !c->m_ready) return fail(c, SL_E_UNSUPPORTED, "M-sampler unavailable: " + c->m_why);
+            st[i].l_sweeps += nl;
+            st[i].m_sweeps += C - nl;
+            if (sl::gate_open(s, sch->burnin, sch->period)) st[i].scored += C;
+        }
+        const int nchunks = (C + c->gpw - 1) / c->gpw;
+        int rc = ensure(c, &c->d_partial, &c->partial_cap, (size_t)(c->M - 1) * c->npos * nchunks);
+        if (!rc && c->m_ready) rc = ensure(c, &c->d_order, &c->order_cap, c->meioses_h.size() * (size_t)C);
+        if (rc) return rc;
+        SL_HIP(c, hipMemcpyAsync(c->d_sweep, &sch->first_sweep, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+        SL_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    if (sch->sweeps > 0) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        SL_HIP(c0, hipStreamBeginCapture(b->origin, hipStreamCaptureModeRelaxed));
+        int rc = SL_OK;
+        hipError_t e = hipEventRecord(b->fork, b->origin);
+        for (int i = 0; i < n && e == hipSuccess && !rc; ++i) {
+            sl_ctx* c = b->ctx[i];
+            e = hipStreamWaitEvent(c->stream, b->fork, 0);
+            if (e == hipSuccess) rc = enqueue_sweep(c, sch);
+            if (e == hipSuccess && !rc) e = hipEventRecord(b->join[i], c->stream);
+            if (e == hipSuccess && !rc) e = hipStreamWaitEvent(b->origin, b->join[i], 0);
+        }
+        hipError_t e2 = hipStreamEndCapture(b->origin, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        for (int k = 0; k < sch->sweeps && e == hipSuccess; ++k) e = hipGraphLaunch(exec, b->origin);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->origin);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(c0, SL_E_DEVICE, std::string("batch graph: ") + hipGetErrorString(e));
+    }
+    int first_bad = SL_OK;
+    for (int i = 0; i < n; ++i) {
+        sl_ctx* c = b->ctx[i];
+        c->count += st[i].scored;
+        if (stats) stats[i] = st[i];
+        std::vector<int32_t> status((size_t)c->n_chains);
+        SL_HIP(c, hipMemcpyAsync(status.data(), c->d_status, status.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                 c->stream));
+        SL_HIP(c, hipStreamSynchronize(c->stream));
+        for (int32_t v : status)
+            if (v != 0 && first_bad == SL_OK) {
+                first_bad = fail(c, v, "a chain met a zero likelihood or an illegal graph (see sl_mcmc_status)");
+            }
+    }
+    return first_bad;
 }
 
 // ---- trait model and the adjoint of the trait peel ----

@@ -32,7 +32,15 @@ struct LodArgs {
     double* partial;             // [chunks][M-1][npos] chunk log-sums
     double* terms;               // optional [G][M-1][npos]
     int arena_bytes, maxbits_bytes;
+    // captured sweeps: when set, score only if the sweep index here is at
+    // least gate_burnin and a multiple of gate_period
+    const uint32_t* gate_sweep;
+    int gate_burnin, gate_period;
 };
+
+__host__ __device__ inline bool gate_open(uint32_t sweep, int burnin, int period) {
+    return (int64_t)sweep >= burnin && sweep % (uint32_t)period == 0;
+}
 
 // The adjoint of the trait peel (FP64): runs an unpruned program compiled
 // with one position per table and no arena reuse, one workgroup of
@@ -53,7 +61,10 @@ hipError_t launch_grad(const LodArgs& A, const GradArgs& Gd, int nodg, size_t ld
 // then the e^(t - state)-weighted sums of the derivatives)
 hipError_t launch_grad_fold(double* state, const double* terms, int G, int n, hipStream_t s);
 hipError_t launch_trait_prob(const LodArgs& A, int precision, int block, size_t lds, double* out, hipStream_t s);
-hipError_t launch_lod_merge(double* acc, const double* partial, int nchunks, int n, hipStream_t s);
+hipError_t launch_lod_merge(double* acc, const double* partial, int nchunks, int n, hipStream_t s,
+                            const uint32_t* gate_sweep = nullptr, int gate_burnin = 0, int gate_period = 1);
+// one thread: (*counter)++ (the device sweep index of captured sweeps)
+hipError_t launch_advance(uint32_t* counter, hipStream_t s);
 hipError_t launch_fill(double* p, double v, int n, hipStream_t s);
 hipError_t configure_lod_lds(int precision, size_t lds);
 hipError_t launch_pack(const int32_t* in, uint32_t* out, int G, int M, int N, int F, int W, hipStream_t s);

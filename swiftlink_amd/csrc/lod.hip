@@ -286,6 +286,7 @@ __device__ double final_log(const LodArgs& A, const T* arena, const uint32_t* ma
 template <typename T, bool RESCALE>
 __global__ void lod_kernel(LodArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
+    if (A.gate_sweep && !gate_open(*A.gate_sweep, A.gate_burnin, A.gate_period)) return;
     T* arena = reinterpret_cast<T*>(smem);
     uint32_t* maxbits = reinterpret_cast<uint32_t*>(smem + A.arena_bytes);
     T* ths = reinterpret_cast<T*>(smem + A.arena_bytes + A.maxbits_bytes);
@@ -603,13 +604,17 @@ __global__ void grad_fold_kernel(double* state, const double* terms, int G, int 
     for (int k = 0; k < 6; ++k) state[(size_t)i * 7 + 1 + k] = V[k];
 }
 
-__global__ void lod_merge_kernel(double* acc, const double* partial, int nchunks, int n) {
+__global__ void lod_merge_kernel(double* acc, const double* partial, int nchunks, int n, const uint32_t* gate_sweep,
+                                 int gate_burnin, int gate_period) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (gate_sweep && !gate_open(*gate_sweep, gate_burnin, gate_period)) return;
     double a = acc[i];
     for (int c = 0; c < nchunks; ++c) a = lse(a, partial[(size_t)c * n + i]);
     acc[i] = a;
 }
+
+__global__ void advance_kernel(uint32_t* counter) { counter[0] += 1u; }
 
 __global__ void fill_kernel(double* p, double v, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -694,8 +699,15 @@ hipError_t launch_trait_prob(const LodArgs& A, int precision, int block, size_t 
     return hipGetLastError();
 }
 
-hipError_t launch_lod_merge(double* acc, const double* partial, int nchunks, int n, hipStream_t s) {
-    hipLaunchKernelGGL(lod_merge_kernel, dim3((n + 255) / 256), dim3(256), 0, s, acc, partial, nchunks, n);
+hipError_t launch_lod_merge(double* acc, const double* partial, int nchunks, int n, hipStream_t s,
+                            const uint32_t* gate_sweep, int gate_burnin, int gate_period) {
+    hipLaunchKernelGGL(lod_merge_kernel, dim3((n + 255) / 256), dim3(256), 0, s, acc, partial, nchunks, n, gate_sweep,
+                       gate_burnin, gate_period);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance(uint32_t* counter, hipStream_t s) {
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, s, counter);
     return hipGetLastError();
 }
 

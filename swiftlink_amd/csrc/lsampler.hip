@@ -34,6 +34,10 @@ namespace sl {
 
 namespace {
 
+// the sweep index: an argument, or a device counter advanced between
+// replays of a captured sweep
+__device__ __forceinline__ uint32_t sweep_of(const SampArgs& A) { return A.sweep_dev ? *A.sweep_dev : A.sweep; }
+
 typedef const __attribute__((address_space(4))) int32_t* cint_p;
 typedef const __attribute__((address_space(4))) double* cdbl_p;
 typedef const __attribute__((address_space(4))) float* cflt_p;
@@ -154,8 +158,8 @@ __global__ void lsampler_kernel(SampArgs A) {
     int parity = A.parity;
     if (A.check_kind) {
         // the chain's sweep kind and first parity (draws 0 and 1 at step kStepSched)
-        if (!(philox_uniform(A.seed, (uint32_t)chain, A.sweep, kStepSched, 0) < A.lprob)) return;
-        const int p0 = philox_uniform(A.seed, (uint32_t)chain, A.sweep, kStepSched, 1) < 0.5 ? 0 : 1;
+        if (!(philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), kStepSched, 0) < A.lprob)) return;
+        const int p0 = philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), kStepSched, 1) < 0.5 ? 0 : 1;
         parity = A.pass == 0 ? p0 : 1 - p0;
     }
     const int locus = parity + 2 * blockIdx.x;
@@ -203,7 +207,7 @@ __global__ void lsampler_kernel(SampArgs A) {
         }
     }
     for (int o = tid; o < n_ops; o += blockDim.x)
-        uop[o] = philox_uniform(A.seed, (uint32_t)chain, A.sweep, (uint32_t)locus, (uint32_t)(n_ops - 1 - o));
+        uop[o] = philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), (uint32_t)locus, (uint32_t)(n_ops - 1 - o));
     for (int i = tid; i < 4 * N; i += blockDim.x) trt[i] = trait_at<T>(A, i >> 2, locus, i & 3);
     __syncthreads();
     const int total_cells = block_exclusive_scan(off, n_ops, scan_tmp);
@@ -437,7 +441,7 @@ __global__ void lsampler_kernel(SampArgs A) {
                 p0 *= (nb == 0u) ? 1.0 - thR : thR;
                 p1 *= (nb == 1u) ? 1.0 - thR : thR;
             }
-            const double u = philox_uniform(A.seed, (uint32_t)chain, A.sweep, (uint32_t)locus,
+            const double u = philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), (uint32_t)locus,
                                             (uint32_t)(n_ops + mcount[b]));
             bit = (u < (p0 / (p0 + p1))) ? 0u : 1u;
         }

@@ -56,6 +56,10 @@ constexpr int G_HETERO = 1, G_HOMOZ_B = 3;  // genotype codes (genotype.h)
 constexpr int kIllegalGraph = -5;
 constexpr int kGenoPrefetch = 4;  // genotype chunks of 64 persons loaded ahead
 
+// the sweep index: an argument, or a device counter advanced between
+// replays of a captured sweep
+__device__ __forceinline__ uint32_t sweep_of(const MsampArgs& A) { return A.sweep_dev ? *A.sweep_dev : A.sweep; }
+
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
 
 __device__ __forceinline__ void wave_sync() {
@@ -494,7 +498,7 @@ __device__ __forceinline__ Mat wave_products(Mat P) {
 }
 
 __device__ __forceinline__ bool is_msweep(const MsampArgs& A, int chain) {
-    return !A.check_kind || !(philox_uniform(A.seed, (uint32_t)chain, A.sweep, kStepSched, 0) < A.lprob);
+    return !A.check_kind || !(philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), kStepSched, 0) < A.lprob);
 }
 
 // flipped-column log-likelihoods: [step parity][variant][chain][locus]
@@ -612,7 +616,7 @@ __device__ __forceinline__ uint32_t locus_map(const MsampArgs& A, int chain, int
     // indicator; a difference (underflow in one branch only) is counted
     const bool draw = w0[0] != 0.0 && w1[0] != 0.0;
     if (draw != (w0[1] != 0.0 && w1[1] != 0.0)) ++*near;
-    const double u = draw ? philox_uniform(A.seed, (uint32_t)chain, A.sweep, (uint32_t)k, ctr) : 0.0;
+    const double u = draw ? philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), (uint32_t)k, ctr) : 0.0;
     uint32_t map = 0;
 #pragma unroll
     for (int bn = 0; bn < 2; ++bn) {
@@ -910,7 +914,7 @@ __global__ void morder_kernel(MsampArgs A, const int32_t* fixed_order) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         o[i] = A.meioses[i];
         if (i > 0) {
-            const double u = philox_uniform(A.seed, (uint32_t)chain, A.sweep, kStepShuffle, (uint32_t)(n - 1 - i));
+            const double u = philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), kStepShuffle, (uint32_t)(n - 1 - i));
             const int j = (int)(u * (double)(i + 1));
             jdx[i] = j > i ? i : j;
         }

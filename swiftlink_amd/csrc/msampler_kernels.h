@@ -41,6 +41,7 @@ struct MsampArgs {
     uint32_t* near_edge;          // [chains] draws within 1e-12 of their threshold
     uint64_t seed;
     uint32_t sweep;
+    const uint32_t* sweep_dev;    // when set, the sweep index is read here (captured sweeps)
     int step;                     // meiosis step whose likelihood jobs this launch runs
     int fb_step;                  // meiosis step whose forward-backward pass this launch runs
     int n_jobs;                   // likelihood jobs (wavefronts) per chain in this launch

@@ -28,6 +28,7 @@ struct SampArgs {
     int32_t* status;             // [chains]
     uint64_t seed;
     uint32_t sweep;
+    const uint32_t* sweep_dev;   // when set, the sweep index is read here (captured sweeps)
     int parity;                  // loci of this parity (check_kind == 0)
     int cells_cap;
     int check_kind;              // 1: act only on chains whose sweep is an L-sweep, and take

@@ -98,6 +98,9 @@ _SIGS = {
     "sl_mcmc_run": (C.c_int, [C.c_void_p, C.POINTER(Schedule), C.POINTER(McmcStats)]),
     "sl_mcmc_likelihood": (C.c_int, [C.c_void_p, f64p]),
     "sl_mcmc_near_edge": (C.c_int, [C.c_void_p, i64p]),
+    "sl_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]),
+    "sl_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Schedule), C.POINTER(McmcStats)]),
+    "sl_batch_destroy": (None, [C.c_void_p]),
     "sl_trait_model_tables": (C.c_int, [C.POINTER(TraitModel), C.c_int32, C.c_int32, f64p, f64p]),
     "sl_set_trait_model": (C.c_int, [C.c_void_p, C.POINTER(TraitModel)]),
     "sl_trait_grad": (C.c_int, [C.c_void_p, i32p, C.c_int32, f64p]),
@@ -342,3 +345,39 @@ class Context:
         b = C.c_double(0)
         self._check(lib().sl_lod_work(self.h, C.byref(f), C.byref(b)), "sl_lod_work")
         return f.value, b.value
+
+
+class Batch:
+    """Contexts (one per pedigree) whose sweeps run as one captured hipGraph
+    (sl_batch_*).  Destroy the batch before its contexts, or use it as a
+    context manager."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = C.c_void_p()
+        rc = lib().sl_batch_create(arr, len(self.contexts), C.byref(h))
+        if rc:
+            raise SlError(f"sl_batch_create failed ({rc}): {lib().sl_last_error(None).decode()}")
+        self.h = h
+
+    def run(self, sweeps: int, burnin: int, period: int, lprob: float, seed: int, first_sweep: int = 0):
+        """Same schedule on every context; returns one stats dict per context."""
+        sch = Schedule(sweeps, burnin, period, lprob, seed, first_sweep)
+        st = (McmcStats * len(self.contexts))()
+        rc = lib().sl_batch_run(self.h, C.byref(sch), st)
+        if rc:
+            msgs = [lib().sl_last_error(c.h).decode() for c in self.contexts]
+            raise SlError(f"sl_batch_run failed ({rc}): {[m for m in msgs if m]}")
+        return [{"l_sweeps": x.l_sweeps, "m_sweeps": x.m_sweeps, "scored": x.scored} for x in st]
+
+    def close(self):
+        if self.h:
+            lib().sl_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
