@@ -209,6 +209,35 @@ def main():
     lod32, k32, lodv32, flops = lod(native.SL_FP32_RESCALED, 20)
     lod64, _, lodv64, _ = lod(native.SL_FP64, 8)
 
+    # ---- 16 pedigrees of uneven size, 4 chains each, one captured sweep per
+    # context replayed side by side (sl_batch_*), shapes from the autotuner ----
+    from swiftlink_amd.tune import autotune
+    bcases = synth.make_batch(tempfile.mkdtemp(prefix="slbatch_"))
+    bctx, tunes = [], []
+    BCH, BK, BW = 4, 10, 2
+    for bcase, bdata in bcases:
+        c = native.Context(bcase, native.SL_FP32_RESCALED)
+        c.mcmc_init(np.repeat(synth.true_graph(bdata, bcase)[None], BCH, axis=0))
+        tu = autotune(c)
+        tunes.append({"people": bcase.N, "snps": bcase.M, "block": tu["block"], "gpw": tu["gpw"],
+                      "msampler_mode": tu["msampler_mode"]})
+        bctx.append(c)
+    bsched = (0, PERIOD, LPROB, SEED)
+    t0 = time.perf_counter()
+    for c in bctx:
+        c.mcmc_run(BK, *bsched)
+    for c in bctx:
+        c.sync()
+    bseq = time.perf_counter() - t0
+    with native.Batch(bctx) as bb:
+        bb.run(BW, *bsched, first_sweep=BK)
+        t0 = time.perf_counter()
+        bstats = bb.run(BK, *bsched, first_sweep=BK + BW)
+        bdt = time.perf_counter() - t0
+    bchains = BCH * len(bctx)
+    for c in bctx:
+        c.close()
+
     # ---- adjoint of the trait peel: LOD and its gradient over resident
     # graphs of the differentiation workload (ped200_snp2000) ----
     acase, adata = synth.make_case(ADJ_CONFIG, tempfile.mkdtemp(prefix="sladj_"))
@@ -279,6 +308,16 @@ def main():
                          "traffic": profile_traffic("lod_kernel", args.config),
                          "flops_per_eval": flops, "launch_ms": k32},
     }
+    res["batch_pedigrees"] = {
+        "workload": f"{len(bcases)} pedigrees of 40-400 people (1-6 loops) and 500-5000 SNPs at 0.1-0.5 cM, "
+                    f"{BCH} chains each, FP32-rescaled samplers and peel, FP64 LOD accumulation, L:M 1:1, "
+                    f"scored every {PERIOD}th sweep",
+        "chains": bchains, "sweeps_per_chain": BK, "sweeps_per_s": bchains * BK / bdt,
+        "sequential_sweeps_per_s": bchains * BK / bseq,
+        "l_sweeps": int(sum(x["l_sweeps"] for x in bstats)), "m_sweeps": int(sum(x["m_sweeps"] for x in bstats)),
+        "note": "sweeps_per_s: one hipGraph-captured sweep per pedigree, replayed on the pedigrees' streams side by "
+                "side (sl_batch_run); sequential_sweeps_per_s: sl_mcmc_run on each pedigree in turn, same shapes",
+        "autotuned": tunes}
     res["adjoint"] = {
         "workload": f"{ADJ_CONFIG}: {acase.N} people, {acase.M} SNPs, {apos} positions; {AG} resident graphs "
                     f"(copies of the gene-drop graph), model penetrance {MOD_MODEL[0]}, frequency {MOD_MODEL[1]}",

@@ -76,8 +76,8 @@ typedef struct {
 typedef struct {
     int32_t precision;            /* SL_FP64 or SL_FP32_RESCALED */
     int32_t device;               /* HIP device ordinal */
-    int32_t block_threads;        /* LOD kernel workgroup size; 0 = autotuned */
-    int32_t graphs_per_block;     /* graphs folded into one LOD workgroup; 0 = default */
+    int32_t block_threads;        /* LOD kernel workgroup size; 0 = the default, 128 */
+    int32_t graphs_per_block;     /* graphs folded into one LOD workgroup; 0 = the default, 4 */
     int32_t msampler_mode;        /* SL_MSAMPLER_*: how an M-sweep's steps are launched */
 } sl_options;
 
@@ -241,19 +241,28 @@ int sl_mcmc_likelihood(sl_ctx* ctx, double* per_locus);
  * the reference's serial one).  No counterpart in the reference. */
 int sl_mcmc_near_edge(sl_ctx* ctx, int64_t* counts);
 
+/* Changes the launch shapes of a context (block_threads, graphs_per_block,
+ * msampler_mode) without touching its chains or accumulators; precision and
+ * device must be those of sl_create.  A new block size recompiles the LOD
+ * program's wave schedule.  The autotuner (swiftlink_amd/tune.py) times the
+ * candidates through this call.  No counterpart in the reference. */
+int sl_set_options(sl_ctx* ctx, const sl_options* opts);
+
 /* ---- Batches: several pedigrees' chains, one captured sweep ----
  *
  * The reference runs one pedigree at a time (LinkageProgram::run,
  * linkage_program.cc:41-71) and one chain per MarkovChain; -R replicates run
  * one after another.  A batch groups contexts (one per pedigree, each holding
- * its replicate chains from sl_mcmc_init) and captures one sweep of all of
- * them into a single hipGraph: each context's branch runs on its own stream
- * (its L-sweep and M-sweep launches for the chains that draw them, then the
- * gated LOD scoring of its chains) and advances a device sweep counter the
- * kernels read, so one graph replay per sweep drives every pedigree at once
- * with no host work between sweeps.  Results equal sl_mcmc_run on each
- * context with the same schedule, bit for bit.  All contexts must be on one
- * device; a context belongs to at most one live batch. */
+ * its replicate chains from sl_mcmc_init) and captures one sweep of each
+ * into a hipGraph on the context's stream: its L-sweep and M-sweep launches
+ * for the chains that draw them, the LOD scoring of its chains gated on the
+ * device, and the advance of a device sweep counter the kernels read.  A
+ * sweep of the batch is one replay of every context's graph, the contexts'
+ * streams running side by side, with no other host work.  Results equal
+ * sl_mcmc_run on each context with the same schedule, bit for bit.  All
+ * contexts must be on one device; a context belongs to at most one live
+ * batch.  HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4 by
+ * default), which bounds how many contexts run at once. */
 typedef struct sl_batch sl_batch;
 
 int sl_batch_create(sl_ctx* const* ctxs, int32_t n_ctx, sl_batch** out);

@@ -33,9 +33,6 @@ struct sl_ctx;
 // contexts whose sweeps are captured together (sl_batch_*)
 struct sl_batch {
     std::vector<sl_ctx*> ctx;
-    hipStream_t origin = nullptr;
-    hipEvent_t fork = nullptr;
-    std::vector<hipEvent_t> join;
 };
 
 struct sl_ctx {
@@ -1094,6 +1091,33 @@ int sl_mcmc_near_edge(sl_ctx* c, int64_t* counts) {
     return SL_OK;
 }
 
+namespace {
+int install_lod_program(sl_ctx* c);
+}
+
+int sl_set_options(sl_ctx* c, const sl_options* o) {
+    if (!c || !o) return SL_E_INVALID;
+    if (o->precision != c->precision || o->device != c->device)
+        return fail(c, SL_E_INVALID, "precision and device are fixed at sl_create");
+    const int block = o->block_threads > 0 ? o->block_threads : 128;
+    if (block % 64 != 0 || block < 64 || block > 1024)
+        return fail(c, SL_E_INVALID, "block_threads must be a multiple of 64 in 64..1024");
+    if ((o->msampler_mode & ~(3 | SL_MSAMPLER_LDS_GRAPHS)) != 0 || (o->msampler_mode & 3) > SL_MSAMPLER_TWO_LAUNCH)
+        return fail(c, SL_E_INVALID, "unknown msampler_mode");
+    if (block != c->block) {
+        const int old = c->block;
+        c->block = block;
+        const int rc = install_lod_program(c);
+        if (rc) {
+            c->block = old;
+            return rc;
+        }
+    }
+    c->gpw = o->graphs_per_block > 0 ? o->graphs_per_block : 4;
+    c->m_mode = o->msampler_mode;
+    return SL_OK;
+}
+
 // ---- batches: one captured sweep over several contexts ----
 
 
@@ -1109,10 +1133,6 @@ int sl_batch_create(sl_ctx* const* ctxs, int32_t n, sl_batch** out) {
     sl_batch* b = new sl_batch();
     b->ctx.assign(ctxs, ctxs + n);
     hipError_t e = hipSetDevice(ctxs[0]->device);
-    if (e == hipSuccess) e = hipStreamCreate(&b->origin);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
-    b->join.assign(n, nullptr);
-    for (int i = 0; i < n && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming);
     for (int i = 0; i < n && e == hipSuccess; ++i)
         if (!ctxs[i]->d_sweep) e = hipMalloc((void**)&ctxs[i]->d_sweep, sizeof(uint32_t));
     if (e != hipSuccess) {
@@ -1127,13 +1147,11 @@ int sl_batch_create(sl_ctx* const* ctxs, int32_t n, sl_batch** out) {
 
 void sl_batch_destroy(sl_batch* b) {
     if (!b) return;
-    if (b->origin) (void)hipStreamSynchronize(b->origin);
     for (sl_ctx* c : b->ctx)
-        if (c && c->batch == b) c->batch = nullptr;
-    for (hipEvent_t e : b->join)
-        if (e) (void)hipEventDestroy(e);
-    if (b->fork) (void)hipEventDestroy(b->fork);
-    if (b->origin) (void)hipStreamDestroy(b->origin);
+        if (c && c->batch == b) {
+            (void)hipStreamSynchronize(c->stream);
+            c->batch = nullptr;
+        }
     delete b;
 }
 
@@ -1188,29 +1206,29 @@ int sl_batch_run(sl_batch* b, const sl_schedule* sch, sl_mcmc_stats* stats) {
         SL_HIP(c, hipStreamSynchronize(c->stream));
     }
     if (sch->sweeps > 0) {
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        SL_HIP(c0, hipStreamBeginCapture(b->origin, hipStreamCaptureModeRelaxed));
+        // one graph per context, captured from its own stream; replays of
+        // different contexts run concurrently on their streams
+        std::vector<hipGraph_t> graphs(n, nullptr);
+        std::vector<hipGraphExec_t> execs(n, nullptr);
+        hipError_t e = hipSuccess;
         int rc = SL_OK;
-        hipError_t e = hipEventRecord(b->fork, b->origin);
         for (int i = 0; i < n && e == hipSuccess && !rc; ++i) {
             sl_ctx* c = b->ctx[i];
-            e = hipStreamWaitEvent(c->stream, b->fork, 0);
-            if (e == hipSuccess) rc = enqueue_sweep(c, sch);
-            if (e == hipSuccess && !rc) e = hipEventRecord(b->join[i], c->stream);
-            if (e == hipSuccess && !rc) e = hipStreamWaitEvent(b->origin, b->join[i], 0);
+            e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+            if (e != hipSuccess) break;
+            rc = enqueue_sweep(c, sch);
+            const hipError_t e2 = hipStreamEndCapture(c->stream, &graphs[i]);
+            if (e == hipSuccess) e = e2;
+            if (e == hipSuccess && !rc) e = hipGraphInstantiate(&execs[i], graphs[i], nullptr, nullptr, 0);
         }
-        hipError_t e2 = hipStreamEndCapture(b->origin, &graph);
-        if (rc) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return rc;
+        for (int k = 0; k < sch->sweeps && e == hipSuccess && !rc; ++k)
+            for (int i = 0; i < n && e == hipSuccess; ++i) e = hipGraphLaunch(execs[i], b->ctx[i]->stream);
+        for (int i = 0; i < n && e == hipSuccess; ++i) e = hipStreamSynchronize(b->ctx[i]->stream);
+        for (int i = 0; i < n; ++i) {
+            if (execs[i]) (void)hipGraphExecDestroy(execs[i]);
+            if (graphs[i]) (void)hipGraphDestroy(graphs[i]);
         }
-        if (e == hipSuccess) e = e2;
-        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        for (int k = 0; k < sch->sweeps && e == hipSuccess; ++k) e = hipGraphLaunch(exec, b->origin);
-        if (e == hipSuccess) e = hipStreamSynchronize(b->origin);
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        if (rc) return rc;
         if (e != hipSuccess) return fail(c0, SL_E_DEVICE, std::string("batch graph: ") + hipGetErrorString(e));
     }
     int first_bad = SL_OK;

@@ -98,6 +98,7 @@ _SIGS = {
     "sl_mcmc_run": (C.c_int, [C.c_void_p, C.POINTER(Schedule), C.POINTER(McmcStats)]),
     "sl_mcmc_likelihood": (C.c_int, [C.c_void_p, f64p]),
     "sl_mcmc_near_edge": (C.c_int, [C.c_void_p, i64p]),
+    "sl_set_options": (C.c_int, [C.c_void_p, C.POINTER(Options)]),
     "sl_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]),
     "sl_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Schedule), C.POINTER(McmcStats)]),
     "sl_batch_destroy": (None, [C.c_void_p]),
@@ -203,6 +204,14 @@ class Context:
         if rc:
             raise SlError(f"sl_create failed ({rc}): {lib().sl_last_error(None).decode()}")
         self.h = h
+        self.precision, self.device = precision, device
+        self.options = {"block": block, "gpw": gpw, "msampler_mode": msampler_mode}
+
+    def set_options(self, block: int = 0, gpw: int = 0, msampler_mode: int = SL_MSAMPLER_AUTO):
+        """New launch shapes (sl_set_options); chains and accumulators are kept."""
+        opts = Options(self.precision, self.device, block, gpw, msampler_mode)
+        self._check(lib().sl_set_options(self.h, C.byref(opts)), "sl_set_options")
+        self.options = {"block": block, "gpw": gpw, "msampler_mode": msampler_mode}
 
     def _check(self, rc, what):
         if rc:

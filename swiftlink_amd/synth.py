@@ -298,6 +298,32 @@ def true_graph(data: dict, case) -> np.ndarray:
     return g
 
 
+def batch_specs(n: int = 16, seed: int = 1003) -> List[SynthSpec]:
+    """The uneven batch: n pedigrees of 40-400 people (4 + people/100
+    generations, 1-6 loops), 500-5000 SNPs at 0.1-0.5 cM, 10% missing."""
+    rng = np.random.default_rng(seed)
+    people = np.linspace(40, 400, n).round().astype(int)
+    rng.shuffle(people)
+    specs = []
+    for i, npeople in enumerate(people):
+        markers = int(round(500 + 4500 * ((i * 7) % n) / max(n - 1, 1)))
+        specs.append(SynthSpec(int(npeople), 4 + int(npeople) // 100, 1 + i % 6, markers, [0.1, 0.2, 0.25, 0.5][i % 4],
+                               missing=0.10, seed=2000 + i))
+    return specs
+
+
+def make_batch(outdir: str, n: int = 16, nlod: int = 5):
+    """Cases and data of the uneven batch (batch_specs)."""
+    from . import linkage
+    out = []
+    for i, spec in enumerate(batch_specs(n)):
+        data = generate(spec)
+        name = f"batch{i:02d}_ped{spec.n_people}_snp{spec.n_markers}"
+        p, m, d = write_linkage(data, outdir, name)
+        out.append((linkage.load(p, m, d, nlod=nlod), data))
+    return out
+
+
 def make_case(name: str, outdir: str, nlod: int = 5, **overrides):
     """Generate config `name`, write its LINKAGE files and load them.
     Returns (case, data)."""

@@ -85,3 +85,26 @@ def test_batch_rejects_shared_context_and_survives_destroyed_member():
     batch.run(5, 0, 1, 0.5, 3)
     batch.close()
     a.close()
+
+
+def test_autotune_keeps_chains_and_results():
+    """The autotuner restores the chains it timed, and any launch shape gives
+    the same chains (and accumulators up to the log-sum order of graph
+    chunks) as the default shapes."""
+    from swiftlink_amd.tune import autotune
+    case, fx = load_golden("ped30_snp100")
+    starts = _starts(fx, 4)
+    with native.Context(case, native.SL_FP64) as a, native.Context(case, native.SL_FP64) as b:
+        a.mcmc_init(starts)
+        tu = autotune(a, blocks=(64, 256), gpws=(1, 4))
+        assert np.array_equal(a.mcmc_graphs(), starts)
+        assert a.options == {"block": tu["block"], "gpw": tu["gpw"], "msampler_mode": tu["msampler_mode"]}
+        b.mcmc_init(starts)
+        for c in (a, b):
+            c.lod_reset()
+            c.mcmc_run(12, *SCHED.values())
+        assert np.array_equal(a.mcmc_graphs(), b.mcmc_graphs())
+        acc_a, cnt_a, _ = a.lod_results()
+        acc_b, cnt_b, _ = b.lod_results()
+        assert cnt_a == cnt_b
+        np.testing.assert_allclose(acc_a, acc_b, rtol=1e-12, atol=1e-9)
