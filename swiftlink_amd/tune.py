@@ -7,8 +7,9 @@ For a context holding its chains, times on the device:
     workgroup folds (its LDS arena is reused across them);
   * one M-sweep of every chain in each M-sampler launch mode (one launch per
     step with speculative likelihoods, or two launches per step).
-The chains' graphs are restored afterwards, so tuning leaves the chain state
-as it found it.  Timing uses the library's HIP-event profiling.  The shapes
+The chains' graphs are restored afterwards (and the LOD accumulators
+cleared), so the chains continue as they were.  Timing uses the library's
+HIP-event profiling and the host clock.  The shapes
 change through sl_set_options; results are identical for every shape.
 """
 from __future__ import annotations
@@ -19,19 +20,19 @@ from typing import Dict, Sequence
 import numpy as np
 
 from . import native
-from .case import pack_graphs
 
 LOD_BLOCKS = (64, 128, 256, 512)
 LOD_GPW = (1, 2, 4, 8)
 M_MODES = (native.SL_MSAMPLER_SPECULATIVE, native.SL_MSAMPLER_TWO_LAUNCH)
 
 
-def _lod_ms(ctx, dev_ptr: int, G: int, reps: int) -> float:
-    ctx.lod_accumulate_packed(dev_ptr, G)  # warm
+def _lod_ms(ctx, graphs: np.ndarray, reps: int) -> float:
+    # the events bracket the peel launch only, not the graphs' upload
+    ctx.lod_accumulate(graphs)  # warm
     ctx.sync()
     ctx.profile(True)
     for _ in range(reps):
-        ctx.lod_accumulate_packed(dev_ptr, G)
+        ctx.lod_accumulate(graphs)
     ms, n = ctx.profile_read(native.SL_KERNEL_LOD)
     ctx.profile(False)
     return ms / max(n, 1)
@@ -52,26 +53,22 @@ def autotune(ctx: "native.Context", blocks: Sequence[int] = LOD_BLOCKS, gpws: Se
              m_modes: Sequence[int] = M_MODES, lod_reps: int = 3, m_sweeps: int = 2) -> Dict:
     """Picks and applies the fastest shapes for ctx's chains; returns the
     choice and every timing (ms)."""
-    import torch
     graphs = ctx.mcmc_graphs()
-    G = graphs.shape[0]
-    dev = torch.from_numpy(pack_graphs(ctx.case, graphs).reshape(-1).view(np.int32)).cuda()
     lod = {}
     for b, g in itertools.product(blocks, gpws):
         try:
             ctx.set_options(b, g, ctx.options["msampler_mode"])
         except native.SlError:
             continue  # e.g. a block the LDS arena cannot hold
-        lod[(b, g)] = _lod_ms(ctx, dev.data_ptr(), G, lod_reps)
+        lod[(b, g)] = _lod_ms(ctx, graphs, lod_reps)
     best_lod = min(lod, key=lod.get)
     msw = {}
-    if ctx.case.F >= 1:
-        for m in m_modes:
-            ctx.set_options(best_lod[0], best_lod[1], m)
-            try:
-                msw[m] = _msweep_ms(ctx, 1, m_sweeps)
-            except native.SlError:
-                continue
+    for m in m_modes:
+        ctx.set_options(best_lod[0], best_lod[1], m)
+        try:
+            msw[m] = _msweep_ms(ctx, 1, m_sweeps)
+        except native.SlError:
+            continue
     best_m = min(msw, key=msw.get) if msw else native.SL_MSAMPLER_AUTO
     ctx.set_options(best_lod[0], best_lod[1], best_m)
     ctx.mcmc_init(graphs)  # the chains as they were
