@@ -77,7 +77,7 @@ typedef struct {
     int32_t precision;            /* SL_FP64 or SL_FP32_RESCALED */
     int32_t device;               /* HIP device ordinal */
     int32_t block_threads;        /* LOD kernel workgroup size; 0 = the default, 128 */
-    int32_t graphs_per_block;     /* graphs folded into one LOD workgroup; 0 = the default, 4 */
+    int32_t graphs_per_block;     /* graphs folded into one LOD workgroup; 0 = the default, 1 */
     int32_t msampler_mode;        /* SL_MSAMPLER_*: how an M-sweep's steps are launched */
 } sl_options;
 

@@ -73,7 +73,7 @@ struct sl_ctx {
     double trait_prob = 0.0;
     bool trait_prob_valid = false;
     int block = 128;
-    int gpw = 4;
+    int gpw = 1;
     size_t lds = 0;
     int arena_bytes = 0, maxbits_bytes = 0;
     // the peel table and disease probabilities (host copies, for recompiles)
@@ -479,7 +479,8 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
     }
     const size_t esz = c->precision == SL_FP64 ? 8 : 4;
     c->arena_bytes = (int)((c->prog.arena_elems * esz + 15) / 16 * 16);
-    c->maxbits_bytes = c->precision == SL_FP64 ? 0 : (int)((c->prog.n_ops * 4 + 15) / 16 * 16);
+    // FP32: one exponent word per table and their running sum
+    c->maxbits_bytes = c->precision == SL_FP64 ? 0 : (int)(((c->prog.n_ops + 1) * 4 + 15) / 16 * 16);
     c->lds = (size_t)c->arena_bytes + c->maxbits_bytes + 4 * sl::kMaxPos * esz + ((c->N + 15) / 16 * 16);
     if (c->lds > 160 * 1024) {
         delete c;
@@ -1113,7 +1114,7 @@ int sl_set_options(sl_ctx* c, const sl_options* o) {
             return rc;
         }
     }
-    c->gpw = o->graphs_per_block > 0 ? o->graphs_per_block : 4;
+    c->gpw = o->graphs_per_block > 0 ? o->graphs_per_block : 1;
     c->m_mode = o->msampler_mode;
     return SL_OK;
 }
@@ -1315,7 +1316,7 @@ int install_lod_program(sl_ctx* c) {
     if (!why.empty()) return fail(c, SL_E_UNSUPPORTED, why);
     const size_t esz = c->precision == SL_FP64 ? 8 : 4;
     const int arena_bytes = (int)((P.arena_elems * esz + 15) / 16 * 16);
-    const int maxbits_bytes = c->precision == SL_FP64 ? 0 : (int)((P.n_ops * 4 + 15) / 16 * 16);
+    const int maxbits_bytes = c->precision == SL_FP64 ? 0 : (int)(((P.n_ops + 1) * 4 + 15) / 16 * 16);
     const size_t lds = (size_t)arena_bytes + maxbits_bytes + 4 * sl::kMaxPos * esz + ((c->N + 15) / 16 * 16);
     if (lds > 160 * 1024) return fail(c, SL_E_UNSUPPORTED, "peel program needs more than 160 KiB of LDS");
     SL_HIP(c, hipStreamSynchronize(c->stream));

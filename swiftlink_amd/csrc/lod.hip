@@ -114,17 +114,36 @@ __device__ __forceinline__ T select4(const T tr[4], uint32_t v) {
     return v == 0u ? tr[0] : v == 1u ? tr[1] : v == 2u ? tr[2] : tr[3];
 }
 
-// One compact cell `c` of op `d` at position `pos`: the precompiled terms give
-// the summed node's trait value, which consumed-table cells to read, and for
-// PARENT ops which (i, j) transmit each child's trait.
-template <typename T, bool NODG>
+// The consumed-table indices of one cell: 4 terms x NP u16, 8-byte aligned
+// (the compiler pads each op's block), read as NP 8-byte words.
+template <int NP>
+struct CellIdx {
+    uint2 w[NP > 0 ? NP : 1];
+    __device__ __forceinline__ void load(const uint16_t* p) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) w[k] = reinterpret_cast<const uint2*>(p)[k];
+    }
+    // entry t*NP + q (compile-time after unrolling)
+    __device__ __forceinline__ int at(int t, int q) const {
+        const int e = t * NP + q;
+        const uint32_t v = (e & 2) ? w[e >> 2].y : w[e >> 2].x;
+        return (int)((v >> (16 * (e & 1))) & 0xFFFFu);
+    }
+};
+
+// One compact cell `c` of op `d` at position `pos`, for an op consuming NP
+// tables: the precompiled terms give the summed node's trait value, which
+// consumed-table cells to read, and for PARENT ops which (i, j) transmit each
+// child's trait.  A child's four transmission factors do not depend on the
+// term, so they are formed once per cell.
+template <typename T, bool NODG, int NP>
 __device__ T eval_cell(const LodArgs& A, const OpDesc& d, const T tr[4], const PrevSet<T>& P, const int* kids,
                        int c, int pos, const T* arena, const uint8_t* code, const Thetas<T>& th) {
-    const int np = d.n_prev;
     const T quarter = (T)0.25;
     const uint4 ti = *reinterpret_cast<const uint4*>(A.tinfo + (size_t)(d.task_base + c) * 4);
     const uint32_t info[4] = {ti.x, ti.y, ti.z, ti.w};
-    const uint16_t* ix = A.idx + d.idx_base + (size_t)c * 4 * np;
+    CellIdx<NP> ix;
+    ix.load(A.idx + d.idx_base + (size_t)c * 4 * NP);
     T total = (T)0;
     if (d.type == kChild) {
         const uint32_t cd = NODG ? 0u : (uint32_t)code[d.peel];
@@ -133,33 +152,35 @@ __device__ T eval_cell(const LodArgs& A, const OpDesc& d, const T tr[4], const P
             if (!(info[t] & 4u)) continue;
             T tmp = select4(tr, info[t] & 3u);
 #pragma unroll
-            for (int q = 0; q < kMaxPrev; ++q)
-                if (q < np) tmp *= arena[P.tab[q] + pos * P.size[q] + ix[t * np + q]] * P.scale[q];
+            for (int q = 0; q < NP; ++q) tmp *= arena[P.tab[q] + pos * P.size[q] + ix.at(t, q)] * P.scale[q];
             if (NODG) tmp *= quarter;
             else tmp *= quarter * rho<T>(cd, t >> 1, t & 1, th);
             total += tmp;
         }
     } else if (d.type == kParent) {
+        T child_prob[4] = {(T)1, (T)1, (T)1, (T)1};
+        for (int ch = 0; ch < d.n_child; ++ch) {
+            T r4[4];
+            const uint32_t cc = NODG ? 0u : (uint32_t)code[kids[ch]];
+#pragma unroll
+            for (int ij = 0; ij < 4; ++ij) r4[ij] = NODG ? quarter : quarter * rho<T>(cc, ij >> 1, ij & 1, th);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const uint32_t m = (info[t] >> (4 + 4 * ch)) & 15u;
+                T ctmp = (T)0;
+#pragma unroll
+                for (int ij = 0; ij < 4; ++ij)
+                    if ((m >> ij) & 1u) ctmp += r4[ij];
+                child_prob[t] *= ctmp;
+            }
+        }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             if (!(info[t] & 4u)) continue;
             T tmp = tr[t];
 #pragma unroll
-            for (int q = 0; q < kMaxPrev; ++q)
-                if (q < np) tmp *= arena[P.tab[q] + pos * P.size[q] + ix[t * np + q]] * P.scale[q];
-            T child_prob = (T)1;
-            for (int ch = 0; ch < d.n_child; ++ch) {
-                const uint32_t m = (info[t] >> (4 + 4 * ch)) & 15u;
-                const uint32_t cc = NODG ? 0u : (uint32_t)code[kids[ch]];
-                T ctmp = (T)0;
-#pragma unroll
-                for (int ij = 0; ij < 4; ++ij) {
-                    if (!((m >> ij) & 1u)) continue;
-                    ctmp += NODG ? quarter : quarter * rho<T>(cc, ij >> 1, ij & 1, th);
-                }
-                child_prob *= ctmp;
-            }
-            tmp *= child_prob;
+            for (int q = 0; q < NP; ++q) tmp *= arena[P.tab[q] + pos * P.size[q] + ix.at(t, q)] * P.scale[q];
+            tmp *= child_prob[t];
             total += tmp;
         }
     } else {
@@ -168,8 +189,7 @@ __device__ T eval_cell(const LodArgs& A, const OpDesc& d, const T tr[4], const P
             if (!(info[t] & 4u)) continue;
             T tmp = tr[t];
 #pragma unroll
-            for (int q = 0; q < kMaxPrev; ++q)
-                if (q < np) tmp *= arena[P.tab[q] + pos * P.size[q] + ix[t * np + q]] * P.scale[q];
+            for (int q = 0; q < NP; ++q) tmp *= arena[P.tab[q] + pos * P.size[q] + ix.at(t, q)] * P.scale[q];
             total += tmp;
         }
     }
@@ -205,11 +225,39 @@ __device__ __forceinline__ const double* trait_ptr<double>(const LodArgs& A) { r
 template <>
 __device__ __forceinline__ const float* trait_ptr<float>(const LodArgs& A) { return A.trait_f; }
 
+// Every (position, cell) item of op d, spread over the wave's lanes; returns
+// the lane's largest value (for the FP32 table exponent).
+template <typename T, bool RESCALE, bool NODG, int NP>
+__device__ float op_items(const LodArgs& A, const OpDesc& d, const T tr[4], const PrevSet<T>& P, const int* kids,
+                          T* arena, const uint8_t* code, const T* ths, int lane) {
+    const int npos = A.npos;
+    const int items = d.size * npos;
+    const float inv = 1.0f / (float)d.size;
+    float mx = 0.0f;
+    for (int it = lane; it < items; it += 64) {
+        int pos = (int)(((float)it + 0.5f) * inv);
+        if (pos >= npos) pos = npos - 1;
+        int c = it - pos * d.size;
+        if (c < 0) { --pos; c += d.size; }
+        else if (c >= d.size) { ++pos; c -= d.size; }
+        Thetas<T> th;
+        if (!NODG) {
+            th.t1 = ths[4 * pos + 0];
+            th.a1 = ths[4 * pos + 1];
+            th.t2 = ths[4 * pos + 2];
+            th.a2 = ths[4 * pos + 3];
+        }
+        const T val = eval_cell<T, NODG, NP>(A, d, tr, P, kids, c, pos, arena, code, th);
+        arena[d.tab_off + pos * d.size + c] = val;
+        if (RESCALE) mx = fmaxf(mx, (float)val);
+    }
+    return mx;
+}
+
 // Evaluates every level of the program for one graph (or none when NODG).
 // `ths` holds the flanking thetas of each position (LDS).
 template <typename T, bool RESCALE, bool NODG>
 __device__ void run_levels(const LodArgs& A, T* arena, uint32_t* maxbits, const uint8_t* code, const T* ths) {
-    const int npos = A.npos;
     const int W = A.waves;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = (int)(threadIdx.x & 63u);
@@ -239,29 +287,25 @@ __device__ void run_levels(const LodArgs& A, T* arena, uint32_t* maxbits, const 
             int kids[7];
 #pragma unroll
             for (int ch = 0; ch < 7; ++ch) kids[ch] = ch < d.n_child ? ((cint_p)(A.children + d.child_off + ch))[0] : 0;
-            const int items = d.size * npos;
-            const float inv = 1.0f / (float)d.size;
             float mx = 0.0f;
-            for (int it = lane; it < items; it += 64) {
-                int pos = (int)(((float)it + 0.5f) * inv);
-                if (pos >= npos) pos = npos - 1;
-                int c = it - pos * d.size;
-                if (c < 0) { --pos; c += d.size; }
-                else if (c >= d.size) { ++pos; c -= d.size; }
-                Thetas<T> th;
-                if (!NODG) {
-                    th.t1 = ths[4 * pos + 0];
-                    th.a1 = ths[4 * pos + 1];
-                    th.t2 = ths[4 * pos + 2];
-                    th.a2 = ths[4 * pos + 3];
-                }
-                const T val = eval_cell<T, NODG>(A, d, tr, P, kids, c, pos, arena, code, th);
-                arena[d.tab_off + pos * d.size + c] = val;
-                if (RESCALE) mx = fmaxf(mx, (float)val);
+            switch (d.n_prev) {
+                case 0: mx = op_items<T, RESCALE, NODG, 0>(A, d, tr, P, kids, arena, code, ths, lane); break;
+                case 1: mx = op_items<T, RESCALE, NODG, 1>(A, d, tr, P, kids, arena, code, ths, lane); break;
+                case 2: mx = op_items<T, RESCALE, NODG, 2>(A, d, tr, P, kids, arena, code, ths, lane); break;
+                case 3: mx = op_items<T, RESCALE, NODG, 3>(A, d, tr, P, kids, arena, code, ths, lane); break;
+                case 4: mx = op_items<T, RESCALE, NODG, 4>(A, d, tr, P, kids, arena, code, ths, lane); break;
+                case 5: mx = op_items<T, RESCALE, NODG, 5>(A, d, tr, P, kids, arena, code, ths, lane); break;
+                case 6: mx = op_items<T, RESCALE, NODG, 6>(A, d, tr, P, kids, arena, code, ths, lane); break;
+                case 7: mx = op_items<T, RESCALE, NODG, 7>(A, d, tr, P, kids, arena, code, ths, lane); break;
+                default: mx = op_items<T, RESCALE, NODG, 8>(A, d, tr, P, kids, arena, code, ths, lane); break;
             }
             if (RESCALE) {
                 const float m = wave_max(mx);
-                if (lane == 0) maxbits[op] = __float_as_uint(m);
+                if (lane == 0) {
+                    maxbits[op] = __float_as_uint(m);
+                    // running sum of the exponents the consumers divide out
+                    if (op != A.last_op) atomicAdd(reinterpret_cast<int*>(maxbits + A.n_ops), exp_from_bits(maxbits[op]));
+                }
             }
         }
         __syncthreads();
@@ -274,12 +318,7 @@ __device__ double final_log(const LodArgs& A, const T* arena, const uint32_t* ma
     const OpDesc last = load_op(A.ops, A.last_op);
     const double L = (double)arena[last.tab_off + pos * last.size];
     double s = 0.0;
-    if (RESCALE) {
-        int e = 0;
-        for (int o = 0; o < A.n_ops; ++o)
-            if (o != A.last_op) e += exp_from_bits(maxbits[o]);
-        s = (double)e * 0.69314718055994530942;
-    }
+    if (RESCALE) s = (double)reinterpret_cast<const int*>(maxbits)[A.n_ops] * 0.69314718055994530942;
     return L > 0.0 ? log(L) + s : -INFINITY;
 }
 
@@ -323,6 +362,7 @@ __global__ void lod_kernel(LodArgs A) {
             const uint32_t p1 = (c1[(b + 1) >> 5] >> ((b + 1) & 31)) & 1u;
             code[p] = (uint8_t)(m0 | (m1 << 1) | (p0 << 2) | (p1 << 3));
         }
+        if (RESCALE && threadIdx.x == 0) maxbits[A.n_ops] = 0u;
         __syncthreads();
         run_levels<T, RESCALE, false>(A, arena, maxbits, code, ths);
         if ((int)threadIdx.x < npos) {
@@ -346,6 +386,8 @@ __global__ void trait_prob_kernel(LodArgs A, double* out) {
     extern __shared__ __align__(16) unsigned char smem[];
     T* arena = reinterpret_cast<T*>(smem);
     uint32_t* maxbits = reinterpret_cast<uint32_t*>(smem + A.arena_bytes);
+    if (RESCALE && threadIdx.x == 0) maxbits[A.n_ops] = 0u;
+    __syncthreads();
     run_levels<T, RESCALE, true>(A, arena, maxbits, nullptr, nullptr);
     if (threadIdx.x == 0) out[0] = final_log<T, RESCALE>(A, arena, maxbits, 0);
 }

@@ -223,6 +223,9 @@ std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int
             lds += 4.0 * d.size * d.n_prev;
             // precompiled terms of every cell
             const int pat_digit = d.type == kChild ? digit_of(ped.father[d.peel]) : -1;
+            // each op's block starts 8-byte aligned: a cell's 4 x n_prev
+            // entries are read as n_prev 8-byte words
+            while (P.idx.size() % 4) P.idx.push_back(0);
             d.idx_base = (int)P.idx.size();
             if (d.n_child > 7) return fmt("op %d: PARENT peel with more than 7 children in the cutset", o);
             for (int c = 0; c < d.size; ++c) {
