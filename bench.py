@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--batch", type=int, default=64, help="descent graphs per LOD-throughput launch")
     ap.add_argument("--config", default="ped150_snp1000")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the 16-pedigree batch and the adjoint workloads (profiling passes)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -211,47 +213,50 @@ def main():
 
     # ---- 16 pedigrees of uneven size, 4 chains each, one captured sweep per
     # context replayed side by side (sl_batch_*), shapes from the autotuner ----
+    extras = not args.no_extras
     from swiftlink_amd.tune import autotune
-    bcases = synth.make_batch(tempfile.mkdtemp(prefix="slbatch_"))
-    bctx, tunes = [], []
-    BCH, BK, BW = 4, 10, 2
-    for bcase, bdata in bcases:
-        c = native.Context(bcase, native.SL_FP32_RESCALED)
-        c.mcmc_init(np.repeat(synth.true_graph(bdata, bcase)[None], BCH, axis=0))
-        tu = autotune(c)
-        tunes.append({"people": bcase.N, "snps": bcase.M, "block": tu["block"], "gpw": tu["gpw"],
-                      "msampler_mode": tu["msampler_mode"]})
-        bctx.append(c)
-    bsched = (0, PERIOD, LPROB, SEED)
-    t0 = time.perf_counter()
-    for c in bctx:
-        c.mcmc_run(BK, *bsched)
-    for c in bctx:
-        c.sync()
-    bseq = time.perf_counter() - t0
-    with native.Batch(bctx) as bb:
-        bb.run(BW, *bsched, first_sweep=BK)
+    if extras:
+        bcases = synth.make_batch(tempfile.mkdtemp(prefix="slbatch_"))
+        bctx, tunes = [], []
+        BCH, BK, BW = 4, 10, 2
+        for bcase, bdata in bcases:
+            c = native.Context(bcase, native.SL_FP32_RESCALED)
+            c.mcmc_init(np.repeat(synth.true_graph(bdata, bcase)[None], BCH, axis=0))
+            tu = autotune(c)
+            tunes.append({"people": bcase.N, "snps": bcase.M, "block": tu["block"], "gpw": tu["gpw"],
+                          "msampler_mode": tu["msampler_mode"]})
+            bctx.append(c)
+        bsched = (0, PERIOD, LPROB, SEED)
         t0 = time.perf_counter()
-        bstats = bb.run(BK, *bsched, first_sweep=BK + BW)
-        bdt = time.perf_counter() - t0
-    bchains = BCH * len(bctx)
-    for c in bctx:
-        c.close()
+        for c in bctx:
+            c.mcmc_run(BK, *bsched)
+        for c in bctx:
+            c.sync()
+        bseq = time.perf_counter() - t0
+        with native.Batch(bctx) as bb:
+            bb.run(BW, *bsched, first_sweep=BK)
+            t0 = time.perf_counter()
+            bstats = bb.run(BK, *bsched, first_sweep=BK + BW)
+            bdt = time.perf_counter() - t0
+        bchains = BCH * len(bctx)
+        for c in bctx:
+            c.close()
 
     # ---- adjoint of the trait peel: LOD and its gradient over resident
     # graphs of the differentiation workload (ped200_snp2000) ----
-    acase, adata = synth.make_case(ADJ_CONFIG, tempfile.mkdtemp(prefix="sladj_"))
-    ag0 = synth.true_graph(adata, acase)
-    AG = 64
-    apacked = torch.from_numpy(pack_graphs(acase, np.repeat(ag0[None], AG, axis=0)).reshape(-1).view(np.int32)).cuda()
-    torch.cuda.synchronize()
-    with native.Context(acase, native.SL_FP64) as actx:
-        actx.set_trait_model(*MOD_MODEL)
-        actx.lod_grad_packed(apacked.data_ptr(), 2)
-        t0 = time.perf_counter()
-        alod, adlod = actx.lod_grad_packed(apacked.data_ptr(), AG)
-        adt = time.perf_counter() - t0
-    apos = (acase.M - 1) * acase.nlod
+    if extras:
+        acase, adata = synth.make_case(ADJ_CONFIG, tempfile.mkdtemp(prefix="sladj_"))
+        ag0 = synth.true_graph(adata, acase)
+        AG = 64
+        apacked = torch.from_numpy(pack_graphs(acase, np.repeat(ag0[None], AG, axis=0)).reshape(-1).view(np.int32)).cuda()
+        torch.cuda.synchronize()
+        with native.Context(acase, native.SL_FP64) as actx:
+            actx.set_trait_model(*MOD_MODEL)
+            actx.lod_grad_packed(apacked.data_ptr(), 2)
+            t0 = time.perf_counter()
+            alod, adlod = actx.lod_grad_packed(apacked.data_ptr(), AG)
+            adt = time.perf_counter() - t0
+        apos = (acase.M - 1) * acase.nlod
 
     # ---- roofline of the meiosis-step kernel (one chain: mspec_kernel, the
     # forward-backward pass of step k beside the founder-allele likelihoods of
@@ -308,26 +313,28 @@ def main():
                          "traffic": profile_traffic("lod_kernel", args.config),
                          "flops_per_eval": flops, "launch_ms": k32},
     }
-    res["batch_pedigrees"] = {
-        "workload": f"{len(bcases)} pedigrees of 40-400 people (1-6 loops) and 500-5000 SNPs at 0.1-0.5 cM, "
-                    f"{BCH} chains each, FP32-rescaled samplers and peel, FP64 LOD accumulation, L:M 1:1, "
-                    f"scored every {PERIOD}th sweep",
-        "chains": bchains, "sweeps_per_chain": BK, "sweeps_per_s": bchains * BK / bdt,
-        "sequential_sweeps_per_s": bchains * BK / bseq,
-        "l_sweeps": int(sum(x["l_sweeps"] for x in bstats)), "m_sweeps": int(sum(x["m_sweeps"] for x in bstats)),
-        "note": "sweeps_per_s: one hipGraph-captured sweep per pedigree, replayed on the pedigrees' streams side by "
-                "side (sl_batch_run); sequential_sweeps_per_s: sl_mcmc_run on each pedigree in turn, same shapes",
-        "autotuned": tunes}
-    res["adjoint"] = {
-        "workload": f"{ADJ_CONFIG}: {acase.N} people, {acase.M} SNPs, {apos} positions; {AG} resident graphs "
-                    f"(copies of the gene-drop graph), model penetrance {MOD_MODEL[0]}, frequency {MOD_MODEL[1]}",
-        "graphs": AG, "positions": apos, "ms": adt * 1e3, "gradient_evals_per_s": AG * apos / adt,
-        "partials_per_eval": 6, "dtype": "f64", "max_lod": float(alod.max()),
-        "note": "sl_lod_grad: ln L and d ln L/d(3 penetrances, frequency, theta_L, theta_R) per (graph, position), "
-                "reduced on the device to the LOD and its gradient at every position"}
+    if extras:
+        res["batch_pedigrees"] = {
+            "workload": f"{len(bcases)} pedigrees of 40-400 people (1-6 loops) and 500-5000 SNPs at 0.1-0.5 cM, "
+                        f"{BCH} chains each, FP32-rescaled samplers and peel, FP64 LOD accumulation, L:M 1:1, "
+                        f"scored every {PERIOD}th sweep",
+            "chains": bchains, "sweeps_per_chain": BK, "sweeps_per_s": bchains * BK / bdt,
+            "sequential_sweeps_per_s": bchains * BK / bseq,
+            "l_sweeps": int(sum(x["l_sweeps"] for x in bstats)), "m_sweeps": int(sum(x["m_sweeps"] for x in bstats)),
+            "note": "sweeps_per_s: one hipGraph-captured sweep per pedigree, replayed on the pedigrees' streams side by "
+                    "side (sl_batch_run); sequential_sweeps_per_s: sl_mcmc_run on each pedigree in turn, same shapes",
+            "autotuned": tunes}
+        res["adjoint"] = {
+            "workload": f"{ADJ_CONFIG}: {acase.N} people, {acase.M} SNPs, {apos} positions; {AG} resident graphs "
+                        f"(copies of the gene-drop graph), model penetrance {MOD_MODEL[0]}, frequency {MOD_MODEL[1]}",
+            "graphs": AG, "positions": apos, "ms": adt * 1e3, "gradient_evals_per_s": AG * apos / adt,
+            "partials_per_eval": 6, "dtype": "f64", "max_lod": float(alod.max()),
+            "note": "sl_lod_grad: ln L and d ln L/d(3 penetrances, frequency, theta_L, theta_R) per (graph, position), "
+                    "reduced on the device to the LOD and its gradient at every position"}
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(tmp, os.path.join(tmp, args.config), case, threads)
+    if not args.no_cpu and extras:
         _, ajac = native.trait_model_tables(MOD_MODEL[0], MOD_MODEL[1], acase.affection, acase.F)
         from swiftlink_amd.linkage import disease_probs
         import dataclasses
