@@ -215,3 +215,29 @@ def test_msweep_fp64_ped150(ped150, mode):
     print(f"ped150 M-sweep: {near} draws within 1e-12 of their threshold")
     assert near == 0
     assert np.array_equal(out, o.msweep(g0, shuffled_order(o, 2024, 0, 1), 2024, 0, 1))
+
+
+@pytest.fixture(scope="module")
+def long_map(tmp_path_factory):
+    """A small pedigree over a long marker map: the forward-backward pass then
+    runs its segments from global memory (M > 8 x 512 in one launch per step)
+    or with eight loci per thread (two launches per step)."""
+    d = tmp_path_factory.mktemp("ped30m5000")
+    case, data = synth.make_case("ped30_snp100", str(d), n_markers=5000, spacing_cm=0.1)
+    return case, synth.true_graph(data, case), Oracle(case)
+
+
+@pytest.mark.parametrize("mode", [native.SL_MSAMPLER_SPECULATIVE, native.SL_MSAMPLER_TWO_LAUNCH])
+def test_msweep_long_map_matches_oracle(long_map, mode):
+    case, g0, o = long_map
+    assert case.M == 5000
+    starts = np.stack([g0, g0])
+    with native.Context(case, native.SL_FP64, msampler_mode=mode) as ctx:
+        ctx.mcmc_init(starts)
+        ctx.msweep(606, 2)
+        out = ctx.mcmc_graphs()
+        assert np.all(ctx.mcmc_status() == 0)
+        near = ctx.mcmc_near_edge()
+    assert np.all(near == 0)
+    for c in range(2):
+        assert np.array_equal(out[c], o.msweep(starts[c], shuffled_order(o, 606, c, 2), 606, c, 2)), c
