@@ -88,8 +88,9 @@ typedef struct {
  * whose speculative workgroup would need more than 160 KiB of LDS always take
  * two launches.
  * SL_MSAMPLER_LDS_GRAPHS may be or-ed in: every founder-allele graph is then
- * built in LDS, as it always is for pedigrees of more than 512 people (smaller
- * ones keep a wavefront's graph in registers); results are identical. */
+ * built in LDS, as it always is for pedigrees of more than 1024 people or 512
+ * founders (smaller ones keep a wavefront's graph in registers); results are
+ * identical. */
 #define SL_MSAMPLER_AUTO 0
 #define SL_MSAMPLER_SPECULATIVE 1
 #define SL_MSAMPLER_TWO_LAUNCH 2

@@ -928,7 +928,7 @@ static sl::MsampArgs msamp_args(sl_ctx* c) {
     A.status = c->d_status;
     A.near_edge = c->d_near;
     A.wave_bytes = c->m_wave_bytes;
-    A.people_chunks = (c->m_mode & SL_MSAMPLER_LDS_GRAPHS) ? 0 : sl::msampler_people_chunks(c->N);
+    A.people_chunks = (c->m_mode & SL_MSAMPLER_LDS_GRAPHS) ? 0 : sl::msampler_people_chunks(c->N, c->F);
     A.ptab = c->d_ptab;
     A.ptab_bytes = c->m_ptab_bytes;
     A.n_chains = c->n_chains;

@@ -240,7 +240,8 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
     return ll;
 }
 
-// The same likelihood for pedigrees of at most 64*KC people, with the wave's
+// The same likelihood for pedigrees of at most 64*KC people and 32*KC
+// founders, with the wave's
 // working set in registers: lane j holds people j, j+64, ... (genotype, edge,
 // the union-find words of the edge's two nodes) and slots j, j+64, ... of the
 // generation-ordered person table.  A round reads the union-find words of
@@ -250,7 +251,7 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
 // summation are those of fag_log_likelihood, so the result is the same.
 template <int KC>
 __device__ double fag_log_likelihood_r(const MsampArgs& A, const uint2* ptab, const WaveArea& w, int l) {
-    constexpr int KN = 2 * KC;  // node chunks (2F <= 2N)
+    constexpr int KN = KC;  // node chunks: the chunk class covers 2F too
     const int lane = lane_id();
     const int N = A.N, F = A.F, NF = A.N - A.F, nodes = 2 * A.F;
     int g[KC];
@@ -934,8 +935,9 @@ __global__ void morder_kernel(MsampArgs A, const int32_t* fixed_order) {
 
 bool msampler_supported(int N, int F) { return F >= 1 && 2 * F <= kMaxFagNodes && N <= 65535; }
 
-int msampler_people_chunks(int N) {
-    return N <= 128 ? 2 : N <= 192 ? 3 : N <= 256 ? 4 : N <= 384 ? 6 : N <= 512 ? 8 : 0;
+int msampler_people_chunks(int N, int F) {
+    const int n = N > 2 * F ? N : 2 * F;  // people and founder alleles, 64 per chunk
+    return n <= 128 ? 2 : n <= 192 ? 3 : n <= 256 ? 4 : n <= 384 ? 6 : n <= 512 ? 8 : n <= 768 ? 12 : n <= 1024 ? 16 : 0;
 }
 
 size_t msampler_wave_bytes(int N, int F, int W) {
@@ -989,6 +991,8 @@ hipError_t configure_msampler_lds(size_t job_lds, size_t spec_lds) {
     if (e == hipSuccess) e = set_lds_kc<4>(job_lds, spec_lds);
     if (e == hipSuccess) e = set_lds_kc<6>(job_lds, spec_lds);
     if (e == hipSuccess) e = set_lds_kc<8>(job_lds, spec_lds);
+    if (e == hipSuccess) e = set_lds_kc<12>(job_lds, spec_lds);
+    if (e == hipSuccess) e = set_lds_kc<16>(job_lds, spec_lds);
     return e;
 }
 
@@ -1007,6 +1011,8 @@ hipError_t launch_mstep(const MsampArgs& A, size_t lds, hipStream_t s) {
         case 4: hipLaunchKernelGGL(mstep_kernel<4>, grid, dim3(kMsampBlock), lds, s, A); break;
         case 6: hipLaunchKernelGGL(mstep_kernel<6>, grid, dim3(kMsampBlock), lds, s, A); break;
         case 8: hipLaunchKernelGGL(mstep_kernel<8>, grid, dim3(kMsampBlock), lds, s, A); break;
+        case 12: hipLaunchKernelGGL(mstep_kernel<12>, grid, dim3(kMsampBlock), lds, s, A); break;
+        case 16: hipLaunchKernelGGL(mstep_kernel<16>, grid, dim3(kMsampBlock), lds, s, A); break;
         default: hipLaunchKernelGGL(mstep_kernel<0>, grid, dim3(kMsampBlock), lds, s, A); break;
     }
     return hipGetLastError();
@@ -1030,6 +1036,8 @@ hipError_t launch_mspec(const MsampArgs& A, size_t lds, hipStream_t s) {
         case 4: launch_mspec_kc<4>(A, lds, s); break;
         case 6: launch_mspec_kc<6>(A, lds, s); break;
         case 8: launch_mspec_kc<8>(A, lds, s); break;
+        case 12: launch_mspec_kc<12>(A, lds, s); break;
+        case 16: launch_mspec_kc<16>(A, lds, s); break;
         default: launch_mspec_kc<0>(A, lds, s); break;
     }
     return hipGetLastError();

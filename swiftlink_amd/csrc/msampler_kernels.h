@@ -50,13 +50,13 @@ struct MsampArgs {
     int check_kind;               // 1: act only on chains whose sweep `sweep` is an M-sweep
     double lprob;                 // L-sweep probability of the sweep schedule
     int wave_bytes;               // LDS bytes per wavefront
-    int people_chunks;            // 2, 3, 4, 6, 8: graphs built in registers for at most 64x that
-                                  // many people; 0: in LDS, any size
+    int people_chunks;            // 2, 3, 4, 6, 8, 12, 16: graphs built in registers for at most 64x
+                                  // that many people and 32x that many founders; 0: in LDS, any size
     int n_chains;
 };
 
 bool msampler_supported(int N, int F);
-int msampler_people_chunks(int N);  // register chunks for N people, 0 when too many
+int msampler_people_chunks(int N, int F);  // register chunks for N people, F founders; 0 when too many
 size_t msampler_wave_bytes(int N, int F, int W);
 size_t msampler_ptab_bytes(int N, int F);
 size_t msampler_lds_bytes(const MsampArgs& A, int block);

@@ -34,7 +34,7 @@ def shuffled_order(o, seed, chain, sweep):
 
 
 # launch modes, each with the founder-allele graphs in registers (the default
-# up to 512 people) and in LDS (the path of larger pedigrees)
+# up to 1024 people) and in LDS (the path of larger pedigrees)
 MODES = [m | g for m in (native.SL_MSAMPLER_SPECULATIVE, native.SL_MSAMPLER_TWO_LAUNCH)
          for g in (0, native.SL_MSAMPLER_LDS_GRAPHS)]
 GRAPHS = [native.SL_MSAMPLER_AUTO, native.SL_MSAMPLER_LDS_GRAPHS]
