@@ -185,6 +185,29 @@ int sl_mcmc_get_graphs(sl_ctx* ctx, int32_t* graphs);
  * meiosis_sampler.cc:33-36); the failing step leaves the graph unchanged. */
 int sl_mcmc_status(sl_ctx* ctx, int32_t* status);
 
+/* Sequential imputation: the starting descent graph of the chains
+ * (SequentialImputation::parallel_run, sequential_imputation.cc:47-114, called
+ * at linkage_program.cc:144-146; option -s gives `iterations`).
+ * iterations >= 1: that many replicates of LocusSampler::sequential_imputation
+ * (locus_sampler2.cc:209-241) run side by side, one workgroup each per locus
+ * step.  Replicate r draws all its uniforms in sequence from the Philox stream
+ * (draw, step=0xFFFFFFFC, sweep=0, chain=r): draw 0 picks the starting locus
+ * s = floor(u M) (get_random_locus, sampler.h:29-31), then every locus step
+ * takes n_ops op draws and one per homozygous-parent meiosis, as
+ * LocusSampler::step does; loci s, s-1..0 (conditioning on l+1), s+1..M-1
+ * (conditioning on l-1).  Its weight is the sum over loci of ln L (natural
+ * log; the reference prints it / ln 10).  iterations == 0: one pass of
+ * LocusSampler::locus_by_locus (:183-207), every locus with neither neighbour,
+ * draws from 0.  The best replicate has the largest weight (the first on
+ * ties).  Outputs (each may be NULL): best_graph int32 [M][2N]; weights
+ * [max(iterations,1)]; graphs int32 [max(iterations,1)][M][2N]; best_index.
+ * n_chains > 0 also makes n_chains chains that all start from the best graph
+ * (as sl_mcmc_init would).  Returns SL_E_ZERO_LIKELIHOOD when a replicate met
+ * a zero locus likelihood (the reference aborts, sequential_imputation.cc:84-88). */
+#define SL_IMPUTE_MAX 65535
+int sl_sequential_imputation(sl_ctx* ctx, uint64_t seed, int32_t iterations, int32_t n_chains, int32_t* best_graph,
+                             double* weights, int32_t* graphs, int32_t* best_index);
+
 /* One L-sweep on every chain: LocusSampler::step (locus_sampler2.cc:128-159)
  * at every locus, loci of parity first_parity ascending then the other parity
  * (non-adjacent loci run concurrently, as run_gpu_lsampler_kernel launched

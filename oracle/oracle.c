@@ -24,6 +24,7 @@ enum { AL_U = 0, AL_A = 1 };
 
 static const uint32_t STEP_SCHED = 0xFFFFFFFFu;
 static const uint32_t STEP_SHUFFLE = 0xFFFFFFFEu;
+static const uint32_t STEP_IMPUTE = 0xFFFFFFFCu;
 
 /* ---------------------------------------------------------------------------
  * Philox4x32-10 (Salmon et al., "Parallel random numbers: as easy as 1, 2, 3",
@@ -587,16 +588,18 @@ static double sampler_peel(const orc_case* c, work_t* w, const int32_t* dg, int 
     return w->tab[c->n_ops - 1][0];
 }
 
-/* sample_homo_mi (locus_sampler2.cc:44-65) */
-static int sample_homo(const orc_case* c, const int32_t* dg, int locus, int person, int parent, stream_t* s) {
+/* sample_homo_mi (locus_sampler2.cc:44-65); use_left / use_right are the
+ * negated ignore_left / ignore_right flags of LocusSampler::set_locus. */
+static int sample_homo(const orc_case* c, const int32_t* dg, int locus, int person, int parent, stream_t* s,
+                       int use_left, int use_right) {
     double p0 = 1.0, p1 = 1.0;
-    if (locus != 0) {
+    if (locus != 0 && use_left) {
         int b = dg_get(c, dg, person, locus - 1, parent);
         double th = c->theta[locus - 1], an = 1.0 - c->theta[locus - 1];
         p0 *= (b == 0) ? an : th;
         p1 *= (b == 1) ? an : th;
     }
-    if (locus != c->M - 1) {
+    if (locus != c->M - 1 && use_right) {
         int b = dg_get(c, dg, person, locus + 1, parent);
         double th = c->theta[locus], an = 1.0 - c->theta[locus];
         p0 *= (b == 0) ? an : th;
@@ -607,16 +610,19 @@ static int sample_homo(const orc_case* c, const int32_t* dg, int locus, int pers
 
 /* sample_mi / sample_hetero_mi (locus_sampler2.cc:32-88) */
 static int sample_mi(const orc_case* c, const int32_t* dg, int locus, int allele, int pt, int person, int parent,
-                     stream_t* s) {
+                     stream_t* s, int use_left, int use_right) {
     if (pt == PT_UA || pt == PT_AU) {
         if (allele == AL_U) return (pt == PT_UA) ? 0 : 1;
         return (pt == PT_UA) ? 1 : 0;
     }
-    return sample_homo(c, dg, locus, person, parent, s);
+    return sample_homo(c, dg, locus, person, parent, s, use_left, use_right);
 }
 
-int orc_lstep(const orc_case* c, int32_t* dg, int locus, uint64_t seed, uint32_t chain, uint32_t sweep,
-              int* ndraws) {
+/* LocusSampler::step (locus_sampler2.cc:128-159) after set_locus(locus,
+ * !use_left, !use_right): the thetas of an ignored side are 1
+ * (sampler_rfunction.h:84-112).  Draws from *s; *L gets the locus likelihood. */
+static int lstep_core(const orc_case* c, int32_t* dg, int locus, int use_left, int use_right, stream_t* s,
+                      double* L_out) {
     work_t w;
     work_init(c, &w, 1);
     int maxchild = 1;
@@ -625,16 +631,15 @@ int orc_lstep(const orc_case* c, int32_t* dg, int locus, uint64_t seed, uint32_t
         if (nch > maxchild) maxchild = nch;
     }
     double* T = (double*)malloc(sizeof(double) * 64 * (size_t)maxchild);
-    /* SamplerRfunction::set_locus_minimal (sampler_rfunction.h:114-137) */
     sthetas_t t = {1.0, 1.0, 1.0, 1.0};
-    if (locus != 0) { t.theta2 = c->theta[locus - 1]; t.antitheta2 = 1.0 - c->theta[locus - 1]; }
-    if (locus != c->M - 1) { t.theta = c->theta[locus]; t.antitheta = 1.0 - c->theta[locus]; }
+    if (locus != 0 && use_left) { t.theta2 = c->theta[locus - 1]; t.antitheta2 = 1.0 - c->theta[locus - 1]; }
+    if (locus != c->M - 1 && use_right) { t.theta = c->theta[locus]; t.antitheta = 1.0 - c->theta[locus]; }
     int rc = 0;
     double L = sampler_peel(c, &w, dg, locus, &t, T, maxchild);
+    if (L_out) *L_out = L;
     if (L == 0.0) {
         rc = -1;
     } else {
-        stream_t s = {seed, chain, sweep, (uint32_t)locus, 0};
         int* pmk = (int*)malloc(sizeof(int) * (size_t)c->N);
         for (int i = 0; i < c->N; ++i) pmk[i] = -1;
         /* reverse peel: SamplerRfunction::sample (sampler_rfunction.cc:159-188) */
@@ -650,7 +655,7 @@ int orc_lstep(const orc_case* c, int32_t* dg, int locus, uint64_t seed, uint32_t
             if (tot != 0.0)
                 for (int i = 0; i < 4; ++i) pd[i] /= tot;
             int last = 0, chosen = -1;
-            double r = next_u(&s);
+            double r = next_u(s);
             double run = 0.0;
             for (int i = 0; i < 4; ++i) {
                 run += pd[i];
@@ -664,19 +669,76 @@ int orc_lstep(const orc_case* c, int32_t* dg, int locus, uint64_t seed, uint32_t
             int tr = pmk[i];
             int mt = pmk[c->mother[i]];
             int mal = (tr == PT_UU || tr == PT_UA) ? AL_U : AL_A;
-            int mmi = sample_mi(c, dg, locus, mal, mt, i, 0, &s);
+            int mmi = sample_mi(c, dg, locus, mal, mt, i, 0, s, use_left, use_right);
             dg[(size_t)2 * c->N * locus + 2 * i + 0] = mmi;
             int pt = pmk[c->father[i]];
             int pal = (tr == PT_UU || tr == PT_AU) ? AL_U : AL_A;
-            int pmi = sample_mi(c, dg, locus, pal, pt, i, 1, &s);
+            int pmi = sample_mi(c, dg, locus, pal, pt, i, 1, s, use_left, use_right);
             dg[(size_t)2 * c->N * locus + 2 * i + 1] = pmi;
         }
-        if (ndraws) *ndraws = (int)s.draw;
         free(pmk);
     }
     free(T);
     work_free(c, &w);
     return rc;
+}
+
+double orc_locus_likelihood(const orc_case* c, const int32_t* dg, int locus, int use_left, int use_right) {
+    work_t w;
+    work_init(c, &w, 1);
+    int maxchild = 1;
+    for (int o = 0; o < c->n_ops; ++o) {
+        int nch = c->child_off[o + 1] - c->child_off[o];
+        if (nch > maxchild) maxchild = nch;
+    }
+    double* T = (double*)malloc(sizeof(double) * 64 * (size_t)maxchild);
+    sthetas_t t = {1.0, 1.0, 1.0, 1.0};
+    if (locus != 0 && use_left) { t.theta2 = c->theta[locus - 1]; t.antitheta2 = 1.0 - c->theta[locus - 1]; }
+    if (locus != c->M - 1 && use_right) { t.theta = c->theta[locus]; t.antitheta = 1.0 - c->theta[locus]; }
+    double L = sampler_peel(c, &w, dg, locus, &t, T, maxchild);
+    free(T);
+    work_free(c, &w);
+    return L;
+}
+
+/* The sweep's L-step: set_locus_minimal (sampler_rfunction.h:114-137) uses
+ * both neighbours; the stream sits at (seed, chain, sweep, step=locus). */
+int orc_lstep(const orc_case* c, int32_t* dg, int locus, uint64_t seed, uint32_t chain, uint32_t sweep,
+              int* ndraws) {
+    stream_t s = {seed, chain, sweep, (uint32_t)locus, 0};
+    int rc = lstep_core(c, dg, locus, 1, 1, &s, NULL);
+    if (ndraws && rc == 0) *ndraws = (int)s.draw;
+    return rc;
+}
+
+/* LocusSampler::sequential_imputation -> start_from (locus_sampler2.cc:209-241)
+ * or, with by_locus, locus_by_locus (:183-207); one stream at
+ * (seed, chain, 0, STEP_IMPUTE) drawn in sequence. */
+int orc_impute(const orc_case* c, int32_t* dg, uint64_t seed, uint32_t chain, int by_locus, double* weight) {
+    stream_t s = {seed, chain, 0, STEP_IMPUTE, 0};
+    double wsum = 0.0, L = 0.0;
+    int M = c->M;
+    if (by_locus) {
+        for (int l = 0; l < M; ++l) {
+            if (lstep_core(c, dg, l, 0, 0, &s, &L)) return -1;
+            wsum += log(L);
+        }
+    } else {
+        int start = (int)(next_u(&s) * (double)M); /* get_random_locus (sampler.h:29-31) as floor(u M) over the stream */
+        if (start >= M) start = M - 1;
+        if (lstep_core(c, dg, start, 0, 0, &s, &L)) return -1;
+        wsum += log(L);
+        for (int l = start - 1; l >= 0; --l) {
+            if (lstep_core(c, dg, l, 0, 1, &s, &L)) return -1;
+            wsum += log(L);
+        }
+        for (int l = start + 1; l < M; ++l) {
+            if (lstep_core(c, dg, l, 1, 0, &s, &L)) return -1;
+            wsum += log(L);
+        }
+    }
+    if (weight) *weight = wsum;
+    return 0;
 }
 
 /* ---------------------------------------------------------------------------

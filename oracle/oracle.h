@@ -83,6 +83,17 @@ int orc_lod_accumulate(const orc_case* c, const int32_t* graphs, int G, double* 
 int orc_lstep(const orc_case* c, int32_t* dg, int locus, uint64_t seed, uint32_t chain, uint32_t sweep,
               int* ndraws);
 
+/* Sequential imputation of one replicate into dg: start_from a locus drawn
+ * first (locus_sampler2.cc:209-241), or locus_by_locus (:183-207) when
+ * by_locus; every uniform in sequence at (seed, chain, 0, step=0xFFFFFFFC).
+ * *weight = sum over loci of ln L.  Returns 0, or -1 on a zero likelihood. */
+int orc_impute(const orc_case* c, int32_t* dg, uint64_t seed, uint32_t chain, int by_locus, double* weight);
+
+/* The L-sampler's forward-peel likelihood of `locus` given dg, conditioning
+ * on column l-1 / l+1 only where use_left / use_right (LocusSampler::set_locus
+ * + the forward loop of step, locus_sampler2.cc:132-135); dg is not changed. */
+double orc_locus_likelihood(const orc_case* c, const int32_t* dg, int locus, int use_left, int use_right);
+
 /* Founder-allele-graph likelihood of column `locus` (founder_allele_graph4.cc:34-442). */
 double orc_fag(const orc_case* c, const int32_t* dg, int locus);
 

@@ -21,6 +21,7 @@
 //   chain  <ped> <map> <dat> <outdir> seed sweeps burnin period lprob
 //   fag    <ped> <map> <dat> <outdir> <graphs> G  per-locus founder-allele-graph likelihood
 //   graphs <ped> <map> <dat> <outdir> seed G stride  random legal graphs (elimination + L-sweeps)
+//   impute <ped> <map> <dat> <outdir> seed R      sequential-imputation replicates (R = 0: locus by locus)
 //   bench  <ped> <map> <dat> <outdir> threads nL nM nG
 // Common options go before the sub-command as key=value: lods=5 peeliters=N
 // peelseed=S.
@@ -129,6 +130,7 @@ int get_random_int(int limit) {
 static const uint32_t STEP_SCHED = 0xFFFFFFFFu;
 static const uint32_t STEP_SHUFFLE = 0xFFFFFFFEu;
 static const uint32_t STEP_SETUP = 0xFFFFFFFDu;
+static const uint32_t STEP_IMPUTE = 0xFFFFFFFCu;
 
 // ---------------------------------------------------------------------------
 // small I/O helpers
@@ -406,6 +408,33 @@ static void cmd_msweep(Case& c, const std::string& out, const std::string& gfile
     }
     write_bin(out + "/graph_out.i32", dg.get_internal_ptr(), dg.get_internal_size());
     write_bin(out + "/draws.i32", &draws[0], draws.size() * 4);
+}
+
+// Sequential imputation (SequentialImputation::parallel_run,
+// sequential_imputation.cc:47-114): replicate r calls
+// LocusSampler::sequential_imputation with the stream at
+// (seed, r, 0, STEP_IMPUTE) drawn in sequence; R = 0 runs locus_by_locus once.
+// Writes every replicate's graph and weight and the chosen index (largest
+// weight, first on ties, as the reference's strict '>' scan).
+static void cmd_impute(Case& c, const std::string& out, uint64_t seed, int R) {
+    const int reps = R > 0 ? R : 1;
+    const size_t per = (size_t)2 * c.N * c.M;
+    std::vector<int32_t> all(per * reps);
+    std::vector<double> w(reps);
+    LocusSampler ls(c.p, c.m, c.psg, 0, c.sex_linked);
+    int best = -1;
+    double best_w = -INFINITY;
+    for (int r = 0; r < reps; ++r) {
+        DescentGraph dg(c.p, c.m, c.sex_linked);
+        stream_set(seed, (uint32_t)r, 0, STEP_IMPUTE);
+        w[r] = R > 0 ? ls.sequential_imputation(dg) : ls.locus_by_locus(dg);
+        memcpy(&all[per * r], dg.get_internal_ptr(), per * sizeof(int32_t));
+        if (w[r] > best_w) { best_w = w[r]; best = r; }
+    }
+    write_bin(out + "/impute_graphs.i32", &all[0], all.size() * 4);
+    write_bin(out + "/impute_weights.f64", &w[0], w.size() * 8);
+    int32_t b = best;
+    write_bin(out + "/impute_best.i32", &b, 4);
 }
 
 // Non-ignorable meioses, in index order (markov_chain.cc:65-80).
@@ -687,6 +716,7 @@ int main(int argc, char** argv) {
         cmd_chain(c, out, rest[0], strtoull(rest[1], 0, 10), atoi(rest[2]), atoi(rest[3]), atoi(rest[4]),
                   atoi(rest[5]), atof(rest[6]));
     else if (cmd == "fag" && nrest >= 2) cmd_fag(c, out, rest[0], atoi(rest[1]));
+    else if (cmd == "impute" && nrest >= 2) cmd_impute(c, out, strtoull(rest[0], 0, 10), atoi(rest[1]));
     else if (cmd == "graphs" && nrest >= 3) cmd_graphs(c, out, strtoull(rest[0], 0, 10), atoi(rest[1]), atoi(rest[2]));
     else if (cmd == "bench" && nrest >= 5)
         cmd_bench(c, out, rest[0], atoi(rest[1]), atoi(rest[2]), atoi(rest[3]), atoi(rest[4]));

@@ -151,6 +151,15 @@ struct sl_ctx {
     double* d_fbuf = nullptr;
     uint8_t* d_maps = nullptr;
     uint32_t* d_near = nullptr;
+    // sequential-imputation replicates
+    uint32_t* d_imp_graphs = nullptr;  // [R][M][W]
+    size_t imp_graphs_cap = 0;
+    uint32_t* d_imp_draw = nullptr;    // [R]
+    size_t imp_draw_cap = 0;
+    double* d_imp_weight = nullptr;    // [R]
+    size_t imp_weight_cap = 0;
+    int32_t* d_imp_status = nullptr;   // [R]
+    size_t imp_status_cap = 0;
     // profiling
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -284,7 +293,8 @@ int release_all(sl_ctx* c) {
                     c->d_trait_f, c->d_ptheta, c->d_logt, c->d_loga, c->d_acc, c->d_scalar,
                     c->d_partial, c->d_packed, c->d_stage, c->d_terms,
                     c->g_ops, c->g_prevs, c->g_children, c->g_tasks, c->g_tinfo, c->g_idx, c->g_level_base,
-                    c->g_sched_off, c->g_sched_ops, c->d_gdp, c->d_jac, c->d_gterms, c->d_gstate, c->d_sweep};
+                    c->g_sched_off, c->g_sched_ops, c->d_gdp, c->d_jac, c->d_gterms, c->d_gstate, c->d_sweep,
+                    c->d_imp_graphs, c->d_imp_draw, c->d_imp_weight, c->d_imp_status};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
@@ -780,11 +790,8 @@ static int need_sampler(sl_ctx* c) {
     return SL_OK;
 }
 
-int sl_mcmc_init(sl_ctx* c, int32_t n_chains, const int32_t* graphs) {
-    if (!c || n_chains <= 0 || !graphs) return SL_E_INVALID;
-    if (!c->samp_ready && !c->m_ready)
-        return fail(c, SL_E_UNSUPPORTED, "samplers unavailable: " + c->samp_why + "; " + c->m_why);
-    int rc;
+// (Re)allocates the per-chain state for n_chains chains.
+static int alloc_chains(sl_ctx* c, int n_chains) {
     const size_t words = (size_t)n_chains * c->M * c->W;
     const size_t cm = (size_t)n_chains * c->M;
     if (n_chains != c->n_chains) {
@@ -804,23 +811,33 @@ int sl_mcmc_init(sl_ctx* c, int32_t n_chains, const int32_t* graphs) {
         SL_HIP(c, hipMalloc((void**)&c->d_near, (size_t)n_chains * sizeof(uint32_t)));
         c->n_chains = n_chains;
     }
-    rc = stage_graphs(c, graphs, n_chains);
-    if (rc) return rc;
-    SL_HIP(c, hipMemcpyAsync(c->d_chains, c->d_packed, words * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
     SL_HIP(c, hipMemsetAsync(c->d_status, 0, (size_t)n_chains * sizeof(int32_t), c->stream));
     SL_HIP(c, hipMemsetAsync(c->d_near, 0, (size_t)n_chains * sizeof(uint32_t), c->stream));
+    return SL_OK;
+}
+
+int sl_mcmc_init(sl_ctx* c, int32_t n_chains, const int32_t* graphs) {
+    if (!c || n_chains <= 0 || !graphs) return SL_E_INVALID;
+    if (!c->samp_ready && !c->m_ready)
+        return fail(c, SL_E_UNSUPPORTED, "samplers unavailable: " + c->samp_why + "; " + c->m_why);
+    int rc = alloc_chains(c, n_chains);
+    if (rc) return rc;
+    rc = stage_graphs(c, graphs, n_chains);
+    if (rc) return rc;
+    const size_t words = (size_t)n_chains * c->M * c->W;
+    SL_HIP(c, hipMemcpyAsync(c->d_chains, c->d_packed, words * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
     SL_HIP(c, hipStreamSynchronize(c->stream));
     return SL_OK;
 }
 
-int sl_mcmc_get_graphs(sl_ctx* c, int32_t* out) {
-    if (!c || !out || c->n_chains <= 0) return SL_E_INVALID;
-    const size_t words = (size_t)c->n_chains * c->M * c->W;
+// Packed graphs [G][M][W] (device) to the reference layout int32 [G][M][2N].
+static int unpack_graphs(sl_ctx* c, const uint32_t* dev, int G, int32_t* out) {
+    const size_t words = (size_t)G * c->M * c->W;
     std::vector<uint32_t> h(words);
-    SL_HIP(c, hipMemcpyAsync(h.data(), c->d_chains, words * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    SL_HIP(c, hipMemcpyAsync(h.data(), dev, words * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     SL_HIP(c, hipStreamSynchronize(c->stream));
     const int nm = 2 * (c->N - c->F);
-    for (int ch = 0; ch < c->n_chains; ++ch)
+    for (int ch = 0; ch < G; ++ch)
         for (int l = 0; l < c->M; ++l) {
             int32_t* col = out + ((size_t)ch * c->M + l) * 2 * c->N;
             const uint32_t* w = h.data() + ((size_t)ch * c->M + l) * c->W;
@@ -828,6 +845,11 @@ int sl_mcmc_get_graphs(sl_ctx* c, int32_t* out) {
             for (int b = 0; b < nm; ++b) col[2 * c->F + b] = (int32_t)((w[b >> 5] >> (b & 31)) & 1u);
         }
     return SL_OK;
+}
+
+int sl_mcmc_get_graphs(sl_ctx* c, int32_t* out) {
+    if (!c || !out || c->n_chains <= 0) return SL_E_INVALID;
+    return unpack_graphs(c, c->d_chains, c->n_chains, out);
 }
 
 int sl_mcmc_status(sl_ctx* c, int32_t* status) {
@@ -891,6 +913,74 @@ static int run_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int first_parity
             SL_HIP(c, hipEventRecord(e1, c->stream));
             c->ev_used.push_back({SL_KERNEL_LSAMPLER, e0, e1, 1});
         }
+    }
+    return SL_OK;
+}
+
+int sl_sequential_imputation(sl_ctx* c, uint64_t seed, int32_t iterations, int32_t n_chains, int32_t* best_graph,
+                             double* weights, int32_t* graphs, int32_t* best_index) {
+    if (!c || iterations < 0 || iterations > SL_IMPUTE_MAX || n_chains < 0) return SL_E_INVALID;
+    int rc = need_sampler(c);
+    if (rc) return rc;
+    const int R = iterations > 0 ? iterations : 1;
+    const size_t per = (size_t)c->M * c->W;
+    if ((rc = ensure(c, &c->d_imp_graphs, &c->imp_graphs_cap, (size_t)R * per)) ||
+        (rc = ensure(c, &c->d_imp_draw, &c->imp_draw_cap, (size_t)R)) ||
+        (rc = ensure(c, &c->d_imp_weight, &c->imp_weight_cap, (size_t)R)) ||
+        (rc = ensure(c, &c->d_imp_status, &c->imp_status_cap, (size_t)R)))
+        return rc;
+    SL_HIP(c, hipMemsetAsync(c->d_imp_graphs, 0, (size_t)R * per * sizeof(uint32_t), c->stream));
+    // replicates draw their starting locus first (draw 0); locus by locus does not
+    SL_HIP(c, hipMemsetD32Async((hipDeviceptr_t)c->d_imp_draw, iterations > 0 ? 1 : 0, (size_t)R, c->stream));
+    SL_HIP(c, hipMemsetAsync(c->d_imp_weight, 0, (size_t)R * sizeof(double), c->stream));
+    SL_HIP(c, hipMemsetAsync(c->d_imp_status, 0, (size_t)R * sizeof(int32_t), c->stream));
+    sl::SampArgs A = samp_args(c);
+    A.graphs = c->d_imp_graphs;
+    A.status = c->d_imp_status;
+    A.seed = seed;
+    A.sweep = 0;
+    A.impute = iterations > 0 ? 1 : 2;
+    A.imp_draw = c->d_imp_draw;
+    A.imp_weight = c->d_imp_weight;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->prof) {
+        e0 = take_event(c);
+        e1 = take_event(c);
+        if (e0 && e1) SL_HIP(c, hipEventRecord(e0, c->stream));
+    }
+    for (int t = 0; t < c->M; ++t) {
+        A.impute_t = t;
+        SL_HIP(c, sl::launch_lsampler(A, c->precision, c->samp_block, c->samp_lds, R, c->stream));
+    }
+    if (c->prof && e0 && e1) {
+        SL_HIP(c, hipEventRecord(e1, c->stream));
+        c->ev_used.push_back({SL_KERNEL_LSAMPLER, e0, e1, c->M});
+    }
+    std::vector<double> w(R);
+    std::vector<int32_t> st(R);
+    SL_HIP(c, hipMemcpyAsync(w.data(), c->d_imp_weight, (size_t)R * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SL_HIP(c, hipMemcpyAsync(st.data(), c->d_imp_status, (size_t)R * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    for (int r = 0; r < R; ++r)
+        if (st[r] != 0)
+            return fail(c, st[r], "sequential imputation: replicate " + std::to_string(r) +
+                                      (st[r] == SL_E_ZERO_LIKELIHOOD ? " met a zero locus likelihood"
+                                                                     : " exceeded the sampler table capacity"));
+    // the best replicate: largest weight, the first on ties (sequential_imputation.cc:92-101)
+    int best = 0;
+    for (int r = 1; r < R; ++r)
+        if (w[r] > w[best]) best = r;
+    if (weights) std::copy(w.begin(), w.end(), weights);
+    if (best_index) *best_index = best;
+    if (graphs && (rc = unpack_graphs(c, c->d_imp_graphs, R, graphs))) return rc;
+    if (best_graph && (rc = unpack_graphs(c, c->d_imp_graphs + (size_t)best * per, 1, best_graph))) return rc;
+    if (n_chains > 0) {
+        if ((rc = alloc_chains(c, n_chains))) return rc;
+        for (int ch = 0; ch < n_chains; ++ch)
+            SL_HIP(c, hipMemcpyAsync(c->d_chains + (size_t)ch * per, c->d_imp_graphs + (size_t)best * per,
+                                     per * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+        SL_HIP(c, hipStreamSynchronize(c->stream));
     }
     return SL_OK;
 }

@@ -22,6 +22,12 @@
 //  4. meiosis indicators (sample_meiosis_indicators, locus_sampler2.cc:93-125):
 //     heterozygous parents fix the bit, homozygous parents draw in person
 //     order after the op draws; the whole column l is rewritten.
+//
+// The same kernel runs sequential imputation (SequentialImputation,
+// sequential_imputation.cc:47-114): one workgroup per replicate, one launch
+// per step of the replicates' locus walks, the walked-from neighbour as the
+// only conditioning column, each replicate's uniforms taken in sequence from
+// its own stream, and ln L of every locus summed into the replicate's weight.
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
@@ -106,6 +112,32 @@ __device__ int block_exclusive_scan(int* a, int n, int* tmp) {
     return total;
 }
 
+// Locus of step A.impute_t of one sequential-imputation replicate and the
+// neighbours it conditions on (LocusSampler::start_from and locus_by_locus,
+// locus_sampler2.cc:183-241): step 0 samples the starting locus s (draw 0,
+// get_random_locus, sampler.h:29-31) with neither neighbour, steps 1..s walk
+// left conditioning on l+1 only, steps s+1..M-1 walk right conditioning on
+// l-1 only.  Locus by locus: step t samples locus t with neither neighbour.
+__device__ int impute_locus(const SampArgs& A, int chain, bool* use_left, bool* use_right) {
+    const int t = A.impute_t, M = A.M;
+    if (A.impute == 2) {
+        *use_left = *use_right = false;
+        return t;
+    }
+    int s = (int)(philox_uniform(A.seed, (uint32_t)chain, A.sweep, kStepImpute, 0) * (double)M);
+    if (s >= M) s = M - 1;
+    if (t == 0) {
+        *use_left = *use_right = false;
+        return s;
+    }
+    if (t <= s) {
+        *use_left = false;
+        return s - t;
+    }
+    *use_right = false;
+    return t;
+}
+
 template <typename T>
 __device__ __forceinline__ T trait_at(const SampArgs& A, int person, int locus, int v);
 template <>
@@ -153,27 +185,43 @@ __global__ void lsampler_kernel(SampArgs A) {
     uint8_t* pmk = mask + N;                            // [N]
     uint8_t* bits = pmk + N;                            // [2NF]
     __shared__ int s_status;
+    __shared__ int s_scale_exp;  // sum of log2 of the table scale factors consumed (RESCALE)
 
     const int chain = blockIdx.y;
-    int parity = A.parity;
-    if (A.check_kind) {
-        // the chain's sweep kind and first parity (draws 0 and 1 at step kStepSched)
-        if (!(philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), kStepSched, 0) < A.lprob)) return;
-        const int p0 = philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), kStepSched, 1) < 0.5 ? 0 : 1;
-        parity = A.pass == 0 ? p0 : 1 - p0;
+    const uint32_t sweep = A.impute ? A.sweep : sweep_of(A);
+    int locus;
+    bool use_left = true, use_right = true;  // condition on column l-1 / l+1
+    uint32_t step_id, draw0 = 0;
+    if (A.impute) {
+        if (A.status[chain] != 0) return;
+        locus = impute_locus(A, chain, &use_left, &use_right);
+        step_id = kStepImpute;
+        draw0 = A.imp_draw[chain];
+    } else {
+        int parity = A.parity;
+        if (A.check_kind) {
+            // the chain's sweep kind and first parity (draws 0 and 1 at step kStepSched)
+            if (!(philox_uniform(A.seed, (uint32_t)chain, sweep, kStepSched, 0) < A.lprob)) return;
+            const int p0 = philox_uniform(A.seed, (uint32_t)chain, sweep, kStepSched, 1) < 0.5 ? 0 : 1;
+            parity = A.pass == 0 ? p0 : 1 - p0;
+        }
+        locus = parity + 2 * blockIdx.x;
+        if (locus >= M || A.status[chain] != 0) return;  // uniform over the workgroup
+        step_id = (uint32_t)locus;
     }
-    const int locus = parity + 2 * blockIdx.x;
-    if (locus >= M || A.status[chain] != 0) return;  // uniform over the workgroup
     uint32_t* g = A.graphs + (size_t)chain * M * A.W;
-    const uint32_t* colL = locus > 0 ? g + (size_t)(locus - 1) * A.W : nullptr;
-    const uint32_t* colR = locus < M - 1 ? g + (size_t)(locus + 1) * A.W : nullptr;
+    const uint32_t* colL = locus > 0 && use_left ? g + (size_t)(locus - 1) * A.W : nullptr;
+    const uint32_t* colR = locus < M - 1 && use_right ? g + (size_t)(locus + 1) * A.W : nullptr;
     const double thL = locus > 0 ? A.theta[locus - 1] : 1.0;  // theta2 (l-1, l)
     const double thR = locus < M - 1 ? A.theta[locus] : 1.0;  // theta (l, l+1)
     const int tid = threadIdx.x;
 
     // 1. masks, table sizes, transmission distributions, op draws
     for (int p = tid; p < N; p += blockDim.x) mask[p] = A.legal_t[(size_t)locus * N + p];
-    if (tid == 0) s_status = 0;
+    if (tid == 0) {
+        s_status = 0;
+        s_scale_exp = 0;
+    }
     __syncthreads();
     for (int o = tid; o < n_ops; o += blockDim.x) {
         cint_p d = (cint_p)(A.ops + o);
@@ -207,7 +255,7 @@ __global__ void lsampler_kernel(SampArgs A) {
         }
     }
     for (int o = tid; o < n_ops; o += blockDim.x)
-        uop[o] = philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), (uint32_t)locus, (uint32_t)(n_ops - 1 - o));
+        uop[o] = philox_uniform(A.seed, (uint32_t)chain, sweep, step_id, draw0 + (uint32_t)(n_ops - 1 - o));
     for (int i = tid; i < 4 * N; i += blockDim.x) trt[i] = trait_at<T>(A, i >> 2, locus, i & 3);
     __syncthreads();
     const int total_cells = block_exclusive_scan(off, n_ops, scan_tmp);
@@ -251,6 +299,7 @@ __global__ void lsampler_kernel(SampArgs A) {
             const int my_off = uni(off[op]);
             // consumed tables: offset, per-digit source and stride at this locus
             int p_off[kMaxPrev], p_nd[kMaxPrev];
+            int scale_exp = 0;
             T p_scale[kMaxPrev];
             int p_src[kMaxPrev][kMaxCut], p_stride[kMaxPrev][kMaxCut];
 #pragma unroll
@@ -269,7 +318,12 @@ __global__ void lsampler_kernel(SampArgs A) {
                     const int pop = pd[0];
                     p_off[q] = uni(off[pop]);
                     p_nd[q] = pd[1];
-                    if (RESCALE) p_scale[q] = (T)scale_from_bits(maxbits[pop]);
+                    if (RESCALE) {
+                        const uint32_t mb = maxbits[pop];
+                        p_scale[q] = (T)scale_from_bits(mb);
+                        const int ef = (int)((mb >> 23) & 0xFFu);
+                        scale_exp += ef ? 126 - ef : 0;
+                    }
 #pragma unroll
                     for (int j = 0; j < kMaxCut; ++j) {
                         if (j < p_nd[q]) {
@@ -299,6 +353,7 @@ __global__ void lsampler_kernel(SampArgs A) {
                     }
                 }
             }
+            if (RESCALE && A.impute && lane == 0 && scale_exp != 0) atomicAdd(&s_scale_exp, scale_exp);
             float mx = 0.0f;
             for (int c = lane; c < size; c += 64) {
                 // mixed-radix decode of the compact cell into trait values (2 bits per digit)
@@ -377,6 +432,13 @@ __global__ void lsampler_kernel(SampArgs A) {
         if (tid == 0) A.status[chain] = s_status;
         return;
     }
+    if (A.impute && tid == 0) {
+        // ln of the locus likelihood (locus_sampler2.cc:221, 228, 235): the
+        // rescaled last table divided by the scale factors it absorbed
+        double w = log((double)tab[off[n_ops - 1]]);
+        if (RESCALE) w -= (double)s_scale_exp * 0.69314718055994530942;
+        A.imp_weight[chain] += w;
+    }
 
     // 3. reverse sampling by reverse levels
     for (int R = 0; R < A.n_rlevels; ++R) {
@@ -418,7 +480,8 @@ __global__ void lsampler_kernel(SampArgs A) {
         mcount[b] = (pt == PT_UU || pt == PT_AA) ? 1 : 0;
     }
     __syncthreads();
-    block_exclusive_scan(mcount, 2 * NF, scan_tmp);
+    const int homo_draws = block_exclusive_scan(mcount, 2 * NF, scan_tmp);
+    if (A.impute && tid == 0) A.imp_draw[chain] = draw0 + (uint32_t)(n_ops + homo_draws);
     for (int b = tid; b < 2 * NF; b += blockDim.x) {
         const int person = F + (b >> 1), side = b & 1;
         const int parent = side ? A.father[person] : A.mother[person];
@@ -441,8 +504,8 @@ __global__ void lsampler_kernel(SampArgs A) {
                 p0 *= (nb == 0u) ? 1.0 - thR : thR;
                 p1 *= (nb == 1u) ? 1.0 - thR : thR;
             }
-            const double u = philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), (uint32_t)locus,
-                                            (uint32_t)(n_ops + mcount[b]));
+            const double u = philox_uniform(A.seed, (uint32_t)chain, sweep, step_id,
+                                            draw0 + (uint32_t)(n_ops + mcount[b]));
             bit = (u < (p0 / (p0 + p1))) ? 0u : 1u;
         }
         bits[b] = (uint8_t)bit;
@@ -483,7 +546,7 @@ hipError_t configure_lsampler_lds(int precision, size_t lds) {
 }
 
 hipError_t launch_lsampler(const SampArgs& A, int precision, int block, size_t lds, int n_chains, hipStream_t s) {
-    const int nloci = A.check_kind ? (A.M + 1) / 2 : (A.M - A.parity + 1) / 2;
+    const int nloci = A.impute ? 1 : A.check_kind ? (A.M + 1) / 2 : (A.M - A.parity + 1) / 2;
     dim3 grid(nloci, n_chains);
     if (precision == 0)
         hipLaunchKernelGGL((lsampler_kernel<double, false>), grid, dim3(block), lds, s, A);

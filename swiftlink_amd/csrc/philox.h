@@ -11,9 +11,12 @@ namespace sl {
 
 // Philox step indices of the draws that are not per-locus or per-meiosis-step:
 // a sweep's kind (draw 0) and first L-sweep parity (draw 1), and the M-sweep's
-// meiosis-order shuffle.
+// meiosis-order shuffle.  Sequential imputation draws every uniform of one
+// replicate in sequence at step kStepImpute (draw 0: the starting locus).
+// 0xFFFFFFFD is the peel-sequence search's stream on the host side.
 constexpr uint32_t kStepSched = 0xFFFFFFFFu;
 constexpr uint32_t kStepShuffle = 0xFFFFFFFEu;
+constexpr uint32_t kStepImpute = 0xFFFFFFFCu;
 
 __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll

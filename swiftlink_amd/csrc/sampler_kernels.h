@@ -34,6 +34,13 @@ struct SampArgs {
     int check_kind;              // 1: act only on chains whose sweep is an L-sweep, and take
     int pass;                    //    the parity of pass 0/1 from the chain's schedule draws
     double lprob;                // L-sweep probability of the sweep schedule
+    // sequential imputation (impute 1: start_from a drawn locus, 2: locus by
+    // locus): one workgroup per replicate (grid y), step impute_t of M; the
+    // replicate's uniforms run in sequence from imp_draw[chain] at step
+    // kStepImpute, and ln L of each locus is added to imp_weight[chain]
+    int impute, impute_t;
+    uint32_t* imp_draw;          // [chains] next draw index
+    double* imp_weight;          // [chains] running sum of ln L
 };
 
 size_t lsampler_lds_bytes(int precision, int cells_cap, int n_ops, int N, int F, int block);

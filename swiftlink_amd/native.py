@@ -93,6 +93,8 @@ _SIGS = {
     "sl_mcmc_init": (C.c_int, [C.c_void_p, C.c_int32, i32p]),
     "sl_mcmc_get_graphs": (C.c_int, [C.c_void_p, i32p]),
     "sl_mcmc_status": (C.c_int, [C.c_void_p, i32p]),
+    "sl_sequential_imputation": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_int32, i32p, f64p, i32p,
+                                           C.POINTER(C.c_int32)]),
     "sl_lsweep": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32]),
     "sl_msweep": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, i32p, C.c_int32]),
     "sl_mcmc_run": (C.c_int, [C.c_void_p, C.POINTER(Schedule), C.POINTER(McmcStats)]),
@@ -320,6 +322,23 @@ class Context:
         out = np.zeros(self.n_chains, np.int32)
         self._check(lib().sl_mcmc_status(self.h, _ptr(out, i32p)), "sl_mcmc_status")
         return out
+
+    def sequential_imputation(self, seed: int, iterations: int, n_chains: int = 0, all_graphs: bool = False):
+        """SequentialImputation::parallel_run on the device: `iterations`
+        replicates (0: one locus-by-locus pass).  Returns (best graph [M][2N],
+        weights, best index, every replicate's graph or None); with n_chains
+        > 0 the chains start from the best graph."""
+        R = max(iterations, 1)
+        best = np.zeros((self.case.M, 2 * self.case.N), np.int32)
+        w = np.zeros(R)
+        allg = np.zeros((R, self.case.M, 2 * self.case.N), np.int32) if all_graphs else None
+        bi = C.c_int32(-1)
+        self._check(lib().sl_sequential_imputation(self.h, seed, iterations, n_chains, _ptr(best, i32p),
+                                                   _ptr(w, f64p), _ptr(allg, i32p) if all_graphs else None,
+                                                   C.byref(bi)), "sl_sequential_imputation")
+        if n_chains > 0:
+            self.n_chains = n_chains
+        return best, w, bi.value, allg
 
     def lsweep(self, seed: int, sweep: int, first_parity: int = 0):
         self._check(lib().sl_lsweep(self.h, seed, sweep, first_parity), "sl_lsweep")

@@ -14,6 +14,9 @@ Each fixture holds:
               Philox stream (seed, chain 0, sweep 5)
   msweep_*    one M-sweep (MeiosisSampler reset + steps) under (seed, 0, sweep 6)
   chain_*     a 40-sweep chain in the shared schedule with scoring
+  impute_*    4 sequential-imputation replicates (LocusSampler::sequential_imputation)
+              under (seed 55, chain r, 0, step 0xFFFFFFFC), their weights and the pick
+  impute0_*   LocusSampler::locus_by_locus under (seed 56, 0, 0, step 0xFFFFFFFC)
 """
 from __future__ import annotations
 
@@ -56,11 +59,13 @@ def make(name, base, work):
     morder = np.random.default_rng(3).permutation(ml).astype(np.int32)
     morder.tofile(os.path.join(work, "morder.i32"))
     gi = os.path.join(work, "graph_init.i32")
-    for sub in ("ls", "ms", "ch"):
+    for sub in ("ls", "ms", "ch", "si", "si0"):
         os.makedirs(os.path.join(work, sub), exist_ok=True)
     run("lsweep", p, m, d, os.path.join(work, "ls"), gi, os.path.join(work, "lorder.i32"), 99, 0, 5)
     run("msweep", p, m, d, os.path.join(work, "ms"), gi, os.path.join(work, "morder.i32"), 99, 0, 6)
     run("chain", p, m, d, os.path.join(work, "ch"), gi, 77, 0, 40, 10, 3, 0.5)
+    run("impute", p, m, d, os.path.join(work, "si"), 55, 4)
+    run("impute", p, m, d, os.path.join(work, "si0"), 56, 0)
     rd = lambda f, dt: np.fromfile(os.path.join(work, f), dt)
     fx = {"case_" + k: v for k, v in case.to_npz_dict().items()}
     fx.update(
@@ -78,6 +83,12 @@ def make(name, base, work):
         chain_out=rd("ch/graph_out.i32", np.int32).reshape(M, 2 * N),
         chain_kinds=rd("ch/sweep_kinds.i32", np.int32), chain_acc=rd("ch/lod_acc.f64", np.float64),
         chain_count=np.array([json.load(open(os.path.join(work, "ch/chain_meta.json")))["count"]]),
+        impute_seed=np.array([55, 4]),
+        impute_graphs=rd("si/impute_graphs.i32", np.int32).reshape(4, M, 2 * N),
+        impute_weights=rd("si/impute_weights.f64", np.float64), impute_best=rd("si/impute_best.i32", np.int32),
+        impute0_seed=np.array([56]),
+        impute0_graph=rd("si0/impute_graphs.i32", np.int32).reshape(M, 2 * N),
+        impute0_weight=rd("si0/impute_weights.f64", np.float64),
     )
     path = os.path.join(OUT, name + ".npz")
     np.savez_compressed(path, **fx)

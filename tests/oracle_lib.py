@@ -47,7 +47,10 @@ def _load():
     lib.orc_log_sum.restype = C.c_double
     lib.orc_lod_accumulate.argtypes = [P, i32p, C.c_int, f64p]
     lib.orc_lstep.argtypes = [P, i32p, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(C.c_int)]
+    lib.orc_impute.argtypes = [P, i32p, C.c_uint64, C.c_uint32, C.c_int, f64p]
     lib.orc_fag.argtypes = [P, i32p, C.c_int]
+    lib.orc_locus_likelihood.argtypes = [P, i32p, C.c_int, C.c_int, C.c_int]
+    lib.orc_locus_likelihood.restype = C.c_double
     lib.orc_fag.restype = C.c_double
     lib.orc_msweep.argtypes = [P, i32p, i32p, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32]
     lib.orc_meiosis_list.argtypes = [P, i32p]
@@ -135,6 +138,30 @@ class Oracle:
         if rc:
             raise RuntimeError("zero likelihood")
         return d, nd.value
+
+    def impute(self, seed, chain, by_locus=False):
+        """One sequential-imputation replicate: (graph [M][2N], weight)."""
+        d = np.zeros((self.case.M, 2 * self.case.N), np.int32)
+        w = np.zeros(1)
+        rc = lib().orc_impute(self.ref, _p(d, i32p), seed, chain, 1 if by_locus else 0, _p(w, f64p))
+        if rc:
+            raise RuntimeError("zero likelihood")
+        return d, float(w[0])
+
+    def locus_likelihood(self, dg, locus, use_left=True, use_right=True) -> float:
+        d = np.ascontiguousarray(dg, np.int32)
+        return lib().orc_locus_likelihood(self.ref, _p(d, i32p), locus, int(use_left), int(use_right))
+
+    def impute_weight(self, dg, start) -> float:
+        """The weight of a replicate that started at `start` and ended at dg:
+        its locus likelihoods recomputed along the same walk."""
+        M = self.case.M
+        w = np.log(self.locus_likelihood(dg, start, False, False))
+        for l in range(start - 1, -1, -1):
+            w += np.log(self.locus_likelihood(dg, l, False, True))
+        for l in range(start + 1, M):
+            w += np.log(self.locus_likelihood(dg, l, True, False))
+        return float(w)
 
     def fag(self, dg, locus) -> float:
         d = np.ascontiguousarray(dg, np.int32)

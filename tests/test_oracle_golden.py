@@ -68,3 +68,38 @@ def test_chain(golden):
 def test_meiosis_list(golden):
     case, fx = golden
     assert np.array_equal(Oracle(case).meioses(), case.meioses())
+
+
+def test_impute(golden):
+    """Sequential-imputation replicates (LocusSampler::sequential_imputation)
+    and the replicate parallel_run keeps."""
+    case, fx = golden
+    o = Oracle(case)
+    seed, R = (int(x) for x in fx["impute_seed"])
+    w = []
+    for r in range(R):
+        g, wr = o.impute(seed, r)
+        assert np.array_equal(g, fx["impute_graphs"][r]), r
+        w.append(wr)
+    assert np.array_equal(np.array(w), fx["impute_weights"])
+    assert int(np.argmax(w)) == int(fx["impute_best"][0])
+
+
+def test_impute_by_locus(golden):
+    case, fx = golden
+    g, w = Oracle(case).impute(int(fx["impute0_seed"][0]), 0, by_locus=True)
+    assert np.array_equal(g, fx["impute0_graph"])
+    assert w == float(fx["impute0_weight"][0])
+
+
+def test_impute_weight_recomputed(golden):
+    """A replicate's weight from its final graph: the locus likelihoods along
+    its walk (each locus conditions only on the neighbour sampled before it,
+    which the later steps never rewrite)."""
+    from oracle_lib import uniform
+    case, fx = golden
+    o = Oracle(case)
+    seed, R = (int(x) for x in fx["impute_seed"])
+    for r in range(R):
+        start = min(int(uniform(seed, r, 0, 0xFFFFFFFC, 0) * case.M), case.M - 1)
+        assert o.impute_weight(fx["impute_graphs"][r], start) == fx["impute_weights"][r]
