@@ -22,6 +22,7 @@
 //   fag    <ped> <map> <dat> <outdir> <graphs> G  per-locus founder-allele-graph likelihood
 //   graphs <ped> <map> <dat> <outdir> seed G stride  random legal graphs (elimination + L-sweeps)
 //   impute <ped> <map> <dat> <outdir> seed R      sequential-imputation replicates (R = 0: locus by locus)
+//   writer <ped> <map> <dat> <outdir> <graphs> G  LinkageWriter output for one and two families
 //   bench  <ped> <map> <dat> <outdir> threads nL nM nG
 // Common options go before the sub-command as key=value: lods=5 peeliters=N
 // peelseed=S.
@@ -49,6 +50,7 @@
 #include "descent_graph.h"
 #include "locus_sampler2.h"
 #include "meiosis_sampler.h"
+#include "linkage_writer.h"
 #include "founder_allele_graph4.h"
 #include "elimination.h"
 #include "markov_chain.h"
@@ -437,6 +439,50 @@ static void cmd_impute(Case& c, const std::string& out, uint64_t seed, int R) {
     write_bin(out + "/impute_best.i32", &b, 4);
 }
 
+// LinkageWriter::write (linkage_writer.cc:14-91) for one family (all G
+// graphs scored into one LODscores) and for two families (graphs [0, G/2)
+// and [G/2, G) scored into two), with the LOD values each holds and the
+// marker names.
+static void cmd_writer(Case& c, const std::string& out, const std::string& gfile, int G) {
+    std::vector<char> gb = read_bin(gfile);
+    const size_t per = (size_t)2 * c.N * c.M;
+    if (gb.size() < per * G * 4 || G < 2) { fprintf(stderr, "graph file too small\n"); exit(2); }
+    const int32_t* gr = (const int32_t*)&gb[0];
+    LODscores all(c.m), fa(c.m), fb(c.m);
+    Peeler p_all(c.p, c.m, c.psg, &all, c.sex_linked);
+    Peeler p_a(c.p, c.m, c.psg, &fa, c.sex_linked);
+    Peeler p_b(c.p, c.m, c.psg, &fb, c.sex_linked);
+    const double tp = p_all.calc_trait_prob();
+    all.set_trait_prob(tp);
+    fa.set_trait_prob(tp);
+    fb.set_trait_prob(tp);
+    DescentGraph dg(c.p, c.m, c.sex_linked);
+    for (int g = 0; g < G; ++g) {
+        load_graph(dg, gr + per * g, c.N, c.M);
+        Peeler& half = g < G / 2 ? p_a : p_b;
+        for (int j = 0; j < c.M - 1; ++j) {
+            p_all.set_locus(j);
+            p_all.process(&dg);
+            half.set_locus(j);
+            half.process(&dg);
+        }
+    }
+    std::vector<LODscores*> one(1, &all), two;
+    two.push_back(&fa);
+    two.push_back(&fb);
+    LinkageWriter(c.m, out + "/linkage1.txt", false).write(one);
+    LinkageWriter(c.m, out + "/linkage2.txt", false).write(two);
+    std::vector<double> v;
+    LODscores* src[3] = {&all, &fa, &fb};
+    for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < c.M - 1; ++i)
+            for (int j = 0; j < c.nlod; ++j) v.push_back(src[k]->get(i, j));
+    write_bin(out + "/writer_lod.f64", &v[0], v.size() * 8);
+    FILE* f = fopen((out + "/marker_names.txt").c_str(), "w");
+    for (int i = 0; i < c.M; ++i) fprintf(f, "%s\n", c.m->get_name(i).c_str());
+    fclose(f);
+}
+
 // Non-ignorable meioses, in index order (markov_chain.cc:65-80).
 static std::vector<int> meiosis_list(Case& c) {
     std::vector<int> v;
@@ -716,6 +762,7 @@ int main(int argc, char** argv) {
         cmd_chain(c, out, rest[0], strtoull(rest[1], 0, 10), atoi(rest[2]), atoi(rest[3]), atoi(rest[4]),
                   atoi(rest[5]), atof(rest[6]));
     else if (cmd == "fag" && nrest >= 2) cmd_fag(c, out, rest[0], atoi(rest[1]));
+    else if (cmd == "writer" && nrest >= 2) cmd_writer(c, out, rest[0], atoi(rest[1]));
     else if (cmd == "impute" && nrest >= 2) cmd_impute(c, out, strtoull(rest[0], 0, 10), atoi(rest[1]));
     else if (cmd == "graphs" && nrest >= 3) cmd_graphs(c, out, strtoull(rest[0], 0, 10), atoi(rest[1]), atoi(rest[2]));
     else if (cmd == "bench" && nrest >= 5)

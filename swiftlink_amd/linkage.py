@@ -17,6 +17,9 @@ MCMC starts, so the GPU library can be driven without the reference:
   the reference's std::sort is not, so comparisons go by person id);
 * genotype elimination masks (elimination.cc:87-125, 393-395);
 * a peel sequence (swiftlink_amd.peelseq).
+
+And the output side: the LOD table the reference writes
+(LinkageWriter::write, linkage_writer.cc:14-91), `write_lod_table`.
 """
 from __future__ import annotations
 
@@ -53,6 +56,49 @@ def read_map(path: str) -> Tuple[List[str], np.ndarray]:
             names.append(t[2])
             pos.append(float(t[1]) / 100.0)
     return names, np.array(pos)
+
+
+def _fmt(x: float) -> str:
+    """A double as std::ostream writes it by default (6 significant
+    digits, %g rules)."""
+    if math.isnan(x):
+        return "nan" if math.copysign(1.0, x) > 0 else "-nan"
+    return format(x, "g")
+
+
+def format_lod_table(names, gpos, partial_theta, family_lods) -> str:
+    """The text of LinkageWriter::write (linkage_writer.cc:52-84).
+
+    names: M marker names; gpos: M genetic positions (Morgans);
+    partial_theta: M-1 partial thetas (genetic_map.cc:58-62);
+    family_lods: one array [M-1][nlod] of LOD scores per pedigree
+    (LODscores::get).  Each interval lists its nlod positions at
+    g_i + inverse_haldane(partial_theta_i (j+1)) cM with the LOD summed over
+    pedigrees in order, then each pedigree's own LOD when there is more than
+    one."""
+    fams = [np.asarray(f, np.float64) for f in family_lods]
+    M = len(names)
+    nlod = fams[0].shape[1]
+    out = ["marker\tposition\tlod\n"]
+    for i in range(M - 1):
+        out.append(f"{names[i]}\t{_fmt(100.0 * float(gpos[i]))}\n")
+        for j in range(nlod):
+            tmp = float(fams[0][i, j])
+            for f in fams[1:]:
+                tmp += float(f[i, j])
+            pos = float(gpos[i]) + _inverse_haldane(float(partial_theta[i]) * (j + 1))
+            line = f"-\t{_fmt(100 * pos)}\t{_fmt(tmp)}"
+            if len(fams) > 1:
+                line += "".join(f"\t{_fmt(float(f[i, j]))}" for f in fams)
+            out.append(line + "\n")
+    out.append(f"{names[M - 1]}\t{_fmt(100.0 * float(gpos[M - 1]))}\n")
+    return "".join(out)
+
+
+def write_lod_table(path: str, names, gpos, partial_theta, family_lods) -> None:
+    """Writes format_lod_table(...) to `path` (truncating it)."""
+    with open(path, "w") as f:
+        f.write(format_lod_table(names, gpos, partial_theta, family_lods))
 
 
 def read_dat(path: str):

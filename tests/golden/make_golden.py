@@ -17,6 +17,8 @@ Each fixture holds:
   impute_*    4 sequential-imputation replicates (LocusSampler::sequential_imputation)
               under (seed 55, chain r, 0, step 0xFFFFFFFC), their weights and the pick
   impute0_*   LocusSampler::locus_by_locus under (seed 56, 0, 0, step 0xFFFFFFFC)
+  writer_*    LinkageWriter's LOD table for the graphs as one family and split
+              into two, the LOD values it printed, marker names and positions
 """
 from __future__ import annotations
 
@@ -64,6 +66,7 @@ def make(name, base, work):
     run("lsweep", p, m, d, os.path.join(work, "ls"), gi, os.path.join(work, "lorder.i32"), 99, 0, 5)
     run("msweep", p, m, d, os.path.join(work, "ms"), gi, os.path.join(work, "morder.i32"), 99, 0, 6)
     run("chain", p, m, d, os.path.join(work, "ch"), gi, 77, 0, 40, 10, 3, 0.5)
+    run("writer", p, m, d, work, os.path.join(work, "graphs.i32"), G)
     run("impute", p, m, d, os.path.join(work, "si"), 55, 4)
     run("impute", p, m, d, os.path.join(work, "si0"), 56, 0)
     rd = lambda f, dt: np.fromfile(os.path.join(work, f), dt)
@@ -89,6 +92,11 @@ def make(name, base, work):
         impute0_seed=np.array([56]),
         impute0_graph=rd("si0/impute_graphs.i32", np.int32).reshape(M, 2 * N),
         impute0_weight=rd("si0/impute_weights.f64", np.float64),
+        writer_text1=np.array([open(os.path.join(work, "linkage1.txt")).read()]),
+        writer_text2=np.array([open(os.path.join(work, "linkage2.txt")).read()]),
+        writer_lod=rd("writer_lod.f64", np.float64).reshape(3, M - 1, case.nlod),
+        writer_names=np.array(open(os.path.join(work, "marker_names.txt")).read().split("\n")[:M]),
+        writer_gpos=rd("gpos.f64", np.float64),
     )
     path = os.path.join(OUT, name + ".npz")
     np.savez_compressed(path, **fx)

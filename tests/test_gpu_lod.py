@@ -115,3 +115,29 @@ def test_ped150_fp32_lod_within_tolerance(ped150):
         ctx.lod_accumulate(graphs)
         acc, count, lod = ctx.lod_results()
     assert np.abs(lod - lod_ref).max() < FP64_LOD_TOL
+
+
+def _table_fields(text):
+    return [line.split("\t") for line in text.splitlines()]
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_lod_table_from_device_scores(name):
+    """Two pedigrees' LOD scores from the device (one context each, the golden
+    graphs split in two) written as the reference's LinkageWriter table: the
+    same rows and positions, LODs within the FP64 tolerance."""
+    from swiftlink_amd import linkage
+    case, fx = load_golden(name)
+    G = len(fx["graphs"])
+    lods = []
+    for part in (fx["graphs"][:G // 2], fx["graphs"][G // 2:]):
+        with native.Context(case, native.SL_FP64) as ctx:
+            ctx.lod_accumulate(part)
+            lods.append(ctx.lod_results()[2].reshape(case.M - 1, case.nlod))
+    text = linkage.format_lod_table(list(fx["writer_names"]), fx["writer_gpos"], case.partial_theta, lods)
+    mine, ref = _table_fields(text), _table_fields(str(fx["writer_text2"][0]))
+    assert len(mine) == len(ref)
+    for a, b in zip(mine, ref):
+        assert len(a) == len(b) and a[:2] == b[:2], (a, b)
+        for x, y in zip(a[2:], b[2:]):
+            assert abs(float(x) - float(y)) <= 1e-5 * max(1.0, abs(float(y))), (a, b)

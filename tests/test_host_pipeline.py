@@ -58,3 +58,27 @@ def test_synthetic_configs_shapes():
         spec = synth.CONFIGS[name]
         ped = synth.make_pedigree(spec, np.random.default_rng(spec.seed))
         assert len(ped.ids) == N and max(ped.generation) + 1 == gens and ped.loops == loops
+
+
+def test_lod_table_text_matches_reference(golden):
+    """format_lod_table against LinkageWriter::write's file, for one family
+    and for two (per-family columns after the summed LOD)."""
+    case, fx = golden
+    names, gpos = list(fx["writer_names"]), fx["writer_gpos"]
+    lod = fx["writer_lod"]
+    assert linkage.format_lod_table(names, gpos, case.partial_theta, [lod[0]]) == str(fx["writer_text1"][0])
+    assert linkage.format_lod_table(names, gpos, case.partial_theta, [lod[1], lod[2]]) == str(fx["writer_text2"][0])
+
+
+def test_lod_table_file_and_formats(tmp_path):
+    """write_lod_table writes the same text; values print as std::ostream
+    does by default (6 significant digits, exponent form outside 1e-4..1e6,
+    -inf for an empty accumulator)."""
+    p = tmp_path / "out.txt"
+    lods = np.array([[1234567.0, 0.000012345, -np.inf]])
+    linkage.write_lod_table(str(p), ["a", "b"], np.array([0.0, 0.01]), np.array([0.002]), [lods])
+    lines = p.read_text().splitlines()
+    assert lines[0] == "marker\tposition\tlod"
+    assert lines[1] == "a\t0"
+    assert [l.split("\t")[2] for l in lines[2:5]] == ["1.23457e+06", "1.2345e-05", "-inf"]
+    assert lines[5] == "b\t1"
