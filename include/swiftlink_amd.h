@@ -208,6 +208,20 @@ int sl_mcmc_status(sl_ctx* ctx, int32_t* status);
 int sl_sequential_imputation(sl_ctx* ctx, uint64_t seed, int32_t iterations, int32_t n_chains, int32_t* best_graph,
                              double* weights, int32_t* graphs, int32_t* best_index);
 
+/* Expected LOD of a pedigree (Elod::run, elod.cc:19-84, option --elod):
+ * `sampler` is a context of the pedigree on the three-locus map marker (0) -
+ * trait (sep/2) - marker (sep) with no genotypes and the trait locus' sampler
+ * probabilities set to the disease probabilities (elod.h:35-88); `scorer` is
+ * a context of the same pedigree on the two-marker map (0, sep) with one
+ * trait position per interval.  Replicate j (0 <= j < replicates) runs
+ * LocusSampler::start_from(dg, 1) (loci 1, 0, 2; one L-sampler workgroup per
+ * replicate, up to SL_IMPUTE_MAX replicates per launch) drawing its uniforms
+ * in sequence at (draw, step=0xFFFFFFFC, sweep=0, chain=j), and its marker
+ * columns 0 and 2 are scored by the trait peel on the scorer (Peeler::process).
+ * The scorer's LOD accumulators are reset first and hold the result after;
+ * *elod (may be NULL) = its LOD at interval 0, position 0 (lod.get(0,0)). */
+int sl_elod(sl_ctx* sampler, sl_ctx* scorer, uint64_t seed, int64_t replicates, double* elod);
+
 /* One L-sweep on every chain: LocusSampler::step (locus_sampler2.cc:128-159)
  * at every locus, loci of parity first_parity ascending then the other parity
  * (non-adjacent loci run concurrently, as run_gpu_lsampler_kernel launched

@@ -714,31 +714,45 @@ int orc_lstep(const orc_case* c, int32_t* dg, int locus, uint64_t seed, uint32_t
 /* LocusSampler::sequential_imputation -> start_from (locus_sampler2.cc:209-241)
  * or, with by_locus, locus_by_locus (:183-207); one stream at
  * (seed, chain, 0, STEP_IMPUTE) drawn in sequence. */
-int orc_impute(const orc_case* c, int32_t* dg, uint64_t seed, uint32_t chain, int by_locus, double* weight) {
-    stream_t s = {seed, chain, 0, STEP_IMPUTE, 0};
+static int impute_walk(const orc_case* c, int32_t* dg, stream_t* s, int by_locus, int start, double* weight) {
     double wsum = 0.0, L = 0.0;
     int M = c->M;
     if (by_locus) {
         for (int l = 0; l < M; ++l) {
-            if (lstep_core(c, dg, l, 0, 0, &s, &L)) return -1;
+            if (lstep_core(c, dg, l, 0, 0, s, &L)) return -1;
             wsum += log(L);
         }
     } else {
-        int start = (int)(next_u(&s) * (double)M); /* get_random_locus (sampler.h:29-31) as floor(u M) over the stream */
-        if (start >= M) start = M - 1;
-        if (lstep_core(c, dg, start, 0, 0, &s, &L)) return -1;
+        if (start < 0) { /* get_random_locus (sampler.h:29-31) as floor(u M) over the stream */
+            start = (int)(next_u(s) * (double)M);
+            if (start >= M) start = M - 1;
+        }
+        if (lstep_core(c, dg, start, 0, 0, s, &L)) return -1;
         wsum += log(L);
         for (int l = start - 1; l >= 0; --l) {
-            if (lstep_core(c, dg, l, 0, 1, &s, &L)) return -1;
+            if (lstep_core(c, dg, l, 0, 1, s, &L)) return -1;
             wsum += log(L);
         }
         for (int l = start + 1; l < M; ++l) {
-            if (lstep_core(c, dg, l, 1, 0, &s, &L)) return -1;
+            if (lstep_core(c, dg, l, 1, 0, s, &L)) return -1;
             wsum += log(L);
         }
     }
     if (weight) *weight = wsum;
     return 0;
+}
+
+int orc_impute(const orc_case* c, int32_t* dg, uint64_t seed, uint32_t chain, int by_locus, double* weight) {
+    stream_t s = {seed, chain, 0, STEP_IMPUTE, 0};
+    return impute_walk(c, dg, &s, by_locus, -1, weight);
+}
+
+/* LocusSampler::start_from(dg, start) (locus_sampler2.cc:214-241), as ELOD
+ * runs it (elod.cc:58): no locus draw, the walk's uniforms from draw 0. */
+int orc_impute_from(const orc_case* c, int32_t* dg, uint64_t seed, uint32_t chain, int start, double* weight) {
+    stream_t s = {seed, chain, 0, STEP_IMPUTE, 0};
+    if (start < 0 || start >= c->M) return -1;
+    return impute_walk(c, dg, &s, 0, start, weight);
 }
 
 /* ---------------------------------------------------------------------------

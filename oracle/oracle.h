@@ -89,6 +89,10 @@ int orc_lstep(const orc_case* c, int32_t* dg, int locus, uint64_t seed, uint32_t
  * *weight = sum over loci of ln L.  Returns 0, or -1 on a zero likelihood. */
 int orc_impute(const orc_case* c, int32_t* dg, uint64_t seed, uint32_t chain, int by_locus, double* weight);
 
+/* The same walk from a given start locus with no locus draw (start_from as
+ * ELOD calls it, elod.cc:58). */
+int orc_impute_from(const orc_case* c, int32_t* dg, uint64_t seed, uint32_t chain, int start, double* weight);
+
 /* The L-sampler's forward-peel likelihood of `locus` given dg, conditioning
  * on column l-1 / l+1 only where use_left / use_right (LocusSampler::set_locus
  * + the forward loop of step, locus_sampler2.cc:132-135); dg is not changed. */

@@ -23,6 +23,8 @@
 //   graphs <ped> <map> <dat> <outdir> seed G stride  random legal graphs (elimination + L-sweeps)
 //   impute <ped> <map> <dat> <outdir> seed R      sequential-imputation replicates (R = 0: locus by locus)
 //   writer <ped> <map> <dat> <outdir> <graphs> G  LinkageWriter output for one and two families
+//   elod   <ped> <map> <dat> <outdir> freq p0 p1 p2 sep R seed keep
+//                                                 ELOD of the first pedigree of <ped> (map, dat unused)
 //   bench  <ped> <map> <dat> <outdir> threads nL nM nG
 // Common options go before the sub-command as key=value: lods=5 peeliters=N
 // peelseed=S.
@@ -51,6 +53,7 @@
 #include "locus_sampler2.h"
 #include "meiosis_sampler.h"
 #include "linkage_writer.h"
+#include "pedigree_parser.h"
 #include "founder_allele_graph4.h"
 #include "elimination.h"
 #include "markov_chain.h"
@@ -483,6 +486,114 @@ static void cmd_writer(Case& c, const std::string& out, const std::string& gfile
     fclose(f);
 }
 
+static void write_ops_json(FILE* f, PeelSequenceGenerator* psg) {
+    std::vector<PeelOperation>& ops = psg->get_peel_order();
+    fprintf(f, "[\n");
+    for (size_t o = 0; o < ops.size(); ++o) {
+        PeelOperation& op = ops[o];
+        fprintf(f, "  {\"type\": %d, \"peel\": %u, \"cutset\": ", (int)op.get_type(), op.get_peelnode());
+        json_int_list(f, op.get_cutset());
+        fprintf(f, ", \"prev\": ");
+        json_int_list(f, op.get_prevfunctions());
+        fprintf(f, ", \"children\": ");
+        json_int_list(f, op.get_children());
+        fprintf(f, "}%s\n", o + 1 < ops.size() ? "," : "");
+    }
+    fprintf(f, " ]");
+}
+
+// ELOD (Elod::run, elod.cc:19-84) of the first pedigree in `pedfile`: the
+// constructor's set-up (elod.h:35-88: the three-locus and two-locus maps, no
+// genotypes, disease probabilities copied into the trait locus) restated with
+// the reference's classes, then the replicate loop with replicate j's
+// uniforms at (seed, j, 0, STEP_IMPUTE).  Writes the ELOD, the accumulator,
+// P(T), the first `keep` replicates' graphs and both maps' tables.
+static void cmd_elod(const char* pedfile, const std::string& out, double freq, double p0, double p1, double p2,
+                     double sep, int reps, uint64_t seed, int keep, int peeliters, uint64_t peelseed) {
+    std::vector<double> pen(3);
+    pen[0] = p0;
+    pen[1] = p1;
+    pen[2] = p2;
+    DiseaseModel dm(freq, pen, false);
+    std::vector<Pedigree> peds;
+    GeneticMap map1(1), map2(1);
+    Snp s0("marker", 0.0);
+    s0.set_minor_freq(0.5);
+    Snp s1("trait", sep / 2);
+    s1.set_minor_freq(freq);
+    Snp s2("marker", sep);
+    s2.set_minor_freq(0.5);
+    map1.add(s0);
+    map1.add(s1);
+    map1.add(s2);
+    map1.add_theta(map1.haldane(sep / 2));
+    map1.add_theta(map1.haldane(sep / 2));
+    map2.add(s0);
+    map2.add(s2);
+    map2.add_theta(map2.haldane(sep));
+    if (!map1.sanity_check() || !map2.sanity_check()) exit(2);
+    PedigreeParser pp(pedfile, peds, dm, map1);
+    pp.set_ignore_genotypes();
+    if (!pp.parse()) exit(2);
+    Pedigree& P = peds[0];
+    const int N = (int)P.num_members(), F = (int)P.num_founders();
+    for (int i = 0; i < N; ++i) {
+        Person* q = P.get_by_index(i);
+        q->clear_genotypes();
+        q->add_genotype(UNTYPED);
+        q->add_genotype(UNTYPED);
+        q->add_genotype(UNTYPED);
+        q->populate_trait_prob_cache(map1);
+        q->copy_disease_probs(1);
+    }
+    srand((unsigned)peelseed);
+    stream_set(peelseed, 0, 0, STEP_SETUP);
+    PeelSequenceGenerator psg(&P, &map1, false, false);
+    psg.build_peel_sequence((unsigned)peeliters);
+    DescentGraph dg1(&P, &map1, false), dg2(&P, &map2, false);
+    LocusSampler ls(&P, &map1, &psg, 0, false);
+    LODscores lod(&map2);
+    Peeler peel(&P, &map2, &psg, &lod, false);
+    peel.calc_trait_prob();
+    peel.set_locus(0);
+    const double tp = peel.calc_trait_prob();
+    lod.set_trait_prob(tp);
+    std::vector<int32_t> kept;
+    for (int j = 0; j < reps; ++j) {
+        stream_set(seed, (uint32_t)j, 0, STEP_IMPUTE);
+        ls.start_from(dg1, 1);
+        dg2.copy_locus(dg1, 0, 0);
+        dg2.copy_locus(dg1, 2, 1);
+        peel.process(&dg2);
+        if (j < keep) kept.insert(kept.end(), dg1.get_internal_ptr(), dg1.get_internal_ptr() + 2 * N * 3);
+    }
+    if (!kept.empty()) write_bin(out + "/elod_graphs.i32", &kept[0], kept.size() * 4);
+    std::vector<int32_t> mother(N), father(N);
+    std::vector<double> dprob((size_t)N * 4), gprob((size_t)N * 3 * 4);
+    for (int i = 0; i < N; ++i) {
+        Person* q = P.get_by_index(i);
+        mother[i] = q->isfounder() ? -1 : (int)q->get_maternalid();
+        father[i] = q->isfounder() ? -1 : (int)q->get_paternalid();
+        for (int t = 0; t < 4; ++t) dprob[i * 4 + t] = q->get_disease_prob((enum phased_trait)t);
+        for (int l = 0; l < 3; ++l)
+            for (int t = 0; t < 4; ++t) gprob[((size_t)i * 3 + l) * 4 + t] = q->get_trait_probability(l, (enum phased_trait)t);
+    }
+    write_bin(out + "/elod_mother.i32", &mother[0], N * 4);
+    write_bin(out + "/elod_father.i32", &father[0], N * 4);
+    write_bin(out + "/elod_disease_prob.f64", &dprob[0], dprob.size() * 8);
+    write_bin(out + "/elod_geno_prob.f64", &gprob[0], gprob.size() * 8);
+    FILE* f = fopen((out + "/elod_meta.json").c_str(), "w");
+    fprintf(f, "{\"elod\": %.17g, \"acc\": %.17g, \"count\": %u, \"trait_prob\": %.17g, \"N\": %d, \"F\": %d,\n",
+            lod.get(0, 0), lod.get_raw(0), lod.get_count(), tp, N, F);
+    fprintf(f, " \"theta3\": [%.17g, %.17g], \"theta2\": %.17g, \"partial2\": %.17g, \"ids\": [", map1.get_theta(0),
+            map1.get_theta(1), map2.get_theta(0), map2.get_theta_partial(0, 1));
+    for (int i = 0; i < N; ++i) fprintf(f, i ? ",\"%s\"" : "\"%s\"", P.get_by_index(i)->get_id().c_str());
+    fprintf(f, "],\n \"ops\": ");
+    write_ops_json(f, &psg);
+    fprintf(f, "\n}\n");
+    fclose(f);
+}
+
 // Non-ignorable meioses, in index order (markov_chain.cc:65-80).
 static std::vector<int> meiosis_list(Case& c) {
     std::vector<int> v;
@@ -749,6 +860,12 @@ int main(int argc, char** argv) {
     int nrest = argc - a - 5;
     omp_set_num_threads(1);
 
+    if (cmd == "elod") {
+        if (nrest < 8) { fprintf(stderr, "elod needs freq p0 p1 p2 sep R seed keep\n"); return 2; }
+        cmd_elod(ped, out, atof(rest[0]), atof(rest[1]), atof(rest[2]), atof(rest[3]), atof(rest[4]), atoi(rest[5]),
+                 strtoull(rest[6], 0, 10), atoi(rest[7]), peeliters, peelseed);
+        return 0;
+    }
     Case c;
     build_case(c, ped, map, dat, lods, peeliters, peelseed);
 

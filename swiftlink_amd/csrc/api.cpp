@@ -160,6 +160,8 @@ struct sl_ctx {
     size_t imp_weight_cap = 0;
     int32_t* d_imp_status = nullptr;   // [R]
     size_t imp_status_cap = 0;
+    uint32_t* d_elod_pairs = nullptr;  // [R][2][W] marker columns of ELOD replicates
+    size_t elod_pairs_cap = 0;
     // profiling
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -294,7 +296,7 @@ int release_all(sl_ctx* c) {
                     c->d_partial, c->d_packed, c->d_stage, c->d_terms,
                     c->g_ops, c->g_prevs, c->g_children, c->g_tasks, c->g_tinfo, c->g_idx, c->g_level_base,
                     c->g_sched_off, c->g_sched_ops, c->d_gdp, c->d_jac, c->d_gterms, c->d_gstate, c->d_sweep,
-                    c->d_imp_graphs, c->d_imp_draw, c->d_imp_weight, c->d_imp_status};
+                    c->d_imp_graphs, c->d_imp_draw, c->d_imp_weight, c->d_imp_status, c->d_elod_pairs};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
@@ -917,21 +919,21 @@ static int run_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int first_parity
     return SL_OK;
 }
 
-int sl_sequential_imputation(sl_ctx* c, uint64_t seed, int32_t iterations, int32_t n_chains, int32_t* best_graph,
-                             double* weights, int32_t* graphs, int32_t* best_index) {
-    if (!c || iterations < 0 || iterations > SL_IMPUTE_MAX || n_chains < 0) return SL_E_INVALID;
-    int rc = need_sampler(c);
-    if (rc) return rc;
-    const int R = iterations > 0 ? iterations : 1;
+// Launches the locus walks of R sequential-imputation replicates (mode 1:
+// drawn start, 2: locus by locus, 3: from `start`) into d_imp_graphs;
+// replicate r draws as Philox chain chain_base + r.  Statuses and weights
+// stay on the device.
+static int run_impute(sl_ctx* c, uint64_t seed, int R, int mode, int start, uint32_t chain_base) {
     const size_t per = (size_t)c->M * c->W;
+    int rc;
     if ((rc = ensure(c, &c->d_imp_graphs, &c->imp_graphs_cap, (size_t)R * per)) ||
         (rc = ensure(c, &c->d_imp_draw, &c->imp_draw_cap, (size_t)R)) ||
         (rc = ensure(c, &c->d_imp_weight, &c->imp_weight_cap, (size_t)R)) ||
         (rc = ensure(c, &c->d_imp_status, &c->imp_status_cap, (size_t)R)))
         return rc;
     SL_HIP(c, hipMemsetAsync(c->d_imp_graphs, 0, (size_t)R * per * sizeof(uint32_t), c->stream));
-    // replicates draw their starting locus first (draw 0); locus by locus does not
-    SL_HIP(c, hipMemsetD32Async((hipDeviceptr_t)c->d_imp_draw, iterations > 0 ? 1 : 0, (size_t)R, c->stream));
+    // replicates with a drawn start take it as draw 0
+    SL_HIP(c, hipMemsetD32Async((hipDeviceptr_t)c->d_imp_draw, mode == 1 ? 1 : 0, (size_t)R, c->stream));
     SL_HIP(c, hipMemsetAsync(c->d_imp_weight, 0, (size_t)R * sizeof(double), c->stream));
     SL_HIP(c, hipMemsetAsync(c->d_imp_status, 0, (size_t)R * sizeof(int32_t), c->stream));
     sl::SampArgs A = samp_args(c);
@@ -939,7 +941,9 @@ int sl_sequential_imputation(sl_ctx* c, uint64_t seed, int32_t iterations, int32
     A.status = c->d_imp_status;
     A.seed = seed;
     A.sweep = 0;
-    A.impute = iterations > 0 ? 1 : 2;
+    A.impute = mode;
+    A.impute_start = start;
+    A.chain_base = chain_base;
     A.imp_draw = c->d_imp_draw;
     A.imp_weight = c->d_imp_weight;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -956,6 +960,23 @@ int sl_sequential_imputation(sl_ctx* c, uint64_t seed, int32_t iterations, int32
         SL_HIP(c, hipEventRecord(e1, c->stream));
         c->ev_used.push_back({SL_KERNEL_LSAMPLER, e0, e1, c->M});
     }
+    return SL_OK;
+}
+
+static int impute_failed(sl_ctx* c, int r, int st) {
+    return fail(c, st, "sequential imputation: replicate " + std::to_string(r) +
+                           (st == SL_E_ZERO_LIKELIHOOD ? " met a zero locus likelihood"
+                                                       : " exceeded the sampler table capacity"));
+}
+
+int sl_sequential_imputation(sl_ctx* c, uint64_t seed, int32_t iterations, int32_t n_chains, int32_t* best_graph,
+                             double* weights, int32_t* graphs, int32_t* best_index) {
+    if (!c || iterations < 0 || iterations > SL_IMPUTE_MAX || n_chains < 0) return SL_E_INVALID;
+    int rc = need_sampler(c);
+    if (rc) return rc;
+    const int R = iterations > 0 ? iterations : 1;
+    const size_t per = (size_t)c->M * c->W;
+    if ((rc = run_impute(c, seed, R, iterations > 0 ? 1 : 2, 0, 0))) return rc;
     std::vector<double> w(R);
     std::vector<int32_t> st(R);
     SL_HIP(c, hipMemcpyAsync(w.data(), c->d_imp_weight, (size_t)R * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -963,10 +984,7 @@ int sl_sequential_imputation(sl_ctx* c, uint64_t seed, int32_t iterations, int32
                              c->stream));
     SL_HIP(c, hipStreamSynchronize(c->stream));
     for (int r = 0; r < R; ++r)
-        if (st[r] != 0)
-            return fail(c, st[r], "sequential imputation: replicate " + std::to_string(r) +
-                                      (st[r] == SL_E_ZERO_LIKELIHOOD ? " met a zero locus likelihood"
-                                                                     : " exceeded the sampler table capacity"));
+        if (st[r] != 0) return impute_failed(c, r, st[r]);
     // the best replicate: largest weight, the first on ties (sequential_imputation.cc:92-101)
     int best = 0;
     for (int r = 1; r < R; ++r)
@@ -981,6 +999,54 @@ int sl_sequential_imputation(sl_ctx* c, uint64_t seed, int32_t iterations, int32
             SL_HIP(c, hipMemcpyAsync(c->d_chains + (size_t)ch * per, c->d_imp_graphs + (size_t)best * per,
                                      per * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
         SL_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return SL_OK;
+}
+
+int sl_elod(sl_ctx* sampler, sl_ctx* scorer, uint64_t seed, int64_t replicates, double* elod) {
+    if (!sampler || !scorer || replicates <= 0 || replicates > (int64_t)UINT32_MAX) return SL_E_INVALID;
+    int rc = need_sampler(sampler);
+    if (rc) return rc;
+    if (sampler->M != 3 || scorer->M != 2 || scorer->npos != 1 || sampler->N != scorer->N ||
+        sampler->F != scorer->F || sampler->device != scorer->device)
+        return fail(sampler, SL_E_INVALID,
+                    "ELOD needs a 3-locus sampler context and a 2-locus, 1-position scorer context of the same "
+                    "pedigree on one device");
+    const int W = sampler->W;
+    // failures on the scorer are reported on the sampler context too
+    auto scorer_failed = [&](int code) { return fail(sampler, code, "ELOD scorer: " + scorer->err); };
+    if ((rc = sl_lod_reset(scorer))) return scorer_failed(rc);
+    const int64_t chunk_max = SL_IMPUTE_MAX;
+    const size_t need = (size_t)std::min<int64_t>(replicates, chunk_max) * 2 * W;
+    if ((rc = ensure(sampler, &sampler->d_elod_pairs, &sampler->elod_pairs_cap, need))) return rc;
+    uint32_t* const pairs = sampler->d_elod_pairs;
+    std::vector<int32_t> st;
+    for (int64_t done = 0; done < replicates;) {
+        const int R = (int)std::min<int64_t>(replicates - done, chunk_max);
+        // start_from(dg1, 1) per replicate (elod.cc:58), replicate j drawing as chain j
+        if ((rc = run_impute(sampler, seed, R, 3, 1, (uint32_t)done))) return rc;
+        // the scorer still reads the previous chunk's pairs
+        SL_HIP(sampler, hipStreamSynchronize(scorer->stream));
+        // dg2 = loci 0 and 2 of dg1 (elod.cc:60-61)
+        const size_t w4 = (size_t)W * sizeof(uint32_t);
+        for (int k = 0; k < 2; ++k)
+            SL_HIP(sampler, hipMemcpy2DAsync(pairs + (size_t)k * W, 2 * w4, sampler->d_imp_graphs + (size_t)2 * k * W,
+                                             3 * w4, w4, (size_t)R, hipMemcpyDeviceToDevice, sampler->stream));
+        st.assign(R, 0);
+        SL_HIP(sampler, hipMemcpyAsync(st.data(), sampler->d_imp_status, (size_t)R * sizeof(int32_t),
+                                       hipMemcpyDeviceToHost, sampler->stream));
+        SL_HIP(sampler, hipStreamSynchronize(sampler->stream));
+        for (int r = 0; r < R; ++r)
+            if (st[r] != 0) return impute_failed(sampler, (int)(done + r), st[r]);
+        // Peeler::process on the two-marker map (elod.cc:63)
+        if ((rc = sl_lod_accumulate_packed(scorer, pairs, R))) return scorer_failed(rc);
+        done += R;
+    }
+    SL_HIP(sampler, hipStreamSynchronize(scorer->stream));
+    if (elod) {
+        double lod = 0.0;
+        if ((rc = sl_lod_results(scorer, nullptr, nullptr, &lod))) return scorer_failed(rc);
+        *elod = lod;
     }
     return SL_OK;
 }

@@ -118,14 +118,19 @@ __device__ int block_exclusive_scan(int* a, int n, int* tmp) {
 // get_random_locus, sampler.h:29-31) with neither neighbour, steps 1..s walk
 // left conditioning on l+1 only, steps s+1..M-1 walk right conditioning on
 // l-1 only.  Locus by locus: step t samples locus t with neither neighbour.
+// impute 3 walks the same way from the given locus A.impute_start
+// (start_from, as ELOD calls it at elod.cc:58) without drawing it.
 __device__ int impute_locus(const SampArgs& A, int chain, bool* use_left, bool* use_right) {
     const int t = A.impute_t, M = A.M;
     if (A.impute == 2) {
         *use_left = *use_right = false;
         return t;
     }
-    int s = (int)(philox_uniform(A.seed, (uint32_t)chain, A.sweep, kStepImpute, 0) * (double)M);
-    if (s >= M) s = M - 1;
+    int s = A.impute_start;
+    if (A.impute == 1) {
+        s = (int)(philox_uniform(A.seed, (uint32_t)chain + A.chain_base, A.sweep, kStepImpute, 0) * (double)M);
+        if (s >= M) s = M - 1;
+    }
     if (t == 0) {
         *use_left = *use_right = false;
         return s;
@@ -192,6 +197,7 @@ __global__ void lsampler_kernel(SampArgs A) {
     int locus;
     bool use_left = true, use_right = true;  // condition on column l-1 / l+1
     uint32_t step_id, draw0 = 0;
+    const uint32_t rng_chain = (uint32_t)chain + (A.impute ? A.chain_base : 0u);  // Philox chain of the draws
     if (A.impute) {
         if (A.status[chain] != 0) return;
         locus = impute_locus(A, chain, &use_left, &use_right);
@@ -255,7 +261,7 @@ __global__ void lsampler_kernel(SampArgs A) {
         }
     }
     for (int o = tid; o < n_ops; o += blockDim.x)
-        uop[o] = philox_uniform(A.seed, (uint32_t)chain, sweep, step_id, draw0 + (uint32_t)(n_ops - 1 - o));
+        uop[o] = philox_uniform(A.seed, rng_chain, sweep, step_id, draw0 + (uint32_t)(n_ops - 1 - o));
     for (int i = tid; i < 4 * N; i += blockDim.x) trt[i] = trait_at<T>(A, i >> 2, locus, i & 3);
     __syncthreads();
     const int total_cells = block_exclusive_scan(off, n_ops, scan_tmp);
@@ -504,7 +510,7 @@ __global__ void lsampler_kernel(SampArgs A) {
                 p0 *= (nb == 0u) ? 1.0 - thR : thR;
                 p1 *= (nb == 1u) ? 1.0 - thR : thR;
             }
-            const double u = philox_uniform(A.seed, (uint32_t)chain, sweep, step_id,
+            const double u = philox_uniform(A.seed, rng_chain, sweep, step_id,
                                             draw0 + (uint32_t)(n_ops + mcount[b]));
             bit = (u < (p0 / (p0 + p1))) ? 0u : 1u;
         }

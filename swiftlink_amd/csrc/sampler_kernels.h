@@ -35,10 +35,12 @@ struct SampArgs {
     int pass;                    //    the parity of pass 0/1 from the chain's schedule draws
     double lprob;                // L-sweep probability of the sweep schedule
     // sequential imputation (impute 1: start_from a drawn locus, 2: locus by
-    // locus): one workgroup per replicate (grid y), step impute_t of M; the
-    // replicate's uniforms run in sequence from imp_draw[chain] at step
-    // kStepImpute, and ln L of each locus is added to imp_weight[chain]
-    int impute, impute_t;
+    // locus, 3: start_from impute_start): one workgroup per replicate (grid
+    // y), step impute_t of M; replicate chain_base + y draws its uniforms in
+    // sequence from imp_draw[y] at step kStepImpute, and ln L of each locus
+    // is added to imp_weight[y]
+    int impute, impute_t, impute_start;
+    uint32_t chain_base;
     uint32_t* imp_draw;          // [chains] next draw index
     double* imp_weight;          // [chains] running sum of ln L
 };

@@ -95,6 +95,7 @@ _SIGS = {
     "sl_mcmc_status": (C.c_int, [C.c_void_p, i32p]),
     "sl_sequential_imputation": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_int32, i32p, f64p, i32p,
                                            C.POINTER(C.c_int32)]),
+    "sl_elod": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int64, f64p]),
     "sl_lsweep": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32]),
     "sl_msweep": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, i32p, C.c_int32]),
     "sl_mcmc_run": (C.c_int, [C.c_void_p, C.POINTER(Schedule), C.POINTER(McmcStats)]),
@@ -373,6 +374,14 @@ class Context:
         b = C.c_double(0)
         self._check(lib().sl_lod_work(self.h, C.byref(f), C.byref(b)), "sl_lod_work")
         return f.value, b.value
+
+
+def run_elod(sampler: "Context", scorer: "Context", seed: int, replicates: int) -> float:
+    """sl_elod: the ELOD of `replicates` replicates; the scorer keeps the
+    accumulators."""
+    out = C.c_double(0.0)
+    sampler._check(lib().sl_elod(sampler.h, scorer.h, seed, replicates, C.byref(out)), "sl_elod")
+    return out.value
 
 
 class Batch:

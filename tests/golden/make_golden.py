@@ -19,9 +19,14 @@ Each fixture holds:
   impute0_*   LocusSampler::locus_by_locus under (seed 56, 0, 0, step 0xFFFFFFFC)
   writer_*    LinkageWriter's LOD table for the graphs as one family and split
               into two, the LOD values it printed, marker names and positions
+  elod_*      Elod::run's set-up and replicate loop (400 replicates, default
+              model, replicate j under (seed 31, j, 0, step 0xFFFFFFFC)): the
+              ELOD, accumulator, P(T), the first 6 replicates' graphs, both
+              maps' tables and the peel sequence it built
 """
 from __future__ import annotations
 
+import dataclasses
 import json
 import os
 import subprocess
@@ -33,7 +38,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 
-from swiftlink_amd.case import Case  # noqa: E402
+from swiftlink_amd.case import Case, PeelOps  # noqa: E402
 from swiftlink_amd import synth  # noqa: E402
 
 REF = os.environ.get("REFSRC", "/root/reference/src")
@@ -61,12 +66,13 @@ def make(name, base, work):
     morder = np.random.default_rng(3).permutation(ml).astype(np.int32)
     morder.tofile(os.path.join(work, "morder.i32"))
     gi = os.path.join(work, "graph_init.i32")
-    for sub in ("ls", "ms", "ch", "si", "si0"):
+    for sub in ("ls", "ms", "ch", "si", "si0", "el"):
         os.makedirs(os.path.join(work, sub), exist_ok=True)
     run("lsweep", p, m, d, os.path.join(work, "ls"), gi, os.path.join(work, "lorder.i32"), 99, 0, 5)
     run("msweep", p, m, d, os.path.join(work, "ms"), gi, os.path.join(work, "morder.i32"), 99, 0, 6)
     run("chain", p, m, d, os.path.join(work, "ch"), gi, 77, 0, 40, 10, 3, 0.5)
     run("writer", p, m, d, work, os.path.join(work, "graphs.i32"), G)
+    run("elod", p, m, d, os.path.join(work, "el"), 0.0001, 0, 0, 1, 0.05, 400, 31, 6)
     run("impute", p, m, d, os.path.join(work, "si"), 55, 4)
     run("impute", p, m, d, os.path.join(work, "si0"), 56, 0)
     rd = lambda f, dt: np.fromfile(os.path.join(work, f), dt)
@@ -97,6 +103,21 @@ def make(name, base, work):
         writer_lod=rd("writer_lod.f64", np.float64).reshape(3, M - 1, case.nlod),
         writer_names=np.array(open(os.path.join(work, "marker_names.txt")).read().split("\n")[:M]),
         writer_gpos=rd("gpos.f64", np.float64),
+    )
+    em = json.load(open(os.path.join(work, "el", "elod_meta.json")))
+    eops = PeelOps.from_lists([o["type"] for o in em["ops"]], [o["peel"] for o in em["ops"]],
+                              [o["cutset"] for o in em["ops"]], [o["prev"] for o in em["ops"]],
+                              [o["children"] for o in em["ops"]])
+    fx.update(
+        elod_params=np.array([0.0001, 0.0, 0.0, 1.0, 0.05]), elod_run=np.array([400, 31]),
+        elod_value=np.array([em["elod"]]), elod_acc=np.array([em["acc"]]), elod_count=np.array([em["count"]]),
+        elod_trait_prob=np.array([em["trait_prob"]]),
+        elod_graphs=rd("el/elod_graphs.i32", np.int32).reshape(6, 3, 2 * N),
+        elod_disease_prob=rd("el/elod_disease_prob.f64", np.float64).reshape(N, 4),
+        elod_geno_prob=rd("el/elod_geno_prob.f64", np.float64).reshape(N, 3, 4),
+        elod_theta3=np.array(em["theta3"]), elod_theta2=np.array([em["theta2"]]),
+        elod_partial2=np.array([em["partial2"]]), elod_ids=np.array(em["ids"]),
+        **{"elod_op_" + f.name: getattr(eops, f.name) for f in dataclasses.fields(PeelOps)},
     )
     path = os.path.join(OUT, name + ".npz")
     np.savez_compressed(path, **fx)

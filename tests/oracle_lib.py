@@ -48,6 +48,7 @@ def _load():
     lib.orc_lod_accumulate.argtypes = [P, i32p, C.c_int, f64p]
     lib.orc_lstep.argtypes = [P, i32p, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(C.c_int)]
     lib.orc_impute.argtypes = [P, i32p, C.c_uint64, C.c_uint32, C.c_int, f64p]
+    lib.orc_impute_from.argtypes = [P, i32p, C.c_uint64, C.c_uint32, C.c_int, f64p]
     lib.orc_fag.argtypes = [P, i32p, C.c_int]
     lib.orc_locus_likelihood.argtypes = [P, i32p, C.c_int, C.c_int, C.c_int]
     lib.orc_locus_likelihood.restype = C.c_double
@@ -145,6 +146,13 @@ class Oracle:
         w = np.zeros(1)
         rc = lib().orc_impute(self.ref, _p(d, i32p), seed, chain, 1 if by_locus else 0, _p(w, f64p))
         if rc:
+            raise RuntimeError("zero likelihood")
+        return d, float(w[0])
+
+    def impute_from(self, seed, chain, start):
+        d = np.zeros((self.case.M, 2 * self.case.N), np.int32)
+        w = np.zeros(1)
+        if lib().orc_impute_from(self.ref, _p(d, i32p), seed, chain, start, _p(w, f64p)):
             raise RuntimeError("zero likelihood")
         return d, float(w[0])
 

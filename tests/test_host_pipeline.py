@@ -82,3 +82,38 @@ def test_lod_table_file_and_formats(tmp_path):
     assert lines[1] == "a\t0"
     assert [l.split("\t")[2] for l in lines[2:5]] == ["1.23457e+06", "1.2345e-05", "-inf"]
     assert lines[5] == "b\t1"
+
+
+def test_elod_cases_match_reference(golden):
+    """Both ELOD maps as Elod's constructor builds them (elod.h:35-88):
+    disease probabilities under the ELOD model, the trait locus' sampler
+    probabilities, thetas and the partial theta, the same peel sequence."""
+    from swiftlink_amd import elod
+    case, fx = golden
+    freq, p0, p1, p2, sep = (float(x) for x in fx["elod_params"])
+    c3, c2 = elod.elod_cases(case, freq, (p0, p1, p2), sep)
+    assert np.array_equal(c3.disease_prob, fx["elod_disease_prob"])
+    assert np.array_equal(c3.geno_prob, fx["elod_geno_prob"])
+    assert np.array_equal(c3.theta, fx["elod_theta3"])
+    assert np.array_equal(c2.theta, fx["elod_theta2"])
+    assert np.array_equal(c2.partial_theta, fx["elod_partial2"])
+    assert np.array_equal(case.ops.type, fx["elod_op_type"]) and np.array_equal(case.ops.cut, fx["elod_op_cut"])
+
+
+def test_elod_oracle_matches_reference(golden):
+    """The oracle's restatement of the replicate loop: the sampled three-locus
+    graphs, P(T) on the two-marker map, the accumulator and the ELOD."""
+    from swiftlink_amd import elod
+    case, fx = golden
+    freq, p0, p1, p2, sep = (float(x) for x in fx["elod_params"])
+    R, seed = (int(x) for x in fx["elod_run"])
+    c3, c2 = elod.elod_cases(case, freq, (p0, p1, p2), sep)
+    o3, o2 = Oracle(c3), Oracle(c2)
+    graphs = np.stack([o3.impute_from(seed, j, 1)[0] for j in range(R)])
+    assert np.array_equal(graphs[:len(fx["elod_graphs"])], fx["elod_graphs"])
+    tp = o2.trait_prob()
+    assert tp == float(fx["elod_trait_prob"][0])
+    acc = o2.lod_accumulate(graphs[:, [0, 2]])
+    assert acc[0] == float(fx["elod_acc"][0])
+    value = (acc[0] - np.log(R) - tp) / np.log(10.0)
+    assert abs(value - float(fx["elod_value"][0])) <= 1e-12 * max(1.0, abs(value))
