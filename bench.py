@@ -19,7 +19,10 @@ host's cores, or the oracle restatement when that binary is absent.  The
 respect to penetrances, disease allele frequency and flanking thetas at every
 position) over 64 resident graphs of ped200_snp2000 (200 people, 7
 generations, 4 loops, 2000 SNPs), beside the oracle's forward-mode gradient
-on one core.
+on one core.  `sequential_imputation` times 1000 replicates of the chains'
+starting-graph search on the bench pedigree, and `elod` 262140 ELOD
+replicates of it under the default ELOD model, each beside the reference's
+own code on one core.
 """
 from __future__ import annotations
 
@@ -42,6 +45,8 @@ FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 LPROB, PERIOD, SEED = 0.5, 10, 20250101
 ADJ_CONFIG = "ped200_snp2000"  # the differentiation workload: 200 people, 7 generations, 4 loops, 2000 SNPs
+SI_REPS = 1000          # sequential-imputation replicates (the reference's -s default)
+ELOD_REPS = 4 * 65535   # ELOD replicates (four launches' worth)
 
 
 def profile_traffic(kernel: str, workload: str):
@@ -90,6 +95,31 @@ def cpu_baseline(tmpdir: str, base: str, case, threads: int):
     return {"value": sweeps / dt, "unit": "sweeps/s", "cores": 1, "kind": "port",
             "sample": f"{sweeps} sweeps of the oracle chain (same schedule, scored every {PERIOD}th), FP64, "
                       f"1 core, {dt:.1f} s"}
+
+
+def cpu_impute_elod_baseline(tmpdir: str, base: str):
+    """The reference's LocusSampler::sequential_imputation (20 replicates)
+    and Elod::run's replicate loop (3000 replicates, default model) on one
+    core, timed inside the harness (set-up excluded)."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "refharness")
+    if not os.path.exists(harness):
+        return None, None
+    p, m, d = base + ".ped", base + ".map", base + ".dat"
+    si_out, el_out = os.path.join(tmpdir, "cpu_si"), os.path.join(tmpdir, "cpu_elod")
+    reps, ereps = 20, 3000
+    subprocess.run([harness, "impute", p, m, d, si_out, str(SEED), str(reps)], check=True,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    subprocess.run([harness, "elod", p, m, d, el_out, "0.0001", "0", "0", "1", "0.05", str(ereps), str(SEED), "0"],
+                   check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    si = json.load(open(os.path.join(si_out, "impute_meta.json")))
+    el = json.load(open(os.path.join(el_out, "elod_meta.json")))
+    return ({"value": si["replicates"] * si["loci"] / si["seconds"], "unit": "replicate-loci/s", "cores": 1,
+             "kind": "reference",
+             "sample": f"{reps} replicates of the reference's sequential_imputation, FP64, 1 core "
+                       f"({si['seconds']:.1f} s; parallel_run spreads replicates over OpenMP threads)"},
+            {"value": ereps / el["seconds"], "unit": "replicates/s", "cores": 1, "kind": "reference",
+             "sample": f"{ereps} replicates of Elod::run's loop (start_from + Peeler::process), FP64, 1 core "
+                       f"({el['seconds']:.1f} s)"})
 
 
 MOD_MODEL = ((0.05, 0.3, 0.85), 0.12)  # trait model of the adjoint workload
@@ -258,6 +288,23 @@ def main():
             adt = time.perf_counter() - t0
         apos = (acase.M - 1) * acase.nlod
 
+    # ---- sequential imputation (the chains' starting graph, 1000 replicates
+    # as the reference's -s default) and ELOD (replicates of start_from at the
+    # trait locus, scored on the two-marker map) on the bench pedigree ----
+    if extras:
+        from swiftlink_amd import elod as elodmod
+        with native.Context(case, native.SL_FP32_RESCALED) as ictx:
+            ictx.sequential_imputation(SEED, 16)
+            t0 = time.perf_counter()
+            _, iw, ibest, _ = ictx.sequential_imputation(SEED, SI_REPS)
+            idt = time.perf_counter() - t0
+        e3, e2 = elodmod.elod_cases(case)
+        with native.Context(e3, native.SL_FP32_RESCALED) as es, native.Context(e2, native.SL_FP64) as esc:
+            native.run_elod(es, esc, SEED, 2048)
+            t0 = time.perf_counter()
+            elod_value = native.run_elod(es, esc, SEED, ELOD_REPS)
+            edt = time.perf_counter() - t0
+
     # ---- roofline of the meiosis-step kernel (one chain: mspec_kernel, the
     # forward-backward pass of step k beside the founder-allele likelihoods of
     # step k+1 for both values of meiosis k's bit): algorithmic bytes per
@@ -331,6 +378,15 @@ def main():
             "partials_per_eval": 6, "dtype": "f64", "max_lod": float(alod.max()),
             "note": "sl_lod_grad: ln L and d ln L/d(3 penetrances, frequency, theta_L, theta_R) per (graph, position), "
                     "reduced on the device to the LOD and its gradient at every position"}
+    if extras:
+        res["sequential_imputation"] = {
+            "workload": f"{args.config}: {SI_REPS} replicates x {case.M} loci (LocusSampler::sequential_imputation), "
+                        f"FP32-rescaled peel, FP64 weights",
+            "seconds": idt, "replicate_loci_per_s": SI_REPS * case.M / idt, "best_weight": float(iw[ibest])}
+        res["elod"] = {
+            "workload": f"{args.config} pedigree, default ELOD model (frequency 1e-4, penetrance 0:0:1, markers "
+                        f"0.05 M apart), {ELOD_REPS} replicates; FP32-rescaled sampler, FP64 scoring",
+            "seconds": edt, "replicates_per_s": ELOD_REPS / edt, "elod": elod_value}
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(tmp, os.path.join(tmp, args.config), case, threads)
@@ -342,6 +398,11 @@ def main():
                        for i in range(acase.N)])
         res["adjoint"]["cpu_baseline"] = cpu_adjoint_baseline(
             dataclasses.replace(acase, disease_prob=dp.reshape(acase.disease_prob.shape)), ag0, ajac, 15000)
+    if not args.no_cpu and extras and "elod" in res:
+        si_cpu, elod_cpu = cpu_impute_elod_baseline(tmp, os.path.join(tmp, args.config))
+        if si_cpu:
+            res["sequential_imputation"]["cpu_baseline"] = si_cpu
+            res["elod"]["cpu_baseline"] = elod_cpu
     print(json.dumps(res))
 
 

@@ -429,6 +429,7 @@ static void cmd_impute(Case& c, const std::string& out, uint64_t seed, int R) {
     LocusSampler ls(c.p, c.m, c.psg, 0, c.sex_linked);
     int best = -1;
     double best_w = -INFINITY;
+    const double t0 = wall();
     for (int r = 0; r < reps; ++r) {
         DescentGraph dg(c.p, c.m, c.sex_linked);
         stream_set(seed, (uint32_t)r, 0, STEP_IMPUTE);
@@ -438,8 +439,12 @@ static void cmd_impute(Case& c, const std::string& out, uint64_t seed, int R) {
     }
     write_bin(out + "/impute_graphs.i32", &all[0], all.size() * 4);
     write_bin(out + "/impute_weights.f64", &w[0], w.size() * 8);
+    const double secs = wall() - t0;
     int32_t b = best;
     write_bin(out + "/impute_best.i32", &b, 4);
+    FILE* f = fopen((out + "/impute_meta.json").c_str(), "w");
+    fprintf(f, "{\"replicates\": %d, \"loci\": %d, \"seconds\": %.6f}\n", reps, c.M, secs);
+    fclose(f);
 }
 
 // LinkageWriter::write (linkage_writer.cc:14-91) for one family (all G
@@ -559,6 +564,7 @@ static void cmd_elod(const char* pedfile, const std::string& out, double freq, d
     const double tp = peel.calc_trait_prob();
     lod.set_trait_prob(tp);
     std::vector<int32_t> kept;
+    const double t0 = wall();
     for (int j = 0; j < reps; ++j) {
         stream_set(seed, (uint32_t)j, 0, STEP_IMPUTE);
         ls.start_from(dg1, 1);
@@ -567,6 +573,7 @@ static void cmd_elod(const char* pedfile, const std::string& out, double freq, d
         peel.process(&dg2);
         if (j < keep) kept.insert(kept.end(), dg1.get_internal_ptr(), dg1.get_internal_ptr() + 2 * N * 3);
     }
+    const double secs = wall() - t0;
     if (!kept.empty()) write_bin(out + "/elod_graphs.i32", &kept[0], kept.size() * 4);
     std::vector<int32_t> mother(N), father(N);
     std::vector<double> dprob((size_t)N * 4), gprob((size_t)N * 3 * 4);
@@ -585,6 +592,7 @@ static void cmd_elod(const char* pedfile, const std::string& out, double freq, d
     FILE* f = fopen((out + "/elod_meta.json").c_str(), "w");
     fprintf(f, "{\"elod\": %.17g, \"acc\": %.17g, \"count\": %u, \"trait_prob\": %.17g, \"N\": %d, \"F\": %d,\n",
             lod.get(0, 0), lod.get_raw(0), lod.get_count(), tp, N, F);
+    fprintf(f, " \"seconds\": %.6f,\n", secs);
     fprintf(f, " \"theta3\": [%.17g, %.17g], \"theta2\": %.17g, \"partial2\": %.17g, \"ids\": [", map1.get_theta(0),
             map1.get_theta(1), map2.get_theta(0), map2.get_theta_partial(0, 1));
     for (int i = 0; i < N; ++i) fprintf(f, i ? ",\"%s\"" : "\"%s\"", P.get_by_index(i)->get_id().c_str());
