@@ -105,7 +105,7 @@ struct sl_ctx {
     sl_batch* batch = nullptr;
     // sampler program and chain state
     sl::SampProgram sprog;
-    int samp_block = 256;  // L-sampler workgroup: 4 wavefronts share each peel level
+    int samp_block = sl::kLsamplerBlock;
     int cells_cap = 0;
     size_t samp_lds = 0;
     sl::SampOp* d_sops = nullptr;

@@ -87,7 +87,7 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 // exclusive prefix sum of a[0..n) in place (LDS), whole workgroup; returns total
-__device__ int block_exclusive_scan(int* a, int n, int* tmp) {
+__device__ __forceinline__ int block_exclusive_scan(int* a, int n, int* tmp) {
     const int T = blockDim.x, t = threadIdx.x;
     const int per = (n + T - 1) / T;
     const int b0 = min(n, t * per), b1 = min(n, b0 + per);
@@ -112,23 +112,23 @@ __device__ int block_exclusive_scan(int* a, int n, int* tmp) {
     return total;
 }
 
-// Locus of step A.impute_t of one sequential-imputation replicate and the
-// neighbours it conditions on (LocusSampler::start_from and locus_by_locus,
+// Locus of step t of one sequential-imputation replicate (mode: the impute
+// field of SampArgs) and the neighbours it conditions on (LocusSampler::start_from and locus_by_locus,
 // locus_sampler2.cc:183-241): step 0 samples the starting locus s (draw 0,
 // get_random_locus, sampler.h:29-31) with neither neighbour, steps 1..s walk
 // left conditioning on l+1 only, steps s+1..M-1 walk right conditioning on
 // l-1 only.  Locus by locus: step t samples locus t with neither neighbour.
-// impute 3 walks the same way from the given locus A.impute_start
-// (start_from, as ELOD calls it at elod.cc:58) without drawing it.
-__device__ int impute_locus(const SampArgs& A, int chain, bool* use_left, bool* use_right) {
-    const int t = A.impute_t, M = A.M;
-    if (A.impute == 2) {
+// Mode 3 walks the same way from the given locus `start` (start_from, as
+// ELOD calls it at elod.cc:58) without drawing it.
+__device__ __forceinline__ int impute_locus(int mode, int t, int M, int start, uint64_t seed, uint32_t rng_chain,
+                                            uint32_t sweep, bool* use_left, bool* use_right) {
+    if (mode == 2) {
         *use_left = *use_right = false;
         return t;
     }
-    int s = A.impute_start;
-    if (A.impute == 1) {
-        s = (int)(philox_uniform(A.seed, (uint32_t)chain + A.chain_base, A.sweep, kStepImpute, 0) * (double)M);
+    int s = start;
+    if (mode == 1) {
+        s = (int)(philox_uniform(seed, rng_chain, sweep, kStepImpute, 0) * (double)M);
         if (s >= M) s = M - 1;
     }
     if (t == 0) {
@@ -154,6 +154,16 @@ __device__ __forceinline__ float trait_at<float>(const SampArgs& A, int person, 
     return ((cflt_p)A.geno_prob_f)[((size_t)person * A.M + locus) * 4 + v];
 }
 
+// probability that a parent of phased trait t transmits allele a (1 = A),
+// from that parent's half of a child's distribution (4 entries: AU then UA,
+// each P(U), P(A)) (get_recombination_distribution, sampler_rfunction.cc:102-157)
+template <typename T>
+__device__ __forceinline__ T parent_factor(const T* half, uint32_t t, uint32_t a) {
+    if (t == PT_UU) return a ? (T)0 : (T)1;
+    if (t == PT_AA) return a ? (T)1 : (T)0;
+    return half[(t == PT_AU ? 0 : 2) + a];
+}
+
 // transmission probability of kid trait `kid` from parent traits (m, p)
 // (transmission_matrix, sampler_rfunction.cc:350-419): mat_dist x pat_dist
 template <typename T>
@@ -161,18 +171,239 @@ __device__ __forceinline__ T trans(const T* dist, uint32_t m, uint32_t p, uint32
     // allele of the kid from each parent: maternal A for AA/AU, paternal A for AA/UA
     const uint32_t am = (kid == PT_AA || kid == PT_AU) ? 1u : 0u;
     const uint32_t ap = (kid == PT_AA || kid == PT_UA) ? 1u : 0u;
-    T md, pd;
-    if (m == PT_UU) md = am ? (T)0 : (T)1;
-    else if (m == PT_AA) md = am ? (T)1 : (T)0;
-    else md = dist[(m == PT_AU ? 0 : 2) + am];
-    if (p == PT_UU) pd = ap ? (T)0 : (T)1;
-    else if (p == PT_AA) pd = ap ? (T)1 : (T)0;
-    else pd = dist[4 + (p == PT_AU ? 0 : 2) + ap];
-    return md * pd;
+    return parent_factor<T>(dist, m, am) * parent_factor<T>(dist + 4, p, ap);
+}
+
+// What one op's evaluation reads and writes (LDS arrays of the workgroup).
+template <typename T>
+struct OpEnv {
+    const SampPrev* prevs;
+    const ChildDesc* children;
+    int impute;
+    int op;
+    cint_p d;               // the op's SampOp words
+    const uint8_t* mask;    // [N] legal trait values at the locus
+    const int* off;         // [n_ops+1] table offsets
+    uint32_t* maxbits;      // [n_ops] FP32 bits of each table's maximum (RESCALE)
+    const T* trt;           // [N][4] trait probabilities at the locus
+    const T* dist;          // [NF][8] transmission distributions
+    T* tab;
+    T* pre;
+    int lane, F;
+    int* scale_exp;         // sum of log2 of the scale factors consumed (sequential imputation)
+};
+
+// One op of the forward peel, evaluated by one wavefront: every compact cell
+// of its cutset at this locus, for each legal value v of the peel node, the
+// presum pre[cell][v] and the table entry tab[cell] = sum_v pre[cell][v]
+// (SamplerRfunction::evaluate_{child,parent}_peel, sampler_rfunction.cc:190-287;
+// Rfunction::evaluate_partner_peel, rfunction.cc:116-142).  NP is the number
+// of consumed tables.  Products keep the reference's order: trait
+// probability, CHILD transmission, consumed tables in order, PARENT children.
+//
+// A consumed table is addressed in its own compact mixed radix, whose digits
+// are digits of this op's cutset (or its peel node).  The cell decode yields
+// each digit's rank within its legal values, so a consumed index is
+// sum_d rank_d * S[q][d] over this op's cutset digits, formed once per cell,
+// plus rank(v) * S[q][k] for the peel node.
+// Index into consumed table q of op descriptor words `pd` for the cell whose
+// digit ranks are packed 2 bits per digit in rk (digit k: rank rv of v).
+__device__ __forceinline__ int consumed_index(cint_p pd, uint32_t cmw, uint32_t rk) {
+    const int nd = pd[1];
+    int idx = 0, st = 1;
+    for (int j = 0; j < nd; ++j) {
+        const int src = pd[2 + j];
+        idx += (int)((rk >> (2 * src)) & 3u) * st;
+        st *= __popc((cmw >> (4 * src)) & 15u);
+    }
+    return idx;
+}
+
+template <typename T, bool RESCALE, int NP_>
+__device__ __forceinline__ void sampler_op(const OpEnv<T>& E) {
+    // NP_ >= 0: that many consumed tables with their strides in registers;
+    // NP_ < 0: any number, indices recomputed from the descriptors
+    constexpr int NP = NP_ < 0 ? 0 : NP_;
+    cint_p d = E.d;
+    const int type = d[0], k = d[1], peel = d[2], n_child = d[4], child_off = d[5];
+    const int prev_off = d[6], mat_digit = d[7], pat_digit = d[8];
+    const int F = E.F;
+    // per-digit legal masks packed 4 bits per digit; digit k is the peel node
+    uint32_t cmw = 0;
+    int size = 1;
+#pragma unroll
+    for (int q = 0; q < kMaxCut; ++q) {
+        if (q < k) {
+            const uint32_t m = (uint32_t)uni(E.mask[d[9 + q]]);
+            cmw |= m << (4 * q);
+            size *= __popc(m);
+        }
+    }
+    const uint32_t pmask = (uint32_t)uni(E.mask[peel]);
+    cmw |= pmask << (4 * k);
+    T tr[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) tr[v] = E.trt[4 * peel + v];
+    const int my_off = uni(E.off[E.op]);
+    // consumed tables: offset, scale, and the stride of each of this op's
+    // digits in the table's radix (0 where the digit is not in the table)
+    int p_off[NP > 0 ? NP : 1], S[NP > 0 ? NP : 1][kMaxCut + 1];
+    T p_scale[NP > 0 ? NP : 1];
+    int scale_exp = 0;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        cint_p pd = (cint_p)(E.prevs + prev_off + q);
+        const int pop = pd[0], nd = pd[1];
+        p_off[q] = uni(E.off[pop]);
+        p_scale[q] = (T)1;
+        if (RESCALE) {
+            const uint32_t mb = E.maxbits[pop];
+            p_scale[q] = (T)scale_from_bits(mb);
+            const int ef = (int)((mb >> 23) & 0xFFu);
+            scale_exp += ef ? 126 - ef : 0;
+        }
+#pragma unroll
+        for (int dd = 0; dd <= kMaxCut; ++dd) S[q][dd] = 0;
+        int st = 1;
+#pragma unroll
+        for (int j = 0; j < kMaxCut; ++j) {
+            if (j < nd) {
+                const int src = pd[2 + j];
+#pragma unroll
+                for (int dd = 0; dd <= kMaxCut; ++dd) S[q][dd] = src == dd ? st : S[q][dd];
+                st *= __popc((cmw >> (4 * src)) & 15u);
+            }
+        }
+    }
+    int PS[NP > 0 ? NP : 1];  // stride of the peel node's digit
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        PS[q] = 0;
+#pragma unroll
+        for (int dd = 0; dd <= kMaxCut; ++dd) PS[q] = dd == k ? S[q][dd] : PS[q];
+    }
+    // PARENT children: distribution offset, and kid digit | other-parent
+    // digit << 4 | (peel node is the mother) << 8
+    int kdist[7], kinfo[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) kdist[c] = kinfo[c] = 0;
+    if (type == kParent) {  // child descriptors only where the op has children
+#pragma unroll
+        for (int c = 0; c < 7; ++c) {
+            if (c < n_child) {
+                cint_p cd = (cint_p)(E.children + child_off + c);
+                kdist[c] = 8 * (cd[0] - F);
+                kinfo[c] = cd[1] | (cd[2] << 4) | ((cd[3] ? 1 : 0) << 8);
+            }
+        }
+    }
+    if (NP_ < 0 && RESCALE) {
+        for (int q = 0; q < d[3]; ++q) {
+            const int ef = (int)((E.maxbits[((cint_p)(E.prevs + prev_off + q))[0]] >> 23) & 0xFFu);
+            scale_exp += ef ? 126 - ef : 0;
+        }
+    }
+    if (RESCALE && E.impute && E.lane == 0 && scale_exp != 0) atomicAdd(E.scale_exp, scale_exp);
+    const T* dist_peel = E.dist + 8 * (peel - F);
+    float mx = 0.0f;
+    for (int c = E.lane; c < size; c += 64) {
+        // mixed-radix decode of the compact cell into trait values (2 bits
+        // per digit) and the consumed tables' base indices
+        uint32_t ext = 0, rk = 0;
+        int base[NP > 0 ? NP : 1];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) base[q] = 0;
+        int rem = c;
+#pragma unroll
+        for (int q = 0; q < kMaxCut; ++q) {
+            if (q < k) {
+                const uint32_t m = (cmw >> (4 * q)) & 15u;
+                const int n = __popc(m);
+                const int quo = (int)(((float)rem + 0.5f) / (float)n);
+                const int r = rem - quo * n;
+                rem = quo;
+                ext |= select_bit(m, (uint32_t)r) << (2 * q);
+                rk |= (uint32_t)r << (2 * q);
+#pragma unroll
+                for (int p = 0; p < NP; ++p) base[p] += r * S[p][q];
+            }
+        }
+        // v-independent transmission parts: CHILD parents' traits; PARENT
+        // each child's trait and the factor of its other parent
+        const uint32_t mt_c = (ext >> (2 * mat_digit)) & 3u, pt_c = (ext >> (2 * pat_digit)) & 3u;
+        uint32_t kid_t[7];
+        T fixed[7];
+        if (type == kParent) {
+#pragma unroll
+            for (int ch = 0; ch < 7; ++ch) {
+                kid_t[ch] = 0;
+                fixed[ch] = (T)0;
+                if (ch < n_child) {
+                    const uint32_t kid = (ext >> (2 * (kinfo[ch] & 15))) & 3u;
+                    const uint32_t other = (ext >> (2 * ((kinfo[ch] >> 4) & 15))) & 3u;
+                    const T* dk = E.dist + kdist[ch];
+                    kid_t[ch] = kid;
+                    // other parent: father when the peel node is the mother
+                    fixed[ch] = (kinfo[ch] >> 8) ? parent_factor<T>(dk + 4, other, (kid == PT_AA || kid == PT_UA) ? 1u : 0u)
+                                                 : parent_factor<T>(dk, other, (kid == PT_AA || kid == PT_AU) ? 1u : 0u);
+                }
+            }
+        }
+        T total = (T)0;
+        T* pc = E.pre + 4 * (size_t)(my_off + c);
+        int rv = 0;  // rank of v among the legal values
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            T tmp = tr[v];
+            const bool legal = ((pmask >> v) & 1u) != 0u;
+            if (!legal || tmp == (T)0) {
+                pc[v] = (T)0;
+                rv += legal ? 1 : 0;
+                continue;
+            }
+            if (type == kChild) tmp *= trans<T>(dist_peel, mt_c, pt_c, (uint32_t)v);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) tmp *= E.tab[p_off[q] + base[q] + rv * PS[q]] * p_scale[q];
+            if (NP_ < 0) {
+                const uint32_t rkv = rk | ((uint32_t)rv << (2 * k));
+                for (int q = 0; q < d[3]; ++q) {
+                    cint_p pd = (cint_p)(E.prevs + prev_off + q);
+                    const int pop = pd[0];
+                    const T sc = RESCALE ? (T)scale_from_bits(E.maxbits[pop]) : (T)1;
+                    tmp *= E.tab[E.off[pop] + consumed_index(pd, cmw, rkv)] * sc;
+                }
+            }
+            if (type == kParent) {
+                T child_prob = (T)1;
+#pragma unroll
+                for (int ch = 0; ch < 7; ++ch) {
+                    if (ch < n_child) {
+                        const uint32_t kid = kid_t[ch];
+                        const T* dk = E.dist + kdist[ch];
+                        const bool pim = (kinfo[ch] >> 8) != 0;
+                        // trans(dist, mt, pt, kid) = md * pd, one of them v's
+                        const T vf = pim ? parent_factor<T>(dk, (uint32_t)v, (kid == PT_AA || kid == PT_AU) ? 1u : 0u)
+                                         : parent_factor<T>(dk + 4, (uint32_t)v, (kid == PT_AA || kid == PT_UA) ? 1u : 0u);
+                        child_prob *= pim ? vf * fixed[ch] : fixed[ch] * vf;
+                    }
+                }
+                tmp *= child_prob;
+            }
+            pc[v] = tmp;
+            total += tmp;
+            ++rv;
+        }
+        E.tab[my_off + c] = total;
+        if (RESCALE) mx = fmaxf(mx, (float)total);
+    }
+    if (RESCALE) {
+        const float m = wave_max(mx);
+        if (E.lane == 0) E.maxbits[E.op] = __float_as_uint(m);
+    }
 }
 
 template <typename T, bool RESCALE>
-__global__ void lsampler_kernel(SampArgs A) {
+__global__ void __launch_bounds__(kLsamplerBlock) lsampler_kernel(SampArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int N = A.N, F = A.F, M = A.M, n_ops = A.n_ops;
     const int NF = N - F;
@@ -200,7 +431,8 @@ __global__ void lsampler_kernel(SampArgs A) {
     const uint32_t rng_chain = (uint32_t)chain + (A.impute ? A.chain_base : 0u);  // Philox chain of the draws
     if (A.impute) {
         if (A.status[chain] != 0) return;
-        locus = impute_locus(A, chain, &use_left, &use_right);
+        locus = impute_locus(A.impute, A.impute_t, M, A.impute_start, A.seed, rng_chain, A.sweep, &use_left,
+                             &use_right);
         step_id = kStepImpute;
         draw0 = A.imp_draw[chain];
     } else {
@@ -282,151 +514,14 @@ __global__ void lsampler_kernel(SampArgs A) {
         for (int s = s0; s < s1; ++s) {
             const int op = ((cint_p)A.sched_ops)[s];
             cint_p d = (cint_p)(A.ops + op);
-            const int type = d[0], k = d[1], peel = d[2], n_prev = d[3], n_child = d[4], child_off = d[5];
-            const int prev_off = d[6], mat_digit = d[7], pat_digit = d[8];
-            // per-digit legal masks packed 4 bits per digit; digit k is the peel node
-            uint32_t cmw = 0;
-            int size = 1;
-            int stride[kMaxCut];
-#pragma unroll
-            for (int q = 0; q < kMaxCut; ++q) {
-                stride[q] = size;
-                if (q < k) {
-                    const uint32_t m = (uint32_t)uni(mask[d[9 + q]]);
-                    cmw |= m << (4 * q);
-                    size *= __popc(m);
-                }
-            }
-            const uint32_t pmask = (uint32_t)uni(mask[peel]);
-            cmw |= pmask << (4 * k);
-            T tr[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) tr[v] = trt[4 * peel + v];
-            const int my_off = uni(off[op]);
-            // consumed tables: offset, per-digit source and stride at this locus
-            int p_off[kMaxPrev], p_nd[kMaxPrev];
-            int scale_exp = 0;
-            T p_scale[kMaxPrev];
-            int p_src[kMaxPrev][kMaxCut], p_stride[kMaxPrev][kMaxCut];
-#pragma unroll
-            for (int q = 0; q < kMaxPrev; ++q) {
-                p_off[q] = 0;
-                p_nd[q] = 0;
-                p_scale[q] = (T)1;
-                int st = 1;
-#pragma unroll
-                for (int j = 0; j < kMaxCut; ++j) {
-                    p_src[q][j] = 0;
-                    p_stride[q][j] = 0;
-                }
-                if (q < n_prev) {
-                    cint_p pd = (cint_p)(A.prevs + prev_off + q);
-                    const int pop = pd[0];
-                    p_off[q] = uni(off[pop]);
-                    p_nd[q] = pd[1];
-                    if (RESCALE) {
-                        const uint32_t mb = maxbits[pop];
-                        p_scale[q] = (T)scale_from_bits(mb);
-                        const int ef = (int)((mb >> 23) & 0xFFu);
-                        scale_exp += ef ? 126 - ef : 0;
-                    }
-#pragma unroll
-                    for (int j = 0; j < kMaxCut; ++j) {
-                        if (j < p_nd[q]) {
-                            const int src = pd[2 + j];
-                            p_src[q][j] = src;
-                            p_stride[q][j] = st;
-                            st *= __popc((cmw >> (4 * src)) & 15u);
-                        }
-                    }
-                }
-            }
-            int kids[7], kid_digit[7], other_digit[7], kid_pim[7];
-#pragma unroll
-            for (int c = 0; c < 7; ++c) {
-                kids[c] = F;
-                kid_digit[c] = other_digit[c] = kid_pim[c] = 0;
-            }
-            if (type == kParent) {  // child descriptors only where the op has children
-#pragma unroll
-                for (int c = 0; c < 7; ++c) {
-                    if (c < n_child) {
-                        cint_p cd = (cint_p)(A.children + child_off + c);
-                        kids[c] = cd[0];
-                        kid_digit[c] = cd[1];
-                        other_digit[c] = cd[2];
-                        kid_pim[c] = cd[3];
-                    }
-                }
-            }
-            if (RESCALE && A.impute && lane == 0 && scale_exp != 0) atomicAdd(&s_scale_exp, scale_exp);
-            float mx = 0.0f;
-            for (int c = lane; c < size; c += 64) {
-                // mixed-radix decode of the compact cell into trait values (2 bits per digit)
-                uint32_t ext = 0;
-                int rem = c;
-#pragma unroll
-                for (int q = 0; q < kMaxCut; ++q) {
-                    if (q < k) {
-                        const uint32_t m = (cmw >> (4 * q)) & 15u;
-                        const int n = __popc(m);
-                        const int quo = (int)(((float)rem + 0.5f) / (float)n);
-                        const int r = rem - quo * n;
-                        rem = quo;
-                        ext |= select_bit(m, (uint32_t)r) << (2 * q);
-                    }
-                }
-                T total = (T)0;
-                T* pc = pre + 4 * (size_t)(my_off + c);
-                for (int v = 0; v < 4; ++v) {
-                    T tmp = tr[v];
-                    if (!((pmask >> v) & 1u) || tmp == (T)0) {
-                        pc[v] = (T)0;
-                        continue;
-                    }
-                    const uint32_t ev = ext | ((uint32_t)v << (2 * k));
-                    if (type == kChild)
-                        tmp *= trans<T>(dist + 8 * (peel - F), (ev >> (2 * mat_digit)) & 3u,
-                                        (ev >> (2 * pat_digit)) & 3u, (uint32_t)v);
-#pragma unroll
-                    for (int q = 0; q < kMaxPrev; ++q) {
-                        if (q < n_prev) {
-                            int idx = 0;
-#pragma unroll
-                            for (int j = 0; j < kMaxCut; ++j) {
-                                if (j < p_nd[q]) {
-                                    const int src = p_src[q][j];
-                                    const uint32_t x = (ev >> (2 * src)) & 3u;
-                                    const uint32_t mm = (cmw >> (4 * src)) & 15u;
-                                    idx += (int)rank_in(mm, x) * p_stride[q][j];
-                                }
-                            }
-                            tmp *= tab[p_off[q] + idx] * p_scale[q];
-                        }
-                    }
-                    if (type == kParent) {
-                        T child_prob = (T)1;
-#pragma unroll
-                        for (int ch = 0; ch < 7; ++ch) {
-                            if (ch < n_child) {
-                                const uint32_t kid = (ev >> (2 * kid_digit[ch])) & 3u;
-                                const uint32_t other = (ev >> (2 * other_digit[ch])) & 3u;
-                                const uint32_t mt = kid_pim[ch] ? (uint32_t)v : other;
-                                const uint32_t pt = kid_pim[ch] ? other : (uint32_t)v;
-                                child_prob *= trans<T>(dist + 8 * (kids[ch] - F), mt, pt, kid);
-                            }
-                        }
-                        tmp *= child_prob;
-                    }
-                    pc[v] = tmp;
-                    total += tmp;
-                }
-                tab[my_off + c] = total;
-                if (RESCALE) mx = fmaxf(mx, (float)total);
-            }
-            if (RESCALE) {
-                const float m = wave_max(mx);
-                if (lane == 0) maxbits[op] = __float_as_uint(m);
+            const OpEnv<T> E{A.prevs, A.children, A.impute, op, d, mask, off, maxbits, trt, dist, tab, pre, lane, F,
+                             &s_scale_exp};
+            switch (d[3]) {  // consumed tables (pedigrees seldom need more than 3)
+                case 0: sampler_op<T, RESCALE, 0>(E); break;
+                case 1: sampler_op<T, RESCALE, 1>(E); break;
+                case 2: sampler_op<T, RESCALE, 2>(E); break;
+                case 3: sampler_op<T, RESCALE, 3>(E); break;
+                default: sampler_op<T, RESCALE, -1>(E); break;
             }
         }
         __syncthreads();

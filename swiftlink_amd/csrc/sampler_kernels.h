@@ -45,6 +45,9 @@ struct SampArgs {
     double* imp_weight;          // [chains] running sum of ln L
 };
 
+// L-sampler workgroup: four wavefronts share each peel level
+constexpr int kLsamplerBlock = 256;
+
 size_t lsampler_lds_bytes(int precision, int cells_cap, int n_ops, int N, int F, int block);
 hipError_t configure_lsampler_lds(int precision, size_t lds);
 hipError_t launch_lsampler(const SampArgs& A, int precision, int block, size_t lds, int n_chains, hipStream_t s);
