@@ -177,12 +177,11 @@ __device__ __forceinline__ T trans(const T* dist, uint32_t m, uint32_t p, uint32
 // What one op's evaluation reads and writes (LDS arrays of the workgroup).
 template <typename T>
 struct OpEnv {
-    const SampPrev* prevs;
-    const ChildDesc* children;
+    const SampPrev* prevs;  // consumed tables past the first kInlinePrev
     int impute;
     int op;
     cint_p d;               // the op's SampOp words
-    const uint8_t* mask;    // [N] legal trait values at the locus
+    const uint32_t* rec;    // the op's record at this locus (kOpRec words)
     const int* off;         // [n_ops+1] table offsets
     uint32_t* maxbits;      // [n_ops] FP32 bits of each table's maximum (RESCALE)
     const T* trt;           // [N][4] trait probabilities at the locus
@@ -225,75 +224,58 @@ __device__ __forceinline__ void sampler_op(const OpEnv<T>& E) {
     // NP_ < 0: any number, indices recomputed from the descriptors
     constexpr int NP = NP_ < 0 ? 0 : NP_;
     cint_p d = E.d;
-    const int type = d[0], k = d[1], peel = d[2], n_child = d[4], child_off = d[5];
+    const int type = d[0], k = d[1], peel = d[2], n_child = d[4];
     const int prev_off = d[6], mat_digit = d[7], pat_digit = d[8];
     const int F = E.F;
-    // per-digit legal masks packed 4 bits per digit; digit k is the peel node
-    uint32_t cmw = 0;
-    int size = 1;
-#pragma unroll
-    for (int q = 0; q < kMaxCut; ++q) {
-        if (q < k) {
-            const uint32_t m = (uint32_t)uni(E.mask[d[9 + q]]);
-            cmw |= m << (4 * q);
-            size *= __popc(m);
-        }
-    }
-    const uint32_t pmask = (uint32_t)uni(E.mask[peel]);
-    cmw |= pmask << (4 * k);
+    // the op's record (built per locus by the whole workgroup): legal masks
+    // packed 4 bits per digit (digit k is the peel node), table offset, size
+    const uint32_t* rec = E.rec;
+    const uint32_t cmw = (uint32_t)uni((int)rec[0]);
+    const uint32_t os = (uint32_t)uni((int)rec[1]);
+    const int my_off = (int)(os & 0xFFFFu), size = (int)(os >> 16);
+    const uint32_t pmask = (cmw >> (4 * k)) & 15u;
     T tr[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) tr[v] = E.trt[4 * peel + v];
-    const int my_off = uni(E.off[E.op]);
     // consumed tables: offset, scale, and the stride of each of this op's
     // digits in the table's radix (0 where the digit is not in the table)
-    int p_off[NP > 0 ? NP : 1], S[NP > 0 ? NP : 1][kMaxCut + 1];
+    int p_off[NP > 0 ? NP : 1], S[NP > 0 ? NP : 1][kMaxCut + 1], PS[NP > 0 ? NP : 1];
     T p_scale[NP > 0 ? NP : 1];
     int scale_exp = 0;
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
-        cint_p pd = (cint_p)(E.prevs + prev_off + q);
-        const int pop = pd[0], nd = pd[1];
-        p_off[q] = uni(E.off[pop]);
+        const uint32_t* rq = rec + 2 + 4 * q;
+        const uint32_t op_ps = (uint32_t)uni((int)rq[0]);
+        p_off[q] = (int)(op_ps & 0xFFFFu);
+        PS[q] = (int)(op_ps >> 16);
+#pragma unroll
+        for (int w = 0; w < 3; ++w) {
+            const uint32_t pair = (uint32_t)uni((int)rq[1 + w]);
+            S[q][2 * w] = (int)(pair & 0xFFFFu);
+            S[q][2 * w + 1] = (int)(pair >> 16);
+        }
+        S[q][6] = (int)((((uint32_t)uni((int)rec[14 + q / 2])) >> (16 * (q & 1))) & 0xFFFFu);
+        S[q][7] = 0;  // digit 7 exists only as the peel node of a 7-person cutset
         p_scale[q] = (T)1;
         if (RESCALE) {
-            const uint32_t mb = E.maxbits[pop];
+            const uint32_t mb = E.maxbits[d[16 + q]];  // SampOp words 16..: consumed ops
             p_scale[q] = (T)scale_from_bits(mb);
             const int ef = (int)((mb >> 23) & 0xFFu);
             scale_exp += ef ? 126 - ef : 0;
         }
-#pragma unroll
-        for (int dd = 0; dd <= kMaxCut; ++dd) S[q][dd] = 0;
-        int st = 1;
-#pragma unroll
-        for (int j = 0; j < kMaxCut; ++j) {
-            if (j < nd) {
-                const int src = pd[2 + j];
-#pragma unroll
-                for (int dd = 0; dd <= kMaxCut; ++dd) S[q][dd] = src == dd ? st : S[q][dd];
-                st *= __popc((cmw >> (4 * src)) & 15u);
-            }
-        }
-    }
-    int PS[NP > 0 ? NP : 1];  // stride of the peel node's digit
-#pragma unroll
-    for (int q = 0; q < NP; ++q) {
-        PS[q] = 0;
-#pragma unroll
-        for (int dd = 0; dd <= kMaxCut; ++dd) PS[q] = dd == k ? S[q][dd] : PS[q];
     }
     // PARENT children: distribution offset, and kid digit | other-parent
     // digit << 4 | (peel node is the mother) << 8
     int kdist[7], kinfo[7];
 #pragma unroll
     for (int c = 0; c < 7; ++c) kdist[c] = kinfo[c] = 0;
-    if (type == kParent) {  // child descriptors only where the op has children
+    if (type == kParent) {  // child records (SampOp words 22..) only where the op has children
 #pragma unroll
         for (int c = 0; c < 7; ++c) {
             if (c < n_child) {
-                cint_p cd = (cint_p)(E.children + child_off + c);
-                kdist[c] = 8 * (cd[0] - F);
-                kinfo[c] = cd[1] | (cd[2] << 4) | ((cd[3] ? 1 : 0) << 8);
+                const uint32_t w = (uint32_t)d[22 + c];
+                kdist[c] = (int)(w & 0xFFFFu);
+                kinfo[c] = (int)(w >> 16);
             }
         }
     }
@@ -408,7 +390,8 @@ __global__ void __launch_bounds__(kLsamplerBlock) lsampler_kernel(SampArgs A) {
     const int N = A.N, F = A.F, M = A.M, n_ops = A.n_ops;
     const int NF = N - F;
     // LDS carve-up (sizes fixed by the host: cells_cap, n_ops, N)
-    T* tab = reinterpret_cast<T*>(smem);
+    uint32_t* oprec = reinterpret_cast<uint32_t*>(smem);  // [n_ops][kOpRec] per-locus op records
+    T* tab = reinterpret_cast<T*>(oprec + kOpRec * (size_t)n_ops);
     T* pre = tab + A.cells_cap;
     T* dist = pre + 4 * (size_t)A.cells_cap;           // [NF][8]
     T* trt = dist + 8 * NF;                             // [N][4] trait probabilities at the locus
@@ -503,6 +486,43 @@ __global__ void __launch_bounds__(kLsamplerBlock) lsampler_kernel(SampArgs A) {
         if (tid == 0) A.status[chain] = -6;
         return;
     }
+    // per-op records at this locus: legal masks, table offset and size, and
+    // each of the first kInlinePrev consumed tables' offset and strides
+    static_assert(kInlinePrev == 3 && kOpRec == 16, "op record layout");
+    for (int o = tid; o < n_ops; o += blockDim.x) {
+        const SampOp* d = A.ops + o;
+        const int k = d->k;
+        uint32_t cmw = 0;
+        for (int q = 0; q < k; ++q) cmw |= (uint32_t)mask[d->cut[q]] << (4 * q);
+        cmw |= (uint32_t)mask[d->peel] << (4 * k);
+        uint32_t* r = oprec + kOpRec * (size_t)o;
+        r[0] = cmw;
+        r[1] = (uint32_t)off[o] | ((uint32_t)(off[o + 1] - off[o]) << 16);
+        uint32_t s6[kInlinePrev];
+        for (int q = 0; q < kInlinePrev; ++q) {
+            uint32_t* rq = r + 2 + 4 * q;
+            uint32_t S[kMaxCut + 1] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (q < d->n_prev) {
+                const uint32_t srcw = d->prev_src[q];
+                const int nd = (int)(srcw & 7u);
+                uint32_t st = 1;
+                for (int j = 0; j < nd; ++j) {
+                    const int src = (int)((srcw >> (3 + 3 * j)) & 7u);
+                    S[src] = st;
+                    st *= __popc((cmw >> (4 * src)) & 15u);
+                }
+                rq[0] = (uint32_t)off[d->prev_op[q]];
+            } else {
+                rq[0] = 0;
+            }
+            rq[0] |= S[k] << 16;
+            for (int w = 0; w < 3; ++w) rq[1 + w] = S[2 * w] | (S[2 * w + 1] << 16);
+            s6[q] = S[6];
+        }
+        r[14] = s6[0] | (s6[1] << 16);
+        r[15] = s6[2];
+    }
+    __syncthreads();
 
     // 2. forward peel by levels
     const int W = A.waves;
@@ -514,8 +534,8 @@ __global__ void __launch_bounds__(kLsamplerBlock) lsampler_kernel(SampArgs A) {
         for (int s = s0; s < s1; ++s) {
             const int op = ((cint_p)A.sched_ops)[s];
             cint_p d = (cint_p)(A.ops + op);
-            const OpEnv<T> E{A.prevs, A.children, A.impute, op, d, mask, off, maxbits, trt, dist, tab, pre, lane, F,
-                             &s_scale_exp};
+            const OpEnv<T> E{A.prevs, A.impute, op, d, oprec + kOpRec * (size_t)op, off, maxbits, trt, dist, tab, pre,
+                             lane, F, &s_scale_exp};
             switch (d[3]) {  // consumed tables (pedigrees seldom need more than 3)
                 case 0: sampler_op<T, RESCALE, 0>(E); break;
                 case 1: sampler_op<T, RESCALE, 1>(E); break;
@@ -628,7 +648,8 @@ __global__ void __launch_bounds__(kLsamplerBlock) lsampler_kernel(SampArgs A) {
 size_t lsampler_lds_bytes(int precision, int cells_cap, int n_ops, int N, int F, int block) {
     const size_t esz = precision == 0 ? 8 : 4;
     const int NF = N - F;
-    size_t b = esz * (size_t)cells_cap * 5 + esz * 8 * (size_t)NF + esz * 4 * (size_t)N;
+    size_t b = 4 * (size_t)kOpRec * n_ops;  // op records
+    b += esz * (size_t)cells_cap * 5 + esz * 8 * (size_t)NF + esz * 4 * (size_t)N;
     b = (b + 7) / 8 * 8;
     b += 8 * (size_t)n_ops;                          // uop
     b += 4 * ((size_t)n_ops + 1 + block + 2 * NF);   // off, scan, mcount

@@ -47,6 +47,12 @@ struct SampArgs {
 
 // L-sampler workgroup: four wavefronts share each peel level
 constexpr int kLsamplerBlock = 256;
+// words of an op's per-locus record in LDS: legal-mask word, table offset |
+// size << 16, then per inline consumed table (words 2+4q..5+4q): offset |
+// peel-digit stride << 16 and the strides of digits 0..5 as 16-bit pairs;
+// words 14-15 the three tables' digit-6 strides.  Offsets and strides fit
+// 16 bits because the tables fit LDS.
+constexpr int kOpRec = 16;
 
 size_t lsampler_lds_bytes(int precision, int cells_cap, int n_ops, int N, int F, int block);
 hipError_t configure_lsampler_lds(int precision, size_t lds);

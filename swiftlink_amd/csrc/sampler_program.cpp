@@ -64,6 +64,13 @@ std::string compile_sampler_program(const PedigreeView& ped, const PeelView& pv,
                 if (src < 0) return sfmt("op %d: consumed table %d has a variable outside the op", o, p);
                 sp.src[j] = src;
             }
+            const int qi = q - pv.prev_off[o];
+            if (qi < kInlinePrev) {
+                d.prev_op[qi] = p;
+                uint32_t w = (uint32_t)sp.nd;
+                for (int j = 0; j < sp.nd; ++j) w |= (uint32_t)sp.src[j] << (3 + 3 * j);
+                d.prev_src[qi] = w;
+            }
             P.prevs.push_back(sp);
         }
         d.child_off = (int)P.children.size();
@@ -82,6 +89,9 @@ std::string compile_sampler_program(const PedigreeView& ped, const PeelView& pv,
                 cd.other_digit = digit_of(cd.peel_is_mother ? ped.father[ch] : ped.mother[ch]);
                 if (cd.other_digit < 0)
                     return sfmt("op %d: child %d of a PARENT peel lacks its other parent in the cutset", o, ch);
+                if (8 * (ch - ped.F) >= 65536) return "pedigree too large for the sampler's child records";
+                d.child[d.n_child] = (uint32_t)(8 * (ch - ped.F)) |
+                                     ((uint32_t)(cd.child_digit | (cd.other_digit << 4) | (cd.peel_is_mother << 8)) << 16);
                 P.children.push_back(cd);
                 ++d.n_child;
             }

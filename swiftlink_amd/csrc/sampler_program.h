@@ -23,6 +23,8 @@
 
 namespace sl {
 
+constexpr int kInlinePrev = 3;  // consumed tables described inside SampOp
+
 struct SampOp {
     int32_t type;
     int32_t k;
@@ -34,7 +36,16 @@ struct SampOp {
     int32_t mat_digit;   // CHILD: cutset digit of the mother
     int32_t pat_digit;   // CHILD: cutset digit of the father
     int32_t cut[kMaxCut];
+    // The same facts packed so that one 128-byte read has them all: the
+    // first kInlinePrev consumed tables' op and (digit count | src digit j
+    // << (3 + 3j)); PARENT children as 8 (person - F) | (child digit | other
+    // parent's digit << 4 | peel node is the mother << 8) << 16.
+    int32_t prev_op[kInlinePrev];
+    uint32_t prev_src[kInlinePrev];
+    uint32_t child[kMaxCut];
+    int32_t pad[3];
 };
+static_assert(sizeof(SampOp) == 128, "SampOp is one 128-byte record");
 
 struct SampPrev {
     int32_t op;
