@@ -23,6 +23,7 @@
 //   graphs <ped> <map> <dat> <outdir> seed G stride  random legal graphs (elimination + L-sweeps)
 //   impute <ped> <map> <dat> <outdir> seed R      sequential-imputation replicates (R = 0: locus by locus)
 //   writer <ped> <map> <dat> <outdir> <graphs> G  LinkageWriter output for one and two families
+//   peelseq <ped> <map> <dat> <outdir>            seconds of the peel-sequence search (build_peel_sequence)
 //   elod   <ped> <map> <dat> <outdir> freq p0 p1 p2 sep R seed keep
 //                                                 ELOD of the first pedigree of <ped> (map, dat unused)
 //   bench  <ped> <map> <dat> <outdir> threads nL nM nG
@@ -189,6 +190,8 @@ struct Case {
     int N, F, M, nlod;
 };
 
+static double g_peel_seconds = 0.0;
+
 static void build_case(Case& c, const char* ped, const char* map, const char* dat, int lods,
                        int peeliters, uint64_t peelseed) {
     struct mcmc_options o;
@@ -208,7 +211,9 @@ static void build_case(Case& c, const char* ped, const char* map, const char* da
     srand((unsigned)peelseed);
     stream_set(peelseed, 0, 0, STEP_SETUP);
     c.psg = new PeelSequenceGenerator(c.p, c.m, c.sex_linked, false);
+    const double t0 = wall();
     c.psg->build_peel_sequence((unsigned)peeliters);
+    g_peel_seconds = wall() - t0;
 }
 
 static void json_int_list(FILE* f, const std::vector<unsigned>& v) {
@@ -878,6 +883,13 @@ int main(int argc, char** argv) {
     build_case(c, ped, map, dat, lods, peeliters, peelseed);
 
     if (cmd == "dump") cmd_dump(c, out);
+    else if (cmd == "peelseq") {
+        FILE* f = fopen((out + "/peelseq.json").c_str(), "w");
+        fprintf(f, "{\"seconds\": %.6f, \"iterations\": %d, \"ops\": ", g_peel_seconds, peeliters);
+        write_ops_json(f, c.psg);
+        fprintf(f, "}\n");
+        fclose(f);
+    }
     else if (cmd == "lod" && nrest >= 2) cmd_lod(c, out, rest[0], atoi(rest[1]));
     else if (cmd == "lsweep" && nrest >= 5)
         cmd_lsweep(c, out, rest[0], rest[1], strtoull(rest[2], 0, 10), atoi(rest[3]), atoi(rest[4]));

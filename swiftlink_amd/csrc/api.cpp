@@ -12,6 +12,7 @@
 #include "../../include/swiftlink_amd.h"
 #include "kernels.h"
 #include "msampler_kernels.h"
+#include "peelseq.h"
 #include "philox.h"
 #include "program.h"
 #include "sampler_kernels.h"
@@ -1669,6 +1670,36 @@ int sl_lod_grad(sl_ctx* c, const uint32_t* dev_graphs, int32_t G, double* lod, d
         lod[i] = (s[0] - lnG - pt[0]) / ln10;
         for (int k = 0; k < 6; ++k) dlod[(size_t)i * 6 + k] = (s[1 + k] - (k < 4 ? pt[1 + k] : 0.0)) / ln10;
     }
+    return SL_OK;
+}
+
+int sl_peel_sequence(int32_t n_people, const int32_t* mother, const int32_t* father, int64_t iterations,
+                     uint64_t seed, uint32_t rand_seed, int32_t* buf, int64_t cap, int64_t* needed, int64_t* cost) {
+    if (n_people <= 0 || !mother || !father || iterations < 0 || (cap > 0 && !buf)) return fail(nullptr, SL_E_INVALID, "bad arguments");
+    int founders = 0;
+    for (int i = 0; i < n_people; ++i) {
+        const bool f = mother[i] < 0;
+        if (f != (father[i] < 0)) return fail(nullptr, SL_E_INVALID, "person with one parent");
+        if (!f && (mother[i] >= n_people || father[i] >= n_people || mother[i] == i || father[i] == i))
+            return fail(nullptr, SL_E_INVALID, "parent index out of range");
+        if (f) {
+            if (i != founders) return fail(nullptr, SL_E_INVALID, "founders must come first");
+            ++founders;
+        }
+    }
+    if (founders == 0 || founders == n_people) return fail(nullptr, SL_E_INVALID, "no founders or no non-founders");
+    std::vector<int32_t> out;
+    const int64_t c = sl::build_peel_sequence(n_people, mother, father, iterations, seed, rand_seed, out);
+    if (needed) *needed = (int64_t)out.size();
+    if (cost) *cost = c;
+    if ((int64_t)out.size() > cap) return buf ? fail(nullptr, SL_E_CAPACITY, "buffer too small") : SL_OK;
+    std::copy(out.begin(), out.end(), buf);
+    return SL_OK;
+}
+
+int sl_person_order(int32_t n_people, const uint8_t* founder, int32_t* order) {
+    if (n_people <= 0 || !founder || !order) return fail(nullptr, SL_E_INVALID, "bad arguments");
+    sl::person_order(n_people, founder, order);
     return SL_OK;
 }
 

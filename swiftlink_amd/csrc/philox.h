@@ -13,9 +13,11 @@ namespace sl {
 // a sweep's kind (draw 0) and first L-sweep parity (draw 1), and the M-sweep's
 // meiosis-order shuffle.  Sequential imputation draws every uniform of one
 // replicate in sequence at step kStepImpute (draw 0: the starting locus).
-// 0xFFFFFFFD is the peel-sequence search's stream on the host side.
+// The host's peel-sequence search draws its swaps in sequence at
+// kStepPeelSearch (chain 0, sweep 0).
 constexpr uint32_t kStepSched = 0xFFFFFFFFu;
 constexpr uint32_t kStepShuffle = 0xFFFFFFFEu;
+constexpr uint32_t kStepPeelSearch = 0xFFFFFFFDu;
 constexpr uint32_t kStepImpute = 0xFFFFFFFCu;
 
 __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {

@@ -13,10 +13,12 @@ MCMC starts, so the GPU library can be driven without the reference:
   ids are resolved, so every person takes the founder branch: the genotype
   mask times the marker prior, normalised (pedigree_parser.cc:141-157,
   person.h:168).  This is reproduced as is;
-* founders-first internal order (pedigree.cc:154-162; this order is stable,
-  the reference's std::sort is not, so comparisons go by person id);
+* founders-first internal order (pedigree.cc:154-162): by default a stable
+  one; with peel="reference" the reference's own (libstdc++ std::sort, not
+  stable) through sl_person_order;
 * genotype elimination masks (elimination.cc:87-125, 393-395);
-* a peel sequence (swiftlink_amd.peelseq).
+* a peel sequence: this package's greedy order (swiftlink_amd.peelseq) or
+  the reference's search (sl_peel_sequence).
 
 And the output side: the LOD table the reference writes
 (LinkageWriter::write, linkage_writer.cc:14-91), `write_lod_table`.
@@ -247,7 +249,14 @@ def eliminate(geno: np.ndarray, mother: np.ndarray, father: np.ndarray, F: int) 
 
 
 def load(ped_path: str, map_path: str, dat_path: str, nlod: int = 5, peel_seed: int = 1,
-         peel_tries: int = 64) -> Case:
+         peel_tries: int = 64, peel: str = "greedy", peel_iterations: int = 1000000) -> Case:
+    """The Case of the first pedigree in the files.  peel="greedy": this
+    package's fast elimination order (peelseq.build, peel_tries random
+    tie-breaks) over a stable founders-first order; peel="reference": the
+    reference's person order (sl_person_order) and its own search
+    (PeelSequenceGenerator::build_peel_sequence, through sl_peel_sequence)
+    with peel_iterations swaps per restart and peel_seed for both its random
+    sources, which gives the reference's tables."""
     names, gpos = read_map(map_path)
     freq, pen, minors = read_dat(dat_path)
     M = len(names)
@@ -257,9 +266,16 @@ def load(ped_path: str, map_path: str, dat_path: str, nlod: int = 5, peel_seed: 
     fam = rows[0][0]
     rows = [r for r in rows if r[0] == fam]  # first pedigree only
     ids = [r[1] for r in rows]
-    founders = [r for r in rows if r[2] == "0" and r[3] == "0"]
-    others = [r for r in rows if not (r[2] == "0" and r[3] == "0")]
-    order = founders + others
+    if peel == "reference":
+        # the reference's own (unstable) founders-first order
+        from . import native
+        perm = native.person_order([1 if (r[2] == "0" and r[3] == "0") else 0 for r in rows])
+        order = [rows[i] for i in perm]
+        founders = [r for r in order if r[2] == "0" and r[3] == "0"]
+    else:
+        founders = [r for r in rows if r[2] == "0" and r[3] == "0"]
+        others = [r for r in rows if not (r[2] == "0" and r[3] == "0")]
+        order = founders + others
     idx = {r[1]: i for i, r in enumerate(order)}
     N, F = len(order), len(founders)
     mother = np.array([-1 if r[3] == "0" else idx[r[3]] for r in order], np.int32)
@@ -280,7 +296,13 @@ def load(ped_path: str, map_path: str, dat_path: str, nlod: int = 5, peel_seed: 
     if (theta == 0).any():
         raise ValueError("zero recombination fraction between adjacent markers (genetic_map.cc:48-53)")
     ptheta = np.array([_haldane(_inverse_haldane(t) / float(nlod + 1)) for t in theta])
-    ops = peelseq.build(N, F, mother, father, seed=peel_seed, tries=peel_tries)
+    if peel == "reference":
+        from . import native
+        ops, _ = native.peel_sequence(mother, father, peel_iterations, seed=peel_seed, rand_seed=peel_seed)
+    elif peel == "greedy":
+        ops = peelseq.build(N, F, mother, father, seed=peel_seed, tries=peel_tries)
+    else:
+        raise ValueError(f"unknown peel search {peel!r}")
     case = Case(N=N, F=F, M=M, nlod=nlod, mother=mother, father=father, sex=sex, affection=aff, typed=typed,
                 geno=geno, disease_prob=dprob, geno_prob=gprob, legal=legal, theta=theta, partial_theta=ptheta,
                 minor=np.array(minors, np.float64), ops=ops, ids=[r[1] for r in order])

@@ -96,6 +96,9 @@ _SIGS = {
     "sl_sequential_imputation": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_int32, i32p, f64p, i32p,
                                            C.POINTER(C.c_int32)]),
     "sl_elod": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int64, f64p]),
+    "sl_person_order": (C.c_int, [C.c_int32, C.POINTER(C.c_uint8), i32p]),
+    "sl_peel_sequence": (C.c_int, [C.c_int32, i32p, i32p, C.c_int64, C.c_uint64, C.c_uint32, i32p, C.c_int64, i64p,
+                                   i64p]),
     "sl_lsweep": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32]),
     "sl_msweep": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, i32p, C.c_int32]),
     "sl_mcmc_run": (C.c_int, [C.c_void_p, C.POINTER(Schedule), C.POINTER(McmcStats)]),
@@ -374,6 +377,44 @@ class Context:
         b = C.c_double(0)
         self._check(lib().sl_lod_work(self.h, C.byref(f), C.byref(b)), "sl_lod_work")
         return f.value, b.value
+
+
+def person_order(founder) -> np.ndarray:
+    """sl_person_order: file positions of the reference's internal order."""
+    f = np.ascontiguousarray(founder, np.uint8)
+    out = np.zeros(len(f), np.int32)
+    rc = lib().sl_person_order(len(f), f.ctypes.data_as(C.POINTER(C.c_uint8)), _ptr(out, i32p))
+    if rc:
+        raise SlError(f"sl_person_order failed ({rc})")
+    return out
+
+
+def peel_sequence(mother, father, iterations: int = 1000000, seed: int = 12345, rand_seed: int = 12345):
+    """sl_peel_sequence: the reference's peel-sequence search on the host.
+    Returns (PeelOps, cost)."""
+    from .case import PeelOps
+    m = np.ascontiguousarray(mother, np.int32)
+    f = np.ascontiguousarray(father, np.int32)
+    need, cost = C.c_int64(0), C.c_int64(0)
+    rc = lib().sl_peel_sequence(len(m), _ptr(m, i32p), _ptr(f, i32p), iterations, seed, rand_seed, None, 0,
+                                C.byref(need), C.byref(cost))
+    if rc:
+        raise SlError(f"sl_peel_sequence failed ({rc}): {lib().sl_last_error(None).decode()}")
+    buf = np.zeros(max(need.value, 1), np.int32)
+    rc = lib().sl_peel_sequence(len(m), _ptr(m, i32p), _ptr(f, i32p), iterations, seed, rand_seed, _ptr(buf, i32p),
+                                len(buf), C.byref(need), C.byref(cost))
+    if rc:
+        raise SlError(f"sl_peel_sequence failed ({rc}): {lib().sl_last_error(None).decode()}")
+    n, pos = int(buf[0]), 1
+    types, peels, cuts, prevs, kids = [], [], [], [], []
+    for _ in range(n):
+        types.append(int(buf[pos])); peels.append(int(buf[pos + 1])); k = int(buf[pos + 2]); pos += 3
+        cuts.append([int(x) for x in buf[pos:pos + k]]); pos += k
+        npv = int(buf[pos]); pos += 1
+        prevs.append([int(x) for x in buf[pos:pos + npv]]); pos += npv
+        nc = int(buf[pos]); pos += 1
+        kids.append([int(x) for x in buf[pos:pos + nc]]); pos += nc
+    return PeelOps.from_lists(types, peels, cuts, prevs, kids), cost.value
 
 
 def run_elod(sampler: "Context", scorer: "Context", seed: int, replicates: int) -> float:

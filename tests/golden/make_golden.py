@@ -19,6 +19,8 @@ Each fixture holds:
   impute0_*   LocusSampler::locus_by_locus under (seed 56, 0, 0, step 0xFFFFFFFC)
   writer_*    LinkageWriter's LOD table for the graphs as one family and split
               into two, the LOD values it printed, marker names and positions
+  (and peelseq_ped150.npz: the reference's peel table of the synthetic
+   150-person pedigree, see make_peelseq)
   elod_*      Elod::run's set-up and replicate loop (400 replicates, default
               model, replicate j under (seed 31, j, 0, step 0xFFFFFFFC)): the
               ELOD, accumulator, P(T), the first 6 replicates' graphs, both
@@ -124,9 +126,26 @@ def make(name, base, work):
     print(f"{name}: N={N} M={M} ops={case.ops.n} -> {os.path.getsize(path)} bytes")
 
 
+def make_peelseq(tmp):
+    """peelseq_ped150.npz: the reference's peel table of the 150-person
+    synthetic pedigree (random downhill search, 2000 swaps per restart,
+    seeds 7) with the parent indices in its internal order."""
+    d = os.path.join(tmp, "ps150")
+    synth.write_config("ped150_snp1000", d)
+    b = os.path.join(d, "ped150_snp1000")
+    run("peeliters=2000", "peelseed=7", "dump", b + ".ped", b + ".map", b + ".dat", os.path.join(d, "dump"))
+    case = Case.from_harness_dir(os.path.join(d, "dump"))
+    fx = {"mother": case.mother, "father": case.father, "params": np.array([2000, 7])}
+    fx.update({"op_" + f.name: getattr(case.ops, f.name) for f in dataclasses.fields(PeelOps)})
+    path = os.path.join(OUT, "peelseq_ped150.npz")
+    np.savez_compressed(path, **fx)
+    print(f"peelseq_ped150: ops={case.ops.n} -> {os.path.getsize(path)} bytes")
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REFSRC={REF}"], check=True)
     with tempfile.TemporaryDirectory() as tmp:
+        make_peelseq(tmp)
         for ex in ("loop", "east"):
             make(ex, os.path.join(EXAMPLES, ex), os.path.join(tmp, ex))
         sdir = os.path.join(tmp, "synth")

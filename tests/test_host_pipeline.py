@@ -117,3 +117,52 @@ def test_elod_oracle_matches_reference(golden):
     assert acc[0] == float(fx["elod_acc"][0])
     value = (acc[0] - np.log(R) - tp) / np.log(10.0)
     assert abs(value - float(fx["elod_value"][0])) <= 1e-12 * max(1.0, abs(value))
+
+
+def _ops_equal(a, b):
+    import dataclasses as dc
+    from swiftlink_amd.case import PeelOps
+    return all(np.array_equal(getattr(a, f.name), getattr(b, f.name)) for f in dc.fields(PeelOps))
+
+
+def test_peel_sequence_matches_reference(golden):
+    """sl_peel_sequence with the harness's seeds (12345) and 20000 swaps per
+    restart gives the reference's peel table op for op."""
+    from swiftlink_amd import native
+    case, fx = golden
+    ops, cost = native.peel_sequence(case.mother, case.father, 20000, 12345, 12345)
+    assert _ops_equal(ops, case.ops)
+    assert cost == int(np.diff(case.ops.cut_off).sum())
+
+
+def test_peel_sequence_ped150_matches_reference():
+    """150 people, 3 loops: the greedy search fails and random restarts of the
+    downhill search run; the table is still the reference's."""
+    from swiftlink_amd import native
+    from swiftlink_amd.case import PeelOps
+    import dataclasses as dc
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "peelseq_ped150.npz"))
+    iters, seed = (int(x) for x in z["params"])
+    ref = PeelOps(**{f.name: z["op_" + f.name] for f in dc.fields(PeelOps)})
+    ops, _ = native.peel_sequence(z["mother"], z["father"], iters, seed, seed)
+    assert _ops_equal(ops, ref)
+
+
+def test_load_with_reference_peel_search_matches_reference_tables():
+    """LINKAGE files -> Case with the reference's search: every table,
+    the peel table included, is the reference harness's."""
+    ref, fx = load_golden("ped30_snp100")
+    with tempfile.TemporaryDirectory() as d:
+        p, m, dat = synth.write_config("ped30_snp100", d)
+        mine = linkage.load(p, m, dat, peel="reference", peel_iterations=20000, peel_seed=12345)
+    assert mine.ids == ref.ids  # the reference's person order
+    _compare(mine, ref)
+    assert _ops_equal(mine.ops, ref.ops)
+
+
+def test_peel_sequence_rejects_bad_pedigrees():
+    from swiftlink_amd import native
+    with pytest.raises(native.SlError):
+        native.peel_sequence(np.array([-1, 0, -1]), np.array([-1, 2, -1]), 10)  # founders not first
+    with pytest.raises(native.SlError):
+        native.peel_sequence(np.array([-1, -1, 0]), np.array([-1, -1, 5]), 10)  # parent out of range
