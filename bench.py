@@ -398,6 +398,31 @@ def main():
                        for i in range(acase.N)])
         res["adjoint"]["cpu_baseline"] = cpu_adjoint_baseline(
             dataclasses.replace(acase, disease_prob=dp.reshape(acase.disease_prob.shape)), ag0, ajac, 15000)
+    if extras:
+        # the reference's peel-sequence search (greedy, then random-restart
+        # downhill) as host C++, beside the reference's own code
+        iters = 5000
+        from swiftlink_amd import linkage
+        rows = linkage.read_ped(os.path.join(tmp, args.config) + ".ped")
+        perm = native.person_order([1 if (r[2] == "0" and r[3] == "0") else 0 for r in rows])
+        pos = {rows[i][1]: k for k, i in enumerate(perm)}
+        pm = np.array([-1 if rows[i][3] == "0" else pos[rows[i][3]] for i in perm], np.int32)
+        pf = np.array([-1 if rows[i][2] == "0" else pos[rows[i][2]] for i in perm], np.int32)
+        t0 = time.perf_counter()
+        _, pcost = native.peel_sequence(pm, pf, iters, 7, 7)
+        pdt = time.perf_counter() - t0
+        res["peel_sequence"] = {"workload": f"{args.config} pedigree, {iters} downhill swaps per restart, seeds 7",
+                                "seconds": pdt, "cost": pcost, "cores": 1}
+        harness = os.path.join(ROOT, "oracle", "_ref", "refharness")
+        if not args.no_cpu and os.path.exists(harness):
+            base = os.path.join(tmp, args.config)
+            out = os.path.join(tmp, "cpu_peel")
+            subprocess.run([harness, f"peeliters={iters}", "peelseed=7", "peelseq", base + ".ped", base + ".map",
+                            base + ".dat", out], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            pj = json.load(open(os.path.join(out, "peelseq.json")))
+            res["peel_sequence"]["cpu_baseline"] = {
+                "value": pj["seconds"], "unit": "s", "cores": 1, "kind": "reference",
+                "sample": f"PeelSequenceGenerator::build_peel_sequence({iters}) on the same pedigree file"}
     if not args.no_cpu and extras and "elod" in res:
         si_cpu, elod_cpu = cpu_impute_elod_baseline(tmp, os.path.join(tmp, args.config))
         if si_cpu:

@@ -380,7 +380,9 @@ int sl_program_export(const sl_pedigree_desc* ped, const sl_peel_desc* peel, int
  * order, founders first (mother/father -1 for founders).  A greedy
  * elimination is tried first; when it fails, random restarts of the random
  * downhill search run `iterations` swaps each until an order passes
- * is_legit.  The shuffles follow the C library's rand() sequence after
+ * is_legit.  `threads` host threads (0: up to 16) cost the next drawn swaps
+ * side by side; the result does not depend on the count.  The shuffles
+ * follow the C library's rand() sequence after
  * srand(rand_seed) and the swaps draw (draw, step=0xFFFFFFFD, sweep=0,
  * chain=0) of the Philox stream in sequence, so that with the same two
  * seeds the table is the reference's.  Host only (no device, no context).
@@ -388,7 +390,8 @@ int sl_program_export(const sl_pedigree_desc* ped, const sl_peel_desc* peel, int
  * cutset[k], n_prev, prev[n_prev], n_child, children[n_child] (the fields of
  * sl_peel_desc); *needed the length, *cost the order's summed cutset sizes
  * (get_cost, :425-435).  SL_E_CAPACITY when cap is too small (call with
- * buf NULL to size it). */
+ * buf NULL to size it); SL_E_UNSUPPORTED when 200 restarts found no
+ * legitimate order (the reference would keep restarting). */
 /* The reference's internal person order (Pedigree::_reorder_family,
  * pedigree.cc:154-162): order[i] = position in the pedigree file of internal
  * person i.  founder[j] != 0 when person j of the file has no parents.  The
@@ -397,7 +400,8 @@ int sl_program_export(const sl_pedigree_desc* ped, const sl_peel_desc* peel, int
 int sl_person_order(int32_t n_people, const uint8_t* founder, int32_t* order);
 
 int sl_peel_sequence(int32_t n_people, const int32_t* mother, const int32_t* father, int64_t iterations,
-                     uint64_t seed, uint32_t rand_seed, int32_t* buf, int64_t cap, int64_t* needed, int64_t* cost);
+                     uint64_t seed, uint32_t rand_seed, int32_t threads, int32_t* buf, int64_t cap, int64_t* needed,
+                     int64_t* cost);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/swiftlink_amd.h"
@@ -1674,7 +1675,9 @@ int sl_lod_grad(sl_ctx* c, const uint32_t* dev_graphs, int32_t G, double* lod, d
 }
 
 int sl_peel_sequence(int32_t n_people, const int32_t* mother, const int32_t* father, int64_t iterations,
-                     uint64_t seed, uint32_t rand_seed, int32_t* buf, int64_t cap, int64_t* needed, int64_t* cost) {
+                     uint64_t seed, uint32_t rand_seed, int32_t threads, int32_t* buf, int64_t cap, int64_t* needed,
+                     int64_t* cost) {
+    if (threads <= 0) threads = (int32_t)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
     if (n_people <= 0 || !mother || !father || iterations < 0 || (cap > 0 && !buf)) return fail(nullptr, SL_E_INVALID, "bad arguments");
     int founders = 0;
     for (int i = 0; i < n_people; ++i) {
@@ -1689,7 +1692,10 @@ int sl_peel_sequence(int32_t n_people, const int32_t* mother, const int32_t* fat
     }
     if (founders == 0 || founders == n_people) return fail(nullptr, SL_E_INVALID, "no founders or no non-founders");
     std::vector<int32_t> out;
-    const int64_t c = sl::build_peel_sequence(n_people, mother, father, iterations, seed, rand_seed, out);
+    const int64_t c = sl::build_peel_sequence(n_people, mother, father, iterations, seed, rand_seed, threads, out);
+    if (c < 0)
+        return fail(nullptr, SL_E_UNSUPPORTED,
+                    "no legitimate peel order after 200 restarts of the downhill search; use more iterations");
     if (needed) *needed = (int64_t)out.size();
     if (cost) *cost = c;
     if ((int64_t)out.size() > cap) return buf ? fail(nullptr, SL_E_CAPACITY, "buffer too small") : SL_OK;

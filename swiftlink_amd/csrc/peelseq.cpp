@@ -23,7 +23,10 @@
 // multiplicative congruence, output (r[i-3] + r[i-31]) >> 1) so that the
 // process's own rand() state is not touched.
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
+#include <deque>
+#include <thread>
 #include <vector>
 
 #include "peelseq.h"
@@ -150,6 +153,40 @@ int64_t order_cost(const Graph& g, const std::vector<int>& order) {
     return cost;
 }
 
+// The same cost with cutsets as bitsets: the cost only counts cutset
+// members, so their order (which the op table needs) does not matter here.
+// Eliminating x joins its cutset C into every member y's (less y and x).
+class BitCost {
+    int N_, W_;
+    std::vector<uint64_t> init_, work_;
+
+  public:
+    explicit BitCost(const Graph& g) : N_(g.N), W_((g.N + 63) / 64), init_((size_t)g.N * W_, 0) {
+        for (int p = 0; p < N_; ++p)
+            for (int x : g.nodes[p].cut) init_[(size_t)p * W_ + (x >> 6)] |= 1ull << (x & 63);
+    }
+    int64_t operator()(const std::vector<int>& order) {
+        work_ = init_;
+        int64_t cost = 0;
+        for (int x : order) {
+            const uint64_t* C = &work_[(size_t)x * W_];
+            for (int w = 0; w < W_; ++w) cost += __builtin_popcountll(C[w]);
+            for (int w = 0; w < W_; ++w) {
+                uint64_t bits = C[w];
+                while (bits) {
+                    const int y = w * 64 + __builtin_ctzll(bits);
+                    bits &= bits - 1;
+                    uint64_t* Y = &work_[(size_t)y * W_];
+                    for (int v = 0; v < W_; ++v) Y[v] |= C[v];
+                    Y[y >> 6] &= ~(1ull << (y & 63));
+                    Y[x >> 6] &= ~(1ull << (x & 63));
+                }
+            }
+        }
+        return cost;
+    }
+};
+
 bool greedy(const Graph& g, LibcRand& rnd, std::vector<int>& out) {
     std::vector<Node> t(g.nodes);
     out.clear();
@@ -179,34 +216,100 @@ bool greedy(const Graph& g, LibcRand& rnd, std::vector<int>& out) {
     return true;
 }
 
-void downhill(const Graph& g, Stream& st, std::vector<int>& cur, int64_t iterations) {
-    int64_t cost = order_cost(g, cur);
+// random_downhill_search (:361-403): swap two random positions, keep the
+// swap when the cost does not rise.  The swaps drawn do not depend on which
+// were kept, so with several threads the next `threads` candidates are
+// costed side by side against the current order; the first one kept is
+// applied and the candidates after it are costed again against the new
+// order.  Every thread count gives the sequential result.
+void downhill(const Graph& g, Stream& st, std::vector<int>& cur, int64_t iterations, int threads) {
     const int n = (int)cur.size();
-    for (int64_t it = 0; it < iterations; ++it) {
+    std::deque<std::pair<int, int>> cand;  // drawn swaps not yet decided
+    auto draw = [&]() {
         int a, b;
         do {
             a = st.integer(n);
             b = st.integer(n);
         } while (a == b);
-        std::swap(cur[a], cur[b]);
-        const int64_t c = order_cost(g, cur);
-        if (c <= cost) {
-            cost = c;
+        cand.emplace_back(a, b);
+    };
+    if (threads <= 1) {
+        BitCost order_cost(g);
+        int64_t cost = order_cost(cur);
+        for (int64_t it = 0; it < iterations; ++it) {
+            draw();
+            const auto [a, b] = cand.front();
+            cand.pop_front();
+            std::swap(cur[a], cur[b]);
+            const int64_t c = order_cost(cur);
+            if (c <= cost) {
+                cost = c;
+                continue;
+            }
+            std::swap(cur[a], cur[b]);
+        }
+        return;
+    }
+    std::vector<BitCost> evals(threads, BitCost(g));
+    std::vector<std::vector<int>> orders(threads);
+    std::vector<int64_t> costs(threads);
+    int64_t cost = evals[0](cur);
+    // a pool that costs candidate t on thread t, one round at a time
+    std::atomic<int64_t> round{0};
+    std::atomic<int> done{0};
+    std::atomic<bool> stop{false};
+    int live = 0;  // candidates in the current round
+    auto work = [&](int t) {
+        int64_t seen = 0;
+        for (;;) {
+            int64_t r;
+            while ((r = round.load(std::memory_order_acquire)) == seen && !stop.load(std::memory_order_acquire))
+                std::this_thread::yield();
+            if (stop.load(std::memory_order_acquire)) return;
+            seen = r;
+            if (t < live) costs[t] = evals[t](orders[t]);
+            done.fetch_add(1, std::memory_order_acq_rel);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(work, t);
+    int64_t it = 0;
+    while (it < iterations) {
+        live = (int)std::min<int64_t>(threads, iterations - it);
+        while ((int)cand.size() < live) draw();
+        for (int t = 0; t < live; ++t) {
+            orders[t] = cur;
+            std::swap(orders[t][cand[t].first], orders[t][cand[t].second]);
+        }
+        done.store(0, std::memory_order_release);
+        round.fetch_add(1, std::memory_order_acq_rel);
+        costs[0] = evals[0](orders[0]);
+        while (done.load(std::memory_order_acquire) < threads - 1) std::this_thread::yield();
+        int kept = -1;
+        for (int t = 0; t < live && kept < 0; ++t)
+            if (costs[t] <= cost) kept = t;
+        if (kept < 0) {
+            it += live;
+            cand.erase(cand.begin(), cand.begin() + live);
             continue;
         }
-        std::swap(cur[a], cur[b]);
+        std::swap(cur[cand[kept].first], cur[cand[kept].second]);
+        cost = costs[kept];
+        it += kept + 1;
+        cand.erase(cand.begin(), cand.begin() + kept + 1);
     }
+    stop.store(true, std::memory_order_release);
+    for (auto& th : pool) th.join();
 }
 
 // is_legit (:480-508): no node may qualify as both a CHILD and a PARENT peel
+// (the test reads only which persons are peeled, not the cutsets)
 bool legit(const Graph& g, const std::vector<int>& order) {
-    std::vector<Node> t(g.nodes);
     std::vector<char> peeled(g.N, 0);
     for (int p : order) {
         if (!g.founder(p) && !(peeled[g.mother[p]] || peeled[g.father[p]]) && !g.children[p].empty() &&
             !g.all_peeled(g.mates[p], peeled) && !g.all_peeled(g.children[p], peeled))
             return false;
-        eliminate(t, p);
         peeled[p] ^= 1;
     }
     return true;
@@ -287,22 +390,27 @@ void person_order(int n, const uint8_t* founder, int32_t* order) {
     std::copy(v.begin(), v.end(), order);
 }
 
+constexpr int kMaxRestarts = 200;
+
 // build_peel_sequence (:273-306)
 int64_t build_peel_sequence(int N, const int32_t* mother, const int32_t* father, int64_t iterations, uint64_t seed,
-                            uint32_t rand_seed, std::vector<int32_t>& out) {
+                            uint32_t rand_seed, int threads, std::vector<int32_t>& out) {
     Graph g(N, mother, father);
     LibcRand rnd(rand_seed);
     Stream st(seed);
     std::vector<int> cur;
     if (!(greedy(g, rnd, cur) && legit(g, cur))) {
-        for (;;) {
+        // the reference restarts until an order passes; a bound keeps a search
+        // too short for the pedigree from running forever
+        for (int restart = 0;; ++restart) {
+            if (restart == kMaxRestarts) return -1;
             cur.resize(N);
             for (int i = 0; i < N; ++i) cur[i] = i;
             for (int i = 1; i < N; ++i) {
                 const int j = rnd() % (i + 1);
                 if (i != j) std::swap(cur[i], cur[j]);
             }
-            downhill(g, st, cur, iterations);
+            downhill(g, st, cur, iterations, threads);
             if (legit(g, cur)) break;
         }
     }

@@ -97,8 +97,8 @@ _SIGS = {
                                            C.POINTER(C.c_int32)]),
     "sl_elod": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int64, f64p]),
     "sl_person_order": (C.c_int, [C.c_int32, C.POINTER(C.c_uint8), i32p]),
-    "sl_peel_sequence": (C.c_int, [C.c_int32, i32p, i32p, C.c_int64, C.c_uint64, C.c_uint32, i32p, C.c_int64, i64p,
-                                   i64p]),
+    "sl_peel_sequence": (C.c_int, [C.c_int32, i32p, i32p, C.c_int64, C.c_uint64, C.c_uint32, C.c_int32, i32p,
+                                   C.c_int64, i64p, i64p]),
     "sl_lsweep": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_int32]),
     "sl_msweep": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, i32p, C.c_int32]),
     "sl_mcmc_run": (C.c_int, [C.c_void_p, C.POINTER(Schedule), C.POINTER(McmcStats)]),
@@ -389,20 +389,21 @@ def person_order(founder) -> np.ndarray:
     return out
 
 
-def peel_sequence(mother, father, iterations: int = 1000000, seed: int = 12345, rand_seed: int = 12345):
+def peel_sequence(mother, father, iterations: int = 1000000, seed: int = 12345, rand_seed: int = 12345,
+                  threads: int = 0):
     """sl_peel_sequence: the reference's peel-sequence search on the host.
     Returns (PeelOps, cost)."""
     from .case import PeelOps
     m = np.ascontiguousarray(mother, np.int32)
     f = np.ascontiguousarray(father, np.int32)
     need, cost = C.c_int64(0), C.c_int64(0)
-    rc = lib().sl_peel_sequence(len(m), _ptr(m, i32p), _ptr(f, i32p), iterations, seed, rand_seed, None, 0,
-                                C.byref(need), C.byref(cost))
-    if rc:
-        raise SlError(f"sl_peel_sequence failed ({rc}): {lib().sl_last_error(None).decode()}")
-    buf = np.zeros(max(need.value, 1), np.int32)
-    rc = lib().sl_peel_sequence(len(m), _ptr(m, i32p), _ptr(f, i32p), iterations, seed, rand_seed, _ptr(buf, i32p),
-                                len(buf), C.byref(need), C.byref(cost))
+    buf = np.zeros(1 + len(m) * (5 + 3 * 16), np.int32)  # enough unless cutsets exceed 16
+    rc = lib().sl_peel_sequence(len(m), _ptr(m, i32p), _ptr(f, i32p), iterations, seed, rand_seed, threads,
+                                _ptr(buf, i32p), len(buf), C.byref(need), C.byref(cost))
+    if rc == SL_E_CAPACITY:  # search again into a buffer of the size it reported
+        buf = np.zeros(need.value, np.int32)
+        rc = lib().sl_peel_sequence(len(m), _ptr(m, i32p), _ptr(f, i32p), iterations, seed, rand_seed, threads,
+                                    _ptr(buf, i32p), len(buf), C.byref(need), C.byref(cost))
     if rc:
         raise SlError(f"sl_peel_sequence failed ({rc}): {lib().sl_last_error(None).decode()}")
     n, pos = int(buf[0]), 1
