@@ -166,3 +166,17 @@ def test_peel_sequence_rejects_bad_pedigrees():
         native.peel_sequence(np.array([-1, 0, -1]), np.array([-1, 2, -1]), 10)  # founders not first
     with pytest.raises(native.SlError):
         native.peel_sequence(np.array([-1, -1, 0]), np.array([-1, -1, 5]), 10)  # parent out of range
+
+
+def test_peel_sequence_threads_give_the_sequential_result():
+    """Candidate swaps costed side by side on several threads: the table and
+    cost equal the one-thread search, and the reference's."""
+    from swiftlink_amd import native
+    from swiftlink_amd.case import PeelOps
+    import dataclasses as dc
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "peelseq_ped150.npz"))
+    iters, seed = (int(x) for x in z["params"])
+    ref = PeelOps(**{f.name: z["op_" + f.name] for f in dc.fields(PeelOps)})
+    one, c1 = native.peel_sequence(z["mother"], z["father"], iters, seed, seed, threads=1)
+    four, c4 = native.peel_sequence(z["mother"], z["father"], iters, seed, seed, threads=4)
+    assert c1 == c4 and _ops_equal(one, four) and _ops_equal(four, ref)
