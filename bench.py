@@ -409,7 +409,7 @@ def main():
         pm = np.array([-1 if rows[i][3] == "0" else pos[rows[i][3]] for i in perm], np.int32)
         pf = np.array([-1 if rows[i][2] == "0" else pos[rows[i][2]] for i in perm], np.int32)
         t0 = time.perf_counter()
-        _, pcost = native.peel_sequence(pm, pf, iters, 7, 7)
+        _, pcost = native.peel_sequence(pm, pf, iters, 7, 7, threads=1)
         pdt = time.perf_counter() - t0
         res["peel_sequence"] = {"workload": f"{args.config} pedigree, {iters} downhill swaps per restart, seeds 7",
                                 "seconds": pdt, "cost": pcost, "cores": 1}
