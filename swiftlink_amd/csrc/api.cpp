@@ -574,6 +574,7 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
             c->samp_lds = sl::lsampler_lds_bytes(c->precision, c->cells_cap, c->sprog.n_ops, c->N, c->F, c->samp_block);
             if (c->samp_lds > 160 * 1024) c->samp_why = "sampler tables need more than 160 KiB of LDS at some locus";
             if (c->cells_cap >= 65536) c->samp_why = "sampler tables exceed the 16-bit offsets of the op records";
+            if (c->N > 65535) c->samp_why = "more than 65535 people for the sampler's 16-bit person records";
         }
         if (c->samp_why.empty()) {
             std::vector<uint8_t> lt((size_t)c->M * c->N);
