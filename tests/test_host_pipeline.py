@@ -180,3 +180,19 @@ def test_peel_sequence_threads_give_the_sequential_result():
     one, c1 = native.peel_sequence(z["mother"], z["father"], iters, seed, seed, threads=1)
     four, c4 = native.peel_sequence(z["mother"], z["father"], iters, seed, seed, threads=4)
     assert c1 == c4 and _ops_equal(one, four) and _ops_equal(four, ref)
+
+
+@pytest.mark.parametrize("ex", ["loop", "east"])
+def test_examples_with_reference_peel_search_match_reference(ex):
+    """The reference's example pedigrees read with the reference's person
+    order and peel search: the same persons in the same order and the same
+    peel table as the reference built."""
+    src = os.path.join("/root/reference/examples", ex)
+    if not os.path.exists(src + ".ped"):
+        pytest.skip("reference examples not mounted (the golden fixtures still pin the search)")
+    ref, fx = load_golden(ex)
+    mine = linkage.load(src + ".ped", src + ".map", src + ".dat", peel="reference", peel_iterations=20000,
+                        peel_seed=12345)
+    assert mine.ids == ref.ids
+    _compare(mine, ref)
+    assert _ops_equal(mine.ops, ref.ops)
