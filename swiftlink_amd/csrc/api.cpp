@@ -115,6 +115,8 @@ struct sl_ctx {
     sl::ChildDesc* d_schildren = nullptr;
     int32_t* d_ssched_off = nullptr;
     int32_t* d_ssched_ops = nullptr;
+    int32_t* d_ssched_off_w = nullptr;  // the schedule over a wide workgroup's wavefronts
+    int32_t* d_ssched_ops_w = nullptr;
     int32_t* d_rlevel_off = nullptr;
     int32_t* d_rlevel_ops = nullptr;
     int32_t* d_mother = nullptr;
@@ -286,7 +288,8 @@ int check_desc(const sl_pedigree_desc* p, const sl_map_desc* m, const sl_peel_de
 }
 
 int release_all(sl_ctx* c) {
-    void* ptrs[] = {c->d_sops, c->d_sprevs, c->d_schildren, c->d_ssched_off, c->d_ssched_ops, c->d_rlevel_off,
+    void* ptrs[] = {c->d_sops, c->d_sprevs, c->d_schildren, c->d_ssched_off, c->d_ssched_ops, c->d_ssched_off_w,
+                    c->d_ssched_ops_w, c->d_rlevel_off,
                     c->d_rlevel_ops, c->d_mother, c->d_father, c->d_legal_t, c->d_gprob_d, c->d_gprob_f,
                     c->d_theta, c->d_chains, c->d_status,
                     c->d_gen_off, c->d_ptab, c->d_geno2, c->d_minor, c->d_meioses, c->d_order,
@@ -571,7 +574,8 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
         c->samp_why = sl::compile_sampler_program(pv, kv, ped->legal, c->M, c->samp_block / 64, &c->sprog, &max_cells);
         if (c->samp_why.empty()) {
             c->cells_cap = (int)((max_cells + 7) / 8 * 8);
-            c->samp_lds = sl::lsampler_lds_bytes(c->precision, c->cells_cap, c->sprog.n_ops, c->N, c->F, c->samp_block);
+            c->samp_lds = sl::lsampler_lds_bytes(c->precision, c->cells_cap, c->sprog.n_ops, c->N, c->F,
+                                                 sl::kLsamplerBlockWide);
             if (c->samp_lds > 160 * 1024) c->samp_why = "sampler tables need more than 160 KiB of LDS at some locus";
             if (c->cells_cap >= 65536) c->samp_why = "sampler tables exceed the 16-bit offsets of the op records";
             if (c->N > 65535) c->samp_why = "more than 65535 people for the sampler's 16-bit person records";
@@ -587,6 +591,8 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
                 (rc = upload(c, &c->d_schildren, c->sprog.children.data(), c->sprog.children.size())) ||
                 (rc = upload(c, &c->d_ssched_off, c->sprog.sched_off.data(), c->sprog.sched_off.size())) ||
                 (rc = upload(c, &c->d_ssched_ops, c->sprog.sched_ops.data(), c->sprog.sched_ops.size())) ||
+                (rc = upload(c, &c->d_ssched_off_w, c->sprog.sched_off_wide.data(), c->sprog.sched_off_wide.size())) ||
+                (rc = upload(c, &c->d_ssched_ops_w, c->sprog.sched_ops_wide.data(), c->sprog.sched_ops_wide.size())) ||
                 (rc = upload(c, &c->d_rlevel_off, c->sprog.rlevel_off.data(), c->sprog.rlevel_off.size())) ||
                 (rc = upload(c, &c->d_rlevel_ops, c->sprog.rlevel_ops.data(), c->sprog.rlevel_ops.size())) ||
                 (rc = upload(c, &c->d_legal_t, lt.data(), lt.size())) ||
@@ -895,6 +901,21 @@ static sl::SampArgs samp_args(sl_ctx* c) {
     return A;
 }
 
+// Workgroup shape of an L-sampler launch of `groups` workgroups: wide (and
+// its schedule) when the launch is too small to fill the device.
+static int lsampler_shape(sl_ctx* c, int64_t groups, sl::SampArgs& A) {
+    if (groups <= sl::kLsamplerWideGroups) {
+        A.sched_off = c->d_ssched_off_w;
+        A.sched_ops = c->d_ssched_ops_w;
+        A.waves = 2 * c->sprog.waves;
+        return sl::kLsamplerBlockWide;
+    }
+    A.sched_off = c->d_ssched_off;
+    A.sched_ops = c->d_ssched_ops;
+    A.waves = c->sprog.waves;
+    return c->samp_block;
+}
+
 // Both parity launches of one L-sweep.  check_kind: every chain takes its
 // kind and first parity from its schedule draws (lprob), first_parity unused.
 static int run_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int first_parity, int check_kind, double lprob,
@@ -905,6 +926,7 @@ static int run_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int first_parity
     A.sweep_dev = sweep_dev;
     A.check_kind = check_kind;
     A.lprob = lprob;
+    const int block = lsampler_shape(c, (int64_t)((c->M + 1) / 2) * c->n_chains, A);
     for (int pass = 0; pass < 2; ++pass) {
         A.parity = pass == 0 ? first_parity : 1 - first_parity;
         A.pass = pass;
@@ -914,7 +936,7 @@ static int run_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int first_parity
             e1 = take_event(c);
             if (e0 && e1) SL_HIP(c, hipEventRecord(e0, c->stream));
         }
-        SL_HIP(c, sl::launch_lsampler(A, c->precision, c->samp_block, c->samp_lds, c->n_chains, c->stream));
+        SL_HIP(c, sl::launch_lsampler(A, c->precision, block, c->samp_lds, c->n_chains, c->stream));
         if (c->prof && e0 && e1) {
             SL_HIP(c, hipEventRecord(e1, c->stream));
             c->ev_used.push_back({SL_KERNEL_LSAMPLER, e0, e1, 1});
@@ -950,6 +972,7 @@ static int run_impute(sl_ctx* c, uint64_t seed, int R, int mode, int start, uint
     A.chain_base = chain_base;
     A.imp_draw = c->d_imp_draw;
     A.imp_weight = c->d_imp_weight;
+    const int block = lsampler_shape(c, R, A);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->prof) {
         e0 = take_event(c);
@@ -958,7 +981,7 @@ static int run_impute(sl_ctx* c, uint64_t seed, int R, int mode, int start, uint
     }
     for (int t = 0; t < c->M; ++t) {
         A.impute_t = t;
-        SL_HIP(c, sl::launch_lsampler(A, c->precision, c->samp_block, c->samp_lds, R, c->stream));
+        SL_HIP(c, sl::launch_lsampler(A, c->precision, block, c->samp_lds, R, c->stream));
     }
     if (c->prof && e0 && e1) {
         SL_HIP(c, hipEventRecord(e1, c->stream));

@@ -385,7 +385,7 @@ __device__ __forceinline__ void sampler_op(const OpEnv<T>& E) {
 }
 
 template <typename T, bool RESCALE>
-__global__ void __launch_bounds__(kLsamplerBlock) lsampler_kernel(SampArgs A) {
+__global__ void __launch_bounds__(kLsamplerBlockWide) lsampler_kernel(SampArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int N = A.N, F = A.F, M = A.M, n_ops = A.n_ops;
     const int NF = N - F;

@@ -112,24 +112,29 @@ std::string compile_sampler_program(const PedigreeView& ped, const PeelView& pv,
     P.n_levels = *std::max_element(level.begin(), level.end()) + 1;
     std::vector<std::vector<int>> by_level(P.n_levels);
     for (int o = 0; o < n; ++o) by_level[level[o]].push_back(o);
-    P.sched_off.assign((size_t)P.n_levels * waves + 1, 0);
-    for (int L = 0; L < P.n_levels; ++L) {
-        std::vector<std::pair<double, int>> cost;
-        for (int o : by_level[L]) cost.push_back({op_cost(P.ops[o]), o});
-        std::stable_sort(cost.begin(), cost.end(), [](auto& a, auto& b) { return a.first > b.first; });
-        std::vector<double> load(waves, 0.0);
-        std::vector<std::vector<int>> per(waves);
-        for (auto& c : cost) {
-            int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            load[w] += c.first;
-            per[w].push_back(c.second);
+    auto schedule = [&](int nw, std::vector<int32_t>& off, std::vector<int32_t>& ops) {
+        off.assign((size_t)P.n_levels * nw + 1, 0);
+        ops.clear();
+        for (int L = 0; L < P.n_levels; ++L) {
+            std::vector<std::pair<double, int>> cost;
+            for (int o : by_level[L]) cost.push_back({op_cost(P.ops[o]), o});
+            std::stable_sort(cost.begin(), cost.end(), [](auto& a, auto& b) { return a.first > b.first; });
+            std::vector<double> load(nw, 0.0);
+            std::vector<std::vector<int>> per(nw);
+            for (auto& c : cost) {
+                int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                load[w] += c.first;
+                per[w].push_back(c.second);
+            }
+            for (int w = 0; w < nw; ++w) {
+                off[(size_t)L * nw + w] = (int)ops.size();
+                ops.insert(ops.end(), per[w].begin(), per[w].end());
+            }
         }
-        for (int w = 0; w < waves; ++w) {
-            P.sched_off[(size_t)L * waves + w] = (int)P.sched_ops.size();
-            P.sched_ops.insert(P.sched_ops.end(), per[w].begin(), per[w].end());
-        }
-    }
-    P.sched_off[(size_t)P.n_levels * waves] = (int)P.sched_ops.size();
+        off[(size_t)P.n_levels * nw] = (int)ops.size();
+    };
+    schedule(waves, P.sched_off, P.sched_ops);
+    schedule(2 * waves, P.sched_off_wide, P.sched_ops_wide);
 
     // reverse levels: an op can sample once every cutset member has been
     // sampled by the (later) op that peels it

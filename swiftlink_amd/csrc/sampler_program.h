@@ -61,6 +61,8 @@ struct SampProgram {
     std::vector<ChildDesc> children;
     std::vector<int32_t> sched_off;    // [n_levels*waves+1] forward schedule
     std::vector<int32_t> sched_ops;
+    std::vector<int32_t> sched_off_wide;  // the same over 2*waves wavefronts
+    std::vector<int32_t> sched_ops_wide;
     std::vector<int32_t> rlevel_off;   // [n_rlevels+1] reverse-sampling levels
     std::vector<int32_t> rlevel_ops;
     int max_ops_per_rlevel = 0;
