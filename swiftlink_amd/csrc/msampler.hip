@@ -582,13 +582,17 @@ __device__ __forceinline__ Mat emission_matrix(const MsampArgs& A, double e0, do
     return Mat{an * e0, th * e1, th * e0, an * e1};
 }
 
-// e0, e1 of locus l from the current and flipped columns' log-likelihoods
-// (the current column's is finite; exp(-inf) = 0 for an illegal flip)
+// e0, e1 of locus l from the current and flipped columns' log-likelihoods,
+// exp(x - max): exactly 1 for the larger state and exp(-|x0 - x1|) for the
+// other (x1 - x0 is exactly -(x0 - x1)), so one exp per locus.  The current
+// column's is finite; exp(-inf) = 0 for an illegal flip.
 __device__ __forceinline__ void emissions(bool cur, double lcur, double lflip, double* e0, double* e1) {
     const double x0 = cur ? lflip : lcur, x1 = cur ? lcur : lflip;
-    const double mx = fmax(x0, x1);
-    *e0 = exp(x0 - mx);
-    *e1 = exp(x1 - mx);
+    const double d = x0 - x1;
+    const bool first = d >= 0.0;
+    const double t = exp(first ? -d : d);
+    *e0 = first ? 1.0 : t;
+    *e1 = first ? t : 1.0;
 }
 
 // 1 when locus l draws a uniform: both weighted states non-zero (:193-203)
