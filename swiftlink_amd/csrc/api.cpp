@@ -902,9 +902,12 @@ static sl::SampArgs samp_args(sl_ctx* c) {
 }
 
 // Workgroup shape of an L-sampler launch of `groups` workgroups: wide (and
-// its schedule) when the launch is too small to fill the device.
+// its schedule) when the launch is too small to fill the device, and always
+// in FP64, whose larger tables already limit how many workgroups share a CU
+// (measured: 64 chains at ped150 take 10.1 ms per FP64 L-sweep wide against
+// 11.9 ms narrow; FP32 is faster narrow there).
 static int lsampler_shape(sl_ctx* c, int64_t groups, sl::SampArgs& A) {
-    if (groups <= sl::kLsamplerWideGroups) {
+    if (groups <= sl::kLsamplerWideGroups || c->precision == SL_FP64) {
         A.sched_off = c->d_ssched_off_w;
         A.sched_ops = c->d_ssched_ops_w;
         A.waves = 2 * c->sprog.waves;
