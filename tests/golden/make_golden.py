@@ -25,6 +25,8 @@ Each fixture holds:
               model, replicate j under (seed 31, j, 0, step 0xFFFFFFFC)): the
               ELOD, accumulator, P(T), the first 6 replicates' graphs, both
               maps' tables and the peel sequence it built
+  elod2_*     the same under penetrance 0.01:0.5:0.9, frequency 0.05, markers 0.1 M
+              apart (300 replicates, seed 41)
 """
 from __future__ import annotations
 
@@ -68,13 +70,14 @@ def make(name, base, work):
     morder = np.random.default_rng(3).permutation(ml).astype(np.int32)
     morder.tofile(os.path.join(work, "morder.i32"))
     gi = os.path.join(work, "graph_init.i32")
-    for sub in ("ls", "ms", "ch", "si", "si0", "el"):
+    for sub in ("ls", "ms", "ch", "si", "si0", "el", "el2"):
         os.makedirs(os.path.join(work, sub), exist_ok=True)
     run("lsweep", p, m, d, os.path.join(work, "ls"), gi, os.path.join(work, "lorder.i32"), 99, 0, 5)
     run("msweep", p, m, d, os.path.join(work, "ms"), gi, os.path.join(work, "morder.i32"), 99, 0, 6)
     run("chain", p, m, d, os.path.join(work, "ch"), gi, 77, 0, 40, 10, 3, 0.5)
     run("writer", p, m, d, work, os.path.join(work, "graphs.i32"), G)
     run("elod", p, m, d, os.path.join(work, "el"), 0.0001, 0, 0, 1, 0.05, 400, 31, 6)
+    run("elod", p, m, d, os.path.join(work, "el2"), 0.05, 0.01, 0.5, 0.9, 0.1, 300, 41, 0)
     run("impute", p, m, d, os.path.join(work, "si"), 55, 4)
     run("impute", p, m, d, os.path.join(work, "si0"), 56, 0)
     rd = lambda f, dt: np.fromfile(os.path.join(work, f), dt)
@@ -120,6 +123,14 @@ def make(name, base, work):
         elod_theta3=np.array(em["theta3"]), elod_theta2=np.array([em["theta2"]]),
         elod_partial2=np.array([em["partial2"]]), elod_ids=np.array(em["ids"]),
         **{"elod_op_" + f.name: getattr(eops, f.name) for f in dataclasses.fields(PeelOps)},
+    )
+    em2 = json.load(open(os.path.join(work, "el2", "elod_meta.json")))
+    fx.update(
+        elod2_params=np.array([0.05, 0.01, 0.5, 0.9, 0.1]), elod2_run=np.array([300, 41]),
+        elod2_value=np.array([em2["elod"]]), elod2_acc=np.array([em2["acc"]]),
+        elod2_trait_prob=np.array([em2["trait_prob"]]),
+        elod2_disease_prob=rd("el2/elod_disease_prob.f64", np.float64).reshape(N, 4),
+        elod2_theta3=np.array(em2["theta3"]), elod2_partial2=np.array([em2["partial2"]]),
     )
     path = os.path.join(OUT, name + ".npz")
     np.savez_compressed(path, **fx)

@@ -39,6 +39,21 @@ def test_elod_matches_reference(name):
     assert value == lod[0]
 
 
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_elod_other_disease_model_matches_reference(name):
+    """Penetrance 0.01:0.5:0.9, frequency 0.05, markers 0.1 M apart."""
+    case, fx = load_golden(name)
+    freq, p0, p1, p2, sep = (float(x) for x in fx["elod2_params"])
+    R, seed = (int(x) for x in fx["elod2_run"])
+    c3, c2 = elod.elod_cases(case, freq, (p0, p1, p2), sep)
+    with native.Context(c3, native.SL_FP64) as s, native.Context(c2, native.SL_FP64) as sc:
+        value = native.run_elod(s, sc, seed, R)
+        acc, count, _ = sc.lod_results()
+    assert count == R
+    assert abs(acc[0] - fx["elod2_acc"][0]) <= 1e-12 * max(1.0, abs(fx["elod2_acc"][0]))
+    assert abs(value - fx["elod2_value"][0]) <= 1e-12 * max(1.0, abs(fx["elod2_value"][0]))
+
+
 def test_elod_total_over_pedigrees():
     """elod.elod sums the pedigrees' ELODs (elod.cc:72-83)."""
     cases = [load_golden(n) for n in ("loop", "east")]

@@ -119,6 +119,28 @@ def test_elod_oracle_matches_reference(golden):
     assert abs(value - float(fx["elod_value"][0])) <= 1e-12 * max(1.0, abs(value))
 
 
+def test_elod_oracle_other_disease_model(golden):
+    """A second disease model (phenocopies, incomplete penetrance, a common
+    allele, markers 0.1 M apart): the disease probabilities, the three-locus
+    thetas, P(T), the accumulator and the ELOD all follow the reference's."""
+    from swiftlink_amd import elod
+    case, fx = golden
+    freq, p0, p1, p2, sep = (float(x) for x in fx["elod2_params"])
+    R, seed = (int(x) for x in fx["elod2_run"])
+    c3, c2 = elod.elod_cases(case, freq, (p0, p1, p2), sep)
+    assert np.array_equal(c3.disease_prob, fx["elod2_disease_prob"])
+    assert np.array_equal(c3.theta, fx["elod2_theta3"])
+    assert np.array_equal(c2.partial_theta, fx["elod2_partial2"])
+    o3, o2 = Oracle(c3), Oracle(c2)
+    graphs = np.stack([o3.impute_from(seed, j, 1)[0] for j in range(R)])
+    tp = o2.trait_prob()
+    assert tp == float(fx["elod2_trait_prob"][0])
+    acc = o2.lod_accumulate(graphs[:, [0, 2]])
+    assert abs(acc[0] - float(fx["elod2_acc"][0])) <= 1e-12 * max(1.0, abs(acc[0]))
+    value = (acc[0] - np.log(R) - tp) / np.log(10.0)
+    assert abs(value - float(fx["elod2_value"][0])) <= 1e-12 * max(1.0, abs(value))
+
+
 def _ops_equal(a, b):
     import dataclasses as dc
     from swiftlink_amd.case import PeelOps
