@@ -98,7 +98,8 @@ typedef struct {
 
 /* Replaces GPULodscores::GPULodscores + gpu_init (gpu_lodscores.h:63-80,
  * gpu_lodscores.cc:119-161): validates and copies the pedigree, map and peel
- * table, compiles the level-scheduled peel program and uploads it. */
+ * table, compiles the level-scheduled peel program and uploads it.
+ * A pedigree of founders only (no meioses) gets SL_E_UNSUPPORTED. */
 int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel_desc* peel,
               const sl_options* opts, sl_ctx** out);
 

@@ -262,6 +262,8 @@ int check_desc(const sl_pedigree_desc* p, const sl_map_desc* m, const sl_peel_de
     if (!p || !m || !k) return fail(nullptr, SL_E_INVALID, "null descriptor");
     if (p->n_people < 1 || p->n_founders < 1 || p->n_founders > p->n_people)
         return fail(nullptr, SL_E_INVALID, "bad person or founder count");
+    if (p->n_founders == p->n_people)
+        return fail(nullptr, SL_E_UNSUPPORTED, "pedigree has no non-founders: no meioses to sample or score");
     if (p->n_markers < 2) return fail(nullptr, SL_E_INVALID, "need at least two markers");
     if (!p->mother || !p->father || !p->typed || !p->genotypes || !p->disease_prob || !p->geno_prob || !p->legal)
         return fail(nullptr, SL_E_INVALID, "null pedigree array");

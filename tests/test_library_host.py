@@ -66,6 +66,19 @@ def test_program_rejects_bad_tables():
             native.program_export(case)
 
 
+def test_create_rejects_founder_only_pedigree():
+    """A pedigree of founders only has no meioses; sl_create refuses it with
+    SL_E_UNSUPPORTED before touching a device."""
+    import dataclasses
+    case, fx = load_golden("loop")
+    F = case.F
+    cut = {k: getattr(case, k)[:F] for k in ("mother", "father", "sex", "affection", "disease_prob", "ids", "typed",
+                                              "geno", "geno_prob", "legal")}
+    sub = dataclasses.replace(case, N=F, **cut)
+    with pytest.raises(native.SlError, match="no non-founders"):
+        native.Context(sub)
+
+
 def test_pack_graphs_layout():
     from swiftlink_amd.case import pack_graphs
     case, fx = load_golden("east")
