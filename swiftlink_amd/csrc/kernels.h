@@ -61,6 +61,9 @@ hipError_t launch_grad(const LodArgs& A, const GradArgs& Gd, int nodg, size_t ld
 // then the e^(t - state)-weighted sums of the derivatives)
 hipError_t launch_grad_fold(double* state, const double* terms, int G, int n, hipStream_t s);
 hipError_t launch_trait_prob(const LodArgs& A, int precision, int block, size_t lds, double* out, hipStream_t s);
+// log-sums nchunks partial terms [nchunks][n] into acc [n]: a thread per
+// accumulator below kMergeTreeChunks chunks, a workgroup per accumulator from there
+constexpr int kMergeTreeChunks = 256;
 hipError_t launch_lod_merge(double* acc, const double* partial, int nchunks, int n, hipStream_t s,
                             const uint32_t* gate_sweep = nullptr, int gate_burnin = 0, int gate_period = 1);
 // one thread: (*counter)++ (the device sweep index of captured sweeps)

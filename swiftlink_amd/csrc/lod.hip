@@ -656,6 +656,29 @@ __global__ void lod_merge_kernel(double* acc, const double* partial, int nchunks
     acc[i] = a;
 }
 
+// The same merge for many chunks (ELOD's replicates at one position): one
+// workgroup per accumulator; thread t log-sums chunks t, t + kMergeBlock, ...
+// in order, then a fixed pairwise tree over the threads' sums.  No atomics, so
+// the result does not depend on scheduling.
+constexpr int kMergeBlock = 256;
+__global__ void __launch_bounds__(kMergeBlock) lod_merge_tree_kernel(double* acc, const double* partial, int nchunks,
+                                                                     int n, const uint32_t* gate_sweep,
+                                                                     int gate_burnin, int gate_period) {
+    __shared__ double s[kMergeBlock];
+    if (gate_sweep && !gate_open(*gate_sweep, gate_burnin, gate_period)) return;  // uniform over the workgroup
+    const int i = blockIdx.x;
+    const int t = threadIdx.x;
+    double a = -INFINITY;
+    for (int c = t; c < nchunks; c += kMergeBlock) a = lse(a, partial[(size_t)c * n + i]);
+    s[t] = a;
+    __syncthreads();
+    for (int h = kMergeBlock / 2; h > 0; h >>= 1) {
+        if (t < h) s[t] = lse(s[t], s[t + h]);
+        __syncthreads();
+    }
+    if (t == 0) acc[i] = lse(acc[i], s[0]);
+}
+
 __global__ void advance_kernel(uint32_t* counter) { counter[0] += 1u; }
 
 __global__ void fill_kernel(double* p, double v, int n) {
@@ -743,8 +766,13 @@ hipError_t launch_trait_prob(const LodArgs& A, int precision, int block, size_t 
 
 hipError_t launch_lod_merge(double* acc, const double* partial, int nchunks, int n, hipStream_t s,
                             const uint32_t* gate_sweep, int gate_burnin, int gate_period) {
-    hipLaunchKernelGGL(lod_merge_kernel, dim3((n + 255) / 256), dim3(256), 0, s, acc, partial, nchunks, n, gate_sweep,
-                       gate_burnin, gate_period);
+    // a thread per accumulator while each has few chunks to fold in sequence
+    if (nchunks < kMergeTreeChunks)
+        hipLaunchKernelGGL(lod_merge_kernel, dim3((n + 255) / 256), dim3(256), 0, s, acc, partial, nchunks, n,
+                           gate_sweep, gate_burnin, gate_period);
+    else
+        hipLaunchKernelGGL(lod_merge_tree_kernel, dim3(n), dim3(kMergeBlock), 0, s, acc, partial, nchunks, n,
+                           gate_sweep, gate_burnin, gate_period);
     return hipGetLastError();
 }
 

@@ -77,6 +77,29 @@ def test_reset_and_empty():
     np.testing.assert_allclose(acc, fx["lod_terms"][0].reshape(-1), rtol=1e-12)
 
 
+def test_merge_of_many_graphs_matches_small_calls():
+    """600 graphs in one call fold through the workgroup-per-accumulator
+    merge (256 chunks or more); the same graphs in calls of 100 through the
+    thread-per-accumulator merge.  Both are log-sums of the same terms, so
+    they agree to rounding, and with the sum of the oracle's per-graph terms."""
+    case, fx = load_golden("east")
+    reps = np.concatenate([fx["graphs"]] * (600 // len(fx["graphs"]) + 1))[:600]
+    with native.Context(case, native.SL_FP64) as big, native.Context(case, native.SL_FP64) as small:
+        big.lod_accumulate(reps)
+        for k in range(0, 600, 100):
+            small.lod_accumulate(reps[k:k + 100])
+        acc_b, n_b, _ = big.lod_results()
+        acc_s, n_s, _ = small.lod_results()
+        terms = big.lod_terms(fx["graphs"])
+    assert n_b == n_s == 600
+    np.testing.assert_allclose(acc_b, acc_s, rtol=1e-12, atol=1e-12)
+    per = np.asarray(terms).reshape(len(fx["graphs"]), -1)
+    idx = np.arange(600) % len(fx["graphs"])
+    m = per.max(axis=0)
+    want = m + np.log(np.exp(per[idx] - m).sum(axis=0))
+    np.testing.assert_allclose(acc_b, want, rtol=1e-12, atol=1e-12)
+
+
 @pytest.fixture(scope="module")
 def ped150(tmp_path_factory):
     d = tmp_path_factory.mktemp("ped150")
