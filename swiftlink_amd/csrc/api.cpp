@@ -904,12 +904,16 @@ static sl::SampArgs samp_args(sl_ctx* c) {
 }
 
 // Workgroup shape of an L-sampler launch of `groups` workgroups: wide (and
-// its schedule) when the launch is too small to fill the device, and always
-// in FP64, whose larger tables already limit how many workgroups share a CU
+// its schedule) when the launch is too small to fill the device, always in
+// FP64, whose larger tables already limit how many workgroups share a CU
 // (measured: 64 chains at ped150 take 10.1 ms per FP64 L-sweep wide against
-// 11.9 ms narrow; FP32 is faster narrow there).
+// 11.9 ms narrow; FP32 is faster narrow there, at 3 cells per op), and when
+// the heaviest locus holds at least kLsamplerWideCells cells per op (ELOD's
+// untyped markers at ped150, 17 per op: 262140 replicates in 0.233 s wide
+// against 0.291 s narrow).
 static int lsampler_shape(sl_ctx* c, int64_t groups, sl::SampArgs& A) {
-    if (groups <= sl::kLsamplerWideGroups || c->precision == SL_FP64) {
+    const bool big_tables = (int64_t)c->cells_cap >= (int64_t)sl::kLsamplerWideCells * c->sprog.n_ops;
+    if (groups <= sl::kLsamplerWideGroups || c->precision == SL_FP64 || big_tables) {
         A.sched_off = c->d_ssched_off_w;
         A.sched_ops = c->d_ssched_ops_w;
         A.waves = 2 * c->sprog.waves;

@@ -47,10 +47,13 @@ struct SampArgs {
 
 // L-sampler workgroup: four wavefronts share each peel level; launches of
 // at most kLsamplerWideGroups workgroups (too few to fill the device) use
-// eight, which shortens a locus's peel
+// eight, which shortens a locus's peel, and so do pedigrees whose tables hold
+// at least kLsamplerWideCells cells per op at their heaviest locus (loci of
+// untyped markers), where each op keeps more lanes busy
 constexpr int kLsamplerBlock = 256;
 constexpr int kLsamplerBlockWide = 512;
 constexpr int kLsamplerWideGroups = 768;
+constexpr int kLsamplerWideCells = 8;
 // words of an op's per-locus record in LDS: legal-mask word, table offset |
 // size << 16, then per inline consumed table (words 2+4q..5+4q): offset |
 // peel-digit stride << 16 and the strides of digits 0..5 as 16-bit pairs;
