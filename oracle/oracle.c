@@ -157,8 +157,26 @@ static double trait_rho(const orc_case* c, const int32_t* dg, int locus, const t
     return tmp;
 }
 
-static double trait_peel(const orc_case* c, work_t* w, const int32_t* dg, int locus, const thetas_t* t) {
+/* cells of op o evaluated by the trait peel: every cutset member's value has
+ * a non-zero disease probability */
+static int trait_cell_valid(const orc_case* c, int o, int cell) {
+    const int32_t* cut = c->cut + c->cut_off[o];
+    int k = c->cut_off[o + 1] - c->cut_off[o];
+    for (int q = 0; q < k; ++q)
+        if (c->disease_prob[4 * cut[q] + ((cell >> (2 * q)) & 3)] == 0.0) return 0;
+    return 1;
+}
+
+/* With scale_exp != NULL, each op's table is multiplied by 2^-e after it is
+ * computed, e the binary exponent of its largest cell (frexp), and the e are
+ * summed into *scale_exp.  A power-of-two factor is exact, so wherever the
+ * unscaled peel stays in the normal range the scaled table is the unscaled
+ * one times an exact power of two, cell for cell; the scaled peel also stays
+ * in range for pedigrees whose likelihood underflows FP64 (about 4^-(N-F)). */
+static double trait_peel(const orc_case* c, work_t* w, const int32_t* dg, int locus, const thetas_t* t,
+                         int* scale_exp) {
     const double trait_prob = 0.25;
+    if (scale_exp) *scale_exp = 0;
     for (int o = 0; o < c->n_ops; ++o) {
         const int32_t* cut = c->cut + c->cut_off[o];
         int k = c->cut_off[o + 1] - c->cut_off[o];
@@ -230,6 +248,18 @@ static double trait_peel(const orc_case* c, work_t* w, const int32_t* dg, int lo
             }
             tab[cell] = total;
         }
+        if (scale_exp) {
+            double mx = 0.0;
+            for (int cell = 0; cell < size; ++cell)
+                if (trait_cell_valid(c, o, cell) && tab[cell] > mx) mx = tab[cell];
+            if (mx > 0.0) {
+                int e;
+                (void)frexp(mx, &e);
+                for (int cell = 0; cell < size; ++cell)
+                    if (trait_cell_valid(c, o, cell)) tab[cell] = ldexp(tab[cell], -e);
+                *scale_exp += e;
+            }
+        }
     }
     return w->tab[c->n_ops - 1][0];
 }
@@ -248,7 +278,17 @@ double orc_trait_likelihood(const orc_case* c, const int32_t* dg, int locus, int
     work_init(c, &w, 0);
     thetas_t t = {0.0, 1.0, 0.0, 1.0};
     if (dg) trait_thetas(c, locus, offset, &t);
-    double r = trait_peel(c, &w, dg, locus, &t);
+    double r = trait_peel(c, &w, dg, locus, &t, NULL);
+    work_free(c, &w);
+    return r;
+}
+
+double orc_trait_likelihood_scaled(const orc_case* c, const int32_t* dg, int locus, int offset, int* scale_exp) {
+    work_t w;
+    work_init(c, &w, 0);
+    thetas_t t = {0.0, 1.0, 0.0, 1.0};
+    if (dg) trait_thetas(c, locus, offset, &t);
+    double r = trait_peel(c, &w, dg, locus, &t, scale_exp);
     work_free(c, &w);
     return r;
 }
@@ -407,11 +447,28 @@ static int lod_terms_w(const orc_case* c, work_t* w, const int32_t* dg, int locu
     for (int o = 1; o <= c->nlod; ++o) {
         thetas_t t;
         trait_thetas(c, locus, o, &t);
-        double L = trait_peel(c, w, dg, locus, &t);
+        double L = trait_peel(c, w, dg, locus, &t, NULL);
         if (L <= 0.0) return -1;
         out[o - 1] = log(L) - recomb_log(c, dg, locus) - trans;
     }
     return 0;
+}
+
+int orc_lod_terms_scaled(const orc_case* c, const int32_t* dg, int locus, double* out) {
+    work_t w;
+    work_init(c, &w, 0);
+    double trans = log(0.5) * (2 * (c->N - c->F));
+    int rc = 0;
+    for (int o = 1; o <= c->nlod; ++o) {
+        thetas_t t;
+        int e = 0;
+        trait_thetas(c, locus, o, &t);
+        double L = trait_peel(c, &w, dg, locus, &t, &e);
+        if (L <= 0.0) { rc = -1; break; }
+        out[o - 1] = log(L) + (double)e * log(2.0) - recomb_log(c, dg, locus) - trans;
+    }
+    work_free(c, &w);
+    return rc;
 }
 
 int orc_lod_terms(const orc_case* c, const int32_t* dg, int locus, double* out) {

@@ -60,6 +60,15 @@ double orc_trait_prob(const orc_case* c);
  * `offset` in 1..nlod (trait_rfunction.cc:28-137, rfunction.cc:116-198). */
 double orc_trait_likelihood(const orc_case* c, const int32_t* dg, int locus, int offset);
 
+/* The same peel with every table rescaled by an exact power of two after it
+ * is computed (2^-e, e the frexp exponent of its largest evaluated cell);
+ * returns the rescaled L and the summed exponents in *scale_exp, so that
+ * L = ldexp(result, *scale_exp) wherever the plain peel does not underflow.
+ * For pedigrees whose likelihood underflows FP64 (N-F in the hundreds) it is
+ * the FP64 value with an unbounded exponent.  The reference traps such
+ * underflows (feenableexcept(FE_UNDERFLOW), main.cc:601). */
+double orc_trait_likelihood_scaled(const orc_case* c, const int32_t* dg, int locus, int offset, int* scale_exp);
+
 /* Gradient of the trait peel likelihood (not log) at interval `locus`,
  * sub-position `offset` in 1..nlod, or of P(T) when dg is NULL: out[0] = L,
  * out[1..6] = dL/d(penetrance 0..2, disease allele frequency, theta_L,
@@ -71,6 +80,9 @@ int orc_trait_grad(const orc_case* c, const int32_t* dg, int locus, int offset, 
 /* Peeler::process (peeler.cc:79-103): out[o] = ln L - R_locus - T, o = 0..nlod-1.
  * Returns 0, or -1 when a likelihood is <= 0. */
 int orc_lod_terms(const orc_case* c, const int32_t* dg, int locus, double* out);
+
+/* orc_lod_terms over the rescaled peel: ln L = ln(rescaled L) + e ln 2. */
+int orc_lod_terms_scaled(const orc_case* c, const int32_t* dg, int locus, double* out);
 
 /* log_sum (logarithms.cc:14-25) and LODscores::add over G graphs
  * (lod_score.h:74-80); acc has (M-1)*nlod entries and is initialised here. */

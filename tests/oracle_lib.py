@@ -42,6 +42,9 @@ def _load():
     lib.orc_trait_likelihood.argtypes = [P, i32p, C.c_int, C.c_int]
     lib.orc_trait_likelihood.restype = C.c_double
     lib.orc_lod_terms.argtypes = [P, i32p, C.c_int, f64p]
+    lib.orc_lod_terms_scaled.argtypes = [P, i32p, C.c_int, f64p]
+    lib.orc_trait_likelihood_scaled.argtypes = [P, i32p, C.c_int, C.c_int, C.POINTER(C.c_int)]
+    lib.orc_trait_likelihood_scaled.restype = C.c_double
     lib.orc_trait_grad.argtypes = [P, i32p, C.c_int, C.c_int, f64p, f64p]
     lib.orc_log_sum.argtypes = [C.c_double, C.c_double]
     lib.orc_log_sum.restype = C.c_double
@@ -119,6 +122,24 @@ class Oracle:
         out = np.zeros(self.case.nlod)
         d = np.ascontiguousarray(dg, np.int32)
         rc = lib().orc_lod_terms(self.ref, _p(d, i32p), locus, _p(out, f64p))
+        if rc:
+            raise RuntimeError("zero likelihood")
+        return out
+
+    def trait_likelihood_scaled(self, dg, locus, offset):
+        """(rescaled L, e): the trait peel with power-of-two table scaling,
+        L = rescaled L * 2^e."""
+        d = None if dg is None else np.ascontiguousarray(dg, np.int32)
+        e = C.c_int(0)
+        v = lib().orc_trait_likelihood_scaled(self.ref, None if d is None else _p(d, i32p), locus, offset,
+                                              C.byref(e))
+        return v, e.value
+
+    def lod_terms_scaled(self, dg, locus) -> np.ndarray:
+        """lod_terms through the rescaled peel (no FP64 underflow)."""
+        out = np.zeros(self.case.nlod)
+        d = np.ascontiguousarray(dg, np.int32)
+        rc = lib().orc_lod_terms_scaled(self.ref, _p(d, i32p), locus, _p(out, f64p))
         if rc:
             raise RuntimeError("zero likelihood")
         return out

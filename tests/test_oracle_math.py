@@ -116,6 +116,44 @@ def test_trait_peel_order_invariance():
         np.testing.assert_allclose(o2.lod_terms(fx["graphs"][2], l), o1.lod_terms(fx["graphs"][2], l), rtol=1e-12)
 
 
+@pytest.mark.parametrize("name", ["loop", "east", "ped30_snp100"])
+def test_scaled_trait_peel_is_the_plain_one_times_a_power_of_two(name):
+    """The rescaled peel multiplies tables by exact powers of two only, so
+    where the plain peel stays in range L = ldexp(rescaled L, e) bit for bit,
+    and the rescaled LOD terms equal the plain ones to rounding of the log."""
+    case, fx = load_golden(name)
+    o = Oracle(case)
+    for g in fx["graphs"][:2]:
+        for l in range(0, case.M - 1, max(1, (case.M - 1) // 7)):
+            for off in range(1, case.nlod + 1):
+                v, e = o.trait_likelihood_scaled(g, l, off)
+                assert 0.5 <= v < 1.0 or v == 0.0
+                assert np.ldexp(v, e) == o.trait_likelihood(g, l, off)
+            np.testing.assert_allclose(o.lod_terms_scaled(g, l), o.lod_terms(g, l), rtol=1e-13, atol=0)
+    v, e = o.trait_likelihood_scaled(None, 0, 1)
+    assert np.ldexp(v, e) == o.trait_likelihood(None, 0, 1)
+
+
+def test_scaled_trait_peel_past_fp64_underflow():
+    """Multiplying every disease probability by 2^-40 multiplies L by
+    exactly 2^(-40 N) (every term of L holds one disease probability per
+    person): past N = 26 the plain FP64 peel underflows to 0, the rescaled
+    one gives ln L - 40 N ln 2."""
+    case, fx = load_golden("ped30_snp100")
+    tiny = dataclasses.replace(case, disease_prob=np.ldexp(case.disease_prob, -40))
+    o, ot = Oracle(case), Oracle(tiny)
+    shift = 40 * case.N * np.log(2.0)
+    g = fx["graphs"][0]
+    for l in (0, 37, 98):
+        assert ot.trait_likelihood(g, l, 1) == 0.0
+        with pytest.raises(RuntimeError):
+            ot.lod_terms(g, l)
+        np.testing.assert_allclose(ot.lod_terms_scaled(g, l), o.lod_terms(g, l) - shift, rtol=1e-13, atol=0)
+        v, e = ot.trait_likelihood_scaled(g, l, 2)
+        v0, e0 = o.trait_likelihood_scaled(g, l, 2)
+        assert v == v0 and e == e0 - 40 * case.N
+
+
 def test_lod_accumulator_closed_form():
     case, fx = load_golden("east")
     o = Oracle(case)
