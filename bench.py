@@ -254,6 +254,7 @@ def main():
             c.mcmc_init(np.repeat(synth.true_graph(bdata, bcase)[None], BCH, axis=0))
             tu = autotune(c)
             tunes.append({"people": bcase.N, "snps": bcase.M, "block": tu["block"], "gpw": tu["gpw"],
+                          "lod_kernel": tu["lod_kernel"],
                           "msampler_mode": tu["msampler_mode"]})
             bctx.append(c)
         bsched = (0, PERIOD, LPROB, SEED)

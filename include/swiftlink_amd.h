@@ -79,7 +79,18 @@ typedef struct {
     int32_t block_threads;        /* LOD kernel workgroup size; 0 = the default, 128 */
     int32_t graphs_per_block;     /* graphs folded into one LOD workgroup; 0 = the default, 1 */
     int32_t msampler_mode;        /* SL_MSAMPLER_*: how an M-sweep's steps are launched */
+    int32_t lod_kernel;           /* SL_LOD_*: which trait-peel kernel scores graphs */
 } sl_options;
+
+/* Trait-peel kernels (results agree to rounding; FP64 terms are identical):
+ * SL_LOD_WORKGROUP runs one workgroup per (interval, chunk of
+ * graphs_per_block graphs) whose wavefronts share each level of the peel;
+ * SL_LOD_LANES runs one wavefront per (interval, position, 64 graphs), one
+ * evaluation per lane, when its per-lane tables fit LDS (else the first);
+ * SL_LOD_AUTO takes the second from 32 graphs per launch. */
+#define SL_LOD_AUTO 0
+#define SL_LOD_WORKGROUP 1
+#define SL_LOD_LANES 2
 
 /* M-sweep launch modes (results are identical): one launch per meiosis step,
  * computing the next step's likelihoods for both values of the current
@@ -174,7 +185,9 @@ int sl_lod_work(sl_ctx* ctx, double* flops_per_eval, double* lds_bytes_per_eval)
 
 /* Uploads n_chains descent graphs (host, int32 [C][M][2N]) as the chain
  * states, packed on the device; clears the per-chain status.  Counterpart of
- * GPUMarkovChain's copy_to_gpu (gpu_markov_chain.h:110). */
+ * GPUMarkovChain's copy_to_gpu (gpu_markov_chain.h:110).  SL_E_INVALID when
+ * n_chains exceeds the device's grid y extent (at most 65535): the sampler
+ * kernels index chains by the grid's y coordinate. */
 int sl_mcmc_init(sl_ctx* ctx, int32_t n_chains, const int32_t* graphs);
 
 /* Copies the chain states back in the reference layout int32 [C][M][2N]
@@ -275,10 +288,18 @@ int sl_mcmc_run(sl_ctx* ctx, const sl_schedule* schedule, sl_mcmc_stats* stats);
  * gets status SL_E_ILLEGAL_GRAPH. */
 int sl_mcmc_likelihood(sl_ctx* ctx, double* per_locus);
 
-/* Per chain: draws since sl_mcmc_init whose uniform lay within 1e-12 of the
- * decision threshold (where a reassociated sum can decide differently from
- * the reference's serial one).  No counterpart in the reference. */
+/* Per chain: draws since sl_mcmc_init whose uniform lay within rounding of
+ * their decision threshold, where the device's sums can decide differently
+ * from the reference's serial FP64 ones: M-sampler thresholds (reassociated
+ * forward-backward scan; 1e-12 in FP64, 1e-5 with the FP32 pass) and
+ * L-sampler CDF partial sums of the op draws (1e-12 in FP64, 1e-5 with FP32
+ * tables).  No counterpart in the reference. */
 int sl_mcmc_near_edge(sl_ctx* ctx, int64_t* counts);
+
+/* The loci of the L-sampler's near-edge draws since sl_mcmc_init: bit l of
+ * bits[c * ceil(M/32) + l/32] is set when chain c drew near a CDF edge at
+ * locus l.  Test hook; no counterpart in the reference. */
+int sl_mcmc_near_edge_loci(sl_ctx* ctx, uint32_t* bits);
 
 /* Changes the launch shapes of a context (block_threads, graphs_per_block,
  * msampler_mode) without touching its chains or accumulators; precision and

@@ -28,6 +28,8 @@ thread_local std::string g_create_error;
 constexpr int64_t kSpecWaves = 8192;
 // graphs per adjoint launch (its grid's z extent)
 constexpr int kMaxGridZ = 65535;
+// graphs from which the lane kernel scores a launch by default (half its lanes busy)
+constexpr int kLaneMinGraphs = 32;
 }
 
 struct sl_ctx;
@@ -62,6 +64,14 @@ struct sl_ctx {
     double* d_loga = nullptr;
     double* d_acc = nullptr;
     double* d_scalar = nullptr;
+    sl::SeqOp* d_seq = nullptr;  // the program's sequential form (lane kernel)
+    uint32_t* d_lstream = nullptr;  // and its descriptor stream
+    bool lod_global = false;     // the workgroup kernel's arena in global-memory slots (too large for LDS)
+    void* d_lgt = nullptr;       // kLodGlobalSlots slots of arena_bytes
+    size_t lgt_cap = 0;
+    size_t lane_lds = 0;         // LDS of a lane-kernel wavefront
+    bool lane_ok = false;        // the lane kernel's arena fits LDS
+    int lod_kernel = SL_LOD_AUTO;
     // growable work buffers
     double* d_partial = nullptr;
     size_t partial_cap = 0;
@@ -110,6 +120,9 @@ struct sl_ctx {
     int samp_block = sl::kLsamplerBlock;
     int cells_cap = 0;
     size_t samp_lds = 0;
+    bool samp_global = false;      // tables and presums in global-memory slots (too large for LDS)
+    void* d_gtables = nullptr;     // kGlobalTableSlots slots of 5 cells_cap entries
+    size_t gtables_cap = 0;
     sl::SampOp* d_sops = nullptr;
     sl::SampPrev* d_sprevs = nullptr;
     sl::ChildDesc* d_schildren = nullptr;
@@ -155,6 +168,7 @@ struct sl_ctx {
     double* d_fbuf = nullptr;
     uint8_t* d_maps = nullptr;
     uint32_t* d_near = nullptr;
+    uint32_t* d_near_loci = nullptr;  // [chains][ceil(M/32)] loci of L-sampler near-edge draws
     // sequential-imputation replicates
     uint32_t* d_imp_graphs = nullptr;  // [R][M][W]
     size_t imp_graphs_cap = 0;
@@ -244,7 +258,76 @@ sl::LodArgs lod_args(sl_ctx* c) {
     A.gpw = c->gpw;
     A.arena_bytes = c->arena_bytes;
     A.maxbits_bytes = c->maxbits_bytes;
+    A.seq = c->d_seq;
+    A.lstream = c->d_lstream;
+    A.lane_arena_bytes = c->prog.seq_elems * sl::kLaneGraphs * (c->precision == SL_FP64 ? 8 : 4);
+    A.gtables = c->lod_global ? c->d_lgt : nullptr;
     return A;
+}
+
+// LDS of a workgroup of the workgroup-per-graph kernel (and P(T)) for a
+// program: the table arena (unless it lives in global memory), the FP32
+// exponent words, the positions' thetas and the per-person codes.
+struct LodLayout {
+    int arena_bytes = 0, maxbits_bytes = 0;
+    size_t lds = 0;
+    bool global = false;
+};
+
+LodLayout lod_layout(const PeelProgram& P, int precision, int N) {
+    LodLayout L;
+    const size_t esz = precision == SL_FP64 ? 8 : 4;
+    L.arena_bytes = (int)((P.arena_elems * esz + 15) / 16 * 16);
+    L.maxbits_bytes = precision == SL_FP64 ? 0 : (int)(((P.n_ops + 1) * 4 + 15) / 16 * 16);
+    const size_t rest = (size_t)L.maxbits_bytes + 4 * sl::kMaxPos * esz + ((N + 15) / 16 * 16);
+    L.lds = (size_t)L.arena_bytes + rest;
+    if (L.lds > 160 * 1024) {
+        L.global = true;
+        L.lds = rest;
+    }
+    return L;
+}
+
+// Which trait-peel kernel scores a launch of G graphs, and in how many chunks
+// (each chunk's log-sum is one partial): the lane kernel, one (graph,
+// position) per lane, when it is asked for or, by default, when there are
+// enough graphs to fill a wavefront's lanes and its arena fits LDS;
+// otherwise the workgroup-per-graph kernel with gpw graphs per chunk.
+bool lane_mode(const sl_ctx* c, int G) {
+    if (!c->lane_ok || c->lod_kernel == SL_LOD_WORKGROUP) return false;
+    return c->lod_kernel == SL_LOD_LANES || G >= kLaneMinGraphs;
+}
+
+int lod_chunks(const sl_ctx* c, int G) {
+    const int per = lane_mode(c, G) ? sl::kLaneGraphs : c->gpw;
+    return (G + per - 1) / per;
+}
+
+hipError_t launch_scoring(sl_ctx* c, sl::LodArgs A, int nchunks) {
+    if (lane_mode(c, A.G)) return sl::launch_lane_lod(A, c->precision, c->lane_lds, nchunks, c->stream);
+    const int nl = c->M - 1;
+    if (!c->lod_global) return sl::launch_lod(A, c->precision, c->block, c->lds, nl, nchunks, c->stream);
+    // one slot per workgroup: at most kLodGlobalSlots workgroups per launch
+    const int cg = std::min(nchunks, sl::kLodGlobalSlots);
+    const int lx = std::max(1, sl::kLodGlobalSlots / cg);
+    for (int c0 = 0; c0 < nchunks; c0 += cg)
+        for (int l0 = 0; l0 < nl; l0 += lx) {
+            A.chunk0 = c0;
+            A.l0 = l0;
+            const hipError_t e =
+                sl::launch_lod(A, c->precision, c->block, c->lds, std::min(lx, nl - l0), std::min(cg, nchunks - c0),
+                               c->stream);
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
+}
+
+// The global-memory slots of the workgroup kernel's arena (allocated with the
+// program, so that scoring itself never allocates).
+int ensure_lod_slots(sl_ctx* c) {
+    if (!c->lod_global) return SL_OK;
+    return ensure(c, reinterpret_cast<unsigned char**>(&c->d_lgt), &c->lgt_cap,
+                  (size_t)sl::kLodGlobalSlots * c->arena_bytes);
 }
 
 hipEvent_t take_event(sl_ctx* c) {
@@ -295,15 +378,16 @@ int release_all(sl_ctx* c) {
                     c->d_rlevel_ops, c->d_mother, c->d_father, c->d_legal_t, c->d_gprob_d, c->d_gprob_f,
                     c->d_theta, c->d_chains, c->d_status,
                     c->d_gen_off, c->d_ptab, c->d_geno2, c->d_minor, c->d_meioses, c->d_order,
-                    c->d_fixed_order, c->d_lcur, c->d_lflip, c->d_fbuf, c->d_maps, c->d_near,
+                    c->d_fixed_order, c->d_lcur, c->d_lflip, c->d_fbuf, c->d_maps, c->d_near, c->d_near_loci,
                     c->d_ops, c->d_prevs, c->d_children, c->d_tasks, c->d_tinfo, c->d_idx, c->d_level_base,
                     c->d_sched_off,
                     c->d_sched_ops, c->d_trait_d,
                     c->d_trait_f, c->d_ptheta, c->d_logt, c->d_loga, c->d_acc, c->d_scalar,
-                    c->d_partial, c->d_packed, c->d_stage, c->d_terms,
+                    c->d_partial, c->d_packed, c->d_stage, c->d_terms, c->d_seq,
                     c->g_ops, c->g_prevs, c->g_children, c->g_tasks, c->g_tinfo, c->g_idx, c->g_level_base,
                     c->g_sched_off, c->g_sched_ops, c->d_gdp, c->d_jac, c->d_gterms, c->d_gstate, c->d_sweep,
-                    c->d_imp_graphs, c->d_imp_draw, c->d_imp_weight, c->d_imp_status, c->d_elod_pairs};
+                    c->d_imp_graphs, c->d_imp_draw, c->d_imp_weight, c->d_imp_status, c->d_elod_pairs,
+                    c->d_gtables, c->d_lgt, c->d_lstream};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
@@ -471,6 +555,11 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
     }
     if (opts && opts->graphs_per_block > 0) c->gpw = opts->graphs_per_block;
     if (opts) c->m_mode = opts->msampler_mode;
+    if (opts) c->lod_kernel = opts->lod_kernel;
+    if (c->lod_kernel < SL_LOD_AUTO || c->lod_kernel > SL_LOD_LANES) {
+        delete c;
+        return fail(nullptr, SL_E_INVALID, "unknown lod_kernel");
+    }
     if ((c->m_mode & ~(3 | SL_MSAMPLER_LDS_GRAPHS)) != 0 || (c->m_mode & 3) > SL_MSAMPLER_TWO_LAUNCH) {
         delete c;
         return fail(nullptr, SL_E_INVALID, "unknown msampler_mode");
@@ -496,15 +585,20 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
         delete c;
         return fail(nullptr, SL_E_UNSUPPORTED, why);
     }
-    const size_t esz = c->precision == SL_FP64 ? 8 : 4;
-    c->arena_bytes = (int)((c->prog.arena_elems * esz + 15) / 16 * 16);
-    // FP32: one exponent word per table and their running sum
-    c->maxbits_bytes = c->precision == SL_FP64 ? 0 : (int)(((c->prog.n_ops + 1) * 4 + 15) / 16 * 16);
-    c->lds = (size_t)c->arena_bytes + c->maxbits_bytes + 4 * sl::kMaxPos * esz + ((c->N + 15) / 16 * 16);
-    if (c->lds > 160 * 1024) {
-        delete c;
-        return fail(nullptr, SL_E_UNSUPPORTED, "peel program needs more than 160 KiB of LDS");
+    {
+        // FP32: one exponent word per table and their running sum
+        const LodLayout L = lod_layout(c->prog, c->precision, c->N);
+        c->arena_bytes = L.arena_bytes;
+        c->maxbits_bytes = L.maxbits_bytes;
+        c->lds = L.lds;
+        c->lod_global = L.global;
+        if (c->lds > 160 * 1024) {
+            delete c;
+            return fail(nullptr, SL_E_UNSUPPORTED, "peel program's exponents and codes need more than 160 KiB of LDS");
+        }
     }
+    c->lane_lds = sl::lane_lds_bytes(c->prog.seq_elems, c->W, c->precision);
+    c->lane_ok = c->lane_lds <= 160 * 1024;
 
     c->theta.assign(map->theta, map->theta + c->M - 1);
     c->ptheta.assign(map->partial_theta, map->partial_theta + c->M - 1);
@@ -518,7 +612,8 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
         delete c;
         return fail(nullptr, SL_E_DEVICE, m);
     }
-    e = sl::configure_lod_lds(c->precision, c->lds);
+    e = sl::configure_lod_lds(c->precision, c->lds, c->lod_global);
+    if (e == hipSuccess && c->lane_ok) e = sl::configure_lane_lds(c->precision, c->lane_lds);
     if (e != hipSuccess) {
         std::string m = std::string("LDS configuration failed: ") + hipGetErrorString(e);
         release_all(c);
@@ -542,12 +637,15 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
         (rc = upload(c, &c->d_level_base, c->prog.level_base.data(), c->prog.level_base.size())) ||
         (rc = upload(c, &c->d_sched_off, c->prog.sched_off.data(), c->prog.sched_off.size())) ||
         (rc = upload(c, &c->d_sched_ops, c->prog.sched_ops.data(), c->prog.sched_ops.size())) ||
+        (rc = upload(c, &c->d_seq, c->prog.seq.data(), c->prog.seq.size())) ||
+        (rc = upload(c, &c->d_lstream, c->prog.lstream.data(), c->prog.lstream.size())) ||
         (rc = upload(c, &c->d_trait_d, ped->disease_prob, 4 * (size_t)c->N)) ||
         (rc = upload(c, &c->d_trait_f, tf.data(), tf.size())) ||
         (rc = upload(c, &c->d_ptheta, c->ptheta.data(), c->ptheta.size())) ||
         (rc = upload(c, &c->d_logt, logt.data(), logt.size())) ||
         (rc = upload(c, &c->d_loga, loga.data(), loga.size())) ||
-        (rc = upload<double>(c, &c->d_acc, nullptr, nacc)) || (rc = upload<double>(c, &c->d_scalar, nullptr, 4))) {
+        (rc = upload<double>(c, &c->d_acc, nullptr, nacc)) || (rc = upload<double>(c, &c->d_scalar, nullptr, 4)) ||
+        (rc = ensure_lod_slots(c))) {
         std::string m = c->err;
         release_all(c);
         delete c;
@@ -577,8 +675,15 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
         if (c->samp_why.empty()) {
             c->cells_cap = (int)((max_cells + 7) / 8 * 8);
             c->samp_lds = sl::lsampler_lds_bytes(c->precision, c->cells_cap, c->sprog.n_ops, c->N, c->F,
-                                                 sl::kLsamplerBlockWide);
-            if (c->samp_lds > 160 * 1024) c->samp_why = "sampler tables need more than 160 KiB of LDS at some locus";
+                                                 sl::kLsamplerBlockWide, false);
+            if (c->samp_lds > 160 * 1024) {
+                // the tables and presums move to global memory; the rest stays in LDS
+                c->samp_global = true;
+                c->samp_lds = sl::lsampler_lds_bytes(c->precision, c->cells_cap, c->sprog.n_ops, c->N, c->F,
+                                                     sl::kLsamplerBlockWide, true);
+                if (c->samp_lds > 160 * 1024)
+                    c->samp_why = "sampler records and per-person arrays need more than 160 KiB of LDS";
+            }
             if (c->cells_cap >= 65536) c->samp_why = "sampler tables exceed the 16-bit offsets of the op records";
             if (c->N > 65535) c->samp_why = "more than 65535 people for the sampler's 16-bit person records";
         }
@@ -586,8 +691,20 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
             std::vector<uint8_t> lt((size_t)c->M * c->N);
             for (int p = 0; p < c->N; ++p)
                 for (int l = 0; l < c->M; ++l) lt[(size_t)l * c->N + p] = ped->legal[(size_t)p * c->M + l];
-            std::vector<float> gf((size_t)c->N * c->M * 4);
-            for (size_t i = 0; i < gf.size(); ++i) gf[i] = (float)ped->geno_prob[i];
+            // sampler trait probabilities locus-major ([M][N][4]): a locus
+            // step reads one contiguous 4N-entry block, in the precision the
+            // context runs in
+            const size_t ng = (size_t)c->N * c->M * 4;
+            std::vector<double> gd(c->precision == SL_FP64 ? ng : 0);
+            std::vector<float> gf(c->precision == SL_FP64 ? 0 : ng);
+            for (int p = 0; p < c->N; ++p)
+                for (int l = 0; l < c->M; ++l)
+                    for (int v = 0; v < 4; ++v) {
+                        const double x = ped->geno_prob[((size_t)p * c->M + l) * 4 + v];
+                        const size_t j = ((size_t)l * c->N + p) * 4 + v;
+                        if (c->precision == SL_FP64) gd[j] = x;
+                        else gf[j] = (float)x;
+                    }
             if ((rc = upload(c, &c->d_sops, c->sprog.ops.data(), c->sprog.ops.size())) ||
                 (rc = upload(c, &c->d_sprevs, c->sprog.prevs.data(), c->sprog.prevs.size())) ||
                 (rc = upload(c, &c->d_schildren, c->sprog.children.data(), c->sprog.children.size())) ||
@@ -598,14 +715,16 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
                 (rc = upload(c, &c->d_rlevel_off, c->sprog.rlevel_off.data(), c->sprog.rlevel_off.size())) ||
                 (rc = upload(c, &c->d_rlevel_ops, c->sprog.rlevel_ops.data(), c->sprog.rlevel_ops.size())) ||
                 (rc = upload(c, &c->d_legal_t, lt.data(), lt.size())) ||
-                (rc = upload(c, &c->d_gprob_d, ped->geno_prob, (size_t)c->N * c->M * 4)) ||
-                (rc = upload(c, &c->d_gprob_f, gf.data(), gf.size()))) {
+                (c->precision == SL_FP64 ? (rc = upload(c, &c->d_gprob_d, gd.data(), gd.size()))
+                                         : (rc = upload(c, &c->d_gprob_f, gf.data(), gf.size())))) {
                 std::string m = c->err;
                 release_all(c);
                 delete c;
                 return fail(nullptr, rc, m);
             }
-            e = sl::configure_lsampler_lds(c->precision, c->samp_lds);
+            // the host tables above go out of scope
+            e = hipStreamSynchronize(c->stream);
+            if (e == hipSuccess) e = sl::configure_lsampler_lds(c->precision, c->samp_lds, c->samp_global);
             if (e != hipSuccess) {
                 std::string m = std::string("sampler LDS configuration failed: ") + hipGetErrorString(e);
                 release_all(c);
@@ -666,7 +785,7 @@ int sl_lod_reset(sl_ctx* c) {
 
 static int lod_run(sl_ctx* c, const uint32_t* dev_graphs, int G, double* terms_dev, const uint32_t* gate = nullptr,
                    int gate_burnin = 0, int gate_period = 1) {
-    const int nchunks = (G + c->gpw - 1) / c->gpw;
+    const int nchunks = lod_chunks(c, G);
     const size_t nacc = (size_t)(c->M - 1) * c->npos;
     int rc = ensure(c, &c->d_partial, &c->partial_cap, nacc * nchunks);
     if (rc) return rc;
@@ -684,7 +803,7 @@ static int lod_run(sl_ctx* c, const uint32_t* dev_graphs, int G, double* terms_d
         e1 = take_event(c);
         if (e0 && e1) SL_HIP(c, hipEventRecord(e0, c->stream));
     }
-    SL_HIP(c, sl::launch_lod(A, c->precision, c->block, c->lds, nchunks, c->stream));
+    SL_HIP(c, launch_scoring(c, A, nchunks));
     if (c->prof && e0 && e1) {
         SL_HIP(c, hipEventRecord(e1, c->stream));
         c->ev_used.push_back({SL_KERNEL_LOD, e0, e1, 1});
@@ -804,13 +923,15 @@ static int need_sampler(sl_ctx* c) {
     return SL_OK;
 }
 
+static int near_words(const sl_ctx* c) { return (c->M + 31) / 32; }
+
 // (Re)allocates the per-chain state for n_chains chains.
 static int alloc_chains(sl_ctx* c, int n_chains) {
     const size_t words = (size_t)n_chains * c->M * c->W;
     const size_t cm = (size_t)n_chains * c->M;
     if (n_chains != c->n_chains) {
         void** bufs[] = {(void**)&c->d_chains, (void**)&c->d_status, (void**)&c->d_lcur, (void**)&c->d_lflip,
-                         (void**)&c->d_fbuf, (void**)&c->d_maps, (void**)&c->d_near};
+                         (void**)&c->d_fbuf, (void**)&c->d_maps, (void**)&c->d_near, (void**)&c->d_near_loci};
         for (void** b : bufs) {
             if (*b) SL_HIP(c, hipFree(*b));
             *b = nullptr;
@@ -823,10 +944,21 @@ static int alloc_chains(sl_ctx* c, int n_chains) {
         SL_HIP(c, hipMalloc((void**)&c->d_fbuf, 2 * cm * sizeof(double)));
         SL_HIP(c, hipMalloc((void**)&c->d_maps, cm));
         SL_HIP(c, hipMalloc((void**)&c->d_near, (size_t)n_chains * sizeof(uint32_t)));
+        SL_HIP(c, hipMalloc((void**)&c->d_near_loci, (size_t)n_chains * near_words(c) * sizeof(uint32_t)));
         c->n_chains = n_chains;
     }
     SL_HIP(c, hipMemsetAsync(c->d_status, 0, (size_t)n_chains * sizeof(int32_t), c->stream));
     SL_HIP(c, hipMemsetAsync(c->d_near, 0, (size_t)n_chains * sizeof(uint32_t), c->stream));
+    SL_HIP(c, hipMemsetAsync(c->d_near_loci, 0, (size_t)n_chains * near_words(c) * sizeof(uint32_t), c->stream));
+    return SL_OK;
+}
+
+// Largest chain count: the sampler kernels take the chain as the grid's y
+// index.
+static int max_chains(sl_ctx* c, int* out) {
+    int gy = 0;
+    SL_HIP(c, hipDeviceGetAttribute(&gy, hipDeviceAttributeMaxGridDimY, c->device));
+    *out = std::min(gy, 65535);
     return SL_OK;
 }
 
@@ -834,6 +966,11 @@ int sl_mcmc_init(sl_ctx* c, int32_t n_chains, const int32_t* graphs) {
     if (!c || n_chains <= 0 || !graphs) return SL_E_INVALID;
     if (!c->samp_ready && !c->m_ready)
         return fail(c, SL_E_UNSUPPORTED, "samplers unavailable: " + c->samp_why + "; " + c->m_why);
+    int cap = 0, rc0 = max_chains(c, &cap);
+    if (rc0) return rc0;
+    if (n_chains > cap)
+        return fail(c, SL_E_INVALID, "n_chains " + std::to_string(n_chains) + " exceeds the " +
+                                         std::to_string(cap) + " chains one launch's grid holds");
     int rc = alloc_chains(c, n_chains);
     if (rc) return rc;
     rc = stage_graphs(c, graphs, n_chains);
@@ -900,6 +1037,9 @@ static sl::SampArgs samp_args(sl_ctx* c) {
     A.graphs = c->d_chains;
     A.status = c->d_status;
     A.cells_cap = c->cells_cap;
+    A.near_edge = c->d_near;
+    A.near_loci = c->d_near_loci;
+    A.near_words = near_words(c);
     return A;
 }
 
@@ -925,6 +1065,39 @@ static int lsampler_shape(sl_ctx* c, int64_t groups, sl::SampArgs& A) {
     return c->samp_block;
 }
 
+// The global-memory slots of the L-sampler's tables, when they do not fit LDS.
+static int ensure_gtables(sl_ctx* c) {
+    if (!c->samp_global) return SL_OK;
+    const size_t esz = c->precision == SL_FP64 ? 8 : 4;
+    const size_t bytes = (size_t)sl::kGlobalTableSlots * 5 * (size_t)c->cells_cap * esz;
+    return ensure(c, reinterpret_cast<unsigned char**>(&c->d_gtables), &c->gtables_cap, bytes);
+}
+
+// One L-sampler launch over a grid of gx loci blocks by gy chains; with the
+// tables in global memory, as several launches of at most kGlobalTableSlots
+// workgroups, one slot each.
+static int launch_samp(sl_ctx* c, sl::SampArgs A, int block, int gx, int gy) {
+    A.block0 = A.chain0 = 0;
+    A.gtables = nullptr;
+    if (!c->samp_global) {
+        SL_HIP(c, sl::launch_lsampler(A, c->precision, block, c->samp_lds, gx, gy, c->stream));
+        return SL_OK;
+    }
+    int rc = ensure_gtables(c);
+    if (rc) return rc;
+    A.gtables = c->d_gtables;
+    const int cg = std::min(gy, sl::kGlobalTableSlots);
+    const int bx = std::max(1, sl::kGlobalTableSlots / cg);
+    for (int c0 = 0; c0 < gy; c0 += cg)
+        for (int b0 = 0; b0 < gx; b0 += bx) {
+            A.chain0 = c0;
+            A.block0 = b0;
+            SL_HIP(c, sl::launch_lsampler(A, c->precision, block, c->samp_lds, std::min(bx, gx - b0),
+                                          std::min(cg, gy - c0), c->stream));
+        }
+    return SL_OK;
+}
+
 // Both parity launches of one L-sweep.  check_kind: every chain takes its
 // kind and first parity from its schedule draws (lprob), first_parity unused.
 static int run_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int first_parity, int check_kind, double lprob,
@@ -945,7 +1118,8 @@ static int run_lsweep(sl_ctx* c, uint64_t seed, uint32_t sweep, int first_parity
             e1 = take_event(c);
             if (e0 && e1) SL_HIP(c, hipEventRecord(e0, c->stream));
         }
-        SL_HIP(c, sl::launch_lsampler(A, c->precision, block, c->samp_lds, c->n_chains, c->stream));
+        const int rc = launch_samp(c, A, block, sl::lsampler_loci(A), c->n_chains);
+        if (rc) return rc;
         if (c->prof && e0 && e1) {
             SL_HIP(c, hipEventRecord(e1, c->stream));
             c->ev_used.push_back({SL_KERNEL_LSAMPLER, e0, e1, 1});
@@ -981,6 +1155,8 @@ static int run_impute(sl_ctx* c, uint64_t seed, int R, int mode, int start, uint
     A.chain_base = chain_base;
     A.imp_draw = c->d_imp_draw;
     A.imp_weight = c->d_imp_weight;
+    A.near_edge = nullptr;  // replicates have no chain counters
+    A.near_loci = nullptr;
     const int block = lsampler_shape(c, R, A);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->prof) {
@@ -990,7 +1166,7 @@ static int run_impute(sl_ctx* c, uint64_t seed, int R, int mode, int start, uint
     }
     for (int t = 0; t < c->M; ++t) {
         A.impute_t = t;
-        SL_HIP(c, sl::launch_lsampler(A, c->precision, block, c->samp_lds, R, c->stream));
+        if ((rc = launch_samp(c, A, block, 1, R))) return rc;
     }
     if (c->prof && e0 && e1) {
         SL_HIP(c, hipEventRecord(e1, c->stream));
@@ -1010,6 +1186,13 @@ int sl_sequential_imputation(sl_ctx* c, uint64_t seed, int32_t iterations, int32
     if (!c || iterations < 0 || iterations > SL_IMPUTE_MAX || n_chains < 0) return SL_E_INVALID;
     int rc = need_sampler(c);
     if (rc) return rc;
+    if (n_chains > 0) {
+        int cap = 0;
+        if ((rc = max_chains(c, &cap))) return rc;
+        if (n_chains > cap)
+            return fail(c, SL_E_INVALID, "n_chains " + std::to_string(n_chains) + " exceeds the " +
+                                             std::to_string(cap) + " chains one launch's grid holds");
+    }
     const int R = iterations > 0 ? iterations : 1;
     const size_t per = (size_t)c->M * c->W;
     if ((rc = run_impute(c, seed, R, iterations > 0 ? 1 : 2, 0, 0))) return rc;
@@ -1124,6 +1307,7 @@ static sl::MsampArgs msamp_args(sl_ctx* c) {
     A.ptab = c->d_ptab;
     A.ptab_bytes = c->m_ptab_bytes;
     A.n_chains = c->n_chains;
+    A.fp32 = c->precision == SL_FP32_RESCALED ? 1 : 0;
     return A;
 }
 
@@ -1275,6 +1459,14 @@ int sl_mcmc_likelihood(sl_ctx* c, double* per_locus) {
     return SL_OK;
 }
 
+int sl_mcmc_near_edge_loci(sl_ctx* c, uint32_t* bits) {
+    if (!c || !bits || c->n_chains <= 0) return SL_E_INVALID;
+    const size_t n = (size_t)c->n_chains * near_words(c);
+    SL_HIP(c, hipMemcpyAsync(bits, c->d_near_loci, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    return SL_OK;
+}
+
 int sl_mcmc_near_edge(sl_ctx* c, int64_t* counts) {
     if (!c || !counts || c->n_chains <= 0) return SL_E_INVALID;
     std::vector<uint32_t> h((size_t)c->n_chains);
@@ -1297,6 +1489,7 @@ int sl_set_options(sl_ctx* c, const sl_options* o) {
         return fail(c, SL_E_INVALID, "block_threads must be a multiple of 64 in 64..1024");
     if ((o->msampler_mode & ~(3 | SL_MSAMPLER_LDS_GRAPHS)) != 0 || (o->msampler_mode & 3) > SL_MSAMPLER_TWO_LAUNCH)
         return fail(c, SL_E_INVALID, "unknown msampler_mode");
+    if (o->lod_kernel < SL_LOD_AUTO || o->lod_kernel > SL_LOD_LANES) return fail(c, SL_E_INVALID, "unknown lod_kernel");
     if (block != c->block) {
         const int old = c->block;
         c->block = block;
@@ -1308,6 +1501,7 @@ int sl_set_options(sl_ctx* c, const sl_options* o) {
     }
     c->gpw = o->graphs_per_block > 0 ? o->graphs_per_block : 1;
     c->m_mode = o->msampler_mode;
+    c->lod_kernel = o->lod_kernel;
     return SL_OK;
 }
 
@@ -1391,9 +1585,10 @@ int sl_batch_run(sl_batch* b, const sl_schedule* sch, sl_mcmc_stats* stats) {
             st[i].m_sweeps += C - nl;
             if (sl::gate_open(s, sch->burnin, sch->period)) st[i].scored += C;
         }
-        const int nchunks = (C + c->gpw - 1) / c->gpw;
+        const int nchunks = lod_chunks(c, C);
         int rc = ensure(c, &c->d_partial, &c->partial_cap, (size_t)(c->M - 1) * c->npos * nchunks);
         if (!rc && c->m_ready) rc = ensure(c, &c->d_order, &c->order_cap, c->meioses_h.size() * (size_t)C);
+        if (!rc && c->samp_ready) rc = ensure_gtables(c);
         if (rc) return rc;
         SL_HIP(c, hipMemcpyAsync(c->d_sweep, &sch->first_sweep, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
         SL_HIP(c, hipStreamSynchronize(c->stream));
@@ -1506,15 +1701,17 @@ int install_lod_program(sl_ctx* c) {
     PeelProgram P;
     std::string why = sl::compile_lod_program(pv, peel_view(c), c->npos, c->block / 64, &P);
     if (!why.empty()) return fail(c, SL_E_UNSUPPORTED, why);
-    const size_t esz = c->precision == SL_FP64 ? 8 : 4;
-    const int arena_bytes = (int)((P.arena_elems * esz + 15) / 16 * 16);
-    const int maxbits_bytes = c->precision == SL_FP64 ? 0 : (int)(((P.n_ops + 1) * 4 + 15) / 16 * 16);
-    const size_t lds = (size_t)arena_bytes + maxbits_bytes + 4 * sl::kMaxPos * esz + ((c->N + 15) / 16 * 16);
-    if (lds > 160 * 1024) return fail(c, SL_E_UNSUPPORTED, "peel program needs more than 160 KiB of LDS");
+    const LodLayout lay = lod_layout(P, c->precision, c->N);
+    const int arena_bytes = lay.arena_bytes, maxbits_bytes = lay.maxbits_bytes;
+    const size_t lds = lay.lds;
+    if (lds > 160 * 1024)
+        return fail(c, SL_E_UNSUPPORTED, "peel program's exponents and codes need more than 160 KiB of LDS");
+    const size_t lane_lds = sl::lane_lds_bytes(P.seq_elems, c->W, c->precision);
     SL_HIP(c, hipStreamSynchronize(c->stream));
     void** bufs[] = {(void**)&c->d_ops, (void**)&c->d_prevs, (void**)&c->d_children, (void**)&c->d_tasks,
                      (void**)&c->d_tinfo, (void**)&c->d_idx, (void**)&c->d_level_base, (void**)&c->d_sched_off,
-                     (void**)&c->d_sched_ops, (void**)&c->d_trait_d, (void**)&c->d_trait_f};
+                     (void**)&c->d_sched_ops, (void**)&c->d_trait_d, (void**)&c->d_trait_f, (void**)&c->d_seq,
+                     (void**)&c->d_lstream};
     for (void** b : bufs) {
         if (*b) SL_HIP(c, hipFree(*b));
         *b = nullptr;
@@ -1523,7 +1720,12 @@ int install_lod_program(sl_ctx* c) {
     c->arena_bytes = arena_bytes;
     c->maxbits_bytes = maxbits_bytes;
     c->lds = lds;
-    SL_HIP(c, sl::configure_lod_lds(c->precision, c->lds));
+    c->lod_global = lay.global;
+    c->lane_lds = lane_lds;
+    c->lane_ok = lane_lds <= 160 * 1024;
+    SL_HIP(c, sl::configure_lod_lds(c->precision, c->lds, c->lod_global));
+    if (int rc2 = ensure_lod_slots(c)) return rc2;
+    if (c->lane_ok) SL_HIP(c, sl::configure_lane_lds(c->precision, c->lane_lds));
     std::vector<float> tf(c->dp_h.size());
     for (size_t i = 0; i < tf.size(); ++i) tf[i] = (float)c->dp_h[i];
     int rc;
@@ -1536,6 +1738,8 @@ int install_lod_program(sl_ctx* c) {
         (rc = upload(c, &c->d_level_base, c->prog.level_base.data(), c->prog.level_base.size())) ||
         (rc = upload(c, &c->d_sched_off, c->prog.sched_off.data(), c->prog.sched_off.size())) ||
         (rc = upload(c, &c->d_sched_ops, c->prog.sched_ops.data(), c->prog.sched_ops.size())) ||
+        (rc = upload(c, &c->d_seq, c->prog.seq.data(), c->prog.seq.size())) ||
+        (rc = upload(c, &c->d_lstream, c->prog.lstream.data(), c->prog.lstream.size())) ||
         (rc = upload(c, &c->d_trait_d, c->dp_h.data(), c->dp_h.size())) ||
         (rc = upload(c, &c->d_trait_f, tf.data(), tf.size())))
         return rc;
@@ -1774,6 +1978,10 @@ int sl_program_export(const sl_pedigree_desc* ped, const sl_peel_desc* peel, int
     for (uint32_t t : P.tinfo) o.push_back((int32_t)t);
     o.push_back((int32_t)P.idx.size());
     for (uint16_t t : P.idx) o.push_back((int32_t)t);
+    // the sequential form: arena extent, then one SeqOp per op in order
+    o.push_back(P.seq_elems);
+    w = reinterpret_cast<const int32_t*>(P.seq.data());
+    o.insert(o.end(), w, w + P.seq.size() * (sizeof(sl::SeqOp) / 4));
     *needed = (int64_t)o.size();
     if (buf && cap >= (int64_t)o.size()) memcpy(buf, o.data(), o.size() * 4);
     return SL_OK;

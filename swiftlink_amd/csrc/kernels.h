@@ -36,6 +36,16 @@ struct LodArgs {
     // least gate_burnin and a multiple of gate_period
     const uint32_t* gate_sweep;
     int gate_burnin, gate_period;
+    // the lane kernel: the program's sequential form and the bytes of its
+    // per-lane table arena (the graph columns follow it in LDS)
+    const SeqOp* seq;
+    const uint32_t* lstream;     // the lane kernel's descriptor stream
+    int lane_arena_bytes;
+    // programs whose table arena exceeds LDS: per-workgroup global-memory
+    // slots of arena_bytes (slot blockIdx.x + gridDim.x blockIdx.y), and the
+    // launch's first interval and chunk
+    void* gtables;
+    int l0, chunk0;
 };
 
 __host__ __device__ inline bool gate_open(uint32_t sweep, int burnin, int period) {
@@ -53,7 +63,23 @@ struct GradArgs {
     int npos;            // trait positions per interval (the program itself has one)
 };
 
-hipError_t launch_lod(const LodArgs& A, int precision, int block, size_t lds, int nchunks, hipStream_t s);
+// grid (blocks_x intervals from A.l0, blocks_y chunks from A.chunk0); the
+// arena in A.gtables slots when that is set
+hipError_t launch_lod(const LodArgs& A, int precision, int block, size_t lds, int blocks_x, int blocks_y,
+                      hipStream_t s);
+// workgroups of one launch whose tables live in global memory
+constexpr int kLodGlobalSlots = 2048;
+// The lane kernel: one wavefront per (interval, position, chunk of
+// kLaneGraphs graphs), one (graph, position) evaluation per lane; chunk k's
+// log-sum goes to partial[k][l][pos] as launch_lod's does.
+constexpr int kLaneGraphs = 64;
+hipError_t launch_lane_lod(const LodArgs& A, int precision, size_t lds, int nchunks, hipStream_t s);
+hipError_t configure_lane_lds(int precision, size_t lds);
+// LDS of a lane-kernel wavefront: the per-lane arena and two graph columns
+inline size_t lane_lds_bytes(int seq_elems, int W, int precision) {
+    const size_t esz = precision == 0 ? 8 : 4;
+    return (size_t)seq_elems * kLaneGraphs * esz + (size_t)2 * W * kLaneGraphs * 4;
+}
 hipError_t configure_grad_lds(size_t lds);
 // nodg: P(T) (grid of one); else grid (M-1, npos, A.G)
 hipError_t launch_grad(const LodArgs& A, const GradArgs& Gd, int nodg, size_t lds, hipStream_t s);
@@ -69,7 +95,8 @@ hipError_t launch_lod_merge(double* acc, const double* partial, int nchunks, int
 // one thread: (*counter)++ (the device sweep index of captured sweeps)
 hipError_t launch_advance(uint32_t* counter, hipStream_t s);
 hipError_t launch_fill(double* p, double v, int n, hipStream_t s);
-hipError_t configure_lod_lds(int precision, size_t lds);
+hipError_t configure_lod_lds(int precision, size_t lds, bool global_tables);
 hipError_t launch_pack(const int32_t* in, uint32_t* out, int G, int M, int N, int F, int W, hipStream_t s);
+
 
 }  // namespace sl

@@ -322,17 +322,23 @@ __device__ double final_log(const LodArgs& A, const T* arena, const uint32_t* ma
     return L > 0.0 ? log(L) + s : -INFINITY;
 }
 
-template <typename T, bool RESCALE>
+// GT: the table arena lives in a global-memory slot of the workgroup
+// (programs whose arena exceeds LDS); the launch covers intervals l0.. and
+// chunks chunk0..
+template <typename T, bool RESCALE, bool GT>
 __global__ void lod_kernel(LodArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     if (A.gate_sweep && !gate_open(*A.gate_sweep, A.gate_burnin, A.gate_period)) return;
-    T* arena = reinterpret_cast<T*>(smem);
-    uint32_t* maxbits = reinterpret_cast<uint32_t*>(smem + A.arena_bytes);
-    T* ths = reinterpret_cast<T*>(smem + A.arena_bytes + A.maxbits_bytes);
-    uint8_t* code = reinterpret_cast<uint8_t*>(smem + A.arena_bytes + A.maxbits_bytes + 4 * kMaxPos * sizeof(T));
+    const size_t lds_arena = GT ? 0 : (size_t)A.arena_bytes;
+    T* arena = GT ? reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(A.gtables) +
+                                         ((size_t)blockIdx.x + (size_t)gridDim.x * blockIdx.y) * A.arena_bytes)
+                  : reinterpret_cast<T*>(smem);
+    uint32_t* maxbits = reinterpret_cast<uint32_t*>(smem + lds_arena);
+    T* ths = reinterpret_cast<T*>(smem + lds_arena + A.maxbits_bytes);
+    uint8_t* code = reinterpret_cast<uint8_t*>(smem + lds_arena + A.maxbits_bytes + 4 * kMaxPos * sizeof(T));
 
-    const int l = blockIdx.x;
-    const int chunk = blockIdx.y;
+    const int l = (int)blockIdx.x + A.l0;
+    const int chunk = (int)blockIdx.y + A.chunk0;
     const int g0 = chunk * A.gpw;
     const int g1 = min(A.G, g0 + A.gpw);
     const int npos = A.npos;
@@ -380,12 +386,233 @@ __global__ void lod_kernel(LodArgs A) {
         A.partial[((size_t)chunk * (A.M - 1) + l) * npos + threadIdx.x] = run;
 }
 
-// P(T): one workgroup, no graph (peeler.cc:65-73)
+// ---- The trait peel with one (graph, position) evaluation per lane ----
+//
+// One wavefront per (interval l, position, block of 64 graphs).  Every lane
+// evaluates the whole peel program for its own graph, walking the ops in the
+// program's sequential order and each op's cells in ascending order, so op
+// and cell descriptors are wave-uniform (scalar loads) and no lane idles on a
+// small table.  Lane j keeps cell c of a table at offset off in LDS word
+// (off + c) * 64 + j: each lane reads and writes only its own column, so the
+// wave needs no barrier.  The arithmetic of a cell is lod_kernel's, term for
+// term, so FP64 terms equal that kernel's bit for bit.  FP32 keeps each lane's
+// tables in range by multiplying a table by 2^-e (exact) when its largest
+// cell leaves [2^-6, 2^6), and sums the e of the lane into its likelihood.
+constexpr int kLanes = kLaneGraphs;
+constexpr int kRenormLo = -6, kRenormHi = 6;
+
+// phased-trait code of person p for the lane: maternal bit at l and l+1, then
+// paternal (the layout of lod_kernel's code[])
+__device__ __forceinline__ uint32_t lane_code(const uint32_t* lc, int F, int p) {
+    const int b = 2 * (p - F);
+    const int w = b >> 5, sh = b & 31;  // b is even: the paternal bit is in the same word
+    const uint32_t x0 = lc[(size_t)(2 * w) * kLanes], x1 = lc[(size_t)(2 * w + 1) * kLanes];
+    return ((x0 >> sh) & 1u) | (((x1 >> sh) & 1u) << 1) | (((x0 >> (sh + 1)) & 1u) << 2) |
+           (((x1 >> (sh + 1)) & 1u) << 3);
+}
+
+// 0.25 * rho (trait_rfunction.cc:9-22) with lod_kernel's operation order
+template <typename T>
+__device__ __forceinline__ T quarter_rho(uint32_t code, int i, int j, const Thetas<T>& th) {
+    return (T)0.25 * rho<T>(code, i, j, th);
+}
+
+// The descriptor stream, read 64 words at a time: lane j holds word j of the
+// current window (w0) and of the next two (w1, w2, in flight), and a uniform
+// word p of the current window is one readlane.  Headers and cells never
+// straddle a window.
+struct LaneStream {
+    const uint32_t* base;
+    uint32_t w0, w1, w2;
+    int cur;  // window of w0
+    __device__ __forceinline__ void init(const uint32_t* b, int lane) {
+        base = b;
+        cur = 0;
+        w0 = b[lane];
+        w1 = b[64 + lane];
+        w2 = b[128 + lane];
+    }
+    // make p's window the current one, prefetching two ahead
+    __device__ __forceinline__ void seek(int p, int lane) {
+        while ((p >> 6) > cur) {
+            w0 = w1;
+            w1 = w2;
+            ++cur;
+            w2 = base[(size_t)(cur + 2) * 64 + lane];
+        }
+    }
+    __device__ __forceinline__ uint32_t at(int p) const {
+        return (uint32_t)__builtin_amdgcn_readlane((int)w0, p & 63);
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ T trait_from(const LaneStream& st, int h, int v) {
+    const uint64_t bits = (uint64_t)st.at(h + 24 + 2 * v) | ((uint64_t)st.at(h + 25 + 2 * v) << 32);
+    return (T)__builtin_bit_cast(double, bits);
+}
+
+// One op of the sequential program for this lane's evaluation.  The header
+// starts at stream word h; returns the word after the op.
+template <typename T, bool RESCALE, int NP>
+__device__ __forceinline__ int lane_op(const LodArgs& A, LaneStream& st, int h, T* la, const uint32_t* lc,
+                                       const Thetas<T>& th, int lane, int* E) {
+    const int type = (int)st.at(h + 0), size = (int)st.at(h + 2), out = (int)st.at(h + 3);
+    const int peel = (int)st.at(h + 4), n_child = (int)st.at(h + 5), stride = (int)st.at(h + 6);
+    int poff[NP > 0 ? NP : 1];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) poff[q] = (int)st.at(h + 8 + q);
+    T tr[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) tr[v] = trait_from<T>(st, h, v);
+    // transmission factors of this lane's graph: the peel node's (CHILD) or
+    // each child's (PARENT), formed once per op
+    T qr[4] = {(T)0, (T)0, (T)0, (T)0};
+    T r4[7][4];
+    if (type == kChild) {
+        const uint32_t cd = lane_code(lc, A.F, peel);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) qr[t] = quarter_rho<T>(cd, t >> 1, t & 1, th);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 7; ++ch) {
+#pragma unroll
+        for (int ij = 0; ij < 4; ++ij) r4[ch][ij] = (T)0;
+        if (type == kParent && ch < n_child) {
+            const uint32_t cd = lane_code(lc, A.F, (int)st.at(h + 16 + ch));
+#pragma unroll
+            for (int ij = 0; ij < 4; ++ij) r4[ch][ij] = quarter_rho<T>(cd, ij >> 1, ij & 1, th);
+        }
+    }
+    T* o = la + (size_t)out * kLanes;
+    float mx = 0.0f;
+    const int cbase = h + kLsHeader;
+    for (int c = 0; c < size; ++c) {
+        const int p = cbase + c * stride;
+        st.seek(p, lane);
+        uint32_t info[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) info[t] = st.at(p + t);
+        uint32_t iw[NP > 0 ? 2 * NP : 1];
+#pragma unroll
+        for (int j = 0; j < 2 * NP; ++j) iw[j] = st.at(p + 4 + j);
+        T total = (T)0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t inf = info[t];
+            if (!(inf & 4u)) continue;  // wave-uniform
+            T tmp = select4(tr, inf & 3u);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const int e = t * NP + q;
+                const int ix = (int)((iw[e >> 1] >> (16 * (e & 1))) & 0xFFFFu);
+                tmp *= la[(size_t)(poff[q] + ix) * kLanes];
+            }
+            if (type == kChild) {
+                tmp *= qr[t];
+            } else if (type == kParent) {
+                T cp = (T)1;
+#pragma unroll
+                for (int ch = 0; ch < 7; ++ch) {
+                    if (ch < n_child) {
+                        const uint32_t m = (inf >> (4 + 4 * ch)) & 15u;
+                        // adding exact zeros leaves the sum of the set terms unchanged
+                        T ct = (T)0;
+#pragma unroll
+                        for (int ij = 0; ij < 4; ++ij) ct += ((m >> ij) & 1u) ? r4[ch][ij] : (T)0;
+                        cp *= ct;
+                    }
+                }
+                tmp *= cp;
+            }
+            total += tmp;
+        }
+        o[(size_t)c * kLanes] = total;
+        if (RESCALE) mx = fmaxf(mx, (float)total);
+    }
+    if (RESCALE) {
+        // largest cell in [2^e, 2^(e+1)); renormalise outside the band
+        int e = mx > 0.0f ? __builtin_amdgcn_frexp_expf(mx) - 1 : 0;
+        if (e >= kRenormLo && e < kRenormHi) e = 0;
+        if (__ballot(e != 0)) {
+            for (int c = 0; c < size; ++c)
+                o[(size_t)c * kLanes] = (T)__builtin_amdgcn_ldexpf((float)o[(size_t)c * kLanes], -e);
+            *E += e;
+        }
+    }
+    return cbase + ((size * stride + 31) & ~31);
+}
+
 template <typename T, bool RESCALE>
+__global__ void __launch_bounds__(kLanes) lane_lod_kernel(LodArgs A) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    if (A.gate_sweep && !gate_open(*A.gate_sweep, A.gate_burnin, A.gate_period)) return;
+    const int l = blockIdx.x, pos = blockIdx.y, chunk = blockIdx.z;
+    const int lane = (int)threadIdx.x;
+    const int g = chunk * kLanes + lane;
+    const bool live = g < A.G;
+    LaneStream st;
+    st.init(A.lstream, lane);
+    T* la = reinterpret_cast<T*>(smem) + lane;
+    uint32_t* lc = reinterpret_cast<uint32_t*>(smem + A.lane_arena_bytes) + lane;
+    // this lane's columns l and l+1, and its recombination count
+    const uint32_t* c0 = A.graphs + ((size_t)(live ? g : 0) * A.M + l) * A.W;
+    int ncross = 0;
+    for (int w = 0; w < A.W; ++w) {
+        const uint32_t x0 = live ? c0[w] : 0u, x1 = live ? c0[A.W + w] : 0u;
+        lc[(size_t)(2 * w) * kLanes] = x0;
+        lc[(size_t)(2 * w + 1) * kLanes] = x1;
+        ncross += __popc(x0 ^ x1);
+    }
+    // set_thetas (trait_rfunction.h:50-56): linear partial thetas
+    const double ptheta = A.partial_theta[l];
+    const double tl = ptheta * (double)(pos + 1), tq = ptheta * (double)(A.npos - pos);
+    Thetas<T> th;
+    th.t1 = (T)tl;
+    th.a1 = (T)(1.0 - tl);
+    th.t2 = (T)tq;
+    th.a2 = (T)(1.0 - tq);
+    const double lt = A.log_theta[l], lant = A.log_anti[l];
+    int E = 0, h = 0, last_out = 0;
+    for (int s = 0; s < A.n_ops; ++s) {
+        st.seek(h, lane);
+        last_out = (int)st.at(h + 3);
+        switch ((int)st.at(h + 1)) {
+            case 0: h = lane_op<T, RESCALE, 0>(A, st, h, la, lc, th, lane, &E); break;
+            case 1: h = lane_op<T, RESCALE, 1>(A, st, h, la, lc, th, lane, &E); break;
+            case 2: h = lane_op<T, RESCALE, 2>(A, st, h, la, lc, th, lane, &E); break;
+            case 3: h = lane_op<T, RESCALE, 3>(A, st, h, la, lc, th, lane, &E); break;
+            case 4: h = lane_op<T, RESCALE, 4>(A, st, h, la, lc, th, lane, &E); break;
+            case 5: h = lane_op<T, RESCALE, 5>(A, st, h, la, lc, th, lane, &E); break;
+            case 6: h = lane_op<T, RESCALE, 6>(A, st, h, la, lc, th, lane, &E); break;
+            case 7: h = lane_op<T, RESCALE, 7>(A, st, h, la, lc, th, lane, &E); break;
+            default: h = lane_op<T, RESCALE, 8>(A, st, h, la, lc, th, lane, &E); break;
+        }
+    }
+    // ln L - R - T (peeler.cc:97-99) and the chunk's log-sum over its graphs
+    const double L = (double)la[(size_t)last_out * kLanes];
+    double term = L > 0.0 ? log(L) + (RESCALE ? (double)E * 0.69314718055994530942 : 0.0) : -INFINITY;
+    const int nm = 2 * (A.N - A.F);
+    term -= (double)ncross * lt + (double)(nm - ncross) * lant;
+    term -= A.trans;
+    if (live && A.terms) A.terms[((size_t)g * (A.M - 1) + l) * A.npos + pos] = term;
+    const double v = live ? term : -INFINITY;
+    double m = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    double sum = (m == -INFINITY || v == -INFINITY) ? 0.0 : exp(v - m);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if (lane == 0 && A.partial)
+        A.partial[((size_t)chunk * (A.M - 1) + l) * A.npos + pos] = m == -INFINITY ? -INFINITY : m + log(sum);
+}
+
+// P(T): one workgroup, no graph (peeler.cc:65-73)
+template <typename T, bool RESCALE, bool GT>
 __global__ void trait_prob_kernel(LodArgs A, double* out) {
     extern __shared__ __align__(16) unsigned char smem[];
-    T* arena = reinterpret_cast<T*>(smem);
-    uint32_t* maxbits = reinterpret_cast<uint32_t*>(smem + A.arena_bytes);
+    T* arena = GT ? reinterpret_cast<T*>(A.gtables) : reinterpret_cast<T*>(smem);
+    uint32_t* maxbits = reinterpret_cast<uint32_t*>(smem + (GT ? 0 : A.arena_bytes));
     if (RESCALE && threadIdx.x == 0) maxbits[A.n_ops] = 0u;
     __syncthreads();
     run_levels<T, RESCALE, true>(A, arena, maxbits, nullptr, nullptr);
@@ -706,31 +933,52 @@ __global__ void pack_graphs_kernel(const int32_t* in, uint32_t* out, int G, int 
 
 }  // namespace
 
-hipError_t configure_lod_lds(int precision, size_t lds) {
+hipError_t configure_lod_lds(int precision, size_t lds, bool global_tables) {
     if (lds <= 64 * 1024) return hipSuccess;
-    hipError_t e;
+    const void* k[2];
     if (precision == 0) {
-        e = hipFuncSetAttribute((const void*)lod_kernel<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)trait_prob_kernel<double, false>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        k[0] = global_tables ? (const void*)lod_kernel<double, false, true> : (const void*)lod_kernel<double, false, false>;
+        k[1] = global_tables ? (const void*)trait_prob_kernel<double, false, true>
+                             : (const void*)trait_prob_kernel<double, false, false>;
     } else {
-        e = hipFuncSetAttribute((const void*)lod_kernel<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void*)trait_prob_kernel<float, true>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        k[0] = global_tables ? (const void*)lod_kernel<float, true, true> : (const void*)lod_kernel<float, true, false>;
+        k[1] = global_tables ? (const void*)trait_prob_kernel<float, true, true>
+                             : (const void*)trait_prob_kernel<float, true, false>;
     }
+    hipError_t e = hipFuncSetAttribute(k[0], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess) e = hipFuncSetAttribute(k[1], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     return e;
 }
 
-hipError_t launch_lod(const LodArgs& A, int precision, int block, size_t lds, int nchunks, hipStream_t s) {
-    dim3 grid(A.M - 1, nchunks);
+hipError_t launch_lod(const LodArgs& A, int precision, int block, size_t lds, int blocks_x, int blocks_y,
+                      hipStream_t s) {
+    dim3 grid(blocks_x, blocks_y);
+    const bool gt = A.gtables != nullptr;
+    if (precision == 0) {
+        if (gt) hipLaunchKernelGGL((lod_kernel<double, false, true>), grid, dim3(block), lds, s, A);
+        else hipLaunchKernelGGL((lod_kernel<double, false, false>), grid, dim3(block), lds, s, A);
+    } else {
+        if (gt) hipLaunchKernelGGL((lod_kernel<float, true, true>), grid, dim3(block), lds, s, A);
+        else hipLaunchKernelGGL((lod_kernel<float, true, false>), grid, dim3(block), lds, s, A);
+    }
+    return hipGetLastError();
+}
+
+hipError_t configure_lane_lds(int precision, size_t lds) {
+    if (lds <= 64 * 1024) return hipSuccess;
     if (precision == 0)
-        hipLaunchKernelGGL((lod_kernel<double, false>), grid, dim3(block), lds, s, A);
+        return hipFuncSetAttribute((const void*)lane_lod_kernel<double, false>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return hipFuncSetAttribute((const void*)lane_lod_kernel<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds);
+}
+
+hipError_t launch_lane_lod(const LodArgs& A, int precision, size_t lds, int nchunks, hipStream_t s) {
+    dim3 grid(A.M - 1, A.npos, nchunks);
+    if (precision == 0)
+        hipLaunchKernelGGL((lane_lod_kernel<double, false>), grid, dim3(kLanes), lds, s, A);
     else
-        hipLaunchKernelGGL((lod_kernel<float, true>), grid, dim3(block), lds, s, A);
+        hipLaunchKernelGGL((lane_lod_kernel<float, true>), grid, dim3(kLanes), lds, s, A);
     return hipGetLastError();
 }
 
@@ -757,10 +1005,14 @@ hipError_t launch_grad_fold(double* state, const double* terms, int G, int n, hi
 }
 
 hipError_t launch_trait_prob(const LodArgs& A, int precision, int block, size_t lds, double* out, hipStream_t s) {
-    if (precision == 0)
-        hipLaunchKernelGGL((trait_prob_kernel<double, false>), dim3(1), dim3(block), lds, s, A, out);
-    else
-        hipLaunchKernelGGL((trait_prob_kernel<float, true>), dim3(1), dim3(block), lds, s, A, out);
+    const bool gt = A.gtables != nullptr;
+    if (precision == 0) {
+        if (gt) hipLaunchKernelGGL((trait_prob_kernel<double, false, true>), dim3(1), dim3(block), lds, s, A, out);
+        else hipLaunchKernelGGL((trait_prob_kernel<double, false, false>), dim3(1), dim3(block), lds, s, A, out);
+    } else {
+        if (gt) hipLaunchKernelGGL((trait_prob_kernel<float, true, true>), dim3(1), dim3(block), lds, s, A, out);
+        else hipLaunchKernelGGL((trait_prob_kernel<float, true, false>), dim3(1), dim3(block), lds, s, A, out);
+    }
     return hipGetLastError();
 }
 

@@ -67,6 +67,12 @@ __device__ __forceinline__ uint32_t rank_in(uint32_t m, uint32_t v) { return __p
 
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// distance from a CDF partial sum within which a draw is counted as near an
+// edge: FP64 tables follow the reference's arithmetic (its rounding only),
+// FP32 tables carry about a float epsilon per product
+template <typename T>
+constexpr double kEdgeTol = sizeof(T) == 4 ? 1e-5 : 1e-12;
+
 __device__ __forceinline__ float scale_from_bits(uint32_t bits) {
     uint32_t ef = (bits >> 23) & 0xFFu;
     if (ef == 0u) return 1.0f;
@@ -147,11 +153,11 @@ template <typename T>
 __device__ __forceinline__ T trait_at(const SampArgs& A, int person, int locus, int v);
 template <>
 __device__ __forceinline__ double trait_at<double>(const SampArgs& A, int person, int locus, int v) {
-    return ((cdbl_p)A.geno_prob_d)[((size_t)person * A.M + locus) * 4 + v];
+    return ((cdbl_p)A.geno_prob_d)[((size_t)locus * A.N + person) * 4 + v];
 }
 template <>
 __device__ __forceinline__ float trait_at<float>(const SampArgs& A, int person, int locus, int v) {
-    return ((cflt_p)A.geno_prob_f)[((size_t)person * A.M + locus) * 4 + v];
+    return ((cflt_p)A.geno_prob_f)[((size_t)locus * A.N + person) * 4 + v];
 }
 
 // probability that a parent of phased trait t transmits allele a (1 = A),
@@ -384,16 +390,28 @@ __device__ __forceinline__ void sampler_op(const OpEnv<T>& E) {
     }
 }
 
-template <typename T, bool RESCALE>
+// GT: the tables and presums live in a global-memory slot of the workgroup
+// (pedigrees whose tables at some locus exceed LDS) instead of LDS.
+template <typename T, bool RESCALE, bool GT>
 __global__ void __launch_bounds__(kLsamplerBlockWide) lsampler_kernel(SampArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int N = A.N, F = A.F, M = A.M, n_ops = A.n_ops;
     const int NF = N - F;
     // LDS carve-up (sizes fixed by the host: cells_cap, n_ops, N)
     uint32_t* oprec = reinterpret_cast<uint32_t*>(smem);  // [n_ops][kOpRec] per-locus op records
-    T* tab = reinterpret_cast<T*>(oprec + kOpRec * (size_t)n_ops);
-    T* pre = tab + A.cells_cap;
-    T* dist = pre + 4 * (size_t)A.cells_cap;           // [NF][8]
+    T* tab;
+    T* pre;
+    T* dist;                                            // [NF][8]
+    if constexpr (GT) {
+        const size_t slot = (size_t)blockIdx.x + (size_t)gridDim.x * blockIdx.y;
+        tab = reinterpret_cast<T*>(A.gtables) + slot * 5 * (size_t)A.cells_cap;
+        pre = tab + A.cells_cap;
+        dist = reinterpret_cast<T*>(oprec + kOpRec * (size_t)n_ops);
+    } else {
+        tab = reinterpret_cast<T*>(oprec + kOpRec * (size_t)n_ops);
+        pre = tab + A.cells_cap;
+        dist = pre + 4 * (size_t)A.cells_cap;
+    }
     T* trt = dist + 8 * NF;                             // [N][4] trait probabilities at the locus
     double* uop = reinterpret_cast<double*>(trt + 4 * N);  // [n_ops]
     int* off = reinterpret_cast<int*>(uop + n_ops);     // [n_ops+1]
@@ -406,7 +424,7 @@ __global__ void __launch_bounds__(kLsamplerBlockWide) lsampler_kernel(SampArgs A
     __shared__ int s_status;
     __shared__ int s_scale_exp;  // sum of log2 of the table scale factors consumed (RESCALE)
 
-    const int chain = blockIdx.y;
+    const int chain = (int)blockIdx.y + A.chain0;
     const uint32_t sweep = A.impute ? A.sweep : sweep_of(A);
     int locus;
     bool use_left = true, use_right = true;  // condition on column l-1 / l+1
@@ -426,7 +444,7 @@ __global__ void __launch_bounds__(kLsamplerBlockWide) lsampler_kernel(SampArgs A
             const int p0 = philox_uniform(A.seed, (uint32_t)chain, sweep, kStepSched, 1) < 0.5 ? 0 : 1;
             parity = A.pass == 0 ? p0 : 1 - p0;
         }
-        locus = parity + 2 * blockIdx.x;
+        locus = parity + 2 * ((int)blockIdx.x + A.block0);
         if (locus >= M || A.status[chain] != 0) return;  // uniform over the workgroup
         step_id = (uint32_t)locus;
     }
@@ -589,6 +607,20 @@ __global__ void __launch_bounds__(kLsamplerBlockWide) lsampler_kernel(SampArgs A
                 if (pdv[v] != (T)0) last = v;
             }
             pmk[peel] = (uint8_t)(chosen >= 0 ? chosen : last);
+            if (A.near_edge) {
+                // a uniform within rounding of a partial sum of the CDF could
+                // choose differently from the reference's serial FP64 walk
+                T cum = (T)0;
+                bool edge = false;
+                for (int v = 0; v < 3; ++v) {
+                    cum += pdv[v];
+                    if (pdv[v] != (T)0 && fabs(u - (double)cum) < kEdgeTol<T>) edge = true;
+                }
+                if (edge) {
+                    atomicAdd(&A.near_edge[chain], 1u);
+                    atomicOr(&A.near_loci[(size_t)chain * A.near_words + (locus >> 5)], 1u << (locus & 31));
+                }
+            }
         }
         __syncthreads();
     }
@@ -645,11 +677,12 @@ __global__ void __launch_bounds__(kLsamplerBlockWide) lsampler_kernel(SampArgs A
 
 }  // namespace
 
-size_t lsampler_lds_bytes(int precision, int cells_cap, int n_ops, int N, int F, int block) {
+size_t lsampler_lds_bytes(int precision, int cells_cap, int n_ops, int N, int F, int block, bool global_tables) {
     const size_t esz = precision == 0 ? 8 : 4;
     const int NF = N - F;
     size_t b = 4 * (size_t)kOpRec * n_ops;  // op records
-    b += esz * (size_t)cells_cap * 5 + esz * 8 * (size_t)NF + esz * 4 * (size_t)N;
+    if (!global_tables) b += esz * (size_t)cells_cap * 5;
+    b += esz * 8 * (size_t)NF + esz * 4 * (size_t)N;
     b = (b + 7) / 8 * 8;
     b += 8 * (size_t)n_ops;                          // uop
     b += 4 * ((size_t)n_ops + 1 + block + 2 * NF);   // off, scan, mcount
@@ -658,22 +691,30 @@ size_t lsampler_lds_bytes(int precision, int cells_cap, int n_ops, int N, int F,
     return (b + 15) / 16 * 16;
 }
 
-hipError_t configure_lsampler_lds(int precision, size_t lds) {
+hipError_t configure_lsampler_lds(int precision, size_t lds, bool global_tables) {
     if (lds <= 64 * 1024) return hipSuccess;
-    if (precision == 0)
-        return hipFuncSetAttribute((const void*)lsampler_kernel<double, false>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    return hipFuncSetAttribute((const void*)lsampler_kernel<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds);
+    const void* k = precision == 0 ? (global_tables ? (const void*)lsampler_kernel<double, false, true>
+                                                    : (const void*)lsampler_kernel<double, false, false>)
+                                   : (global_tables ? (const void*)lsampler_kernel<float, true, true>
+                                                    : (const void*)lsampler_kernel<float, true, false>);
+    return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-hipError_t launch_lsampler(const SampArgs& A, int precision, int block, size_t lds, int n_chains, hipStream_t s) {
-    const int nloci = A.impute ? 1 : A.check_kind ? (A.M + 1) / 2 : (A.M - A.parity + 1) / 2;
-    dim3 grid(nloci, n_chains);
-    if (precision == 0)
-        hipLaunchKernelGGL((lsampler_kernel<double, false>), grid, dim3(block), lds, s, A);
-    else
-        hipLaunchKernelGGL((lsampler_kernel<float, true>), grid, dim3(block), lds, s, A);
+int lsampler_loci(const SampArgs& A) {
+    return A.impute ? 1 : A.check_kind ? (A.M + 1) / 2 : (A.M - A.parity + 1) / 2;
+}
+
+hipError_t launch_lsampler(const SampArgs& A, int precision, int block, size_t lds, int blocks_x, int blocks_y,
+                           hipStream_t s) {
+    dim3 grid(blocks_x, blocks_y);
+    const bool gt = A.gtables != nullptr;
+    if (precision == 0) {
+        if (gt) hipLaunchKernelGGL((lsampler_kernel<double, false, true>), grid, dim3(block), lds, s, A);
+        else hipLaunchKernelGGL((lsampler_kernel<double, false, false>), grid, dim3(block), lds, s, A);
+    } else {
+        if (gt) hipLaunchKernelGGL((lsampler_kernel<float, true, true>), grid, dim3(block), lds, s, A);
+        else hipLaunchKernelGGL((lsampler_kernel<float, true, false>), grid, dim3(block), lds, s, A);
+    }
     return hipGetLastError();
 }
 

@@ -54,6 +54,9 @@ namespace {
 
 constexpr int G_HETERO = 1, G_HOMOZ_B = 3;  // genotype codes (genotype.h)
 constexpr int kIllegalGraph = -5;
+// |ln L(flipped) - ln L(current)| below which both states stay non-zero in
+// the FP64 pass (exp(-708) is still a normal double)
+constexpr double kFp64DrawLimit = 708.0;
 constexpr int kGenoPrefetch = 4;  // genotype chunks of 64 persons loaded ahead
 
 // the sweep index: an argument, or a device counter advanced between
@@ -418,13 +421,21 @@ __device__ double fag_log_likelihood_r(const MsampArgs& A, const uint2* ptab, co
     return ll;
 }
 
-struct Mat {
-    double a, b, c, d;  // [[a, b], [c, d]]
+// The forward-backward pass runs in FP64 (SL_FP64 contexts: the reference's
+// arithmetic, draws decided as it decides them) or in FP32 (FP32-rescaled
+// contexts): 2x2 products keep their largest entry in [0.5, 1) by an exact
+// power of two in both.  The founder-allele log-likelihoods it reads are FP64
+// in both.
+template <typename T>
+struct MatT {
+    T a, b, c, d;  // [[a, b], [c, d]]
 };
+using Mat = MatT<double>;
 
 // product with an exact power-of-two rescale keeping the largest entry in
 // [0.5, 1): for a normal largest entry m with biased exponent E the factor is
-// 2^(1022-E), built from its bits; a subnormal m takes frexp/ldexp
+// 2^(1022-E) (FP64) or 2^(126-E) (FP32), built from its bits; a subnormal m
+// takes frexp/ldexp
 __device__ __forceinline__ Mat mmul(const Mat& x, const Mat& y) {
     Mat r{x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d, x.c * y.a + x.d * y.c, x.c * y.b + x.d * y.d};
     const double m = fmax(fmax(r.a, r.b), fmax(r.c, r.d));
@@ -442,6 +453,27 @@ __device__ __forceinline__ Mat mmul(const Mat& x, const Mat& y) {
         r.b = ldexp(r.b, -e);
         r.c = ldexp(r.c, -e);
         r.d = ldexp(r.d, -e);
+    }
+    return r;
+}
+
+__device__ __forceinline__ MatT<float> mmul(const MatT<float>& x, const MatT<float>& y) {
+#pragma clang fp contract(fast)
+    MatT<float> r{x.a * y.a + x.b * y.c, x.a * y.b + x.b * y.d, x.c * y.a + x.d * y.c, x.c * y.b + x.d * y.d};
+    const float m = fmaxf(fmaxf(r.a, r.b), fmaxf(r.c, r.d));
+    const uint32_t E = (__float_as_uint(m) >> 23) & 0xFFu;
+    if (E != 0u && E < 253u) {
+        const float f = __uint_as_float((253u - E) << 23);
+        r.a *= f;
+        r.b *= f;
+        r.c *= f;
+        r.d *= f;
+    } else if (m > 0.0f && E == 0u) {
+        const int e = __builtin_amdgcn_frexp_expf(m);
+        r.a = __builtin_amdgcn_ldexpf(r.a, -e);
+        r.b = __builtin_amdgcn_ldexpf(r.b, -e);
+        r.c = __builtin_amdgcn_ldexpf(r.c, -e);
+        r.d = __builtin_amdgcn_ldexpf(r.d, -e);
     }
     return r;
 }
@@ -465,31 +497,41 @@ __device__ __forceinline__ uint32_t u32_dpp(uint32_t old, uint32_t x) {
 }
 
 template <int CTRL, int ROWS>
-__device__ __forceinline__ double f64_dpp(double old, double x) {
+__device__ __forceinline__ double t_dpp(double old, double x) {
     const uint64_t o = __builtin_bit_cast(uint64_t, old), u = __builtin_bit_cast(uint64_t, x);
     const uint32_t lo = u32_dpp<CTRL, ROWS>((uint32_t)o, (uint32_t)u);
     const uint32_t hi = u32_dpp<CTRL, ROWS>((uint32_t)(o >> 32), (uint32_t)(u >> 32));
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-template <int CTRL, int ROWS = 0xf>
-__device__ __forceinline__ Mat mat_dpp(const Mat& x) {  // the identity where there is no source
-    return Mat{f64_dpp<CTRL, ROWS>(1.0, x.a), f64_dpp<CTRL, ROWS>(0.0, x.b), f64_dpp<CTRL, ROWS>(0.0, x.c),
-               f64_dpp<CTRL, ROWS>(1.0, x.d)};
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float t_dpp(float old, float x) {
+    return __uint_as_float(u32_dpp<CTRL, ROWS>(__float_as_uint(old), __float_as_uint(x)));
+}
+
+template <int CTRL, int ROWS = 0xf, typename T>
+__device__ __forceinline__ MatT<T> mat_dpp(const MatT<T>& x) {  // the identity where there is no source
+    return MatT<T>{t_dpp<CTRL, ROWS>((T)1, x.a), t_dpp<CTRL, ROWS>((T)0, x.b), t_dpp<CTRL, ROWS>((T)0, x.c),
+                   t_dpp<CTRL, ROWS>((T)1, x.d)};
 }
 
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t x, int lane) {
     return (uint32_t)__builtin_amdgcn_readlane((int)x, lane);
 }
 
-__device__ __forceinline__ double readlane_f64(double x, int lane) {
+__device__ __forceinline__ double readlane_t(double x, int lane) {
     const uint64_t u = __builtin_bit_cast(uint64_t, x);
     const uint32_t lo = readlane_u32((uint32_t)u, lane), hi = readlane_u32((uint32_t)(u >> 32), lane);
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+__device__ __forceinline__ float readlane_t(float x, int lane) {
+    return __uint_as_float(readlane_u32(__float_as_uint(x), lane));
+}
+
 // inclusive left-to-right products across the wavefront
-__device__ __forceinline__ Mat wave_products(Mat P) {
+template <typename T>
+__device__ __forceinline__ MatT<T> wave_products(MatT<T> P) {
     P = mmul(mat_dpp<dpp_row_shr(1)>(P), P);
     P = mmul(mat_dpp<dpp_row_shr(2)>(P), P);
     P = mmul(mat_dpp<dpp_row_shr(4)>(P), P);
@@ -576,39 +618,62 @@ struct StepView {
 
 // locus matrix from the two states' likelihoods scaled by the larger one:
 // T(theta_{l-1}) diag(e_l), A_0 = diag(e_0)
-__device__ __forceinline__ Mat emission_matrix(const MsampArgs& A, double e0, double e1, int l) {
-    if (l == 0) return Mat{e0, 0.0, 0.0, e1};
-    const double th = A.theta[l - 1], an = 1.0 - A.theta[l - 1];
-    return Mat{an * e0, th * e1, th * e0, an * e1};
+template <typename T>
+__device__ __forceinline__ MatT<T> emission_matrix(const MsampArgs& A, T e0, T e1, int l) {
+    if (l == 0) return MatT<T>{e0, (T)0, (T)0, e1};
+    const T th = (T)A.theta[l - 1], an = (T)(1.0 - A.theta[l - 1]);
+    return MatT<T>{an * e0, th * e1, th * e0, an * e1};
 }
+
+__device__ __forceinline__ double exp_t(double x) { return exp(x); }
+__device__ __forceinline__ float exp_t(float x) { return expf(x); }
 
 // e0, e1 of locus l from the current and flipped columns' log-likelihoods,
 // exp(x - max): exactly 1 for the larger state and exp(-|x0 - x1|) for the
 // other (x1 - x0 is exactly -(x0 - x1)), so one exp per locus.  The current
-// column's is finite; exp(-inf) = 0 for an illegal flip.
-__device__ __forceinline__ void emissions(bool cur, double lcur, double lflip, double* e0, double* e1) {
+// column's is finite; exp(-inf) = 0 for an illegal flip.  The difference is
+// formed in FP64 in both precisions.  Returns whether both states are
+// possible in FP64 terms (the FP32 pass draws where the FP64 one would).
+template <typename T>
+__device__ __forceinline__ bool emissions(bool cur, double lcur, double lflip, T* e0, T* e1) {
     const double x0 = cur ? lflip : lcur, x1 = cur ? lcur : lflip;
     const double d = x0 - x1;
     const bool first = d >= 0.0;
-    const double t = exp(first ? -d : d);
-    *e0 = first ? 1.0 : t;
-    *e1 = first ? t : 1.0;
+    const T t = exp_t((T)(first ? -d : d));
+    *e0 = first ? (T)1 : t;
+    *e1 = first ? t : (T)1;
+    return fabs(d) < kFp64DrawLimit;
 }
 
-// 1 when locus l draws a uniform: both weighted states non-zero (:193-203)
-__device__ __forceinline__ uint32_t locus_draws(const MsampArgs& A, double f0, double f1, int l) {
-    const double t = l < A.M - 1 ? A.theta[l] : 1.0, a = l < A.M - 1 ? 1.0 - A.theta[l] : 1.0;
-    return (f0 * a != 0.0 && f1 * t != 0.0) ? 1u : 0u;
+// 1 when locus l draws a uniform: both weighted states non-zero (:193-203).
+// FP64 decides it from the forward values as the reference does; FP32 takes
+// the FP64 emission test (`both`), so that it consumes the same draws.
+template <typename T>
+__device__ __forceinline__ uint32_t locus_draws(const MsampArgs& A, T f0, T f1, int l, bool both) {
+    if constexpr (sizeof(T) == 4) {
+        return both ? 1u : 0u;
+    } else {
+        const double t = l < A.M - 1 ? A.theta[l] : 1.0, a = l < A.M - 1 ? 1.0 - A.theta[l] : 1.0;
+        return (f0 * a != 0.0 && f1 * t != 0.0) ? 1u : 0u;
+    }
 }
+
+// uniforms within this distance of a threshold are counted as near an edge:
+// the reassociated FP64 sums move thresholds by rounding; FP32 by its epsilon
+// times the depth of the product scan
+template <typename T>
+__device__ __forceinline__ double edge_tol() { return sizeof(T) == 4 ? 1e-5 : 1e-12; }
 
 // 4-bit decision map of locus l: bit bn = new indicator when the next locus's
-// is bn; bit 2+bn = that decision's uniform lay within 1e-12 of its threshold.
-// A drawing locus uses draw index ctr of step k.
-__device__ __forceinline__ uint32_t locus_map(const MsampArgs& A, int chain, int k, double f0, double f1, int l,
-                                              uint32_t ctr, uint32_t* near) {
-    double w0[2], w1[2];
+// is bn; bit 2+bn = that decision's uniform lay within edge_tol of its
+// threshold.  A drawing locus (draws, from locus_draws) uses draw index ctr of
+// step k.
+template <typename T>
+__device__ __forceinline__ uint32_t locus_map(const MsampArgs& A, int chain, int k, T f0, T f1, int l,
+                                              uint32_t ctr, bool draws, uint32_t* near) {
+    T w0[2], w1[2];
     if (l < A.M - 1) {
-        const double th = A.theta[l], an = 1.0 - A.theta[l];
+        const T th = (T)A.theta[l], an = (T)(1.0 - A.theta[l]);
         w0[0] = f0 * an;
         w1[0] = f1 * th;
         w0[1] = f0 * th;
@@ -617,23 +682,27 @@ __device__ __forceinline__ uint32_t locus_map(const MsampArgs& A, int chain, int
         w0[0] = w0[1] = f0;
         w1[0] = w1[1] = f1;
     }
-    // with theta in (0, 1) whether a locus draws does not depend on the next
-    // indicator; a difference (underflow in one branch only) is counted
-    const bool draw = w0[0] != 0.0 && w1[0] != 0.0;
-    if (draw != (w0[1] != 0.0 && w1[1] != 0.0)) ++*near;
-    const double u = draw ? philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), (uint32_t)k, ctr) : 0.0;
+    if constexpr (sizeof(T) == 8) {
+        // with theta in (0, 1) whether a locus draws does not depend on the next
+        // indicator; a difference (underflow in one branch only) is counted
+        if (draws != (w0[1] != 0.0 && w1[1] != 0.0)) ++*near;
+    } else {
+        // an FP32 underflow of a state the FP64 pass keeps is counted
+        if (draws && !(w0[0] != 0.0f && w1[0] != 0.0f && w0[1] != 0.0f && w1[1] != 0.0f)) ++*near;
+    }
+    const double u = draws ? philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), (uint32_t)k, ctr) : 0.0;
     uint32_t map = 0;
 #pragma unroll
     for (int bn = 0; bn < 2; ++bn) {
         uint32_t b, edge = 0;
-        if (w0[bn] == 0.0) {
+        if (w0[bn] == (T)0) {
             b = 1u;
-        } else if (w1[bn] == 0.0) {
+        } else if (w1[bn] == (T)0) {
             b = 0u;
         } else {
-            const double thr = w0[bn] / (w0[bn] + w1[bn]);
+            const double thr = (double)(w0[bn] / (w0[bn] + w1[bn]));
             b = u < thr ? 0u : 1u;
-            edge = fabs(u - thr) < 1e-12 ? 1u : 0u;
+            edge = fabs(u - thr) < edge_tol<T>() ? 1u : 0u;
         }
         map |= (b << bn) | (edge << (2 + bn));
     }
@@ -643,24 +712,25 @@ __device__ __forceinline__ uint32_t locus_map(const MsampArgs& A, int chain, int
 // exclusive left-to-right product of the threads' segment products: within
 // wavefronts by DPP, across them through LDS (by one row of lanes when there
 // are more than four)
-__device__ __forceinline__ Mat scan_products(Mat P, Mat* wtot) {
+template <typename T>
+__device__ __forceinline__ MatT<T> scan_products(MatT<T> P, MatT<T>* wtot) {
     const int lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     P = wave_products(P);
     if (lane == 63) wtot[wave] = P;
     __syncthreads();
-    Mat R{1.0, 0.0, 0.0, 1.0};
+    MatT<T> R{(T)1, (T)0, (T)0, (T)1};
     if (nw <= 4) {
         for (int q = 0; q < wave; ++q) R = mmul(R, wtot[q]);
     } else {
-        Mat T = lane < nw ? wtot[lane] : Mat{1.0, 0.0, 0.0, 1.0};
-        T = mmul(mat_dpp<dpp_row_shr(1)>(T), T);
-        T = mmul(mat_dpp<dpp_row_shr(2)>(T), T);
-        T = mmul(mat_dpp<dpp_row_shr(4)>(T), T);
-        T = mmul(mat_dpp<dpp_row_shr(8)>(T), T);
+        MatT<T> Q = lane < nw ? wtot[lane] : MatT<T>{(T)1, (T)0, (T)0, (T)1};
+        Q = mmul(mat_dpp<dpp_row_shr(1)>(Q), Q);
+        Q = mmul(mat_dpp<dpp_row_shr(2)>(Q), Q);
+        Q = mmul(mat_dpp<dpp_row_shr(4)>(Q), Q);
+        Q = mmul(mat_dpp<dpp_row_shr(8)>(Q), Q);
         const int wu = __builtin_amdgcn_readfirstlane(wave);
         if (wu > 0)
-            R = Mat{readlane_f64(T.a, wu - 1), readlane_f64(T.b, wu - 1), readlane_f64(T.c, wu - 1),
-                    readlane_f64(T.d, wu - 1)};
+            R = MatT<T>{readlane_t(Q.a, wu - 1), readlane_t(Q.b, wu - 1), readlane_t(Q.c, wu - 1),
+                        readlane_t(Q.d, wu - 1)};
     }
     return mmul(R, mat_dpp<kDppWaveShr1>(P));
 }
@@ -709,9 +779,11 @@ __device__ __forceinline__ uint32_t bit_from_above(uint32_t comp, uint32_t* wmap
 // t owns a contiguous segment of loci.  Scans run within wavefronts by lane
 // shuffles and across them through LDS.  SM > 0: segments of at most SM loci
 // whose likelihoods, forward probabilities and maps stay in registers; SM = 0:
-// any length, through the chain's global scratch.
-template <int SM>
-__device__ void forward_backward(const MsampArgs& A, int chain, int k, Mat* wtot, uint32_t* wcnt, uint32_t* wmap) {
+// any length, through the chain's global scratch.  T: the arithmetic of the
+// forward values and thresholds.
+template <int SM, typename T>
+__device__ void forward_backward(const MsampArgs& A, int chain, int k, MatT<T>* wtot, uint32_t* wcnt,
+                                 uint32_t* wmap) {
     const int B = blockDim.x, tid = threadIdx.x, M = A.M;
     const int m = order_at(A, chain, k);
     StepView sv;
@@ -751,29 +823,30 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int k, Mat* wtot
                 }
             }
         }
-        double e0[SM], e1[SM], lfv[SM], f0[SM], f1[SM];
-        uint32_t curm = 0, drm = 0;
+        T e0[SM], e1[SM], f0[SM], f1[SM];
+        double lfv[SM];
+        uint32_t curm = 0, drm = 0, both = 0;
 #pragma unroll
         for (int s = 0; s < SM; ++s) {
             const bool cur = (gw[s] & bm) != 0;
             lfv[s] = (pw[s] & sv.pbm) ? lf1[s] : lf0[s];
-            e0[s] = e1[s] = 0.0;
-            if (a0 + s < a1) emissions(cur, lcv[s], lfv[s], &e0[s], &e1[s]);
+            e0[s] = e1[s] = (T)0;
+            if (a0 + s < a1) both |= (emissions(cur, lcv[s], lfv[s], &e0[s], &e1[s]) ? 1u : 0u) << s;
             curm |= (cur ? 1u : 0u) << s;
         }
-        Mat P{1.0, 0.0, 0.0, 1.0};
+        MatT<T> P{(T)1, (T)0, (T)0, (T)1};
 #pragma unroll
         for (int s = 0; s < SM; ++s)
             if (a0 + s < a1) P = mmul(P, emission_matrix(A, e0[s], e1[s], a0 + s));
-        Mat R = scan_products(P, wtot);
+        MatT<T> R = scan_products(P, wtot);
 #pragma unroll
         for (int s = 0; s < SM; ++s) {
-            f0[s] = f1[s] = 0.0;
+            f0[s] = f1[s] = (T)0;
             if (a0 + s < a1) {
                 R = mmul(R, emission_matrix(A, e0[s], e1[s], a0 + s));
                 f0[s] = R.a + R.c;
                 f1[s] = R.b + R.d;
-                drm |= locus_draws(A, f0[s], f1[s], a0 + s) << s;
+                drm |= locus_draws(A, f0[s], f1[s], a0 + s, ((both >> s) & 1u) != 0u) << s;
             }
         }
         // the backward pass runs from the last locus down, so a drawing
@@ -785,7 +858,8 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int k, Mat* wtot
         for (int s = 0; s < SM; ++s) {
             mp[s] = 0u;
             if (a0 + s < a1)
-                mp[s] = locus_map(A, chain, k, f0[s], f1[s], a0 + s, cnt + (uint32_t)__popc(drm >> (s + 1)), &near);
+                mp[s] = locus_map(A, chain, k, f0[s], f1[s], a0 + s, cnt + (uint32_t)__popc(drm >> (s + 1)),
+                                  ((drm >> s) & 1u) != 0u, &near);
         }
         uint32_t comp = 2u;  // identity
 #pragma unroll
@@ -808,26 +882,32 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int k, Mat* wtot
     } else {
         double* fb = A.fbuf + (size_t)chain * M * 2;
         uint8_t* mp = A.maps + (size_t)chain * M;
-        auto locus = [&](int l) {
-            double e0, e1;
-            emissions((g[(size_t)l * A.W + word] & bm) != 0, lc[l], sv.lf(g, l), &e0, &e1);
+        auto locus = [&](int l, bool* both) {
+            T e0, e1;
+            const bool b = emissions((g[(size_t)l * A.W + word] & bm) != 0, lc[l], sv.lf(g, l), &e0, &e1);
+            if (both) *both = b;
             return emission_matrix(A, e0, e1, l);
         };
-        Mat P{1.0, 0.0, 0.0, 1.0};
-        for (int l = a0; l < a1; ++l) P = mmul(P, locus(l));
-        Mat R = scan_products(P, wtot);
+        MatT<T> P{(T)1, (T)0, (T)0, (T)1};
+        for (int l = a0; l < a1; ++l) P = mmul(P, locus(l, nullptr));
+        MatT<T> R = scan_products(P, wtot);
         uint32_t draws = 0;
         for (int l = a0; l < a1; ++l) {
-            R = mmul(R, locus(l));
-            fb[2 * l] = R.a + R.c;
-            fb[2 * l + 1] = R.b + R.d;
-            draws += locus_draws(A, R.a + R.c, R.b + R.d, l);
+            bool both;
+            R = mmul(R, locus(l, &both));
+            fb[2 * l] = (double)(R.a + R.c);
+            fb[2 * l + 1] = (double)(R.b + R.d);
+            draws += locus_draws(A, R.a + R.c, R.b + R.d, l, both);
         }
         uint32_t cnt = sum_above(draws, wcnt);
         uint32_t comp = 2u;
         for (int l = a1 - 1; l >= a0; --l) {
-            const uint32_t map = locus_map(A, chain, k, fb[2 * l], fb[2 * l + 1], l, cnt, &near);
-            cnt += locus_draws(A, fb[2 * l], fb[2 * l + 1], l);
+            bool both;
+            (void)locus(l, &both);
+            const T f0 = (T)fb[2 * l], f1 = (T)fb[2 * l + 1];
+            const bool dr = locus_draws(A, f0, f1, l, both) != 0u;
+            const uint32_t map = locus_map(A, chain, k, f0, f1, l, cnt, dr, &near);
+            cnt += dr ? 1u : 0u;
             mp[l] = (uint8_t)map;
             comp = compose(map & 3u, comp);
         }
@@ -876,25 +956,25 @@ __global__ void __launch_bounds__(kMsampBlock) mstep_kernel(MsampArgs A) {
 }
 
 // forward-backward pass of step fb_step only; one workgroup per chain
-template <int SM>
+template <int SM, typename T>
 __global__ void __launch_bounds__(kFbBlock) mfb_kernel(MsampArgs A) {
-    __shared__ Mat wtot[kFbBlock / 64];
+    __shared__ MatT<T> wtot[kFbBlock / 64];
     __shared__ uint32_t wcnt[kFbBlock / 64], wmap[kFbBlock / 64];
     __shared__ int s_skip;
     if (skip_chain(A, blockIdx.x, &s_skip)) return;
-    forward_backward<SM>(A, blockIdx.x, A.fb_step, wtot, wcnt, wmap);
+    forward_backward<SM, T>(A, blockIdx.x, A.fb_step, wtot, wcnt, wmap);
 }
 
 // one speculative launch: workgroup 0 of each chain runs step fb_step's
 // forward-backward pass while the others run the jobs of step fb_step + 1
-template <int SM, int KC>
+template <int SM, int KC, typename T>
 __global__ void __launch_bounds__(kSpecBlock) mspec_kernel(MsampArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ Mat wtot[kSpecBlock / 64];
+    __shared__ MatT<T> wtot[kSpecBlock / 64];
     __shared__ uint32_t wcnt[kSpecBlock / 64], wmap[kSpecBlock / 64];
     __shared__ int s_skip;
     if (skip_chain(A, blockIdx.y, &s_skip)) return;
-    if (blockIdx.x == 0) forward_backward<SM>(A, blockIdx.y, A.fb_step, wtot, wcnt, wmap);
+    if (blockIdx.x == 0) forward_backward<SM, T>(A, blockIdx.y, A.fb_step, wtot, wcnt, wmap);
     else run_jobs<KC>(A, blockIdx.y, blockIdx.x - 1, smem);
 }
 
@@ -962,28 +1042,52 @@ hipError_t set_lds(const void* kernel, size_t b) {
     return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b);
 }
 
-template <int KC>
-hipError_t set_lds_kc(size_t job, size_t spec) {
-    hipError_t e = set_lds((const void*)mstep_kernel<KC>, job);
-    if (spec == 0) return e;
-    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<0, KC>, spec);
-    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<1, KC>, spec);
-    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<2, KC>, spec);
-    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<4, KC>, spec);
-    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<8, KC>, spec);
+template <int KC, typename T>
+hipError_t set_lds_spec(size_t spec) {
+    hipError_t e = set_lds((const void*)mspec_kernel<0, KC, T>, spec);
+    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<1, KC, T>, spec);
+    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<2, KC, T>, spec);
+    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<4, KC, T>, spec);
+    if (e == hipSuccess) e = set_lds((const void*)mspec_kernel<8, KC, T>, spec);
     return e;
 }
 
 template <int KC>
+hipError_t set_lds_kc(size_t job, size_t spec) {
+    hipError_t e = set_lds((const void*)mstep_kernel<KC>, job);
+    if (spec == 0) return e;
+    if (e == hipSuccess) e = set_lds_spec<KC, double>(spec);
+    if (e == hipSuccess) e = set_lds_spec<KC, float>(spec);
+    return e;
+}
+
+template <int KC, typename T>
 void launch_mspec_kc(const MsampArgs& A, size_t lds, hipStream_t s) {
     const int nw = kSpecBlock / 64;
     const int seg = (A.M + kSpecBlock - 1) / kSpecBlock;
     const dim3 grid(1 + (A.n_jobs + nw - 1) / nw, A.n_chains), block(kSpecBlock);
-    if (seg <= 1) hipLaunchKernelGGL((mspec_kernel<1, KC>), grid, block, lds, s, A);
-    else if (seg <= 2) hipLaunchKernelGGL((mspec_kernel<2, KC>), grid, block, lds, s, A);
-    else if (seg <= 4) hipLaunchKernelGGL((mspec_kernel<4, KC>), grid, block, lds, s, A);
-    else if (seg <= 8) hipLaunchKernelGGL((mspec_kernel<8, KC>), grid, block, lds, s, A);
-    else hipLaunchKernelGGL((mspec_kernel<0, KC>), grid, block, lds, s, A);
+    if (seg <= 1) hipLaunchKernelGGL((mspec_kernel<1, KC, T>), grid, block, lds, s, A);
+    else if (seg <= 2) hipLaunchKernelGGL((mspec_kernel<2, KC, T>), grid, block, lds, s, A);
+    else if (seg <= 4) hipLaunchKernelGGL((mspec_kernel<4, KC, T>), grid, block, lds, s, A);
+    else if (seg <= 8) hipLaunchKernelGGL((mspec_kernel<8, KC, T>), grid, block, lds, s, A);
+    else hipLaunchKernelGGL((mspec_kernel<0, KC, T>), grid, block, lds, s, A);
+}
+
+template <int KC>
+void launch_mspec_kc(const MsampArgs& A, size_t lds, hipStream_t s) {
+    if (A.fp32) launch_mspec_kc<KC, float>(A, lds, s);
+    else launch_mspec_kc<KC, double>(A, lds, s);
+}
+
+template <typename T>
+void launch_mfb_t(const MsampArgs& A, hipStream_t s) {
+    const int seg = (A.M + kFbBlock - 1) / kFbBlock;
+    const dim3 grid(A.n_chains), block(kFbBlock);
+    if (seg <= 1) hipLaunchKernelGGL((mfb_kernel<1, T>), grid, block, 0, s, A);
+    else if (seg <= 2) hipLaunchKernelGGL((mfb_kernel<2, T>), grid, block, 0, s, A);
+    else if (seg <= 4) hipLaunchKernelGGL((mfb_kernel<4, T>), grid, block, 0, s, A);
+    else if (seg <= 8) hipLaunchKernelGGL((mfb_kernel<8, T>), grid, block, 0, s, A);
+    else hipLaunchKernelGGL((mfb_kernel<0, T>), grid, block, 0, s, A);
 }
 
 }  // namespace
@@ -1023,13 +1127,8 @@ hipError_t launch_mstep(const MsampArgs& A, size_t lds, hipStream_t s) {
 }
 
 hipError_t launch_mfb(const MsampArgs& A, hipStream_t s) {
-    const int seg = (A.M + kFbBlock - 1) / kFbBlock;
-    const dim3 grid(A.n_chains), block(kFbBlock);
-    if (seg <= 1) hipLaunchKernelGGL(mfb_kernel<1>, grid, block, 0, s, A);
-    else if (seg <= 2) hipLaunchKernelGGL(mfb_kernel<2>, grid, block, 0, s, A);
-    else if (seg <= 4) hipLaunchKernelGGL(mfb_kernel<4>, grid, block, 0, s, A);
-    else if (seg <= 8) hipLaunchKernelGGL(mfb_kernel<8>, grid, block, 0, s, A);
-    else hipLaunchKernelGGL(mfb_kernel<0>, grid, block, 0, s, A);
+    if (A.fp32) launch_mfb_t<float>(A, s);
+    else launch_mfb_t<double>(A, s);
     return hipGetLastError();
 }
 

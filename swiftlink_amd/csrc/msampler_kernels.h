@@ -53,6 +53,7 @@ struct MsampArgs {
     int people_chunks;            // 2, 3, 4, 6, 8, 12, 16: graphs built in registers for at most 64x
                                   // that many people and 32x that many founders; 0: in LDS, any size
     int n_chains;
+    int fp32;                     // 1: the forward-backward pass in FP32 (FP32-rescaled contexts)
 };
 
 bool msampler_supported(int N, int F);

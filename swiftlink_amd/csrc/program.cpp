@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <map>
 
 namespace sl {
@@ -311,6 +312,116 @@ std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int
     }
     P.sched_off[(size_t)P.n_levels * waves] = (int)P.sched_ops.size();
     P.arena_elems = arena.peak;
+    {
+        // sequential order: post-order of the consumption tree from the last
+        // op, each op's consumed subtrees visited in decreasing order of
+        // (peak live cells of the subtree - cells of its table), which keeps
+        // the live set small (the ordering rule of Sethi and Ullman)
+        std::vector<std::vector<int>> kids(n);
+        for (int o = 0; o < n; ++o)
+            for (int q = pv.prev_off[o]; q < pv.prev_off[o + 1]; ++q) kids[o].push_back(pv.prev[q]);
+        std::vector<int> need(n, 0), post;
+        post.reserve(n);
+        // iterative post-order: need[] of the children first
+        std::vector<std::pair<int, int>> st{{n - 1, 0}};
+        while (!st.empty()) {
+            auto& top = st.back();
+            if (top.second < (int)kids[top.first].size()) {
+                const int ch = kids[top.first][top.second++];
+                st.push_back({ch, 0});
+                continue;
+            }
+            const int o = top.first;
+            st.pop_back();
+            std::vector<int>& ch = kids[o];
+            std::stable_sort(ch.begin(), ch.end(),
+                             [&](int a, int b) { return need[a] - size[a] > need[b] - size[b]; });
+            int run = 0, peak = 0;
+            for (int c : ch) {
+                peak = std::max(peak, run + need[c]);
+                run += size[c];
+            }
+            need[o] = std::max(peak, run + size[o]);
+        }
+        // the evaluation order: children in the sorted order, then the op
+        std::vector<std::pair<int, int>> st2{{n - 1, 0}};
+        while (!st2.empty()) {
+            auto& top = st2.back();
+            if (top.second < (int)kids[top.first].size()) {
+                const int ch = kids[top.first][top.second++];
+                st2.push_back({ch, 0});
+                continue;
+            }
+            post.push_back(top.first);
+            st2.pop_back();
+        }
+        if ((int)post.size() != n) return "sequential order does not reach every op";
+        Arena sa;
+        std::vector<int> soff(n, -1);
+        P.seq.clear();
+        for (int o : post) {
+            soff[o] = sa.alloc(size[o]);
+            for (int q = pv.prev_off[o]; q < pv.prev_off[o + 1]; ++q) sa.release(soff[pv.prev[q]], size[pv.prev[q]]);
+        }
+        for (int o : post) {
+            const OpDesc& d = P.ops[o];
+            SeqOp s{};
+            s.type = d.type;
+            s.n_prev = d.n_prev;
+            s.size = d.size;
+            s.out_off = soff[o];
+            s.idx_base = d.idx_base;
+            s.tinfo_base = d.task_base * 4;
+            s.peel = d.peel;
+            s.n_child = d.n_child;
+            for (int q = 0; q < d.n_prev; ++q) s.prev_off[q] = soff[P.prevs[d.prev_off + q].op];
+            for (int ch = 0; ch < d.n_child; ++ch) s.child[ch] = P.children[d.child_off + ch].person;
+            P.seq.push_back(s);
+        }
+        P.seq_elems = sa.peak;
+        // the descriptor stream of the lane kernel
+        std::vector<uint32_t>& S = P.lstream;
+        S.clear();
+        auto pad_to = [&](size_t m) {
+            while (S.size() % m) S.push_back(0u);
+        };
+        for (size_t i = 0; i < post.size(); ++i) {
+            const int o = post[i];
+            const OpDesc& d = P.ops[o];
+            const SeqOp& so = P.seq[i];
+            const int words = 4 + 2 * d.n_prev;
+            const int stride = words <= 8 ? 8 : words <= 16 ? 16 : 32;
+            const size_t h = S.size();
+            S.resize(h + kLsHeader, 0u);
+            S[h + 0] = (uint32_t)d.type;
+            S[h + 1] = (uint32_t)d.n_prev;
+            S[h + 2] = (uint32_t)d.size;
+            S[h + 3] = (uint32_t)so.out_off;
+            S[h + 4] = (uint32_t)d.peel;
+            S[h + 5] = (uint32_t)d.n_child;
+            S[h + 6] = (uint32_t)stride;
+            for (int q = 0; q < d.n_prev; ++q) S[h + 8 + q] = (uint32_t)so.prev_off[q];
+            for (int ch = 0; ch < d.n_child; ++ch) S[h + 16 + ch] = (uint32_t)so.child[ch];
+            for (int v = 0; v < 4; ++v) {
+                uint64_t bits;
+                const double x = ped.disease_prob[4 * d.peel + v];
+                memcpy(&bits, &x, 8);
+                S[h + 24 + 2 * v] = (uint32_t)bits;
+                S[h + 25 + 2 * v] = (uint32_t)(bits >> 32);
+            }
+            for (int c = 0; c < d.size; ++c) {
+                const size_t b = S.size();
+                S.resize(b + stride, 0u);
+                for (int t = 0; t < 4; ++t) S[b + t] = P.tinfo[(size_t)(d.task_base + c) * 4 + t];
+                for (int e = 0; e < 4 * d.n_prev; ++e) {
+                    const uint32_t ix = P.idx[(size_t)d.idx_base + (size_t)c * 4 * d.n_prev + e];
+                    S[b + 4 + e / 2] |= ix << (16 * (e & 1));
+                }
+            }
+            pad_to(32);
+        }
+        S.resize(S.size() + 3 * 64, 0u);
+    }
     P.flops_per_eval = flops;
     P.lds_bytes_per_eval = lds;  // in elements; scaled by the element size by the caller
     return "";

@@ -29,6 +29,7 @@ namespace sl {
 constexpr int kMaxCut = 7;       // cutset size limit: k+1 digits fit 16 bits
 constexpr int kMaxPrev = 8;
 constexpr int kMaxPos = 16;      // trait positions per interval
+constexpr int kLsHeader = 32;    // words of an op header in the lane kernel's stream
 
 enum OpType : int32_t { kChild = 1, kParent = 2, kPartner = 3, kLast = 4 };
 
@@ -63,6 +64,25 @@ struct ChildDesc {
     int32_t peel_is_mother;
 };
 
+// One op of the sequential form of a program: a single wavefront evaluates
+// every op in this order for 64 (graph, position) evaluations at once, one
+// per lane, each lane keeping its own tables in its own column of the LDS
+// arena (cell c of a table at offset out_off lives at word (out_off+c)*64 +
+// lane).  Offsets are in cells of one evaluation.
+struct SeqOp {
+    int32_t type;
+    int32_t n_prev;
+    int32_t size;                 // compact cells
+    int32_t out_off;              // arena cell offset of this op's table
+    int32_t idx_base;             // into idx[] (as OpDesc)
+    int32_t tinfo_base;           // into tinfo[]: 4 words per cell
+    int32_t peel;                 // peel-node person
+    int32_t n_child;              // PARENT
+    int32_t prev_off[kMaxPrev];   // arena cell offsets of the consumed tables, in prev order
+    int32_t child[7];             // PARENT: children in cutset order
+    int32_t pad;
+};
+
 // Host-side compiled program.
 struct PeelProgram {
     int n_people = 0, n_founders = 0;
@@ -87,6 +107,20 @@ struct PeelProgram {
     std::vector<int32_t> sched_ops;     // ops of level L run by wave w: sched_off[L*waves+w] ..
     std::vector<uint8_t> lod_mask;      // [N] trait values with non-zero disease prob
     int arena_elems = 0;                // per position-set (all positions)
+    // sequential form: ops in an order that keeps few tables alive (each
+    // op's consumed subtrees in decreasing order of their peak minus their
+    // table), first-fit arena offsets, and the arena's extent in cells
+    std::vector<SeqOp> seq;
+    int seq_elems = 0;
+    // The lane kernel's descriptor stream: per op in sequential order a
+    // 32-word header (kLsHeader: type, n_prev, size, out_off, peel, n_child,
+    // cell stride, 0, consumed-table offsets [8], children [7], 0, the peel
+    // node's four disease probabilities as FP64 bit pairs), then each cell's
+    // four term words and its 4 x n_prev consumed-cell indices (u16 pairs) in
+    // a stride of 8, 16 or 32 words, the op padded to 32 words; three zero
+    // windows of 64 words close it (the stream is read 64 words at a time,
+    // prefetched two windows ahead).
+    std::vector<uint32_t> lstream;
     double flops_per_eval = 0.0;        // algorithmic flops per (graph, interval, position)
     double lds_bytes_per_eval = 0.0;    // table reads per evaluation, in elements
 };

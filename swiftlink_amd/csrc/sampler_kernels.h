@@ -21,8 +21,8 @@ struct SampArgs {
     const int32_t* mother;       // [N]
     const int32_t* father;       // [N]
     const uint8_t* legal_t;      // [M][N] genotype-elimination masks
-    const double* geno_prob_d;   // [N][M][4] sampler trait probabilities
-    const float* geno_prob_f;
+    const double* geno_prob_d;   // [M][N][4] sampler trait probabilities (FP64 contexts)
+    const float* geno_prob_f;    // the same in FP32 (FP32-rescaled contexts)
     const double* theta;         // [M-1]
     uint32_t* graphs;            // [chains][M][W]
     int32_t* status;             // [chains]
@@ -43,6 +43,18 @@ struct SampArgs {
     uint32_t chain_base;
     uint32_t* imp_draw;          // [chains] next draw index
     double* imp_weight;          // [chains] running sum of ln L
+    // a launch covers loci blocks block0.. (locus parity + 2 (blockIdx.x +
+    // block0)) of chains chain0.. (blockIdx.y + chain0)
+    int block0, chain0;
+    // per chain: draws near a CDF edge (counted) and the loci where they fell
+    // (bit l of near_loci[chain][near_words]); unset for imputation
+    uint32_t* near_edge;
+    uint32_t* near_loci;
+    int near_words;
+    // when set, each workgroup's tables and presums live in global memory,
+    // slot (blockIdx.x + gridDim.x blockIdx.y) of 5 cells_cap entries
+    // (pedigrees whose tables exceed LDS at some locus)
+    void* gtables;
 };
 
 // L-sampler workgroup: four wavefronts share each peel level; launches of
@@ -61,8 +73,14 @@ constexpr int kLsamplerWideCells = 8;
 // 16 bits because the tables fit LDS.
 constexpr int kOpRec = 16;
 
-size_t lsampler_lds_bytes(int precision, int cells_cap, int n_ops, int N, int F, int block);
-hipError_t configure_lsampler_lds(int precision, size_t lds);
-hipError_t launch_lsampler(const SampArgs& A, int precision, int block, size_t lds, int n_chains, hipStream_t s);
+size_t lsampler_lds_bytes(int precision, int cells_cap, int n_ops, int N, int F, int block, bool global_tables);
+hipError_t configure_lsampler_lds(int precision, size_t lds, bool global_tables);
+// loci blocks of one launch's grid (1 for a sequential-imputation step)
+int lsampler_loci(const SampArgs& A);
+hipError_t launch_lsampler(const SampArgs& A, int precision, int block, size_t lds, int blocks_x, int blocks_y,
+                           hipStream_t s);
+// workgroups of one launch when the tables live in global memory (bounds
+// the slots' memory)
+constexpr int kGlobalTableSlots = 2048;
 
 }  // namespace sl

@@ -21,6 +21,7 @@ SL_FP64 = 0
 SL_FP32_RESCALED = 1
 SL_MSAMPLER_AUTO, SL_MSAMPLER_SPECULATIVE, SL_MSAMPLER_TWO_LAUNCH = 0, 1, 2
 SL_MSAMPLER_LDS_GRAPHS = 4  # or-ed into a mode: founder-allele graphs built in LDS
+SL_LOD_AUTO, SL_LOD_WORKGROUP, SL_LOD_LANES = 0, 1, 2
 SL_KERNEL_LOD, SL_KERNEL_LSAMPLER, SL_KERNEL_MSAMPLER, SL_KERNEL_MSAMPLER_FB = 0, 1, 2, 3
 SL_OK, SL_E_INVALID, SL_E_UNSUPPORTED, SL_E_DEVICE = 0, -1, -2, -3
 SL_E_ZERO_LIKELIHOOD, SL_E_ILLEGAL_GRAPH, SL_E_CAPACITY = -4, -5, -6
@@ -50,7 +51,7 @@ class PeelDesc(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("device", C.c_int32), ("block_threads", C.c_int32),
-                ("graphs_per_block", C.c_int32), ("msampler_mode", C.c_int32)]
+                ("graphs_per_block", C.c_int32), ("msampler_mode", C.c_int32), ("lod_kernel", C.c_int32)]
 
 
 class Schedule(C.Structure):
@@ -104,6 +105,7 @@ _SIGS = {
     "sl_mcmc_run": (C.c_int, [C.c_void_p, C.POINTER(Schedule), C.POINTER(McmcStats)]),
     "sl_mcmc_likelihood": (C.c_int, [C.c_void_p, f64p]),
     "sl_mcmc_near_edge": (C.c_int, [C.c_void_p, i64p]),
+    "sl_mcmc_near_edge_loci": (C.c_int, [C.c_void_p, u32p]),
     "sl_set_options": (C.c_int, [C.c_void_p, C.POINTER(Options)]),
     "sl_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]),
     "sl_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Schedule), C.POINTER(McmcStats)]),
@@ -200,10 +202,10 @@ class Context:
     """One sl_ctx: a pedigree, its map and peel program on one GPU."""
 
     def __init__(self, case: Case, precision: int = SL_FP64, device: int = 0, block: int = 0, gpw: int = 0,
-                 msampler_mode: int = SL_MSAMPLER_AUTO):
+                 msampler_mode: int = SL_MSAMPLER_AUTO, lod_kernel: int = SL_LOD_AUTO):
         self.case = case
         self._d = _Descs(case)
-        opts = Options(precision, device, block, gpw, msampler_mode)
+        opts = Options(precision, device, block, gpw, msampler_mode, lod_kernel)
         h = C.c_void_p()
         rc = lib().sl_create(C.byref(self._d.ped), C.byref(self._d.map), C.byref(self._d.peel), C.byref(opts),
                              C.byref(h))
@@ -211,13 +213,14 @@ class Context:
             raise SlError(f"sl_create failed ({rc}): {lib().sl_last_error(None).decode()}")
         self.h = h
         self.precision, self.device = precision, device
-        self.options = {"block": block, "gpw": gpw, "msampler_mode": msampler_mode}
+        self.options = {"block": block, "gpw": gpw, "msampler_mode": msampler_mode, "lod_kernel": lod_kernel}
 
-    def set_options(self, block: int = 0, gpw: int = 0, msampler_mode: int = SL_MSAMPLER_AUTO):
+    def set_options(self, block: int = 0, gpw: int = 0, msampler_mode: int = SL_MSAMPLER_AUTO,
+                    lod_kernel: int = SL_LOD_AUTO):
         """New launch shapes (sl_set_options); chains and accumulators are kept."""
-        opts = Options(self.precision, self.device, block, gpw, msampler_mode)
+        opts = Options(self.precision, self.device, block, gpw, msampler_mode, lod_kernel)
         self._check(lib().sl_set_options(self.h, C.byref(opts)), "sl_set_options")
-        self.options = {"block": block, "gpw": gpw, "msampler_mode": msampler_mode}
+        self.options = {"block": block, "gpw": gpw, "msampler_mode": msampler_mode, "lod_kernel": lod_kernel}
 
     def _check(self, rc, what):
         if rc:
@@ -371,6 +374,14 @@ class Context:
         out = np.zeros(self.n_chains, np.int64)
         self._check(lib().sl_mcmc_near_edge(self.h, _ptr(out, i64p)), "sl_mcmc_near_edge")
         return out
+
+    def mcmc_near_edge_loci(self) -> np.ndarray:
+        """[C][M] booleans: loci where the L-sampler drew near a CDF edge."""
+        nw = (self.case.M + 31) // 32
+        bits = np.zeros((self.n_chains, nw), np.uint32)
+        self._check(lib().sl_mcmc_near_edge_loci(self.h, _ptr(bits, u32p)), "sl_mcmc_near_edge_loci")
+        b = np.unpackbits(bits.view(np.uint8), axis=1, bitorder="little")
+        return b[:, :self.case.M].astype(bool)
 
     def lod_work(self):
         f = C.c_double(0)

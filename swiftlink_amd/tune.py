@@ -4,7 +4,8 @@ For a context holding its chains, times on the device:
   * the LOD scoring of the chains' graphs for every LOD workgroup shape
     (block_threads x graphs_per_block): the block size sets how many
     wavefronts share a peel level, graphs_per_block how many graphs one
-    workgroup folds (its LDS arena is reused across them);
+    workgroup folds (its LDS arena is reused across them); and the lane
+    kernel, one (graph, position) evaluation per lane;
   * one M-sweep of every chain in each M-sampler launch mode (one launch per
     step with speculative likelihoods, or two launches per step).
 The chains' graphs are restored afterwards (and the LOD accumulators
@@ -52,26 +53,31 @@ def _msweep_ms(ctx, seed: int, sweeps: int) -> float:
 def autotune(ctx: "native.Context", blocks: Sequence[int] = LOD_BLOCKS, gpws: Sequence[int] = LOD_GPW,
              m_modes: Sequence[int] = M_MODES, lod_reps: int = 3, m_sweeps: int = 2) -> Dict:
     """Picks and applies the fastest shapes for ctx's chains; returns the
-    choice and every timing (ms)."""
+    choice and every timing (ms).  The LOD candidates are the workgroup
+    kernel at every (block, graphs per block) and the lane kernel (one
+    evaluation per lane), which serves any chain count at which it fits."""
     graphs = ctx.mcmc_graphs()
     lod = {}
-    for b, g in itertools.product(blocks, gpws):
+    cands = [(b, g, native.SL_LOD_WORKGROUP) for b, g in itertools.product(blocks, gpws)]
+    cands.append((ctx.options["block"] or 128, ctx.options["gpw"] or 1, native.SL_LOD_LANES))
+    for b, g, k in cands:
         try:
-            ctx.set_options(b, g, ctx.options["msampler_mode"])
+            ctx.set_options(b, g, ctx.options["msampler_mode"], k)
         except native.SlError:
             continue  # e.g. a block the LDS arena cannot hold
-        lod[(b, g)] = _lod_ms(ctx, graphs, lod_reps)
+        lod[(b, g, k)] = _lod_ms(ctx, graphs, lod_reps)
     best_lod = min(lod, key=lod.get)
     msw = {}
     for m in m_modes:
-        ctx.set_options(best_lod[0], best_lod[1], m)
+        ctx.set_options(best_lod[0], best_lod[1], m, best_lod[2])
         try:
             msw[m] = _msweep_ms(ctx, 1, m_sweeps)
         except native.SlError:
             continue
     best_m = min(msw, key=msw.get) if msw else native.SL_MSAMPLER_AUTO
-    ctx.set_options(best_lod[0], best_lod[1], best_m)
+    ctx.set_options(best_lod[0], best_lod[1], best_m, best_lod[2])
     ctx.mcmc_init(graphs)  # the chains as they were
     ctx.lod_reset()
-    return {"block": best_lod[0], "gpw": best_lod[1], "msampler_mode": best_m,
-            "lod_ms": {f"{b}x{g}": v for (b, g), v in lod.items()}, "msweep_ms": {int(k): v for k, v in msw.items()}}
+    name = lambda b, g, k: "lanes" if k == native.SL_LOD_LANES else f"{b}x{g}"
+    return {"block": best_lod[0], "gpw": best_lod[1], "msampler_mode": best_m, "lod_kernel": best_lod[2],
+            "lod_ms": {name(*key): v for key, v in lod.items()}, "msweep_ms": {int(k): v for k, v in msw.items()}}

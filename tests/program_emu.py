@@ -14,6 +14,8 @@ import numpy as np
 OPD = 12   # ints per OpDesc
 PRD = 12   # ints per PrevDesc
 CHD = 4    # ints per ChildDesc
+SQD = 24   # ints per SeqOp: type, n_prev, size, out_off, idx_base, tinfo_base, peel, n_child,
+           # prev_off[8], child[7], pad
 
 AFF = 0x9C
 KT = 0x6C
@@ -51,7 +53,11 @@ class Program:
         p += 1 + nti
         nix = w[p]
         self.idx = w[p + 1: p + 1 + nix]
-        assert p + 1 + nix == len(w)
+        p += 1 + nix
+        self.seq_elems = w[p]
+        p += 1
+        self.seq = [w[p + SQD * i: p + SQD * (i + 1)] for i in range(self.n_ops)]
+        assert p + SQD * self.n_ops == len(w)
 
     def check_terms(self, case):
         """The precompiled term words and table indices agree with what the
@@ -211,4 +217,106 @@ class Program:
             Lv = float(arena[last[4] + pos * last[5]])
             e = sum(scale(o, pos)[1] for o in range(self.n_ops) if o != self.last_op) if fp32 else 0
             out.append(math.log(Lv) + e * math.log(2.0))
+        return np.array(out)
+
+    # ------------------------------------------------------------ sequential form
+    def check_seq(self):
+        """The sequential form runs every op once, after the tables it
+        consumes, and no two tables alive at the same time share arena cells;
+        every touched cell lies inside seq_elems."""
+        by_op = {}
+        for s in self.seq:
+            typ, n_prev, size, out, idx_base, tinfo_base, peel, n_child = s[:8]
+            by_op[tinfo_base // 4] = s
+        live = {}  # out offset -> (size, op position)
+        done_offsets = []
+        for pos, s in enumerate(self.seq):
+            typ, n_prev, size, out = s[:4]
+            assert 0 <= out and out + size <= self.seq_elems
+            consumed = s[8:8 + n_prev]
+            for off in consumed:
+                assert off in live, (pos, off)
+            for off, (sz, _) in live.items():
+                if off in consumed:
+                    continue
+                assert out + size <= off or off + sz <= out, ("overlap", pos)
+            for off in consumed:
+                # the op's output may not overlap what it reads
+                sz = live[off][0]
+                assert out + size <= off or off + sz <= out, ("reads its own output", pos)
+            for off in consumed:
+                del live[off]
+            live[out] = (size, pos)
+        assert len(live) == 1 and self.seq[-1][2] == 1
+
+    def run_seq(self, case, dg, locus, fp32=False):
+        """The lane kernel's arithmetic for one (graph, position) per lane:
+        ops in sequential order, cells in order, FP32 tables renormalised by a
+        power of two when their largest cell leaves [2^-6, 2^6).  Returns ln L
+        per position."""
+        npos = case.nlod
+        N, F = case.N, case.F
+        ftype = np.float32 if fp32 else float
+        trait = case.disease_prob.astype(np.float32) if fp32 else case.disease_prob
+        code = [0] * N
+        col0, col1 = dg[locus], dg[locus + 1]
+        for p in range(F, N):
+            code[p] = int(col0[2 * p]) | (int(col1[2 * p]) << 1) | (int(col0[2 * p + 1]) << 2) | (int(col1[2 * p + 1]) << 3)
+        pt = case.partial_theta[locus]
+        q = ftype(0.25)
+        out = []
+        for pos in range(npos):
+            tl, tr = pt * (pos + 1), pt * (npos - pos)
+            t1, a1, t2, a2 = ftype(tl), ftype(1.0 - tl), ftype(tr), ftype(1.0 - tr)
+
+            def qrho(cd, i, j):
+                r = ftype(1.0)
+                r = ftype(r * (a1 if (cd & 1) == i else t1))
+                r = ftype(r * (a2 if ((cd >> 1) & 1) == i else t2))
+                r = ftype(r * (a1 if ((cd >> 2) & 1) == j else t1))
+                r = ftype(r * (a2 if ((cd >> 3) & 1) == j else t2))
+                return ftype(q * r)
+
+            arena = [ftype(0)] * self.seq_elems
+            E = 0
+            for s in self.seq:
+                typ, n_prev, size, out_off, idx_base, tinfo_base, peel, n_child = s[:8]
+                poff = s[8:8 + n_prev]
+                kids = s[16:16 + n_child]
+                qr = [qrho(code[peel], t >> 1, t & 1) for t in range(4)] if typ == 1 else None
+                r4 = [[qrho(code[kk], ij >> 1, ij & 1) for ij in range(4)] for kk in kids]
+                mx = ftype(0)
+                for c in range(size):
+                    total = ftype(0.0)
+                    for term in range(4):
+                        info = self.tinfo[tinfo_base + 4 * c + term]
+                        if not info & 4:
+                            continue
+                        tmp = ftype(trait[peel, info & 3])
+                        for qq in range(n_prev):
+                            ix = self.idx[idx_base + (c * 4 + term) * n_prev + qq]
+                            tmp = ftype(tmp * arena[poff[qq] + ix])
+                        if typ == 1:
+                            tmp = ftype(tmp * qr[term])
+                        elif typ == 2:
+                            cp = ftype(1.0)
+                            for ch in range(n_child):
+                                m = (info >> (4 + 4 * ch)) & 15
+                                ct = ftype(0.0)
+                                for ij in range(4):
+                                    if (m >> ij) & 1:
+                                        ct = ftype(ct + r4[ch][ij])
+                                cp = ftype(cp * ct)
+                            tmp = ftype(tmp * cp)
+                        total = ftype(total + tmp)
+                    arena[out_off + c] = total
+                    mx = max(mx, total)
+                if fp32 and mx > 0:
+                    e = math.frexp(float(mx))[1] - 1  # mx in [2^e, 2^(e+1))
+                    if e < -6 or e >= 6:
+                        for c in range(size):
+                            arena[out_off + c] = ftype(arena[out_off + c] * ftype(2.0 ** -e))
+                        E += e
+            L = float(arena[self.seq[-1][3]])
+            out.append(math.log(L) + E * math.log(2.0))
         return np.array(out)

@@ -98,7 +98,8 @@ def test_autotune_keeps_chains_and_results():
         a.mcmc_init(starts)
         tu = autotune(a, blocks=(64, 256), gpws=(1, 4))
         assert np.array_equal(a.mcmc_graphs(), starts)
-        assert a.options == {"block": tu["block"], "gpw": tu["gpw"], "msampler_mode": tu["msampler_mode"]}
+        assert a.options == {"block": tu["block"], "gpw": tu["gpw"], "msampler_mode": tu["msampler_mode"],
+                             "lod_kernel": tu["lod_kernel"]}
         b.mcmc_init(starts)
         for c in (a, b):
             c.lod_reset()

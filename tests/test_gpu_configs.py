@@ -1,0 +1,352 @@
+"""Parity at the full workload shapes the metric names, against the oracle.
+
+* 16 pedigrees of 40-400 people (1-6 loops) and 500-5000 SNPs at 0.1-0.5 cM,
+  4 chains each (64 chains): the captured batch equals sl_mcmc_run on every
+  context bit for bit, in FP64 and FP32-rescaled; FP64 chains of the
+  smallest, a middle and the 400-person pedigree equal the oracle's chains
+  (the 400-person one: whole L-sweeps, and M-steps of chosen meioses); the
+  FP32 trait peel on their graphs is within 1e-4 LOD of the FP64 oracle.
+* 200 people, 7 generations, 4 loops, 2000 SNPs: the adjoint of the trait
+  peel against the oracle's forward-mode gradient at 200 sampled (graph,
+  interval, position) triples, the LOD gradient against central finite
+  differences with step 1e-6 (relative error at most 1e-5), and a MOD ascent
+  that raises the LOD at every accepted step.
+* 600 people, 8 generations, 10 loops, 20000 SNPs at 0.1 cM, 20% missing:
+  the context builds; an FP64 L-sweep equals the oracle's steps at sampled
+  loci of both parities; the founder-allele likelihood of every locus equals
+  the oracle's; FP64 M-steps of chosen meioses equal the oracle's; the FP32
+  trait peel is within 1e-4 LOD of the oracle's power-of-two-rescaled FP64
+  peel (the plain FP64 likelihood, about 2^-1076 here, underflows).
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle
+from swiftlink_amd import native, synth
+from swiftlink_amd.case import pack_graphs
+from swiftlink_amd.mod import maximise
+
+pytestmark = pytest.mark.gpu
+
+LN10 = np.log(10.0)
+LOD_BAR = 1e-4          # |dLOD| of FP32-rescaled against FP64
+GRAD_BAR = 1e-5         # relative error of the LOD gradient against central differences
+MODEL = ((0.05, 0.3, 0.85), 0.12)
+
+
+def _device(case, graphs):
+    import torch
+    return torch.from_numpy(pack_graphs(case, graphs).reshape(-1).view(np.int32)).cuda()
+
+
+def _flipped(case, g0, n, flips, seed):
+    """n graphs: g0, then copies with `flips` random meiosis bits flipped."""
+    rng = np.random.default_rng(seed)
+    out = np.repeat(g0[None], n, axis=0)
+    nm = 2 * (case.N - case.F)
+    for g in out[1:]:
+        g[rng.integers(0, case.M, flips), 2 * case.F + rng.integers(0, nm, flips)] ^= 1
+    return out
+
+
+def _lsweep_spot_check(o, case, g_in, g_out, seed, sweep, p0, n, rng):
+    """An L-sweep visits parity p0, then the other; a locus's new column
+    depends only on its neighbours and its own draws, so a first-parity locus
+    is checked from the input graph and a second-parity one from the output
+    graph's (already checked) neighbours."""
+    first = np.arange(p0, case.M, 2)
+    second = np.arange(1 - p0, case.M, 2)
+    for l in rng.choice(first, min(n, len(first)), replace=False):
+        want, _ = o.lstep(g_in, int(l), seed, 0, sweep)
+        assert np.array_equal(g_out[l], want[l]), ("first parity", int(l))
+    for l in rng.choice(second, min(n, len(second)), replace=False):
+        want, _ = o.lstep(g_out, int(l), seed, 0, sweep)
+        assert np.array_equal(g_out[l], want[l]), ("second parity", int(l))
+
+
+# ---------------------------------------------------------------- 16 pedigrees
+
+@pytest.fixture(scope="module")
+def batch16(tmp_path_factory):
+    cases = synth.make_batch(str(tmp_path_factory.mktemp("batch16")))
+    return [(case, synth.true_graph(data, case)) for case, data in cases]
+
+
+SCHED = dict(burnin=2, period=2, lprob=0.5, seed=4242)
+
+
+@pytest.mark.parametrize("precision", [native.SL_FP64, native.SL_FP32_RESCALED])
+def test_batch16_equals_per_context_runs(batch16, precision):
+    sweeps, chains = 6, 4
+    print(f"batch16: {len(batch16)} pedigrees x {chains} chains, precision {precision}", flush=True)
+
+    def contexts():
+        out = []
+        for case, g0 in batch16:
+            c = native.Context(case, precision)
+            c.mcmc_init(np.repeat(g0[None], chains, axis=0))
+            c.lod_reset()
+            out.append(c)
+        return out
+
+    bctx = contexts()
+    try:
+        with native.Batch(bctx) as b:
+            bstats = b.run(sweeps, SCHED["burnin"], SCHED["period"], SCHED["lprob"], SCHED["seed"])
+        got = [(c.mcmc_graphs(), c.lod_results()[:2], c.mcmc_status(), c.mcmc_near_edge()) for c in bctx]
+    finally:
+        for c in bctx:
+            c.close()
+    ectx = contexts()
+    try:
+        for i, c in enumerate(ectx):
+            st = c.mcmc_run(sweeps, SCHED["burnin"], SCHED["period"], SCHED["lprob"], SCHED["seed"])
+            g, (acc, cnt), status, near = got[i]
+            assert np.array_equal(g, c.mcmc_graphs()), i
+            eacc, ecnt, _ = c.lod_results()
+            assert cnt == ecnt and np.array_equal(acc, eacc), i
+            assert np.array_equal(status, c.mcmc_status()) and np.all(status == 0)
+            assert np.array_equal(near, c.mcmc_near_edge())
+            assert bstats[i] == st
+    finally:
+        for c in ectx:
+            c.close()
+
+
+def _by_size(batch16):
+    order = sorted(range(len(batch16)), key=lambda i: batch16[i][0].N)
+    return order[0], order[len(order) // 2], order[-1]
+
+
+def test_batch16_fp64_chains_match_oracle(batch16):
+    """Whole FP64 chains (L- and M-sweeps with scoring) of the smallest and a
+    middle pedigree against the oracle's MarkovChain restatement."""
+    small, mid, _ = _by_size(batch16)
+    for i, sweeps in ((small, 6), (mid, 3)):
+        case, g0 = batch16[i]
+        print(f"batch16 oracle chain: N={case.N} M={case.M} sweeps={sweeps}", flush=True)
+        seed, burnin, period, lprob = 808, 1, 1, 0.5
+        with native.Context(case, native.SL_FP64) as ctx:
+            ctx.mcmc_init(g0[None])
+            ctx.lod_reset()
+            st = ctx.mcmc_run(sweeps, burnin, period, lprob, seed)
+            out = ctx.mcmc_graphs()[0]
+            acc, cnt, _ = ctx.lod_results()
+            near = ctx.mcmc_near_edge()
+        g, acc_ref, cnt_ref, kinds = Oracle(case).chain(g0, seed, 0, sweeps, burnin, period, lprob)
+        assert st["l_sweeps"] == int((kinds == 0).sum()) and st["m_sweeps"] == int((kinds == 1).sum())
+        assert near[0] == 0
+        assert np.array_equal(out, g)
+        assert cnt == cnt_ref
+        np.testing.assert_allclose(acc, acc_ref, rtol=0, atol=1e-9)
+
+
+def test_batch16_largest_pedigree_matches_oracle(batch16):
+    """The 400-person pedigree: a whole FP64 L-sweep (two L-sweeps of the
+    oracle's chain with every sweep an L-sweep) and M-steps of three chosen
+    meioses."""
+    _, _, big = _by_size(batch16)
+    case, g0 = batch16[big]
+    o = Oracle(case)
+    print(f"batch16 largest: N={case.N} M={case.M}", flush=True)
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(g0[None])
+        ctx.lod_reset()
+        ctx.mcmc_run(2, 0, 1, 1.0, 99)
+        out = ctx.mcmc_graphs()[0]
+        g, _, _, kinds = o.chain(g0, 99, 0, 2, 0, 1, 1.0)
+        assert np.all(kinds == 0)
+        assert np.array_equal(out, g)
+        order = o.meioses()[[0, len(o.meioses()) // 2, -1]]
+        ctx.msweep(5, 1, order)
+        mout = ctx.mcmc_graphs()[0]
+        assert ctx.mcmc_near_edge()[0] == 0
+    assert np.array_equal(mout, o.msweep(g, order, 5, 0, 1))
+
+
+def test_batch16_fp32_lod_within_bar(batch16):
+    """FP32-rescaled LOD terms against the FP64 oracle at about 150 intervals
+    spread over the map of the smallest, middle and largest pedigree, for two
+    graphs."""
+    for i in _by_size(batch16):
+        case, g0 = batch16[i]
+        graphs = _flipped(case, g0, 2, 50, i)
+        o = Oracle(case)
+        with native.Context(case, native.SL_FP32_RESCALED) as ctx:
+            t32 = ctx.lod_terms(graphs)
+        step = max(1, (case.M - 1) // 150)
+        worst = 0.0
+        for g in range(2):
+            for l in range(0, case.M - 1, step):
+                worst = max(worst, np.abs(t32[g, l] - o.lod_terms(graphs[g], l)).max() / LN10)
+        print(f"batch16 fp32 lod: N={case.N} M={case.M} max|dLOD|={worst:.2e}", flush=True)
+        assert worst <= LOD_BAR
+
+
+# ---------------------------------------------------------------- adjoint, 200 people
+
+@pytest.fixture(scope="module")
+def ped200(tmp_path_factory):
+    case, data = synth.make_case("ped200_snp2000", str(tmp_path_factory.mktemp("ped200")))
+    g0 = synth.true_graph(data, case)
+    return case, _flipped(case, g0, 8, 200, 3)
+
+
+def test_ped200_adjoint_matches_oracle_gradient(ped200):
+    case, graphs = ped200
+    pen, freq = MODEL
+    dp, jac = native.trait_model_tables(pen, freq, case.affection, case.F)
+    oc = Oracle(dataclasses.replace(case, disease_prob=dp.reshape(case.disease_prob.shape)))
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.set_trait_model(pen, freq)
+        out = ctx.trait_grad(graphs[:2])
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        g, l, pos = int(rng.integers(0, 2)), int(rng.integers(0, case.M - 1)), int(rng.integers(0, case.nlod))
+        ref = oc.trait_grad(graphs[g], l, pos + 1, jac)
+        dref = ref[1:] / ref[0]
+        np.testing.assert_allclose(out[g, l, pos, 1:], dref, rtol=1e-9, atol=1e-9 * np.abs(dref).max())
+
+
+def test_ped200_lod_gradient_matches_central_differences(ped200):
+    """dLOD/d(penetrance 0..2, frequency) against (LOD(x+h) - LOD(x-h))/2h
+    with h = 1e-6, and the theta_L/theta_R pair against the difference in the
+    map's partial theta p (every position's thetas are (o+1) p and (n-o) p);
+    relative error at most 1e-5 of the largest derivative."""
+    case, graphs = ped200
+    G = graphs.shape[0]
+    dev = _device(case, graphs)
+    pen, freq = MODEL
+    h = 1e-6
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.set_trait_model(pen, freq)
+        lod, dlod = ctx.lod_grad_packed(dev.data_ptr(), G)
+        assert np.all(np.isfinite(lod))
+        for k in range(4):
+            vals = []
+            for s in (h, -h):
+                p = list(pen) + [freq]
+                p[k] += s
+                ctx.set_trait_model(tuple(p[:3]), p[3])
+                vals.append(ctx.lod_grad_packed(dev.data_ptr(), G)[0])
+            fd = (vals[0] - vals[1]) / (2 * h)
+            err = np.abs(dlod[..., k] - fd).max() / np.abs(fd).max()
+            print(f"ped200 dLOD/d param {k}: max rel err {err:.2e}", flush=True)
+            assert err <= GRAD_BAR
+    vals = []
+    for s in (1 + h, 1 - h):
+        c2 = dataclasses.replace(case, partial_theta=case.partial_theta * s)
+        with native.Context(c2, native.SL_FP64) as ctx:
+            ctx.set_trait_model(pen, freq)
+            vals.append(ctx.lod_grad_packed(dev.data_ptr(), G)[0])
+    fd = (vals[0] - vals[1]) / (2 * h * case.partial_theta[:, None])
+    o = np.arange(1, case.nlod + 1)[None, :]
+    comb = o * dlod[..., 4] + (case.nlod + 1 - o) * dlod[..., 5]
+    err = np.abs(comb - fd).max() / np.abs(fd).max()
+    print(f"ped200 dLOD/d partial theta: max rel err {err:.2e}", flush=True)
+    assert err <= GRAD_BAR
+
+
+def test_ped200_mod_ascent_raises_the_lod(ped200):
+    case, graphs = ped200
+    dev = _device(case, graphs)
+    with native.Context(case, native.SL_FP64) as ctx:
+        r = maximise(ctx, dev.data_ptr(), graphs.shape[0], (0.1, 0.2, 0.6), 0.05, steps=8)
+        lod, _ = ctx.lod_grad_packed(dev.data_ptr(), graphs.shape[0])
+    assert len(r.history) >= 2
+    assert all(b > a for a, b in zip(r.history, r.history[1:]))
+    assert r.mod == pytest.approx(lod.max(), rel=1e-12)
+
+
+# ---------------------------------------------------------------- 600 people, 20000 SNPs
+
+@pytest.fixture(scope="module")
+def ped600(tmp_path_factory):
+    case, data = synth.make_case("ped600_snp20000", str(tmp_path_factory.mktemp("ped600")))
+    return case, synth.true_graph(data, case), Oracle(case)
+
+
+def test_ped600_fp64_samplers_match_oracle(ped600):
+    case, g0, o = ped600
+    rng = np.random.default_rng(600)
+    print(f"ped600: N={case.N} F={case.F} M={case.M}", flush=True)
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(g0[None])
+        ctx.lsweep(61, 0, 1)
+        out = ctx.mcmc_graphs()[0]
+        assert np.all(ctx.mcmc_status() == 0)
+        lik = ctx.mcmc_likelihood()[0]
+        order = o.meioses()[[3, len(o.meioses()) // 3, -2]]
+        ctx.msweep(62, 1, order)
+        mout = ctx.mcmc_graphs()[0]
+        assert ctx.mcmc_near_edge()[0] == 0
+    _lsweep_spot_check(o, case, g0, out, 61, 0, 1, 40, rng)
+    ref = np.array([o.fag(out, l) for l in range(case.M)])
+    assert np.all(ref > 0) and np.all(lik > 0)
+    np.testing.assert_allclose(lik, ref, rtol=1e-12, atol=0)
+    assert np.array_equal(mout, o.msweep(out, order, 62, 0, 1))
+
+
+def test_ped600_fp32_lod_within_bar_of_rescaled_oracle(ped600):
+    case, g0, o = ped600
+    graphs = _flipped(case, g0, 2, 400, 6)
+    with native.Context(case, native.SL_FP32_RESCALED) as ctx:
+        t32 = ctx.lod_terms(graphs)
+    assert np.all(np.isfinite(t32))
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for g in range(2):
+        for l in rng.choice(case.M - 1, 60, replace=False):
+            worst = max(worst, np.abs(t32[g, l] - o.lod_terms_scaled(graphs[g], int(l))).max() / LN10)
+    print(f"ped600 fp32 lod: max|dLOD| against the rescaled FP64 oracle = {worst:.2e}", flush=True)
+    assert worst <= LOD_BAR
+
+
+def test_ped600_fp32_chains_stay_legal(ped600):
+    """Two FP32-rescaled chains through four sweeps with scoring: legal at
+    every locus, finite LOD at every position."""
+    case, g0, _ = ped600
+    with native.Context(case, native.SL_FP32_RESCALED) as ctx:
+        ctx.mcmc_init(np.stack([g0, g0]))
+        ctx.lod_reset()
+        ctx.mcmc_run(4, 0, 2, 0.5, 1234)
+        assert np.all(ctx.mcmc_status() == 0)
+        assert np.all(ctx.mcmc_likelihood() > 0)
+        _, cnt, lod = ctx.lod_results()
+    assert cnt == 4 and np.all(np.isfinite(lod))
+
+
+def test_lod_tables_past_lds_use_global_slots(tmp_path):
+    """The 400-person pedigree of the batch with 16 positions per interval:
+    its FP64 table arena (every position folded into each table) exceeds the
+    160 KiB of LDS, so the workgroup kernel keeps it in global-memory slots.
+    Terms at spot intervals and P(T) against the oracle; FP32 (whose arena
+    fits) within the LOD bar."""
+    from swiftlink_amd import linkage
+    spec = next(s for s in synth.batch_specs() if s.n_people == 400)
+    data = synth.generate(spec)
+    p, m, d = synth.write_linkage(data, str(tmp_path), "ped400_16pos")
+    case = linkage.load(p, m, d, nlod=16)
+    words = native.program_export(case, 16)
+    assert int(words[3]) * 8 > 160 * 1024  # the FP64 arena of all 16 positions
+    g0 = synth.true_graph(data, case)
+    graphs = _flipped(case, g0, 3, 60, 16)
+    o = Oracle(case)
+    print(f"global LOD tables: N={case.N} M={case.M} arena={int(words[3]) * 8} B", flush=True)
+    with native.Context(case, native.SL_FP64, lod_kernel=native.SL_LOD_WORKGROUP) as ctx:
+        assert abs(ctx.trait_prob() - o.trait_prob()) < 1e-12
+        t64 = ctx.lod_terms(graphs)
+        ctx.lod_accumulate(graphs)
+        acc, cnt, _ = ctx.lod_results()
+    with native.Context(case, native.SL_FP32_RESCALED) as ctx:
+        t32 = ctx.lod_terms(graphs)
+    for l in range(0, case.M - 1, 97):
+        for g in range(3):
+            ref = o.lod_terms(graphs[g], l)
+            np.testing.assert_allclose(t64[g, l], ref, rtol=1e-12, atol=1e-10)
+            assert np.abs(t32[g, l] - ref).max() / LN10 <= LOD_BAR
+    assert cnt == 3
+    m_ = t64.max(axis=0)
+    np.testing.assert_allclose(acc.reshape(m_.shape), m_ + np.log(np.exp(t64 - m_).sum(axis=0)), rtol=1e-12)

@@ -98,3 +98,29 @@ def test_elod_rejects_mismatched_contexts():
             native.run_elod(sc, s, 1, 10)  # sampler and scorer swapped
         with pytest.raises(native.SlError):
             native.run_elod(s, sc, 1, 0)
+
+
+def test_elod_ped150_fp64_matches_oracle_and_fp32_close(tmp_path):
+    """ped150 (150 people, 3 loops) under the default ELOD model: 3000
+    replicates in FP64 against the oracle's start_from replicates scored by
+    its trait peel, and the FP32-sampled ELOD against that FP64 value.  An
+    FP32 replicate differs from its FP64 twin only where a uniform falls
+    within float rounding of a CDF edge, so the bound is 1e-3 absolute on an
+    ELOD of order 1."""
+    from swiftlink_amd import synth
+    case, _ = synth.make_case("ped150_snp1000", str(tmp_path))
+    R, seed = 3000, 21
+    c3, c2 = elod.elod_cases(case)
+    o3, o2 = Oracle(c3), Oracle(c2)
+    graphs = np.stack([o3.impute_from(seed, j, 1)[0] for j in range(R)])
+    acc = o2.lod_accumulate(graphs[:, [0, 2]])[0]
+    want = (acc - np.log(R) - o2.trait_prob()) / np.log(10.0)
+    with native.Context(c3, native.SL_FP64) as s, native.Context(c2, native.SL_FP64) as sc:
+        v64 = native.run_elod(s, sc, seed, R)
+        dacc, count, _ = sc.lod_results()
+    assert count == R
+    assert abs(dacc[0] - acc) <= 1e-12 * max(1.0, abs(acc))
+    assert abs(v64 - want) <= 1e-11 * max(1.0, abs(want))
+    with native.Context(c3, native.SL_FP32_RESCALED) as s, native.Context(c2, native.SL_FP64) as sc:
+        v32 = native.run_elod(s, sc, seed, R)
+    assert abs(v32 - v64) <= 1e-3, (v32, v64)

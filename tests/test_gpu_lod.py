@@ -164,3 +164,66 @@ def test_lod_table_from_device_scores(name):
         assert len(a) == len(b) and a[:2] == b[:2], (a, b)
         for x, y in zip(a[2:], b[2:]):
             assert abs(float(x) - float(y)) <= 1e-5 * max(1.0, abs(float(y))), (a, b)
+
+
+def _many(fx, n):
+    reps = -(-n // len(fx["graphs"]))
+    return np.concatenate([fx["graphs"]] * reps)[:n]
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_lane_kernel_fp64_terms_equal_workgroup_kernel(name):
+    """The lane kernel (one (graph, position) per lane, 70 graphs: one full
+    and one partial block of 64) gives the workgroup kernel's FP64 terms bit
+    for bit: each cell's arithmetic is the same, only the lanes differ."""
+    case, fx = load_golden(name)
+    graphs = _many(fx, 70)
+    with native.Context(case, native.SL_FP64, lod_kernel=native.SL_LOD_LANES) as a, \
+            native.Context(case, native.SL_FP64, lod_kernel=native.SL_LOD_WORKGROUP) as b:
+        ta, tb = a.lod_terms(graphs), b.lod_terms(graphs)
+    assert np.array_equal(ta, tb)
+    np.testing.assert_allclose(ta[:len(fx["graphs"])], fx["lod_terms"], rtol=1e-12, atol=1e-10)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_lane_kernel_fp32_and_accumulators(name):
+    """FP32 lane kernel: terms within the LOD bar of the reference's, and the
+    accumulated LOD of 70 graphs within the bar of the FP64 workgroup
+    kernel's; FP64 accumulators of both kernels agree to rounding of the
+    log-sum order."""
+    case, fx = load_golden(name)
+    graphs = _many(fx, 70)
+    with native.Context(case, native.SL_FP32_RESCALED, lod_kernel=native.SL_LOD_LANES) as ctx:
+        t32 = ctx.lod_terms(fx["graphs"])
+        ctx.lod_accumulate(graphs)
+        _, c32, lod32 = ctx.lod_results()
+    assert np.abs(t32 - fx["lod_terms"]).max() / np.log(10.0) < FP32_LOD_TOL
+    accs = []
+    for kernel in (native.SL_LOD_LANES, native.SL_LOD_WORKGROUP):
+        with native.Context(case, native.SL_FP64, lod_kernel=kernel) as ctx:
+            ctx.lod_accumulate(graphs)
+            acc, c64, lod64 = ctx.lod_results()
+            accs.append((acc, lod64))
+    assert c32 == c64 == 70
+    np.testing.assert_allclose(accs[0][0], accs[1][0], rtol=1e-13, atol=1e-12)
+    assert np.abs(lod32 - accs[1][1]).max() < FP32_LOD_TOL
+
+
+def test_lane_kernel_ped150_fp32_and_fp64(ped150):
+    """150 people, 96 graphs (one and a half blocks): FP64 lane terms equal the
+    workgroup kernel's; FP32 lane terms within the bar of the oracle's at
+    every 50th interval."""
+    case, legal, oracle = ped150
+    rng = np.random.default_rng(150)
+    graphs = np.concatenate([legal] * 24)
+    nm = 2 * (case.N - case.F)
+    for g in graphs[len(legal):]:
+        g[rng.integers(0, case.M, 30), 2 * case.F + rng.integers(0, nm, 30)] ^= 1
+    with native.Context(case, native.SL_FP64, lod_kernel=native.SL_LOD_LANES) as a, \
+            native.Context(case, native.SL_FP64, lod_kernel=native.SL_LOD_WORKGROUP) as b:
+        assert np.array_equal(a.lod_terms(graphs), b.lod_terms(graphs))
+    with native.Context(case, native.SL_FP32_RESCALED, lod_kernel=native.SL_LOD_LANES) as ctx:
+        t32 = ctx.lod_terms(graphs[:8])
+    for gi in range(8):
+        for l in range(0, case.M - 1, 50):
+            assert np.abs(t32[gi, l] - oracle.lod_terms(graphs[gi], l)).max() / np.log(10.0) < FP32_LOD_TOL

@@ -241,3 +241,60 @@ def test_msweep_long_map_matches_oracle(long_map, mode):
     assert np.all(near == 0)
     for c in range(2):
         assert np.array_equal(out[c], o.msweep(starts[c], shuffled_order(o, 606, c, 2), 606, c, 2)), c
+
+
+def _fp32_vs_fp64_steps(case, g0, o, mode, steps, seed):
+    """One M-step at a time (sl_msweep with a one-meiosis order) in an FP64 and
+    an FP32-rescaled context from the same graph: the FP32 forward-backward
+    pass consumes the same draws and must give the FP64 pass's graph whenever
+    it counted no draw within 1e-5 of its threshold.  Returns the number of
+    steps with such a draw."""
+    meioses = o.meioses()
+    g = g0
+    flagged = 0
+    with native.Context(case, native.SL_FP64, msampler_mode=mode) as c64, \
+            native.Context(case, native.SL_FP32_RESCALED, msampler_mode=mode) as c32:
+        for j in range(steps):
+            m = meioses[(7 * j) % len(meioses)]
+            outs = []
+            for ctx in (c64, c32):
+                ctx.mcmc_init(g[None])
+                ctx.msweep(seed, j, [m])
+                assert np.all(ctx.mcmc_status() == 0)
+                outs.append((ctx.mcmc_graphs()[0], int(ctx.mcmc_near_edge()[0])))
+            (g64, n64), (g32, n32) = outs
+            assert n64 == 0
+            if n32 == 0:
+                assert np.array_equal(g32, g64), (j, m)
+            else:
+                flagged += 1
+            g = g64
+    return flagged
+
+
+@pytest.mark.parametrize("mode", [native.SL_MSAMPLER_SPECULATIVE, native.SL_MSAMPLER_TWO_LAUNCH])
+def test_msweep_fp32_equals_fp64_but_counted_edges_ped150(ped150, mode):
+    case, g0, o = ped150
+    flagged = _fp32_vs_fp64_steps(case, g0, o, mode, 60, 3141)
+    print(f"ped150 FP32 M-steps with a draw near an edge: {flagged} of 60")
+    assert flagged <= 6
+
+
+def test_msweep_fp32_long_map(long_map):
+    """5000 loci: the FP32 pass through the global-memory segments (one
+    launch per step) against FP64."""
+    case, g0, o = long_map
+    flagged = _fp32_vs_fp64_steps(case, g0, o, native.SL_MSAMPLER_SPECULATIVE, 12, 2718)
+    assert flagged <= 2
+
+
+def test_msweep_fp32_whole_sweep_keeps_graph_legal(ped150):
+    case, g0, o = ped150
+    with native.Context(case, native.SL_FP32_RESCALED) as ctx:
+        ctx.mcmc_init(np.stack([g0, g0]))
+        for s in range(3):
+            ctx.msweep(77, s)
+        out = ctx.mcmc_graphs()
+        assert np.all(ctx.mcmc_status() == 0)
+    for c in range(2):
+        assert min(o.fag(out[c], l) for l in range(case.M)) > 0

@@ -104,3 +104,39 @@ def test_program_ped150_terms_and_bounds(tmp_path):
     for l in (0, 600):
         ref = [np.log(o.trait_likelihood(g, l, off)) for off in range(1, case.nlod + 1)]
         assert np.array_equal(P.run(case, g, l), ref)
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_sequential_program_fp64_bit_exact_and_fp32_within_bar(name):
+    """The lane kernel's sequential form: every table alive at once has its
+    own arena cells, and executed with the kernel's arithmetic (one evaluation
+    per lane) it gives the oracle's likelihood bit for bit in FP64 and within
+    1e-5 LOD in FP32 with power-of-two renormalisation."""
+    case, fx = load_golden(name)
+    P = Program(native.program_export(case))
+    P.check_seq()
+    o = Oracle(case)
+    for g in fx["graphs"][:2]:
+        for l in (0, case.M // 2, case.M - 2):
+            ref = [np.log(o.trait_likelihood(g, l, off)) for off in range(1, case.nlod + 1)]
+            assert np.array_equal(P.run_seq(case, g, l), ref)
+            assert np.abs(P.run_seq(case, g, l, fp32=True) - ref).max() / np.log(10) < 1e-5
+
+
+def test_sequential_program_ped150_and_underflowing_scale(tmp_path):
+    """150 people: the sequential form's arena layout and FP64 values; with
+    every disease probability scaled by 2^-40 (a likelihood of about 2^-6000
+    times the original, far below FP32 and FP64 range) the FP32 renormalised
+    execution still gives ln L - 40 N ln 2 within the LOD bar."""
+    import dataclasses
+    from swiftlink_amd import synth
+    case, data = synth.make_case("ped150_snp1000", str(tmp_path))
+    P = Program(native.program_export(case))
+    P.check_seq()
+    g = synth.true_graph(data, case)
+    o = Oracle(case)
+    ref = [np.log(o.trait_likelihood(g, 300, off)) for off in range(1, case.nlod + 1)]
+    assert np.array_equal(P.run_seq(case, g, 300), ref)
+    tiny = dataclasses.replace(case, disease_prob=np.ldexp(case.disease_prob, -40))
+    got = P.run_seq(tiny, g, 300, fp32=True)
+    assert np.abs(got - (np.array(ref) - 40 * case.N * np.log(2.0))).max() / np.log(10) < 1e-4
