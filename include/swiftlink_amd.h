@@ -3,8 +3,9 @@
  *
  * C ABI: plain pointers, sizes and status codes.  The library copies every
  * descriptor at sl_create, owns all device memory, writes results into caller
- * buffers, never aborts or exits, and keeps no global state (one context per
- * host thread; a context serialises its own work on one HIP stream).
+ * buffers, never aborts or exits, and keeps no global state besides the fixed
+ * per-device array of sl_lod_accumulate_packed_noalloc (one context per host
+ * thread; a context serialises its own work on one HIP stream).
  *
  * Conventions (identical to the reference CPU path):
  *   persons     internal order, founders first (pedigree.cc:154-162)
@@ -142,6 +143,21 @@ int sl_lod_accumulate(sl_ctx* ctx, const int32_t* graphs, int32_t n_graphs);
 /* Same for graphs already on the device in the packed layout
  * (uint32 [G][M][W]); no host transfer. */
 int sl_lod_accumulate_packed(sl_ctx* ctx, const uint32_t* dev_graphs, int32_t n_graphs);
+
+/* sl_lod_accumulate_packed without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels run; their per-chunk partial log-sums go to a fixed array of
+ * SL_NOALLOC_SCRATCH_BYTES that the library holds on each device it runs on
+ * until the process ends, taking the graphs in as many groups as that array
+ * needs (each group is merged before the next).  Returns SL_E_CAPACITY when
+ * one chunk's partials, (M-1) x lods_per_interval x 8 bytes, exceed the
+ * array.  Every context on a device shares that device's array: calls that
+ * overlap on one device (contexts on different streams or host threads) race
+ * and must be serialised by the caller.  Nothing in the array needs to start
+ * at zero. */
+#define SL_NOALLOC_SCRATCH_BYTES (8u << 20)
+int sl_lod_accumulate_packed_noalloc(sl_ctx* ctx, const uint32_t* dev_graphs, int32_t n_graphs);
 
 /* Waits for queued work.  Replaces GPULodscores::block_until_finished
  * (gpu_lodscores.h:119). */

@@ -21,6 +21,9 @@
 
 using sl::PeelProgram;
 
+static_assert((size_t)SL_NOALLOC_SCRATCH_BYTES == sl::kNoallocScratchDoubles * sizeof(double),
+              "the header's scratch size is the device array's");
+
 namespace {
 thread_local std::string g_create_error;
 // speculative M-steps double the likelihood wavefronts; used while they fit
@@ -72,6 +75,9 @@ struct sl_ctx {
     size_t lane_lds = 0;         // LDS of a lane-kernel wavefront
     bool lane_ok = false;        // the lane kernel's arena fits LDS
     int lod_kernel = SL_LOD_AUTO;
+    // this device's fixed array of the allocation-free LOD entry point (not
+    // owned: it lives until the process ends)
+    double* scratch = nullptr;
     // growable work buffers
     double* d_partial = nullptr;
     size_t partial_cap = 0;
@@ -614,8 +620,9 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
     }
     e = sl::configure_lod_lds(c->precision, c->lds, c->lod_global);
     if (e == hipSuccess && c->lane_ok) e = sl::configure_lane_lds(c->precision, c->lane_lds);
+    if (e == hipSuccess) e = sl::noalloc_scratch(c->device, &c->scratch);
     if (e != hipSuccess) {
-        std::string m = std::string("LDS configuration failed: ") + hipGetErrorString(e);
+        std::string m = std::string("LDS or scratch set-up failed: ") + hipGetErrorString(e);
         release_all(c);
         delete c;
         return fail(nullptr, SL_E_DEVICE, m);
@@ -842,6 +849,36 @@ int sl_lod_accumulate_packed(sl_ctx* c, const uint32_t* dev_graphs, int32_t G) {
     if (G == 0) return SL_OK;
     int rc = lod_run(c, dev_graphs, G, nullptr);
     if (rc) return rc;
+    c->count += G;
+    return SL_OK;
+}
+
+int sl_lod_accumulate_packed_noalloc(sl_ctx* c, const uint32_t* dev_graphs, int32_t G) {
+    if (!c || (!dev_graphs && G > 0) || G < 0) return SL_E_INVALID;
+    if (G == 0) return SL_OK;
+    if (!c->scratch) return fail(c, SL_E_DEVICE, "no scratch array on this device");
+    // graphs go in groups whose chunk partials fit the array; each group is
+    // scored and merged before the next overwrites the partials (one stream)
+    const size_t nacc = (size_t)(c->M - 1) * c->npos;
+    const size_t max_chunks = (size_t)SL_NOALLOC_SCRATCH_BYTES / sizeof(double) / nacc;
+    if (max_chunks == 0)
+        return fail(c, SL_E_CAPACITY,
+                    "allocation-free LOD: one chunk's partials ((M-1) x positions x 8 bytes = " +
+                        std::to_string(nacc * sizeof(double)) + " bytes) exceed the " +
+                        std::to_string((size_t)SL_NOALLOC_SCRATCH_BYTES) + "-byte scratch array");
+    const int per = lane_mode(c, G) ? sl::kLaneGraphs : c->gpw;
+    const int group = (int)std::min<size_t>((size_t)G, std::min<size_t>(max_chunks, 65535) * (size_t)per);
+    sl::LodArgs A = lod_args(c);
+    A.partial = c->scratch;
+    A.terms = nullptr;
+    for (int g0 = 0; g0 < G; g0 += group) {
+        const int n = std::min(group, G - g0);
+        const int nchunks = (n + per - 1) / per;
+        A.graphs = dev_graphs + (size_t)g0 * c->M * c->W;
+        A.G = n;
+        SL_HIP(c, launch_scoring(c, A, nchunks));
+        SL_HIP(c, sl::launch_lod_merge(c->d_acc, c->scratch, nchunks, (int)nacc, c->stream));
+    }
     c->count += G;
     return SL_OK;
 }

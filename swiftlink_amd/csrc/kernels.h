@@ -98,5 +98,12 @@ hipError_t launch_fill(double* p, double v, int n, hipStream_t s);
 hipError_t configure_lod_lds(int precision, size_t lds, bool global_tables);
 hipError_t launch_pack(const int32_t* in, uint32_t* out, int G, int M, int N, int F, int W, hipStream_t s);
 
+// The allocation-free LOD entry point keeps its chunk partials in a fixed
+// device array of kNoallocScratchDoubles doubles (8 MiB) per device;
+// noalloc_scratch gives its address on `device`, which must be current on
+// the first call for that device.
+constexpr size_t kNoallocScratchDoubles = (size_t)1 << 20;
+constexpr int kMaxDevices = 64;
+hipError_t noalloc_scratch(int device, double** out);
 
 }  // namespace sl

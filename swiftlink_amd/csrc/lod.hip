@@ -26,11 +26,38 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "program.h"
 #include "kernels.h"
 
+// Partial log-sums of the allocation-free LOD entry point: one array per
+// device, held until the process ends.  Host code never passes this name to a
+// kernel (on the host it is only a stand-in for the symbol); it passes the
+// device address hipGetSymbolAddress returned for the current device.
+static __device__ __align__(256) double scratch[sl::kNoallocScratchDoubles];
+
 namespace sl {
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(int device, double** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        // the caller has made `device` current
+        double* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
 
 namespace {
 

@@ -83,6 +83,7 @@ _SIGS = {
     "sl_lod_reset": (C.c_int, [C.c_void_p]),
     "sl_lod_accumulate": (C.c_int, [C.c_void_p, i32p, C.c_int32]),
     "sl_lod_accumulate_packed": (C.c_int, [C.c_void_p, u32p, C.c_int32]),
+    "sl_lod_accumulate_packed_noalloc": (C.c_int, [C.c_void_p, u32p, C.c_int32]),
     "sl_sync": (C.c_int, [C.c_void_p]),
     "sl_lod_results": (C.c_int, [C.c_void_p, f64p, i64p, f64p]),
     "sl_lod_terms": (C.c_int, [C.c_void_p, i32p, C.c_int32, f64p]),
@@ -259,6 +260,12 @@ class Context:
     def lod_accumulate_packed(self, dev_ptr: int, n: int):
         self._check(lib().sl_lod_accumulate_packed(self.h, C.cast(C.c_void_p(dev_ptr), u32p), n),
                     "sl_lod_accumulate_packed")
+
+    def lod_accumulate_packed_noalloc(self, dev_ptr: int, n: int):
+        """sl_lod_accumulate_packed_noalloc: the same scoring with no device
+        allocation (partials in the library's fixed per-device array)."""
+        self._check(lib().sl_lod_accumulate_packed_noalloc(self.h, C.cast(C.c_void_p(dev_ptr), u32p), n),
+                    "sl_lod_accumulate_packed_noalloc")
 
     def sync(self):
         self._check(lib().sl_sync(self.h), "sl_sync")
