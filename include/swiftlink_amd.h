@@ -297,6 +297,20 @@ typedef struct {
  * stops changing at the failing step; sl_mcmc_status says which). */
 int sl_mcmc_run(sl_ctx* ctx, const sl_schedule* schedule, sl_mcmc_stats* stats);
 
+/* CODA trace (options -T/-P, markov_chain.cc:83-91, 362-370): with enable
+ * != 0, every later sl_mcmc_run records, after each scored sweep s, one line
+ * "s+1, likelihood" per chain, the likelihood being DescentGraph::
+ * get_likelihood (descent_graph.cc:184-189): 2(N-F) ln 1/2 plus the
+ * recombination log-probability of every interval plus ln of the
+ * founder-allele likelihood of every locus (-inf when a locus is illegal).
+ * The records are cleared by this call and by sl_mcmc_init.  Not recorded
+ * in batches.  Needs the M-sampler's likelihood kernels. */
+int sl_mcmc_trace(sl_ctx* ctx, int32_t enable);
+
+/* *n = the number of records; when cap >= *n, iterations[r] = s+1 and
+ * loglik[r * chains + c] the likelihood of chain c (either may be NULL). */
+int sl_mcmc_trace_read(sl_ctx* ctx, int64_t cap, int64_t* n, int32_t* iterations, double* loglik);
+
 /* Founder-allele-graph likelihood of every chain's current graph at every
  * locus into per_locus [C][M] (FounderAlleleGraph4::reset + likelihood per
  * locus, the legality check of DescentGraph::_sum_prior_prob,

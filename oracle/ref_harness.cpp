@@ -655,7 +655,8 @@ static void cmd_chain(Case& c, const std::string& out, const std::string& gfile,
     double trait_prob = peeler.calc_trait_prob();
     lod.set_trait_prob(trait_prob);
     std::vector<int> mlist = meiosis_list(c);
-    std::vector<int32_t> kinds;
+    std::vector<int32_t> kinds, coda_iter;
+    std::vector<double> coda;
     std::vector<int> order;
     for (int s = 0; s < sweeps; ++s) {
         stream_set(seed, chain, (uint32_t)s, STEP_SCHED);
@@ -681,6 +682,9 @@ static void cmd_chain(Case& c, const std::string& out, const std::string& gfile,
             }
         }
         if (s >= burnin && (s % period) == 0) {
+            // the CODA trace line of a scored sweep (markov_chain.cc:362-370)
+            coda_iter.push_back(s + 1);
+            coda.push_back(dg.get_likelihood());
             for (int j = 0; j < c.M - 1; ++j) {
                 peeler.set_locus(j);
                 peeler.process(&dg);
@@ -697,6 +701,10 @@ static void cmd_chain(Case& c, const std::string& out, const std::string& gfile,
     write_bin(out + "/sweep_kinds.i32", &kinds[0], kinds.size() * 4);
     write_bin(out + "/lod_acc.f64", &acc[0], acc.size() * 8);
     write_bin(out + "/lod_final.f64", &lodv[0], lodv.size() * 8);
+    if (!coda.empty()) {
+        write_bin(out + "/coda.f64", &coda[0], coda.size() * 8);
+        write_bin(out + "/coda_iter.i32", &coda_iter[0], coda_iter.size() * 4);
+    }
     FILE* f = fopen((out + "/chain_meta.json").c_str(), "w");
     fprintf(f, "{\"count\": %u, \"trait_prob\": %.17g}\n", lod.get_count(), trait_prob);
     fclose(f);

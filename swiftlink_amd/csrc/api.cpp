@@ -175,6 +175,11 @@ struct sl_ctx {
     uint8_t* d_maps = nullptr;
     uint32_t* d_near = nullptr;
     uint32_t* d_near_loci = nullptr;  // [chains][ceil(M/32)] loci of L-sampler near-edge draws
+    // CODA trace: a likelihood per chain at every scored sweep of sl_mcmc_run
+    bool trace_on = false;
+    double* d_trace = nullptr;        // [records][chains]
+    size_t trace_cap = 0;             // doubles
+    std::vector<int32_t> trace_iter;  // the record's sweep index + 1
     // sequential-imputation replicates
     uint32_t* d_imp_graphs = nullptr;  // [R][M][W]
     size_t imp_graphs_cap = 0;
@@ -267,6 +272,7 @@ sl::LodArgs lod_args(sl_ctx* c) {
     A.seq = c->d_seq;
     A.lstream = c->d_lstream;
     A.lane_arena_bytes = c->prog.seq_elems * sl::kLaneGraphs * (c->precision == SL_FP64 ? 8 : 4);
+    A.lane_zero = 0;
     A.gtables = c->lod_global ? c->d_lgt : nullptr;
     return A;
 }
@@ -393,7 +399,7 @@ int release_all(sl_ctx* c) {
                     c->g_ops, c->g_prevs, c->g_children, c->g_tasks, c->g_tinfo, c->g_idx, c->g_level_base,
                     c->g_sched_off, c->g_sched_ops, c->d_gdp, c->d_jac, c->d_gterms, c->d_gstate, c->d_sweep,
                     c->d_imp_graphs, c->d_imp_draw, c->d_imp_weight, c->d_imp_status, c->d_elod_pairs,
-                    c->d_gtables, c->d_lgt, c->d_lstream};
+                    c->d_gtables, c->d_lgt, c->d_lstream, c->d_trace};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
@@ -603,8 +609,8 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
             return fail(nullptr, SL_E_UNSUPPORTED, "peel program's exponents and codes need more than 160 KiB of LDS");
         }
     }
-    c->lane_lds = sl::lane_lds_bytes(c->prog.seq_elems, c->W, c->precision);
-    c->lane_ok = c->lane_lds <= 160 * 1024;
+    c->lane_lds = sl::lane_lds_bytes(c->prog.seq_elems, c->W);
+    c->lane_ok = c->precision == SL_FP32_RESCALED && c->lane_lds <= 160 * 1024;
 
     c->theta.assign(map->theta, map->theta + c->M - 1);
     c->ptheta.assign(map->partial_theta, map->partial_theta + c->M - 1);
@@ -1012,6 +1018,7 @@ int sl_mcmc_init(sl_ctx* c, int32_t n_chains, const int32_t* graphs) {
     if (rc) return rc;
     rc = stage_graphs(c, graphs, n_chains);
     if (rc) return rc;
+    c->trace_iter.clear();  // records of the previous chains
     const size_t words = (size_t)n_chains * c->M * c->W;
     SL_HIP(c, hipMemcpyAsync(c->d_chains, c->d_packed, words * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
     SL_HIP(c, hipStreamSynchronize(c->stream));
@@ -1348,6 +1355,35 @@ static sl::MsampArgs msamp_args(sl_ctx* c) {
     return A;
 }
 
+// The trace buffer grown to hold `n` doubles, keeping its records.
+static int grow_trace(sl_ctx* c, size_t n) {
+    if (n <= c->trace_cap) return SL_OK;
+    const size_t cap = std::max(n, 2 * c->trace_cap);
+    double* nb = nullptr;
+    SL_HIP(c, hipMalloc((void**)&nb, cap * sizeof(double)));
+    const size_t used = c->trace_iter.size() * (size_t)c->n_chains;
+    if (used) SL_HIP(c, hipMemcpyAsync(nb, c->d_trace, used * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    SL_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->d_trace) SL_HIP(c, hipFree(c->d_trace));
+    c->d_trace = nb;
+    c->trace_cap = cap;
+    return SL_OK;
+}
+
+// One trace record after sweep s: every chain's current-column likelihoods,
+// then their sum with the transmission and recombination terms.
+static int record_trace(sl_ctx* c, uint32_t s) {
+    sl::MsampArgs A = msamp_args(c);
+    A.with_lcur = 1;
+    A.n_jobs = c->M;
+    SL_HIP(c, sl::launch_mstep(A, c->m_lds, c->stream));
+    const double trans = std::log(0.5) * (2.0 * (c->N - c->F));
+    SL_HIP(c, sl::launch_trace(A, c->d_logt, c->d_loga, trans, c->d_trace + c->trace_iter.size() * (size_t)c->n_chains,
+                               c->stream));
+    c->trace_iter.push_back((int32_t)s + 1);
+    return SL_OK;
+}
+
 // One M-sweep: order, current-column likelihoods, one launch per meiosis step.
 static int run_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* dev_order, int nm, int check_kind,
                       double lprob, const uint32_t* sweep_dev = nullptr) {
@@ -1431,6 +1467,14 @@ int sl_mcmc_run(sl_ctx* c, const sl_schedule* sch, sl_mcmc_stats* stats) {
     if (c->n_chains <= 0) return fail(c, SL_E_INVALID, "no chains: call sl_mcmc_init first");
     const int C = c->n_chains;
     sl_mcmc_stats st{0, 0, 0};
+    if (c->trace_on) {
+        if (!c->m_ready) return fail(c, SL_E_UNSUPPORTED, "trace needs the founder-allele likelihoods: " + c->m_why);
+        int64_t scored = 0;
+        for (int k = 0; k < sch->sweeps; ++k)
+            if (sl::gate_open(sch->first_sweep + (uint32_t)k, sch->burnin, sch->period)) ++scored;
+        const int rc = grow_trace(c, (c->trace_iter.size() + (size_t)scored) * (size_t)C);
+        if (rc) return rc;
+    }
     for (int k = 0; k < sch->sweeps; ++k) {
         const uint32_t s = sch->first_sweep + (uint32_t)k;
         int nl = 0, p0 = 0;
@@ -1459,6 +1503,7 @@ int sl_mcmc_run(sl_ctx* c, const sl_schedule* sch, sl_mcmc_stats* stats) {
             if (rc) return rc;
             c->count += C;
             st.scored += C;
+            if (c->trace_on && (rc = record_trace(c, s))) return rc;
         }
     }
     if (stats) *stats = st;
@@ -1493,6 +1538,29 @@ int sl_mcmc_likelihood(sl_ctx* c, double* per_locus) {
         if (st[i] != 0) now[i] = st[i];
     SL_HIP(c, hipMemcpyAsync(c->d_status, now.data(), now.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     SL_HIP(c, hipStreamSynchronize(c->stream));
+    return SL_OK;
+}
+
+int sl_mcmc_trace(sl_ctx* c, int32_t enable) {
+    if (!c) return SL_E_INVALID;
+    if (enable && !c->m_ready)
+        return fail(c, SL_E_UNSUPPORTED, "trace needs the founder-allele likelihoods: " + c->m_why);
+    c->trace_on = enable != 0;
+    c->trace_iter.clear();
+    return SL_OK;
+}
+
+int sl_mcmc_trace_read(sl_ctx* c, int64_t cap, int64_t* n, int32_t* iterations, double* loglik) {
+    if (!c || !n) return SL_E_INVALID;
+    const int64_t rec = (int64_t)c->trace_iter.size();
+    *n = rec;
+    if (rec == 0 || cap < rec || (!iterations && !loglik)) return SL_OK;
+    if (iterations) std::copy(c->trace_iter.begin(), c->trace_iter.end(), iterations);
+    if (loglik) {
+        SL_HIP(c, hipMemcpyAsync(loglik, c->d_trace, (size_t)rec * c->n_chains * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+        SL_HIP(c, hipStreamSynchronize(c->stream));
+    }
     return SL_OK;
 }
 
@@ -1610,6 +1678,7 @@ int sl_batch_run(sl_batch* b, const sl_schedule* sch, sl_mcmc_stats* stats) {
         sl_ctx* c = b->ctx[i];
         if (c->n_chains <= 0) return fail(c, SL_E_INVALID, "no chains: call sl_mcmc_init first");
         if (c->prof) return fail(c, SL_E_INVALID, "profiling is not available in a batch");
+        if (c->trace_on) return fail(c, SL_E_INVALID, "the likelihood trace is not recorded in a batch");
         const int C = c->n_chains;
         for (int k = 0; k < sch->sweeps; ++k) {
             const uint32_t s = sch->first_sweep + (uint32_t)k;
@@ -1743,7 +1812,7 @@ int install_lod_program(sl_ctx* c) {
     const size_t lds = lay.lds;
     if (lds > 160 * 1024)
         return fail(c, SL_E_UNSUPPORTED, "peel program's exponents and codes need more than 160 KiB of LDS");
-    const size_t lane_lds = sl::lane_lds_bytes(P.seq_elems, c->W, c->precision);
+    const size_t lane_lds = sl::lane_lds_bytes(P.seq_elems, c->W);
     SL_HIP(c, hipStreamSynchronize(c->stream));
     void** bufs[] = {(void**)&c->d_ops, (void**)&c->d_prevs, (void**)&c->d_children, (void**)&c->d_tasks,
                      (void**)&c->d_tinfo, (void**)&c->d_idx, (void**)&c->d_level_base, (void**)&c->d_sched_off,
@@ -1759,7 +1828,7 @@ int install_lod_program(sl_ctx* c) {
     c->lds = lds;
     c->lod_global = lay.global;
     c->lane_lds = lane_lds;
-    c->lane_ok = lane_lds <= 160 * 1024;
+    c->lane_ok = c->precision == SL_FP32_RESCALED && lane_lds <= 160 * 1024;
     SL_HIP(c, sl::configure_lod_lds(c->precision, c->lds, c->lod_global));
     if (int rc2 = ensure_lod_slots(c)) return rc2;
     if (c->lane_ok) SL_HIP(c, sl::configure_lane_lds(c->precision, c->lane_lds));

@@ -41,6 +41,7 @@ struct LodArgs {
     const SeqOp* seq;
     const uint32_t* lstream;     // the lane kernel's descriptor stream
     int lane_arena_bytes;
+    int lane_zero;               // 0 (a mask the lane kernel applies to the lane index)
     // programs whose table arena exceeds LDS: per-workgroup global-memory
     // slots of arena_bytes (slot blockIdx.x + gridDim.x blockIdx.y), and the
     // launch's first interval and chunk
@@ -69,16 +70,16 @@ hipError_t launch_lod(const LodArgs& A, int precision, int block, size_t lds, in
                       hipStream_t s);
 // workgroups of one launch whose tables live in global memory
 constexpr int kLodGlobalSlots = 2048;
-// The lane kernel: one wavefront per (interval, position, chunk of
-// kLaneGraphs graphs), one (graph, position) evaluation per lane; chunk k's
-// log-sum goes to partial[k][l][pos] as launch_lod's does.
+// The lane kernel (FP32-rescaled contexts): one wavefront per (interval,
+// position, chunk of kLaneGraphs graphs), one (graph, position) evaluation
+// per lane; chunk k's log-sum goes to partial[k][l][pos] as launch_lod's does.
 constexpr int kLaneGraphs = 64;
 hipError_t launch_lane_lod(const LodArgs& A, int precision, size_t lds, int nchunks, hipStream_t s);
 hipError_t configure_lane_lds(int precision, size_t lds);
-// LDS of a lane-kernel wavefront: the per-lane arena and two graph columns
-inline size_t lane_lds_bytes(int seq_elems, int W, int precision) {
-    const size_t esz = precision == 0 ? 8 : 4;
-    return (size_t)seq_elems * kLaneGraphs * esz + (size_t)2 * W * kLaneGraphs * 4;
+// LDS of a lane-kernel wavefront (FP32): the per-lane arena, two graph
+// columns and the descriptor stream's ring of four 64-word windows
+inline size_t lane_lds_bytes(int seq_elems, int W) {
+    return (size_t)seq_elems * kLaneGraphs * 4 + (size_t)2 * W * kLaneGraphs * 4 + 4 * 64 * 4;
 }
 hipError_t configure_grad_lds(size_t lds);
 // nodg: P(T) (grid of one); else grid (M-1, npos, A.G)

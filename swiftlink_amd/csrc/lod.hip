@@ -444,133 +444,181 @@ __device__ __forceinline__ T quarter_rho(uint32_t code, int i, int j, const Thet
     return (T)0.25 * rho<T>(code, i, j, th);
 }
 
-// The descriptor stream, read 64 words at a time: lane j holds word j of the
-// current window (w0) and of the next two (w1, w2, in flight), and a uniform
-// word p of the current window is one readlane.  Headers and cells never
-// straddle a window.
+// The descriptor stream, read through a ring of four 64-word windows in the
+// wavefront's LDS: the window being read, the one after it, and the next one
+// in flight in one register (`pend`), written into the ring when the reader
+// moves on.  Headers and cells never straddle a window, and every word is a
+// broadcast LDS read (all lanes read the same address).
+constexpr int kRingWindows = 4;
+
 struct LaneStream {
-    const uint32_t* base;
-    uint32_t w0, w1, w2;
-    int cur;  // window of w0
-    __device__ __forceinline__ void init(const uint32_t* b, int lane) {
+    const uint32_t* base;   // the stream in global memory
+    uint32_t* ring;         // kRingWindows x 64 words of LDS
+    uint32_t pend;          // this lane's word of window cur + 2 (in flight)
+    int cur;                // the window being read
+    __device__ __forceinline__ void init(const uint32_t* b, uint32_t* r, int lane) {
         base = b;
+        ring = r;
         cur = 0;
-        w0 = b[lane];
-        w1 = b[64 + lane];
-        w2 = b[128 + lane];
+        ring[lane] = b[lane];
+        ring[64 + lane] = b[64 + lane];
+        pend = b[128 + lane];
     }
-    // make p's window the current one, prefetching two ahead
     __device__ __forceinline__ void seek(int p, int lane) {
         while ((p >> 6) > cur) {
-            w0 = w1;
-            w1 = w2;
             ++cur;
-            w2 = base[(size_t)(cur + 2) * 64 + lane];
+            ring[((cur + 1) % kRingWindows) * 64 + lane] = pend;
+            pend = base[(size_t)(cur + 2) * 64 + lane];
         }
     }
-    __device__ __forceinline__ uint32_t at(int p) const {
-        return (uint32_t)__builtin_amdgcn_readlane((int)w0, p & 63);
+    __device__ __forceinline__ const uint32_t* at(int p) const { return ring + ((p >> 6) % kRingWindows) * 64 + (p & 63); }
+};
+
+// The lane's arena word at a byte offset (the stream's offsets include the
+// x64 lane stride)
+__device__ __forceinline__ float arena_at(const unsigned char* la, uint32_t off) {
+    return *reinterpret_cast<const float*>(la + off);
+}
+
+// v if the mask word is all ones, 0 if it is zero (exact, branch-free)
+__device__ __forceinline__ float keep(float v, uint32_t mask) { return __uint_as_float(__float_as_uint(v) & mask); }
+
+// Descriptor words of one cell, read as broadcasts into vector registers (the
+// offset `dz` is a zero the compiler cannot prove uniform, so the words stay
+// in VGPRs and feed vector address arithmetic directly).
+template <int NW>
+struct CellWords {
+    uint32_t w[NW];
+    __device__ __forceinline__ void load(const uint32_t* rec, int dz) {
+#pragma unroll
+        for (int k = 0; k < NW; ++k) w[k] = rec[k + dz];
     }
 };
 
-template <typename T>
-__device__ __forceinline__ T trait_from(const LaneStream& st, int h, int v) {
-    const uint64_t bits = (uint64_t)st.at(h + 24 + 2 * v) | ((uint64_t)st.at(h + 25 + 2 * v) << 32);
-    return (T)__builtin_bit_cast(double, bits);
-}
-
-// One op of the sequential program for this lane's evaluation.  The header
-// starts at stream word h; returns the word after the op.
-template <typename T, bool RESCALE, int NP>
-__device__ __forceinline__ int lane_op(const LodArgs& A, LaneStream& st, int h, T* la, const uint32_t* lc,
-                                       const Thetas<T>& th, int lane, int* E) {
-    const int type = (int)st.at(h + 0), size = (int)st.at(h + 2), out = (int)st.at(h + 3);
-    const int peel = (int)st.at(h + 4), n_child = (int)st.at(h + 5), stride = (int)st.at(h + 6);
-    int poff[NP > 0 ? NP : 1];
-#pragma unroll
-    for (int q = 0; q < NP; ++q) poff[q] = (int)st.at(h + 8 + q);
-    T tr[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) tr[v] = trait_from<T>(st, h, v);
-    // transmission factors of this lane's graph: the peel node's (CHILD) or
-    // each child's (PARENT), formed once per op
-    T qr[4] = {(T)0, (T)0, (T)0, (T)0};
-    T r4[7][4];
-    if (type == kChild) {
-        const uint32_t cd = lane_code(lc, A.F, peel);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) qr[t] = quarter_rho<T>(cd, t >> 1, t & 1, th);
-    }
-#pragma unroll
-    for (int ch = 0; ch < 7; ++ch) {
-#pragma unroll
-        for (int ij = 0; ij < 4; ++ij) r4[ch][ij] = (T)0;
-        if (type == kParent && ch < n_child) {
-            const uint32_t cd = lane_code(lc, A.F, (int)st.at(h + 16 + ch));
-#pragma unroll
-            for (int ij = 0; ij < 4; ++ij) r4[ch][ij] = quarter_rho<T>(cd, ij >> 1, ij & 1, th);
-        }
-    }
-    T* o = la + (size_t)out * kLanes;
-    float mx = 0.0f;
-    const int cbase = h + kLsHeader;
+// One op's cells for the lane's evaluation, specialised on the op type, the
+// number of consumed tables and (PARENT) of children.  Every term of every
+// cell is formed (a term whose trait probability is zero adds an exact zero),
+// so the loop has no data-dependent branch; the next cell's descriptor words
+// are read before the current cell is evaluated.
+template <int TYPE, int NP, int NC>
+__device__ __forceinline__ void lane_cells(LaneStream& st, int cbase, int size, int stride, const float tr[4],
+                                           const float qr[4], const float A0[7], const float A1[7],
+                                           const float B0[7], const float B1[7], const unsigned char* la, float* o,
+                                           float* mx, int lane, int dz) {
+    constexpr int G = 1 + NP;
+    CellWords<4 * G> cur, nxt;
+    st.seek(cbase, lane);
+    cur.load(st.at(cbase), dz);
     for (int c = 0; c < size; ++c) {
-        const int p = cbase + c * stride;
-        st.seek(p, lane);
-        uint32_t info[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) info[t] = st.at(p + t);
-        uint32_t iw[NP > 0 ? 2 * NP : 1];
-#pragma unroll
-        for (int j = 0; j < 2 * NP; ++j) iw[j] = st.at(p + 4 + j);
-        T total = (T)0;
+        if (c + 1 < size) {
+            const int pn = cbase + (c + 1) * stride;
+            st.seek(pn, lane);
+            nxt.load(st.at(pn), dz);
+        }
+        float total = 0.0f;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const uint32_t inf = info[t];
-            if (!(inf & 4u)) continue;  // wave-uniform
-            T tmp = select4(tr, inf & 3u);
+            float tmp = TYPE == kChild ? __uint_as_float(cur.w[t * G]) : tr[t];
 #pragma unroll
-            for (int q = 0; q < NP; ++q) {
-                const int e = t * NP + q;
-                const int ix = (int)((iw[e >> 1] >> (16 * (e & 1))) & 0xFFFFu);
-                tmp *= la[(size_t)(poff[q] + ix) * kLanes];
-            }
-            if (type == kChild) {
+            for (int q = 0; q < NP; ++q) tmp *= arena_at(la, cur.w[t * G + 1 + q]);
+            if (TYPE == kChild) {
                 tmp *= qr[t];
-            } else if (type == kParent) {
-                T cp = (T)1;
+            } else if (TYPE == kParent) {
+                // per child: (sum over allowed i of its maternal factor) x
+                // (sum over allowed j of its paternal factor), 0.25 in A
+                float cp = 1.0f;
 #pragma unroll
-                for (int ch = 0; ch < 7; ++ch) {
-                    if (ch < n_child) {
-                        const uint32_t m = (inf >> (4 + 4 * ch)) & 15u;
-                        // adding exact zeros leaves the sum of the set terms unchanged
-                        T ct = (T)0;
-#pragma unroll
-                        for (int ij = 0; ij < 4; ++ij) ct += ((m >> ij) & 1u) ? r4[ch][ij] : (T)0;
-                        cp *= ct;
-                    }
+                for (int ch = 0; ch < NC; ++ch) {
+                    const uint32_t code = cur.w[t * G] >> (4 * ch);
+                    const float a = keep(A0[ch], 0u - (code & 1u)) + keep(A1[ch], 0u - ((code >> 1) & 1u));
+                    const float b = keep(B0[ch], 0u - ((code >> 2) & 1u)) + keep(B1[ch], 0u - ((code >> 3) & 1u));
+                    cp *= a * b;
                 }
                 tmp *= cp;
             }
             total += tmp;
         }
         o[(size_t)c * kLanes] = total;
-        if (RESCALE) mx = fmaxf(mx, (float)total);
+        *mx = fmaxf(*mx, total);
+        if (c + 1 < size) cur = nxt;
     }
-    if (RESCALE) {
-        // largest cell in [2^e, 2^(e+1)); renormalise outside the band
-        int e = mx > 0.0f ? __builtin_amdgcn_frexp_expf(mx) - 1 : 0;
-        if (e >= kRenormLo && e < kRenormHi) e = 0;
-        if (__ballot(e != 0)) {
-            for (int c = 0; c < size; ++c)
-                o[(size_t)c * kLanes] = (T)__builtin_amdgcn_ldexpf((float)o[(size_t)c * kLanes], -e);
-            *E += e;
+}
+
+template <int NP, int NC>
+__device__ __forceinline__ void lane_parent(LaneStream& st, int cbase, int size, int stride, const float tr[4],
+                                            const float qr[4], const float A0[7], const float A1[7], const float B0[7],
+                                            const float B1[7], const unsigned char* la, float* o, float* mx, int lane,
+                                            int dz) {
+    lane_cells<kParent, NP, NC>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, la, o, mx, lane, dz);
+}
+
+// One op of the sequential program; the header starts at stream word h.
+// Returns the word after the op.
+template <int NP>
+__device__ __forceinline__ int lane_op(const LodArgs& A, LaneStream& st, int h, float* la, const uint32_t* lc,
+                                       const Thetas<float>& th, int lane, int dz, int* E) {
+    const uint32_t* hd = st.at(h);
+    const int type = __builtin_amdgcn_readfirstlane((int)hd[0]);
+    const int size = __builtin_amdgcn_readfirstlane((int)hd[2]);
+    const int out = __builtin_amdgcn_readfirstlane((int)hd[3]);
+    const int peel = __builtin_amdgcn_readfirstlane((int)hd[4]);
+    const int n_child = __builtin_amdgcn_readfirstlane((int)hd[5]);
+    const int stride = __builtin_amdgcn_readfirstlane((int)hd[6]);
+    float tr[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) tr[v] = __uint_as_float(hd[24 + v]);
+    // transmission factors of this lane's graph
+    float qr[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float A0[7], A1[7], B0[7], B1[7];
+    if (type == kChild) {
+        const uint32_t cd = lane_code(lc, A.F, peel);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) qr[t] = quarter_rho<float>(cd, t >> 1, t & 1, th);
+    }
+#pragma unroll
+    for (int ch = 0; ch < 7; ++ch) {
+        A0[ch] = A1[ch] = B0[ch] = B1[ch] = 0.0f;
+        if (type == kParent && ch < n_child) {
+            // rho(i, j) = [maternal factors of i] x [paternal factors of j]:
+            // the sum over the product set of allowed (i, j) is a product of
+            // two sums
+            const uint32_t cd = lane_code(lc, A.F, __builtin_amdgcn_readfirstlane((int)hd[16 + ch]));
+            A0[ch] = 0.25f * (((cd & 1u) == 0u ? th.a1 : th.t1) * (((cd >> 1) & 1u) == 0u ? th.a2 : th.t2));
+            A1[ch] = 0.25f * (((cd & 1u) == 1u ? th.a1 : th.t1) * (((cd >> 1) & 1u) == 1u ? th.a2 : th.t2));
+            B0[ch] = (((cd >> 2) & 1u) == 0u ? th.a1 : th.t1) * (((cd >> 3) & 1u) == 0u ? th.a2 : th.t2);
+            B1[ch] = (((cd >> 2) & 1u) == 1u ? th.a1 : th.t1) * (((cd >> 3) & 1u) == 1u ? th.a2 : th.t2);
         }
+    }
+    float* o = la + (size_t)out * kLanes;
+    const unsigned char* lb = reinterpret_cast<const unsigned char*>(la);
+    float mx = 0.0f;
+    const int cbase = h + kLsHeader;
+    if (type == kChild) {
+        lane_cells<kChild, NP, 0>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz);
+    } else if (type == kParent) {
+        switch (n_child) {
+            case 0: lane_parent<NP, 0>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
+            case 1: lane_parent<NP, 1>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
+            case 2: lane_parent<NP, 2>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
+            case 3: lane_parent<NP, 3>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
+            case 4: lane_parent<NP, 4>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
+            case 5: lane_parent<NP, 5>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
+            case 6: lane_parent<NP, 6>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
+            default: lane_parent<NP, 7>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
+        }
+    } else {
+        lane_cells<kPartner, NP, 0>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz);
+    }
+    // largest cell in [2^e, 2^(e+1)); renormalise outside the band
+    int e = mx > 0.0f ? __builtin_amdgcn_frexp_expf(mx) - 1 : 0;
+    if (e >= kRenormLo && e < kRenormHi) e = 0;
+    if (__ballot(e != 0)) {
+        for (int c = 0; c < size; ++c) o[(size_t)c * kLanes] = __builtin_amdgcn_ldexpf(o[(size_t)c * kLanes], -e);
+        *E += e;
     }
     return cbase + ((size * stride + 31) & ~31);
 }
 
-template <typename T, bool RESCALE>
 __global__ void __launch_bounds__(kLanes) lane_lod_kernel(LodArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     if (A.gate_sweep && !gate_open(*A.gate_sweep, A.gate_burnin, A.gate_period)) return;
@@ -578,10 +626,11 @@ __global__ void __launch_bounds__(kLanes) lane_lod_kernel(LodArgs A) {
     const int lane = (int)threadIdx.x;
     const int g = chunk * kLanes + lane;
     const bool live = g < A.G;
-    LaneStream st;
-    st.init(A.lstream, lane);
-    T* la = reinterpret_cast<T*>(smem) + lane;
+    float* la = reinterpret_cast<float*>(smem) + lane;
     uint32_t* lc = reinterpret_cast<uint32_t*>(smem + A.lane_arena_bytes) + lane;
+    uint32_t* ring = reinterpret_cast<uint32_t*>(smem + A.lane_arena_bytes + (size_t)8 * A.W * kLanes);
+    LaneStream st;
+    st.init(A.lstream, ring, lane);
     // this lane's columns l and l+1, and its recombination count
     const uint32_t* c0 = A.graphs + ((size_t)(live ? g : 0) * A.M + l) * A.W;
     int ncross = 0;
@@ -594,31 +643,35 @@ __global__ void __launch_bounds__(kLanes) lane_lod_kernel(LodArgs A) {
     // set_thetas (trait_rfunction.h:50-56): linear partial thetas
     const double ptheta = A.partial_theta[l];
     const double tl = ptheta * (double)(pos + 1), tq = ptheta * (double)(A.npos - pos);
-    Thetas<T> th;
-    th.t1 = (T)tl;
-    th.a1 = (T)(1.0 - tl);
-    th.t2 = (T)tq;
-    th.a2 = (T)(1.0 - tq);
+    Thetas<float> th;
+    th.t1 = (float)tl;
+    th.a1 = (float)(1.0 - tl);
+    th.t2 = (float)tq;
+    th.a2 = (float)(1.0 - tq);
     const double lt = A.log_theta[l], lant = A.log_anti[l];
+    // a zero that the compiler cannot prove uniform (a kernel argument masks
+    // the lane index)
+    const int dz = lane & A.lane_zero;
     int E = 0, h = 0, last_out = 0;
     for (int s = 0; s < A.n_ops; ++s) {
         st.seek(h, lane);
-        last_out = (int)st.at(h + 3);
-        switch ((int)st.at(h + 1)) {
-            case 0: h = lane_op<T, RESCALE, 0>(A, st, h, la, lc, th, lane, &E); break;
-            case 1: h = lane_op<T, RESCALE, 1>(A, st, h, la, lc, th, lane, &E); break;
-            case 2: h = lane_op<T, RESCALE, 2>(A, st, h, la, lc, th, lane, &E); break;
-            case 3: h = lane_op<T, RESCALE, 3>(A, st, h, la, lc, th, lane, &E); break;
-            case 4: h = lane_op<T, RESCALE, 4>(A, st, h, la, lc, th, lane, &E); break;
-            case 5: h = lane_op<T, RESCALE, 5>(A, st, h, la, lc, th, lane, &E); break;
-            case 6: h = lane_op<T, RESCALE, 6>(A, st, h, la, lc, th, lane, &E); break;
-            case 7: h = lane_op<T, RESCALE, 7>(A, st, h, la, lc, th, lane, &E); break;
-            default: h = lane_op<T, RESCALE, 8>(A, st, h, la, lc, th, lane, &E); break;
+        const uint32_t* hd = st.at(h);
+        last_out = __builtin_amdgcn_readfirstlane((int)hd[3]);
+        switch (__builtin_amdgcn_readfirstlane((int)hd[1])) {
+            case 0: h = lane_op<0>(A, st, h, la, lc, th, lane, dz, &E); break;
+            case 1: h = lane_op<1>(A, st, h, la, lc, th, lane, dz, &E); break;
+            case 2: h = lane_op<2>(A, st, h, la, lc, th, lane, dz, &E); break;
+            case 3: h = lane_op<3>(A, st, h, la, lc, th, lane, dz, &E); break;
+            case 4: h = lane_op<4>(A, st, h, la, lc, th, lane, dz, &E); break;
+            case 5: h = lane_op<5>(A, st, h, la, lc, th, lane, dz, &E); break;
+            case 6: h = lane_op<6>(A, st, h, la, lc, th, lane, dz, &E); break;
+            case 7: h = lane_op<7>(A, st, h, la, lc, th, lane, dz, &E); break;
+            default: h = lane_op<8>(A, st, h, la, lc, th, lane, dz, &E); break;
         }
     }
     // ln L - R - T (peeler.cc:97-99) and the chunk's log-sum over its graphs
     const double L = (double)la[(size_t)last_out * kLanes];
-    double term = L > 0.0 ? log(L) + (RESCALE ? (double)E * 0.69314718055994530942 : 0.0) : -INFINITY;
+    double term = L > 0.0 ? log(L) + (double)E * 0.69314718055994530942 : -INFINITY;
     const int nm = 2 * (A.N - A.F);
     term -= (double)ncross * lt + (double)(nm - ncross) * lant;
     term -= A.trans;
@@ -992,20 +1045,15 @@ hipError_t launch_lod(const LodArgs& A, int precision, int block, size_t lds, in
 }
 
 hipError_t configure_lane_lds(int precision, size_t lds) {
+    if (precision == 0) return hipErrorInvalidValue;  // the lane kernel is FP32
     if (lds <= 64 * 1024) return hipSuccess;
-    if (precision == 0)
-        return hipFuncSetAttribute((const void*)lane_lod_kernel<double, false>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    return hipFuncSetAttribute((const void*)lane_lod_kernel<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds);
+    return hipFuncSetAttribute((const void*)lane_lod_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 hipError_t launch_lane_lod(const LodArgs& A, int precision, size_t lds, int nchunks, hipStream_t s) {
+    if (precision == 0) return hipErrorInvalidValue;
     dim3 grid(A.M - 1, A.npos, nchunks);
-    if (precision == 0)
-        hipLaunchKernelGGL((lane_lod_kernel<double, false>), grid, dim3(kLanes), lds, s, A);
-    else
-        hipLaunchKernelGGL((lane_lod_kernel<float, true>), grid, dim3(kLanes), lds, s, A);
+    hipLaunchKernelGGL(lane_lod_kernel, grid, dim3(kLanes), lds, s, A);
     return hipGetLastError();
 }
 

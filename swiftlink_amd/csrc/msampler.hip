@@ -1015,7 +1015,62 @@ __global__ void morder_kernel(MsampArgs A, const int32_t* fixed_order) {
     }
 }
 
+// The CODA trace line of each chain (markov_chain.cc:362-370):
+// DescentGraph::get_likelihood (descent_graph.cc:184-189), the transmission
+// 2(N-F) ln 1/2 plus the recombination log-probability of every interval
+// (:195-242), plus ln of the founder-allele likelihood of every locus
+// (:244-267) from the current columns' log-likelihoods in lcur; -inf when a
+// locus is illegal.  One workgroup per chain.
+constexpr int kTraceBlock = 256;
+__global__ void __launch_bounds__(kTraceBlock) trace_kernel(MsampArgs A, const double* log_theta,
+                                                            const double* log_anti, double trans, double* out) {
+    __shared__ double red[2][kTraceBlock / 64];
+    __shared__ int s_bad;
+    const int chain = blockIdx.x, tid = threadIdx.x, M = A.M, W = A.W;
+    const uint32_t* g = A.graphs + (size_t)chain * M * W;
+    const double* lc = A.lcur + (size_t)chain * M;
+    const int nm = 2 * (A.N - A.F);
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    double rec = 0.0, prior = 0.0;
+    bool bad = false;
+    for (int l = tid; l < M; l += kTraceBlock) {
+        const double x = lc[l];
+        if (x == -INFINITY) bad = true;
+        else prior += x;
+        if (l + 1 < M) {
+            int nc = 0;
+            for (int w = 0; w < W; ++w) nc += __popc(g[(size_t)l * W + w] ^ g[(size_t)(l + 1) * W + w]);
+            rec += (double)nc * log_theta[l] + (double)(nm - nc) * log_anti[l];
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        rec += __shfl_xor(rec, off);
+        prior += __shfl_xor(prior, off);
+    }
+    if (bad) s_bad = 1;
+    if (lane_id() == 0) {
+        red[0][tid >> 6] = rec;
+        red[1][tid >> 6] = prior;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double r = 0.0, p = 0.0;
+        for (int w = 0; w < kTraceBlock / 64; ++w) {
+            r += red[0][w];
+            p += red[1][w];
+        }
+        out[chain] = s_bad ? -INFINITY : (trans + r) + p;  // log_product(transmission, prior)
+    }
+}
+
 }  // namespace
+
+hipError_t launch_trace(const MsampArgs& A, const double* log_theta, const double* log_anti, double trans, double* out,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(trace_kernel, dim3(A.n_chains), dim3(kTraceBlock), 0, s, A, log_theta, log_anti, trans, out);
+    return hipGetLastError();
+}
 
 bool msampler_supported(int N, int F) { return F >= 1 && 2 * F <= kMaxFagNodes && N <= 65535; }
 

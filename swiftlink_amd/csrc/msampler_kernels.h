@@ -66,5 +66,9 @@ hipError_t launch_morder(const MsampArgs& A, const int32_t* fixed_order, hipStre
 hipError_t launch_mstep(const MsampArgs& A, size_t lds, hipStream_t s);
 hipError_t launch_mfb(const MsampArgs& A, hipStream_t s);
 hipError_t launch_mspec(const MsampArgs& A, size_t lds, hipStream_t s);
+// per chain: the CODA trace likelihood of its graph into out[chain], from
+// lcur holding the current columns' founder-allele log-likelihoods
+hipError_t launch_trace(const MsampArgs& A, const double* log_theta, const double* log_anti, double trans, double* out,
+                        hipStream_t s);
 
 }  // namespace sl

@@ -379,7 +379,14 @@ std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int
             P.seq.push_back(s);
         }
         P.seq_elems = sa.peak;
-        // the descriptor stream of the lane kernel
+        // the descriptor stream of the lane kernel (FP32): per op a header,
+        // then per cell four term groups [w, byte offsets of the NP consumed
+        // cells in the lane arena], w being the kid value's disease
+        // probability as float bits (CHILD), or for PARENT per child the
+        // allowed transmitted alleles, maternal bits 0-1 and paternal bits 2-3
+        // of a 4-bit field (the kid trait fixes each parent's allele
+        // independently), or unused.  Each cell takes a power-of-two number of
+        // words so that none straddles a 64-word window.
         std::vector<uint32_t>& S = P.lstream;
         S.clear();
         auto pad_to = [&](size_t m) {
@@ -389,8 +396,10 @@ std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int
             const int o = post[i];
             const OpDesc& d = P.ops[o];
             const SeqOp& so = P.seq[i];
-            const int words = 4 + 2 * d.n_prev;
-            const int stride = words <= 8 ? 8 : words <= 16 ? 16 : 32;
+            const int group = 1 + d.n_prev;
+            int stride = 8;
+            while (stride < 4 * group) stride *= 2;
+            if (stride > 64) return "a lane-kernel cell record exceeds 64 words";
             const size_t h = S.size();
             S.resize(h + kLsHeader, 0u);
             S[h + 0] = (uint32_t)d.type;
@@ -403,19 +412,40 @@ std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int
             for (int q = 0; q < d.n_prev; ++q) S[h + 8 + q] = (uint32_t)so.prev_off[q];
             for (int ch = 0; ch < d.n_child; ++ch) S[h + 16 + ch] = (uint32_t)so.child[ch];
             for (int v = 0; v < 4; ++v) {
-                uint64_t bits;
-                const double x = ped.disease_prob[4 * d.peel + v];
-                memcpy(&bits, &x, 8);
-                S[h + 24 + 2 * v] = (uint32_t)bits;
-                S[h + 25 + 2 * v] = (uint32_t)(bits >> 32);
+                const float f = (float)ped.disease_prob[4 * d.peel + v];
+                memcpy(&S[h + 24 + v], &f, 4);
             }
             for (int c = 0; c < d.size; ++c) {
                 const size_t b = S.size();
                 S.resize(b + stride, 0u);
-                for (int t = 0; t < 4; ++t) S[b + t] = P.tinfo[(size_t)(d.task_base + c) * 4 + t];
-                for (int e = 0; e < 4 * d.n_prev; ++e) {
-                    const uint32_t ix = P.idx[(size_t)d.idx_base + (size_t)c * 4 * d.n_prev + e];
-                    S[b + 4 + e / 2] |= ix << (16 * (e & 1));
+                for (int t = 0; t < 4; ++t) {
+                    const uint32_t info = P.tinfo[(size_t)(d.task_base + c) * 4 + t];
+                    const bool valid = (info & 4u) != 0u;
+                    uint32_t w = 0;
+                    if (d.type == kChild) {
+                        const float f = valid ? (float)ped.disease_prob[4 * d.peel + (info & 3u)] : 0.0f;
+                        memcpy(&w, &f, 4);
+                    } else if (d.type == kParent) {
+                        for (int ch = 0; ch < d.n_child; ++ch) {
+                            const uint32_t m = (info >> (4 + 4 * ch)) & 15u;  // bit 2i+j: (i, j) gives the kid's trait
+                            uint32_t I = 0, J = 0;
+                            for (int ij = 0; ij < 4; ++ij)
+                                if ((m >> ij) & 1u) {
+                                    I |= 1u << (ij >> 1);
+                                    J |= 1u << (ij & 1);
+                                }
+                            // the allowed (i, j) form the product set I x J
+                            for (int ij = 0; ij < 4; ++ij)
+                                if ((((m >> ij) & 1u) != 0u) != (((I >> (ij >> 1)) & (J >> (ij & 1)) & 1u) != 0u))
+                                    return "a PARENT transmission mask is not a product set";
+                            w |= (I | (J << 2)) << (4 * ch);
+                        }
+                    }
+                    S[b + t * group] = w;
+                    for (int q = 0; q < d.n_prev; ++q) {
+                        const uint32_t ix = P.idx[(size_t)d.idx_base + ((size_t)c * 4 + t) * d.n_prev + q];
+                        S[b + t * group + 1 + q] = ((uint32_t)so.prev_off[q] + ix) * 64u * 4u;
+                    }
                 }
             }
             pad_to(32);

@@ -112,14 +112,15 @@ struct PeelProgram {
     // table), first-fit arena offsets, and the arena's extent in cells
     std::vector<SeqOp> seq;
     int seq_elems = 0;
-    // The lane kernel's descriptor stream: per op in sequential order a
-    // 32-word header (kLsHeader: type, n_prev, size, out_off, peel, n_child,
-    // cell stride, 0, consumed-table offsets [8], children [7], 0, the peel
-    // node's four disease probabilities as FP64 bit pairs), then each cell's
-    // four term words and its 4 x n_prev consumed-cell indices (u16 pairs) in
-    // a stride of 8, 16 or 32 words, the op padded to 32 words; three zero
-    // windows of 64 words close it (the stream is read 64 words at a time,
-    // prefetched two windows ahead).
+    // The lane kernel's descriptor stream (FP32): per op in sequential order
+    // a 32-word header (kLsHeader: type, n_prev, size, out_off, peel,
+    // n_child, cell stride, 0, consumed-table offsets [8], children [7], 0,
+    // the peel node's four disease probabilities as float bits, 0 x 4), then
+    // per cell four term groups of 1 + n_prev words (a term word, then the
+    // byte offsets of the consumed cells in the 64-lane arena) in a stride of
+    // 8, 16, 32 or 64 words, the op padded to 32 words; three zero windows of
+    // 64 words close it (the stream is read 64 words at a time, one window
+    // ahead).
     std::vector<uint32_t> lstream;
     double flops_per_eval = 0.0;        // algorithmic flops per (graph, interval, position)
     double lds_bytes_per_eval = 0.0;    // table reads per evaluation, in elements

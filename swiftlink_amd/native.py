@@ -107,6 +107,8 @@ _SIGS = {
     "sl_mcmc_likelihood": (C.c_int, [C.c_void_p, f64p]),
     "sl_mcmc_near_edge": (C.c_int, [C.c_void_p, i64p]),
     "sl_mcmc_near_edge_loci": (C.c_int, [C.c_void_p, u32p]),
+    "sl_mcmc_trace": (C.c_int, [C.c_void_p, C.c_int32]),
+    "sl_mcmc_trace_read": (C.c_int, [C.c_void_p, C.c_int64, i64p, i32p, f64p]),
     "sl_set_options": (C.c_int, [C.c_void_p, C.POINTER(Options)]),
     "sl_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]),
     "sl_batch_run": (C.c_int, [C.c_void_p, C.POINTER(Schedule), C.POINTER(McmcStats)]),
@@ -124,6 +126,12 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBPATH):
             raise SlError(f"{LIBPATH} is not built (run `make -C {HERE}` or __graft_entry__.build())")
+        # torch ships its own HIP runtime under the file name libamdhip64.so;
+        # loading it first makes the library bind to that same copy (same
+        # soname), so the process holds one HIP runtime.  Loaded the other way
+        # round, the process would open the GPU through two runtimes and
+        # torch's own device initialisation fails.
+        import torch  # noqa: F401
         l = C.CDLL(LIBPATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(l, name)
@@ -381,6 +389,21 @@ class Context:
         out = np.zeros(self.n_chains, np.int64)
         self._check(lib().sl_mcmc_near_edge(self.h, _ptr(out, i64p)), "sl_mcmc_near_edge")
         return out
+
+    def mcmc_trace(self, enable: bool = True):
+        """Records the CODA likelihood of every chain at every scored sweep."""
+        self._check(lib().sl_mcmc_trace(self.h, 1 if enable else 0), "sl_mcmc_trace")
+
+    def mcmc_trace_read(self):
+        """(iterations [R], loglik [R][C]) of the recorded trace."""
+        n = C.c_int64(0)
+        self._check(lib().sl_mcmc_trace_read(self.h, 0, C.byref(n), None, None), "sl_mcmc_trace_read")
+        it = np.zeros(n.value, np.int32)
+        ll = np.zeros((n.value, self.n_chains))
+        if n.value:
+            self._check(lib().sl_mcmc_trace_read(self.h, n.value, C.byref(n), _ptr(it, i32p), _ptr(ll, f64p)),
+                        "sl_mcmc_trace_read")
+        return it, ll
 
     def mcmc_near_edge_loci(self) -> np.ndarray:
         """[C][M] booleans: loci where the L-sampler drew near a CDF edge."""

@@ -13,7 +13,8 @@ Each fixture holds:
   lsweep_*    one L-sweep (LocusSampler::step over a locus order) under the
               Philox stream (seed, chain 0, sweep 5)
   msweep_*    one M-sweep (MeiosisSampler reset + steps) under (seed, 0, sweep 6)
-  chain_*     a 40-sweep chain in the shared schedule with scoring
+  chain_*     a 40-sweep chain in the shared schedule with scoring, and its CODA
+              trace: DescentGraph::get_likelihood at every scored sweep
   impute_*    4 sequential-imputation replicates (LocusSampler::sequential_imputation)
               under (seed 55, chain r, 0, step 0xFFFFFFFC), their weights and the pick
   impute0_*   LocusSampler::locus_by_locus under (seed 56, 0, 0, step 0xFFFFFFFC)
@@ -97,6 +98,7 @@ def make(name, base, work):
         chain_out=rd("ch/graph_out.i32", np.int32).reshape(M, 2 * N),
         chain_kinds=rd("ch/sweep_kinds.i32", np.int32), chain_acc=rd("ch/lod_acc.f64", np.float64),
         chain_count=np.array([json.load(open(os.path.join(work, "ch/chain_meta.json")))["count"]]),
+        chain_coda=rd("ch/coda.f64", np.float64), chain_coda_iter=rd("ch/coda_iter.i32", np.int32),
         impute_seed=np.array([55, 4]),
         impute_graphs=rd("si/impute_graphs.i32", np.int32).reshape(4, M, 2 * N),
         impute_weights=rd("si/impute_weights.f64", np.float64), impute_best=rd("si/impute_best.i32", np.int32),

@@ -117,3 +117,67 @@ def test_chain_fp32_stays_legal_and_scores():
         ref = np.array([o.fag(out[c], l) for l in range(case.M)])
         assert np.all(ref > 0)
         assert np.all(np.abs(lik[c] / ref - 1.0) <= 1e-12)
+
+
+def _coda_value(o, case, dg):
+    """DescentGraph::get_likelihood (descent_graph.cc:184-215) of one graph
+    from its parts: 2(N-F) ln 1/2 + the recombination log-probability
+    between adjacent loci + the sum of ln FAG over loci (the oracle's
+    founder-allele-graph likelihood)."""
+    nm = 2 * (case.N - case.F)
+    m = dg[:, 2 * case.F:2 * case.N]
+    cross = (m[1:] != m[:-1]).sum(axis=1)
+    rec = (cross * np.log(case.theta) + (nm - cross) * np.log1p(-case.theta)).sum()
+    return nm * np.log(0.5) + rec + sum(np.log(o.fag(dg, l)) for l in range(case.M))
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_coda_trace_matches_reference(name):
+    """The trace holds the reference chain's CODA likelihoods
+    (markov_chain.cc:361-369 writes get_likelihood at each scored sweep): the
+    same iterations, and values within 1e-9 relative of the golden ones; the
+    last row (the final sweep is scored) equals the final graph's likelihood
+    computed from its parts."""
+    case, fx = load_golden(name)
+    seed, chain, sweeps, burnin, period = (int(x) for x in fx["chain_params"])
+    lprob = float(fx["chain_lprob"][0])
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(fx["graph_init"][None])
+        ctx.mcmc_trace(True)
+        ctx.mcmc_run(sweeps, burnin, period, lprob, seed)
+        it, ll = ctx.mcmc_trace_read()
+        out = ctx.mcmc_graphs()[0]
+    assert np.array_equal(it, fx["chain_coda_iter"])
+    np.testing.assert_allclose(ll[:, 0], fx["chain_coda"], rtol=1e-9, atol=0)
+    assert it[-1] == sweeps
+    o = Oracle(case)
+    assert abs(ll[-1, 0] - _coda_value(o, case, out)) <= 1e-9 * abs(ll[-1, 0])
+
+
+def test_coda_trace_spans_runs_and_chains():
+    """Several chains and a schedule split over two runs record the same trace
+    as one run; chain 0 (same start, same Philox stream) repeats the golden
+    values; re-initialising clears the trace; turning it off records
+    nothing."""
+    case, fx = load_golden("ped30_snp100")
+    seed, _, sweeps, burnin, period = (int(x) for x in fx["chain_params"])
+    lprob = float(fx["chain_lprob"][0])
+    starts = np.stack([fx["graph_init"], fx["graphs"][0], fx["graphs"][1]])
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(starts)
+        ctx.mcmc_trace(True)
+        ctx.mcmc_run(sweeps, burnin, period, lprob, seed)
+        it1, ll1 = ctx.mcmc_trace_read()
+        ctx.mcmc_init(starts)
+        assert ctx.mcmc_trace_read()[0].size == 0
+        ctx.mcmc_run(17, burnin, period, lprob, seed)
+        ctx.mcmc_run(sweeps - 17, burnin, period, lprob, seed, first_sweep=17)
+        it2, ll2 = ctx.mcmc_trace_read()
+        ctx.mcmc_init(starts)
+        ctx.mcmc_trace(False)
+        ctx.mcmc_run(sweeps, burnin, period, lprob, seed)
+        assert ctx.mcmc_trace_read()[0].size == 0
+    assert ll1.shape == (len(fx["chain_coda_iter"]), 3)
+    assert np.array_equal(it1, it2) and np.array_equal(ll1, ll2)
+    np.testing.assert_allclose(ll1[:, 0], fx["chain_coda"], rtol=1e-9, atol=0)
+    assert np.all(np.isfinite(ll1))

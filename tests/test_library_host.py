@@ -27,6 +27,20 @@ def test_library_exports_header_symbols():
     assert sorted(native.exported_symbols()) == names
 
 
+def test_library_shares_torch_hip_runtime():
+    """Loading the library before anything imports torch still leaves one HIP
+    runtime in the process (two copies each open the GPU, and torch's own
+    device initialisation then fails)."""
+    import subprocess
+    import sys
+    code = ("import re\nfrom swiftlink_amd import native\nnative.lib()\nimport torch\n"
+            "m = open('/proc/self/maps').read()\n"
+            "print(len(set(re.findall(r'/\\S*libamdhip64\\S*', m))), len(set(re.findall(r'/\\S*libhsa-runtime64\\S*', m))))")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split() == ["1", "1"]
+
+
 @pytest.mark.parametrize("name", GOLDEN_CASES)
 def test_program_bounds_and_fp64_bit_exact(name):
     case, fx = load_golden(name)
