@@ -419,10 +419,11 @@ int sl_trait_prob_grad(sl_ctx* ctx, double* out);
  * Requires sl_set_trait_model. */
 int sl_lod_grad(sl_ctx* ctx, const uint32_t* dev_graphs, int32_t n_graphs, double* lod, double* dlod);
 
-/* Compiles a peel table into the level-scheduled program the LOD kernel
- * runs and serialises it into buf (int32 words; *needed gets the length).
- * Host only, no device work: lets the program's indexing be checked on a
- * machine without a GPU.  No counterpart in the reference. */
+/* Compiles a peel table into the programs the LOD kernels run (the
+ * level-scheduled form, the sequential form and the lane kernel's descriptor
+ * stream) and serialises them into buf (int32 words; *needed gets the
+ * length).  Host only, no device work: lets the programs' indexing be checked
+ * on a machine without a GPU.  No counterpart in the reference. */
 int sl_program_export(const sl_pedigree_desc* ped, const sl_peel_desc* peel, int32_t npos, int32_t* buf,
                       int64_t cap, int64_t* needed);
 

@@ -609,8 +609,8 @@ int sl_create(const sl_pedigree_desc* ped, const sl_map_desc* map, const sl_peel
             return fail(nullptr, SL_E_UNSUPPORTED, "peel program's exponents and codes need more than 160 KiB of LDS");
         }
     }
-    c->lane_lds = sl::lane_lds_bytes(c->prog.seq_elems, c->W);
-    c->lane_ok = c->precision == SL_FP32_RESCALED && c->lane_lds <= 160 * 1024;
+    c->lane_lds = sl::lane_lds_bytes(c->prog.seq_elems, c->N - c->F);
+    c->lane_ok = c->precision == SL_FP32_RESCALED && c->prog.lane_stream_ok && c->lane_lds <= 160 * 1024;
 
     c->theta.assign(map->theta, map->theta + c->M - 1);
     c->ptheta.assign(map->partial_theta, map->partial_theta + c->M - 1);
@@ -1812,7 +1812,7 @@ int install_lod_program(sl_ctx* c) {
     const size_t lds = lay.lds;
     if (lds > 160 * 1024)
         return fail(c, SL_E_UNSUPPORTED, "peel program's exponents and codes need more than 160 KiB of LDS");
-    const size_t lane_lds = sl::lane_lds_bytes(P.seq_elems, c->W);
+    const size_t lane_lds = sl::lane_lds_bytes(P.seq_elems, c->N - c->F);
     SL_HIP(c, hipStreamSynchronize(c->stream));
     void** bufs[] = {(void**)&c->d_ops, (void**)&c->d_prevs, (void**)&c->d_children, (void**)&c->d_tasks,
                      (void**)&c->d_tinfo, (void**)&c->d_idx, (void**)&c->d_level_base, (void**)&c->d_sched_off,
@@ -1828,7 +1828,7 @@ int install_lod_program(sl_ctx* c) {
     c->lds = lds;
     c->lod_global = lay.global;
     c->lane_lds = lane_lds;
-    c->lane_ok = c->precision == SL_FP32_RESCALED && lane_lds <= 160 * 1024;
+    c->lane_ok = c->precision == SL_FP32_RESCALED && c->prog.lane_stream_ok && lane_lds <= 160 * 1024;
     SL_HIP(c, sl::configure_lod_lds(c->precision, c->lds, c->lod_global));
     if (int rc2 = ensure_lod_slots(c)) return rc2;
     if (c->lane_ok) SL_HIP(c, sl::configure_lane_lds(c->precision, c->lane_lds));
@@ -2088,6 +2088,9 @@ int sl_program_export(const sl_pedigree_desc* ped, const sl_peel_desc* peel, int
     o.push_back(P.seq_elems);
     w = reinterpret_cast<const int32_t*>(P.seq.data());
     o.insert(o.end(), w, w + P.seq.size() * (sizeof(sl::SeqOp) / 4));
+    // the lane kernel's descriptor stream (empty when it cannot run the program)
+    o.push_back((int32_t)P.lstream.size());
+    for (uint32_t x : P.lstream) o.push_back((int32_t)x);
     *needed = (int64_t)o.size();
     if (buf && cap >= (int64_t)o.size()) memcpy(buf, o.data(), o.size() * 4);
     return SL_OK;

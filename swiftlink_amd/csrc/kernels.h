@@ -76,10 +76,15 @@ constexpr int kLodGlobalSlots = 2048;
 constexpr int kLaneGraphs = 64;
 hipError_t launch_lane_lod(const LodArgs& A, int precision, size_t lds, int nchunks, hipStream_t s);
 hipError_t configure_lane_lds(int precision, size_t lds);
-// LDS of a lane-kernel wavefront (FP32): the per-lane arena, two graph
-// columns and the descriptor stream's ring of four 64-word windows
-inline size_t lane_lds_bytes(int seq_elems, int W) {
-    return (size_t)seq_elems * kLaneGraphs * 4 + (size_t)2 * W * kLaneGraphs * 4 + 4 * 64 * 4;
+// The lane kernel's transmission table: Z[16 codes][16 masks] and Q[16][4]
+constexpr int kZWords = 256 + 64;
+constexpr int kLaneRingWindows = 2;
+// LDS of a lane-kernel wavefront (FP32): the per-lane arena, each lane's
+// 4-bit phased-trait codes of the nf non-founders, the transmission table and
+// the descriptor stream's ring of 64-word windows
+inline size_t lane_lds_bytes(int seq_elems, int nf) {
+    return (size_t)seq_elems * kLaneGraphs * 4 + (size_t)((nf + 7) / 8) * kLaneGraphs * 4 + kZWords * 4 +
+           kLaneRingWindows * 64 * 4;
 }
 hipError_t configure_grad_lds(size_t lds);
 // nodg: P(T) (grid of one); else grid (M-1, npos, A.G)

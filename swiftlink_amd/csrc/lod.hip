@@ -415,41 +415,62 @@ __global__ void lod_kernel(LodArgs A) {
 
 // ---- The trait peel with one (graph, position) evaluation per lane ----
 //
-// One wavefront per (interval l, position, block of 64 graphs).  Every lane
-// evaluates the whole peel program for its own graph, walking the ops in the
-// program's sequential order and each op's cells in ascending order, so op
-// and cell descriptors are wave-uniform (scalar loads) and no lane idles on a
-// small table.  Lane j keeps cell c of a table at offset off in LDS word
+// One wavefront per (interval l, position, block of 64 graphs), FP32.  Every
+// lane evaluates the whole peel program for its own graph, walking the ops in
+// the program's sequential order and each op's cells in ascending order, so
+// op and cell descriptors are wave-uniform and no lane idles on a small
+// table.  Lane j keeps cell c of a table at offset off in LDS word
 // (off + c) * 64 + j: each lane reads and writes only its own column, so the
-// wave needs no barrier.  The arithmetic of a cell is lod_kernel's, term for
-// term, so FP64 terms equal that kernel's bit for bit.  FP32 keeps each lane's
-// tables in range by multiplying a table by 2^-e (exact) when its largest
-// cell leaves [2^-6, 2^6), and sums the e of the lane into its likelihood.
+// wave needs no barrier.  Each lane keeps its tables in range by multiplying
+// a table by 2^-e (exact) when its largest cell leaves [2^-6, 2^6), and sums
+// the e into its likelihood.
+//
+// A wavefront's LDS: the lanes' arena, then each lane's phased-trait codes
+// (one 4-bit code per non-founder), the transmission table Z (below), and the
+// descriptor ring.
 constexpr int kLanes = kLaneGraphs;
 constexpr int kRenormLo = -6, kRenormHi = 6;
 
-// phased-trait code of person p for the lane: maternal bit at l and l+1, then
-// paternal (the layout of lod_kernel's code[])
-__device__ __forceinline__ uint32_t lane_code(const uint32_t* lc, int F, int p) {
-    const int b = 2 * (p - F);
-    const int w = b >> 5, sh = b & 31;  // b is even: the paternal bit is in the same word
-    const uint32_t x0 = lc[(size_t)(2 * w) * kLanes], x1 = lc[(size_t)(2 * w + 1) * kLanes];
-    return ((x0 >> sh) & 1u) | (((x1 >> sh) & 1u) << 1) | (((x0 >> (sh + 1)) & 1u) << 2) |
-           (((x1 >> (sh + 1)) & 1u) << 3);
+// The transmission factors of the interval's position (trait_rfunction.cc:9-22),
+// shared by the wavefront: for a child whose phased-trait code is `code`
+// (maternal bit at l and l+1, then paternal) and a set of transmitted alleles
+// `mask` (maternal allele i allowed if bit i, paternal j if bit 2+j),
+//   Z[code][mask] = (sum over allowed i of 0.25 f_i(code)) x (sum over allowed j of g_j(code)),
+// where f_i = (bit 0 == i ? 1-theta_L : theta_L) (bit 1 == i ? 1-theta_R : theta_R)
+// and g_j the same over bits 2-3: the PARENT transmission sum of one child
+// factorises this way because the allowed (i, j) form a product set.  Q[code][t]
+// = Z[code][{i = t>>1} x {j = t&1}] = 0.25 rho(code, i, j), a CHILD op's factor.
+
+__device__ __forceinline__ float zentry(int e, const Thetas<float>& th) {
+    int code, mask;
+    if (e < 256) {
+        code = e >> 4;
+        mask = e & 15;
+    } else {
+        code = (e - 256) >> 2;
+        const int t = (e - 256) & 3;
+        mask = (1 << (t >> 1)) | (4 << (t & 1));
+    }
+    float a = 0.0f, b = 0.0f;
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+        const float f = 0.25f * (((code & 1) == v ? th.a1 : th.t1) * (((code >> 1) & 1) == v ? th.a2 : th.t2));
+        const float g = ((code >> 2) & 1) == v ? th.a1 : th.t1;
+        const float g2 = ((code >> 3) & 1) == v ? th.a2 : th.t2;
+        if ((mask >> v) & 1) a += f;
+        if ((mask >> (2 + v)) & 1) b += g * g2;
+    }
+    return a * b;
 }
 
-// 0.25 * rho (trait_rfunction.cc:9-22) with lod_kernel's operation order
-template <typename T>
-__device__ __forceinline__ T quarter_rho(uint32_t code, int i, int j, const Thetas<T>& th) {
-    return (T)0.25 * rho<T>(code, i, j, th);
-}
-
-// The descriptor stream, read through a ring of four 64-word windows in the
-// wavefront's LDS: the window being read, the one after it, and the next one
-// in flight in one register (`pend`), written into the ring when the reader
-// moves on.  Headers and cells never straddle a window, and every word is a
-// broadcast LDS read (all lanes read the same address).
-constexpr int kRingWindows = 4;
+// The descriptor stream, read through a ring of two 64-word windows in the
+// wavefront's LDS: the window being read and the one after it, with the next
+// one in flight in one register (`pend`), written into the ring when the
+// reader moves on.  Headers and 32-word cell blocks never straddle a window,
+// the reader only moves forward and reads an op's header before seeking to
+// its cells, and every word is a broadcast LDS read (all lanes read the same
+// address).
+constexpr int kRingWindows = kLaneRingWindows;
 
 struct LaneStream {
     const uint32_t* base;   // the stream in global memory
@@ -467,156 +488,186 @@ struct LaneStream {
     __device__ __forceinline__ void seek(int p, int lane) {
         while ((p >> 6) > cur) {
             ++cur;
-            ring[((cur + 1) % kRingWindows) * 64 + lane] = pend;
+            ring[(((unsigned)cur + 1u) % kRingWindows) * 64u + lane] = pend;
             pend = base[(size_t)(cur + 2) * 64 + lane];
         }
     }
-    __device__ __forceinline__ const uint32_t* at(int p) const { return ring + ((p >> 6) % kRingWindows) * 64 + (p & 63); }
+    __device__ __forceinline__ const uint32_t* at(int p) const {
+        return ring + (((unsigned)p >> 6) % kRingWindows) * 64u + ((unsigned)p & 63u);
+    }
 };
 
-// The lane's arena word at a byte offset (the stream's offsets include the
+// The lane's arena value at a byte offset (the stream's offsets include the
 // x64 lane stride)
 __device__ __forceinline__ float arena_at(const unsigned char* la, uint32_t off) {
     return *reinterpret_cast<const float*>(la + off);
 }
 
-// v if the mask word is all ones, 0 if it is zero (exact, branch-free)
-__device__ __forceinline__ float keep(float v, uint32_t mask) { return __uint_as_float(__float_as_uint(v) & mask); }
+// The lane's phased-trait code of non-founder r (4 bits, 8 per word)
+__device__ __forceinline__ uint32_t lane_nibble(const uint32_t* lc, int r) {
+    return (lc[(size_t)(r >> 3) * kLanes] >> (4 * (r & 7))) & 15u;
+}
 
-// Descriptor words of one cell, read as broadcasts into vector registers (the
-// offset `dz` is a zero the compiler cannot prove uniform, so the words stay
-// in VGPRs and feed vector address arithmetic directly).
-template <int NW>
-struct CellWords {
-    uint32_t w[NW];
-    __device__ __forceinline__ void load(const uint32_t* rec, int dz) {
-#pragma unroll
-        for (int k = 0; k < NW; ++k) w[k] = rec[k + dz];
-    }
-};
-
-// One op's cells for the lane's evaluation, specialised on the op type, the
-// number of consumed tables and (PARENT) of children.  Every term of every
-// cell is formed (a term whose trait probability is zero adds an exact zero),
-// so the loop has no data-dependent branch; the next cell's descriptor words
-// are read before the current cell is evaluated.
+// One 32-word block of an op's cells (CPB cells), specialised on the op type,
+// the number of consumed tables and (PARENT) of children.  The descriptor
+// words are read as broadcasts into vector registers (`dz` is a zero the
+// compiler cannot prove uniform, so they stay in VGPRs and feed the address
+// arithmetic directly); then every consumed value and transmission factor of
+// the block is read before any product is formed.  Every term of every cell
+// is formed: a term whose trait probability is zero adds an exact zero.
 template <int TYPE, int NP, int NC>
-__device__ __forceinline__ void lane_cells(LaneStream& st, int cbase, int size, int stride, const float tr[4],
-                                           const float qr[4], const float A0[7], const float A1[7],
-                                           const float B0[7], const float B1[7], const unsigned char* la, float* o,
-                                           float* mx, int lane, int dz) {
+__device__ __forceinline__ void lane_block(const uint32_t* blk, int dz, const float tr[4], const float qr[4],
+                                           const unsigned char* const zb[4], const unsigned char* la,
+                                           unsigned char* ob, float* mx) {
     constexpr int G = 1 + NP;
-    CellWords<4 * G> cur, nxt;
-    st.seek(cbase, lane);
-    cur.load(st.at(cbase), dz);
-    for (int c = 0; c < size; ++c) {
-        if (c + 1 < size) {
-            const int pn = cbase + (c + 1) * stride;
-            st.seek(pn, lane);
-            nxt.load(st.at(pn), dz);
-        }
-        float total = 0.0f;
+    constexpr int STRIDE = NP <= 1 ? 8 : 16;
+    constexpr int CPB = 32 / STRIDE;
+    constexpr int NV = NP > 0 ? NP : 1;
+    constexpr int NZ = NC > 0 ? NC : 1;
+    uint32_t d[CPB][4 * G];
+#pragma unroll
+    for (int k = 0; k < CPB; ++k)
+#pragma unroll
+        for (int x = 0; x < 4 * G; ++x) d[k][x] = blk[k * STRIDE + x + dz];
+    float v[CPB][4][NV];
+    float z[CPB][4][NZ];
+#pragma unroll
+    for (int k = 0; k < CPB; ++k)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            float tmp = TYPE == kChild ? __uint_as_float(cur.w[t * G]) : tr[t];
 #pragma unroll
-            for (int q = 0; q < NP; ++q) tmp *= arena_at(la, cur.w[t * G + 1 + q]);
-            if (TYPE == kChild) {
-                tmp *= qr[t];
-            } else if (TYPE == kParent) {
-                // per child: (sum over allowed i of its maternal factor) x
-                // (sum over allowed j of its paternal factor), 0.25 in A
-                float cp = 1.0f;
+            for (int q = 0; q < NP; ++q) v[k][t][q] = arena_at(la, d[k][t * G + 1 + q]);
 #pragma unroll
-                for (int ch = 0; ch < NC; ++ch) {
-                    const uint32_t code = cur.w[t * G] >> (4 * ch);
-                    const float a = keep(A0[ch], 0u - (code & 1u)) + keep(A1[ch], 0u - ((code >> 1) & 1u));
-                    const float b = keep(B0[ch], 0u - ((code >> 2) & 1u)) + keep(B1[ch], 0u - ((code >> 3) & 1u));
-                    cp *= a * b;
-                }
-                tmp *= cp;
-            }
-            total += tmp;
+            for (int ch = 0; ch < NC; ++ch)
+                z[k][t][ch] = *reinterpret_cast<const float*>(zb[ch] + ((d[k][t * G] >> (4 * ch)) & 15u) * 4u);
         }
-        o[(size_t)c * kLanes] = total;
+#pragma unroll
+    for (int k = 0; k < CPB; ++k) {
+        float total;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            float tmp = TYPE == kChild ? __uint_as_float(d[k][t * G]) * qr[t] : tr[t];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) tmp *= v[k][t][q];
+#pragma unroll
+            for (int ch = 0; ch < NC; ++ch) tmp *= z[k][t][ch];
+            total = t == 0 ? tmp : total + tmp;
+        }
+        *reinterpret_cast<float*>(ob + k * kLanes * 4) = total;
         *mx = fmaxf(*mx, total);
-        if (c + 1 < size) cur = nxt;
     }
 }
 
-template <int NP, int NC>
-__device__ __forceinline__ void lane_parent(LaneStream& st, int cbase, int size, int stride, const float tr[4],
-                                            const float qr[4], const float A0[7], const float A1[7], const float B0[7],
-                                            const float B1[7], const unsigned char* la, float* o, float* mx, int lane,
-                                            int dz) {
-    lane_cells<kParent, NP, NC>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, la, o, mx, lane, dz);
+template <int TYPE, int NP, int NC>
+__device__ __forceinline__ void lane_blocks(LaneStream& st, int cbase, int nblk, int lane, int dz, const float tr[4],
+                                            const float qr[4], const unsigned char* const zb[4],
+                                            const unsigned char* la, unsigned char* ob, float* mx) {
+    constexpr int CPB = NP <= 1 ? 4 : 2;
+    for (int b = 0; b < nblk; ++b) {
+        const int p = cbase + 32 * b;
+        st.seek(p, lane);
+        lane_block<TYPE, NP, NC>(st.at(p), dz, tr, qr, zb, la, ob + (size_t)b * CPB * kLanes * 4, mx);
+    }
+}
+
+// Ops with 4 to 7 consumed tables or more than 3 children: one cell per block
+// (stride 32), counts taken at run time.
+__device__ __forceinline__ float lane_blocks_any(LaneStream& st, const uint32_t* hd, int cbase, int nblk, int type,
+                                              int np, int nc, int lane, float4 trv, float4 qrv, const uint32_t* lc,
+                                              const uint32_t* zt, const unsigned char* la, unsigned char* ob,
+                                              float mx) {
+    const float tr[4] = {trv.x, trv.y, trv.z, trv.w}, qr[4] = {qrv.x, qrv.y, qrv.z, qrv.w};
+    const int G = 1 + np;
+    const int stride = __builtin_amdgcn_readfirstlane((int)hd[6]);
+    const int cpb = 32 / stride;
+    int kid[7];
+#pragma unroll
+    for (int ch = 0; ch < 7; ++ch) kid[ch] = ch < nc ? (int)lane_nibble(lc, __builtin_amdgcn_readfirstlane((int)hd[8 + ch])) : 0;
+    for (int b = 0; b < nblk; ++b) {
+        const int p = cbase + 32 * b;
+        st.seek(p, lane);
+        const uint32_t* blk = st.at(p);
+        for (int k = 0; k < cpb; ++k) {
+            const uint32_t* rec = blk + k * stride;
+            float total = 0.0f;
+            for (int t = 0; t < 4; ++t) {
+                const uint32_t w = rec[t * G];
+                float tmp = type == kChild ? __uint_as_float(w) * qr[t] : tr[t];
+                for (int q = 0; q < np; ++q) tmp *= arena_at(la, rec[t * G + 1 + q]);
+#pragma unroll
+                for (int ch = 0; ch < 7; ++ch)
+                    if (ch < nc) tmp *= reinterpret_cast<const float*>(zt)[kid[ch] * 16 + ((w >> (4 * ch)) & 15u)];
+                total += tmp;
+            }
+            *reinterpret_cast<float*>(ob + ((size_t)b * cpb + k) * kLanes * 4) = total;
+            mx = fmaxf(mx, total);
+        }
+    }
+    return mx;
 }
 
 // One op of the sequential program; the header starts at stream word h.
 // Returns the word after the op.
 template <int NP>
-__device__ __forceinline__ int lane_op(const LodArgs& A, LaneStream& st, int h, float* la, const uint32_t* lc,
-                                       const Thetas<float>& th, int lane, int dz, int* E) {
+__device__ __forceinline__ int lane_op(LaneStream& st, int h, unsigned char* la, const uint32_t* lc,
+                                       const uint32_t* zt, int lane, int dz, int* E) {
     const uint32_t* hd = st.at(h);
     const int type = __builtin_amdgcn_readfirstlane((int)hd[0]);
+    const int np = __builtin_amdgcn_readfirstlane((int)hd[1]);
     const int size = __builtin_amdgcn_readfirstlane((int)hd[2]);
     const int out = __builtin_amdgcn_readfirstlane((int)hd[3]);
-    const int peel = __builtin_amdgcn_readfirstlane((int)hd[4]);
-    const int n_child = __builtin_amdgcn_readfirstlane((int)hd[5]);
-    const int stride = __builtin_amdgcn_readfirstlane((int)hd[6]);
-    float tr[4];
+    const int nc = __builtin_amdgcn_readfirstlane((int)hd[5]);
+    const int nblk = __builtin_amdgcn_readfirstlane((int)hd[7]);
+    float tr[4], qr[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int v = 0; v < 4; ++v) tr[v] = __uint_as_float(hd[24 + v]);
-    // transmission factors of this lane's graph
-    float qr[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float A0[7], A1[7], B0[7], B1[7];
+    for (int v = 0; v < 4; ++v) tr[v] = __uint_as_float(hd[16 + v]);
+    const unsigned char* zb[4] = {nullptr, nullptr, nullptr, nullptr};
     if (type == kChild) {
-        const uint32_t cd = lane_code(lc, A.F, peel);
+        // the kid's transmission factors for (i, j) = (t>>1, t&1)
+        const uint32_t cd = lane_nibble(lc, __builtin_amdgcn_readfirstlane((int)hd[4]));
+        const float4 q = *reinterpret_cast<const float4*>(zt + 256 + cd * 4);
+        qr[0] = q.x;
+        qr[1] = q.y;
+        qr[2] = q.z;
+        qr[3] = q.w;
+    } else if (type == kParent) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) qr[t] = quarter_rho<float>(cd, t >> 1, t & 1, th);
+        for (int ch = 0; ch < 4; ++ch)
+            if (ch < nc)
+                zb[ch] = reinterpret_cast<const unsigned char*>(
+                    zt + lane_nibble(lc, __builtin_amdgcn_readfirstlane((int)hd[8 + ch])) * 16);
     }
-#pragma unroll
-    for (int ch = 0; ch < 7; ++ch) {
-        A0[ch] = A1[ch] = B0[ch] = B1[ch] = 0.0f;
-        if (type == kParent && ch < n_child) {
-            // rho(i, j) = [maternal factors of i] x [paternal factors of j]:
-            // the sum over the product set of allowed (i, j) is a product of
-            // two sums
-            const uint32_t cd = lane_code(lc, A.F, __builtin_amdgcn_readfirstlane((int)hd[16 + ch]));
-            A0[ch] = 0.25f * (((cd & 1u) == 0u ? th.a1 : th.t1) * (((cd >> 1) & 1u) == 0u ? th.a2 : th.t2));
-            A1[ch] = 0.25f * (((cd & 1u) == 1u ? th.a1 : th.t1) * (((cd >> 1) & 1u) == 1u ? th.a2 : th.t2));
-            B0[ch] = (((cd >> 2) & 1u) == 0u ? th.a1 : th.t1) * (((cd >> 3) & 1u) == 0u ? th.a2 : th.t2);
-            B1[ch] = (((cd >> 2) & 1u) == 1u ? th.a1 : th.t1) * (((cd >> 3) & 1u) == 1u ? th.a2 : th.t2);
-        }
-    }
-    float* o = la + (size_t)out * kLanes;
-    const unsigned char* lb = reinterpret_cast<const unsigned char*>(la);
+    unsigned char* ob = la + (size_t)out * kLanes * 4;
     float mx = 0.0f;
     const int cbase = h + kLsHeader;
-    if (type == kChild) {
-        lane_cells<kChild, NP, 0>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz);
+    if constexpr (NP > 3) {
+        mx = lane_blocks_any(st, hd, cbase, nblk, type, np, nc, lane, make_float4(tr[0], tr[1], tr[2], tr[3]),
+                             make_float4(qr[0], qr[1], qr[2], qr[3]), lc, zt, la, ob, mx);
+    } else if (type == kParent && nc > 3) {
+        mx = lane_blocks_any(st, hd, cbase, nblk, type, np, nc, lane, make_float4(tr[0], tr[1], tr[2], tr[3]),
+                             make_float4(qr[0], qr[1], qr[2], qr[3]), lc, zt, la, ob, mx);
+    } else if (type == kChild) {
+        lane_blocks<kChild, NP, 0>(st, cbase, nblk, lane, dz, tr, qr, zb, la, ob, &mx);
     } else if (type == kParent) {
-        switch (n_child) {
-            case 0: lane_parent<NP, 0>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
-            case 1: lane_parent<NP, 1>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
-            case 2: lane_parent<NP, 2>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
-            case 3: lane_parent<NP, 3>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
-            case 4: lane_parent<NP, 4>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
-            case 5: lane_parent<NP, 5>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
-            case 6: lane_parent<NP, 6>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
-            default: lane_parent<NP, 7>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz); break;
+        switch (nc) {
+            case 0: lane_blocks<kParent, NP, 0>(st, cbase, nblk, lane, dz, tr, qr, zb, la, ob, &mx); break;
+            case 1: lane_blocks<kParent, NP, 1>(st, cbase, nblk, lane, dz, tr, qr, zb, la, ob, &mx); break;
+            case 2: lane_blocks<kParent, NP, 2>(st, cbase, nblk, lane, dz, tr, qr, zb, la, ob, &mx); break;
+            default: lane_blocks<kParent, NP, 3>(st, cbase, nblk, lane, dz, tr, qr, zb, la, ob, &mx); break;
         }
     } else {
-        lane_cells<kPartner, NP, 0>(st, cbase, size, stride, tr, qr, A0, A1, B0, B1, lb, o, &mx, lane, dz);
+        lane_blocks<kPartner, NP, 0>(st, cbase, nblk, lane, dz, tr, qr, zb, la, ob, &mx);
     }
-    // largest cell in [2^e, 2^(e+1)); renormalise outside the band
+    // largest cell in [2^e, 2^(e+1)); renormalise outside the band (the
+    // block-padding cells past `size` are left as they are: nothing reads them)
     int e = mx > 0.0f ? __builtin_amdgcn_frexp_expf(mx) - 1 : 0;
     if (e >= kRenormLo && e < kRenormHi) e = 0;
     if (__ballot(e != 0)) {
+        float* o = reinterpret_cast<float*>(ob);
         for (int c = 0; c < size; ++c) o[(size_t)c * kLanes] = __builtin_amdgcn_ldexpf(o[(size_t)c * kLanes], -e);
         *E += e;
     }
-    return cbase + ((size * stride + 31) & ~31);
+    return cbase + 32 * nblk;
 }
 
 __global__ void __launch_bounds__(kLanes) lane_lod_kernel(LodArgs A) {
@@ -626,20 +677,14 @@ __global__ void __launch_bounds__(kLanes) lane_lod_kernel(LodArgs A) {
     const int lane = (int)threadIdx.x;
     const int g = chunk * kLanes + lane;
     const bool live = g < A.G;
-    float* la = reinterpret_cast<float*>(smem) + lane;
+    const int nf = A.N - A.F;
+    const int ncw = (nf + 7) >> 3;
+    unsigned char* la = smem + (size_t)lane * 4;
     uint32_t* lc = reinterpret_cast<uint32_t*>(smem + A.lane_arena_bytes) + lane;
-    uint32_t* ring = reinterpret_cast<uint32_t*>(smem + A.lane_arena_bytes + (size_t)8 * A.W * kLanes);
+    uint32_t* zt = reinterpret_cast<uint32_t*>(smem + A.lane_arena_bytes + (size_t)ncw * kLanes * 4);
+    uint32_t* ring = zt + kZWords;
     LaneStream st;
     st.init(A.lstream, ring, lane);
-    // this lane's columns l and l+1, and its recombination count
-    const uint32_t* c0 = A.graphs + ((size_t)(live ? g : 0) * A.M + l) * A.W;
-    int ncross = 0;
-    for (int w = 0; w < A.W; ++w) {
-        const uint32_t x0 = live ? c0[w] : 0u, x1 = live ? c0[A.W + w] : 0u;
-        lc[(size_t)(2 * w) * kLanes] = x0;
-        lc[(size_t)(2 * w + 1) * kLanes] = x1;
-        ncross += __popc(x0 ^ x1);
-    }
     // set_thetas (trait_rfunction.h:50-56): linear partial thetas
     const double ptheta = A.partial_theta[l];
     const double tl = ptheta * (double)(pos + 1), tq = ptheta * (double)(A.npos - pos);
@@ -648,6 +693,25 @@ __global__ void __launch_bounds__(kLanes) lane_lod_kernel(LodArgs A) {
     th.a1 = (float)(1.0 - tl);
     th.t2 = (float)tq;
     th.a2 = (float)(1.0 - tq);
+    for (int e = lane; e < kZWords; e += kLanes) zt[e] = __float_as_uint(zentry(e, th));
+    // this lane's phased-trait codes at (l, l+1) and its recombination count
+    const uint32_t* c0 = A.graphs + ((size_t)(live ? g : 0) * A.M + l) * A.W;
+    int ncross = 0;
+    for (int k = 0; k < ncw; ++k) {
+        // non-founders 8k .. 8k+7 sit in bits 16(k&1) .. of column word k>>1
+        const int w = k >> 1, sh = 16 * (k & 1);
+        const uint32_t x0 = live ? c0[w] : 0u, x1 = live ? c0[A.W + w] : 0u;
+        if ((k & 1) == 0) ncross += __popc(x0 ^ x1);
+        uint32_t word = 0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint32_t b0 = x0 >> (sh + 2 * r), b1 = x1 >> (sh + 2 * r);
+            const uint32_t code = (b0 & 1u) | ((b1 & 1u) << 1) | ((b0 & 2u) << 1) | ((b1 & 2u) << 2);
+            word |= code << (4 * r);
+        }
+        lc[(size_t)k * kLanes] = word;
+    }
+    __syncthreads();
     const double lt = A.log_theta[l], lant = A.log_anti[l];
     // a zero that the compiler cannot prove uniform (a kernel argument masks
     // the lane index)
@@ -658,19 +722,15 @@ __global__ void __launch_bounds__(kLanes) lane_lod_kernel(LodArgs A) {
         const uint32_t* hd = st.at(h);
         last_out = __builtin_amdgcn_readfirstlane((int)hd[3]);
         switch (__builtin_amdgcn_readfirstlane((int)hd[1])) {
-            case 0: h = lane_op<0>(A, st, h, la, lc, th, lane, dz, &E); break;
-            case 1: h = lane_op<1>(A, st, h, la, lc, th, lane, dz, &E); break;
-            case 2: h = lane_op<2>(A, st, h, la, lc, th, lane, dz, &E); break;
-            case 3: h = lane_op<3>(A, st, h, la, lc, th, lane, dz, &E); break;
-            case 4: h = lane_op<4>(A, st, h, la, lc, th, lane, dz, &E); break;
-            case 5: h = lane_op<5>(A, st, h, la, lc, th, lane, dz, &E); break;
-            case 6: h = lane_op<6>(A, st, h, la, lc, th, lane, dz, &E); break;
-            case 7: h = lane_op<7>(A, st, h, la, lc, th, lane, dz, &E); break;
-            default: h = lane_op<8>(A, st, h, la, lc, th, lane, dz, &E); break;
+            case 0: h = lane_op<0>(st, h, la, lc, zt, lane, dz, &E); break;
+            case 1: h = lane_op<1>(st, h, la, lc, zt, lane, dz, &E); break;
+            case 2: h = lane_op<2>(st, h, la, lc, zt, lane, dz, &E); break;
+            case 3: h = lane_op<3>(st, h, la, lc, zt, lane, dz, &E); break;
+            default: h = lane_op<4>(st, h, la, lc, zt, lane, dz, &E); break;
         }
     }
     // ln L - R - T (peeler.cc:97-99) and the chunk's log-sum over its graphs
-    const double L = (double)la[(size_t)last_out * kLanes];
+    const double L = (double)*reinterpret_cast<const float*>(la + (size_t)last_out * kLanes * 4);
     double term = L > 0.0 ? log(L) + (double)E * 0.69314718055994530942 : -INFINITY;
     const int nm = 2 * (A.N - A.F);
     term -= (double)ncross * lt + (double)(nm - ncross) * lant;

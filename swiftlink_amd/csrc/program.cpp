@@ -356,13 +356,56 @@ std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int
             st2.pop_back();
         }
         if ((int)post.size() != n) return "sequential order does not reach every op";
-        Arena sa;
-        std::vector<int> soff(n, -1);
-        P.seq.clear();
-        for (int o : post) {
-            soff[o] = sa.alloc(size[o]);
-            for (int q = pv.prev_off[o]; q < pv.prev_off[o + 1]; ++q) sa.release(soff[pv.prev[q]], size[pv.prev[q]]);
+        // the lane kernel evaluates an op's cells in blocks of 32 stream
+        // words (lane_cells_per_block cells), so each table is allocated
+        // whole blocks of cells
+        std::vector<int> psize(n);
+        for (int o = 0; o < n; ++o) {
+            const int cpb = lane_cells_per_block(P.ops[o].n_prev);
+            psize[o] = (size[o] + cpb - 1) / cpb * cpb;
         }
+        // arena offsets, placed offline: a table lives from its op's position
+        // in the order to its consumer's (inclusive, so an op's table never
+        // overlaps the tables it consumes); tables are placed largest first,
+        // each at the lowest offset clear of every placed table whose life
+        // overlaps its own.  First fit in evaluation order is the fallback
+        // when it needs fewer cells.
+        std::vector<int> born(n), dies(n, n);
+        for (int i = 0; i < n; ++i) born[post[i]] = i;
+        for (int o = 0; o < n; ++o)
+            for (int q = pv.prev_off[o]; q < pv.prev_off[o + 1]; ++q) dies[pv.prev[q]] = born[o];
+        std::vector<int> by(n);
+        for (int o = 0; o < n; ++o) by[o] = o;
+        std::stable_sort(by.begin(), by.end(), [&](int a, int b) { return psize[a] > psize[b]; });
+        // first fit in evaluation order, the other candidate
+        std::vector<int> ff(n, -1);
+        Arena sa;
+        for (int o : post) {
+            ff[o] = sa.alloc(psize[o]);
+            for (int q = pv.prev_off[o]; q < pv.prev_off[o + 1]; ++q) sa.release(ff[pv.prev[q]], psize[pv.prev[q]]);
+        }
+        std::vector<int> soff(n, -1);
+        std::vector<int> placed;
+        int peak_cells = 0;
+        for (int o : by) {
+            std::vector<std::pair<int, int>> busy;
+            for (int u : placed)
+                if (born[u] <= dies[o] && born[o] <= dies[u]) busy.push_back({soff[u], soff[u] + psize[u]});
+            std::sort(busy.begin(), busy.end());
+            int off = 0;
+            for (auto& r : busy) {
+                if (r.first >= off + psize[o]) break;
+                off = std::max(off, r.second);
+            }
+            soff[o] = off;
+            placed.push_back(o);
+            peak_cells = std::max(peak_cells, off + psize[o]);
+        }
+        if (sa.peak < peak_cells) {
+            soff = ff;
+            peak_cells = sa.peak;
+        }
+        P.seq.clear();
         for (int o : post) {
             const OpDesc& d = P.ops[o];
             SeqOp s{};
@@ -378,46 +421,49 @@ std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int
             for (int ch = 0; ch < d.n_child; ++ch) s.child[ch] = P.children[d.child_off + ch].person;
             P.seq.push_back(s);
         }
-        P.seq_elems = sa.peak;
-        // the descriptor stream of the lane kernel (FP32): per op a header,
-        // then per cell four term groups [w, byte offsets of the NP consumed
-        // cells in the lane arena], w being the kid value's disease
-        // probability as float bits (CHILD), or for PARENT per child the
-        // allowed transmitted alleles, maternal bits 0-1 and paternal bits 2-3
-        // of a 4-bit field (the kid trait fixes each parent's allele
-        // independently), or unused.  Each cell takes a power-of-two number of
-        // words so that none straddles a 64-word window.
+        P.seq_elems = peak_cells;
+        // The descriptor stream of the lane kernel (FP32).  Per op in
+        // sequential order a 32-word header, then the op's cells in blocks of
+        // 32 words: a cell is four term groups [w, byte offsets of the n_prev
+        // consumed cells in the 64-lane arena] in a stride of 8 or 16 words
+        // (lane_cell_stride), so a block holds 4 or 2 cells (an op with more
+        // than 3 consumed tables: one cell of 32 words), and cells past the
+        // op's size in its last block are zero.  w is the kid value's disease
+        // probability as float bits (CHILD); for PARENT, per child a 4-bit
+        // field of the allowed transmitted alleles, maternal bits 0-1 and
+        // paternal bits 2-3 (the kid trait fixes each parent's allele
+        // independently); otherwise 0.
         std::vector<uint32_t>& S = P.lstream;
         S.clear();
-        auto pad_to = [&](size_t m) {
-            while (S.size() % m) S.push_back(0u);
-        };
-        for (size_t i = 0; i < post.size(); ++i) {
+        P.lane_stream_ok = true;
+        for (size_t i = 0; i < post.size() && P.lane_stream_ok; ++i) {
             const int o = post[i];
             const OpDesc& d = P.ops[o];
             const SeqOp& so = P.seq[i];
             const int group = 1 + d.n_prev;
-            int stride = 8;
-            while (stride < 4 * group) stride *= 2;
-            if (stride > 64) return "a lane-kernel cell record exceeds 64 words";
+            if (d.n_prev > kLaneMaxPrev) {
+                P.lane_stream_ok = false;  // the lane kernel takes at most 7 consumed tables
+                break;
+            }
+            const int stride = lane_cell_stride(d.n_prev), cpb = lane_cells_per_block(d.n_prev);
+            const int nblk = (d.size + cpb - 1) / cpb;
             const size_t h = S.size();
-            S.resize(h + kLsHeader, 0u);
+            S.resize(h + kLsHeader + (size_t)nblk * 32, 0u);
             S[h + 0] = (uint32_t)d.type;
             S[h + 1] = (uint32_t)d.n_prev;
             S[h + 2] = (uint32_t)d.size;
             S[h + 3] = (uint32_t)so.out_off;
-            S[h + 4] = (uint32_t)d.peel;
+            S[h + 4] = (uint32_t)(d.peel - ped.F);
             S[h + 5] = (uint32_t)d.n_child;
             S[h + 6] = (uint32_t)stride;
-            for (int q = 0; q < d.n_prev; ++q) S[h + 8 + q] = (uint32_t)so.prev_off[q];
-            for (int ch = 0; ch < d.n_child; ++ch) S[h + 16 + ch] = (uint32_t)so.child[ch];
+            S[h + 7] = (uint32_t)nblk;
+            for (int ch = 0; ch < d.n_child; ++ch) S[h + 8 + ch] = (uint32_t)(so.child[ch] - ped.F);
             for (int v = 0; v < 4; ++v) {
                 const float f = (float)ped.disease_prob[4 * d.peel + v];
-                memcpy(&S[h + 24 + v], &f, 4);
+                memcpy(&S[h + 16 + v], &f, 4);
             }
             for (int c = 0; c < d.size; ++c) {
-                const size_t b = S.size();
-                S.resize(b + stride, 0u);
+                const size_t b = h + kLsHeader + (size_t)(c / cpb) * 32 + (size_t)(c % cpb) * stride;
                 for (int t = 0; t < 4; ++t) {
                     const uint32_t info = P.tinfo[(size_t)(d.task_base + c) * 4 + t];
                     const bool valid = (info & 4u) != 0u;
@@ -448,8 +494,8 @@ std::string compile_lod_program(const PedigreeView& ped, const PeelView& pv, int
                     }
                 }
             }
-            pad_to(32);
         }
+        if (!P.lane_stream_ok) S.clear();
         S.resize(S.size() + 3 * 64, 0u);
     }
     P.flops_per_eval = flops;

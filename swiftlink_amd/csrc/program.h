@@ -30,6 +30,12 @@ constexpr int kMaxCut = 7;       // cutset size limit: k+1 digits fit 16 bits
 constexpr int kMaxPrev = 8;
 constexpr int kMaxPos = 16;      // trait positions per interval
 constexpr int kLsHeader = 32;    // words of an op header in the lane kernel's stream
+constexpr int kLaneMaxPrev = 7;  // consumed tables of an op the lane kernel takes
+
+// Lane kernel cell records: 4 x (1 + n_prev) words in a stride of 8 or 16
+// words (32 above 3 consumed tables), packed into 32-word blocks.
+inline int lane_cell_stride(int n_prev) { return n_prev <= 1 ? 8 : n_prev <= 3 ? 16 : 32; }
+inline int lane_cells_per_block(int n_prev) { return 32 / lane_cell_stride(n_prev); }
 
 enum OpType : int32_t { kChild = 1, kParent = 2, kPartner = 3, kLast = 4 };
 
@@ -113,15 +119,14 @@ struct PeelProgram {
     std::vector<SeqOp> seq;
     int seq_elems = 0;
     // The lane kernel's descriptor stream (FP32): per op in sequential order
-    // a 32-word header (kLsHeader: type, n_prev, size, out_off, peel,
-    // n_child, cell stride, 0, consumed-table offsets [8], children [7], 0,
-    // the peel node's four disease probabilities as float bits, 0 x 4), then
-    // per cell four term groups of 1 + n_prev words (a term word, then the
-    // byte offsets of the consumed cells in the 64-lane arena) in a stride of
-    // 8, 16, 32 or 64 words, the op padded to 32 words; three zero windows of
-    // 64 words close it (the stream is read 64 words at a time, one window
-    // ahead).
+    // a 32-word header (kLsHeader: type, n_prev, size, out_off, peel - F,
+    // n_child, cell stride, blocks, children - F [7], 0, the peel node's four
+    // disease probabilities as float bits, 0 x 12), then the cells in 32-word
+    // blocks (lane_cell_stride); three zero windows of 64 words close it (the
+    // stream is read 64 words at a time, one window ahead).  Empty, and
+    // lane_stream_ok false, when an op consumes more than kLaneMaxPrev tables.
     std::vector<uint32_t> lstream;
+    bool lane_stream_ok = false;
     double flops_per_eval = 0.0;        // algorithmic flops per (graph, interval, position)
     double lds_bytes_per_eval = 0.0;    // table reads per evaluation, in elements
 };

@@ -57,7 +57,10 @@ class Program:
         self.seq_elems = w[p]
         p += 1
         self.seq = [w[p + SQD * i: p + SQD * (i + 1)] for i in range(self.n_ops)]
-        assert p + SQD * self.n_ops == len(w)
+        p += SQD * self.n_ops
+        nls = w[p]
+        self.lstream = np.array(w[p + 1: p + 1 + nls], dtype=np.int64).astype(np.uint32)
+        assert p + 1 + nls == len(w)
 
     def check_terms(self, case):
         """The precompiled term words and table indices agree with what the
@@ -319,4 +322,79 @@ class Program:
                         E += e
             L = float(arena[self.seq[-1][3]])
             out.append(math.log(L) + E * math.log(2.0))
+        return np.array(out)
+
+    # ------------------------------------------------------------ lane stream
+    def run_lane_stream(self, case, dg, locus):
+        """The FP32 lane kernel's execution of its descriptor stream for one
+        graph at one interval: per position, the wavefront's transmission
+        table Z[code][mask] / Q[code][t], the lane's 4-bit codes, the ops'
+        headers and 32-word cell blocks (padding cells included), and the
+        power-of-two renormalisation.  Returns ln L per position."""
+        f32 = np.float32
+        S = self.lstream
+        assert S.size > 0
+        N, F = case.N, case.F
+        col0, col1 = dg[locus], dg[locus + 1]
+        code = [int(col0[2 * p]) | (int(col1[2 * p]) << 1) | (int(col0[2 * p + 1]) << 2) | (int(col1[2 * p + 1]) << 3)
+                for p in range(F, N)]
+        pt = case.partial_theta[locus]
+        npos = case.nlod
+        out = []
+        for pos in range(npos):
+            tl, tq = pt * (pos + 1), pt * (npos - pos)
+            t1, a1, t2, a2 = f32(tl), f32(1.0 - tl), f32(tq), f32(1.0 - tq)
+
+            def zentry(cd, mask):
+                a, b = f32(0), f32(0)
+                for v in range(2):
+                    f = f32(f32(0.25) * f32((a1 if (cd & 1) == v else t1) * (a2 if ((cd >> 1) & 1) == v else t2)))
+                    g = f32((a1 if ((cd >> 2) & 1) == v else t1) * (a2 if ((cd >> 3) & 1) == v else t2))
+                    if (mask >> v) & 1:
+                        a = f32(a + f)
+                    if (mask >> (2 + v)) & 1:
+                        b = f32(b + g)
+                return f32(a * b)
+
+            Z = [[zentry(cd, m) for m in range(16)] for cd in range(16)]
+            Q = [[Z[cd][(1 << (t >> 1)) | (4 << (t & 1))] for t in range(4)] for cd in range(16)]
+            arena = np.zeros(self.seq_elems, f32)
+            E, h, last = 0, 0, 0
+            for _ in range(self.n_ops):
+                hd = S[h:h + 32]
+                typ, npv, size, outo, peel_r, nc, stride, nblk = (int(x) for x in hd[:8])
+                tr = hd[16:20].view(np.float32)
+                last = outo
+                qr = Q[code[peel_r]] if typ == 1 else None
+                kid = [code[int(hd[8 + ch])] for ch in range(nc)] if typ == 2 else []
+                G, cpb = 1 + npv, 32 // stride
+                assert stride == (8 if npv <= 1 else 16 if npv <= 3 else 32)
+                mx = f32(0)
+                for b in range(nblk):
+                    for k in range(cpb):
+                        c = b * cpb + k
+                        rec = S[h + 32 + 32 * b + k * stride: h + 32 + 32 * b + (k + 1) * stride]
+                        total = f32(0)
+                        for t in range(4):
+                            w = int(rec[t * G])
+                            tmp = f32(rec[t * G: t * G + 1].view(np.float32)[0] * qr[t]) if typ == 1 else tr[t]
+                            for q in range(npv):
+                                off = int(rec[t * G + 1 + q])
+                                assert off % 256 == 0 and off // 256 < self.seq_elems
+                                tmp = f32(tmp * arena[off // 256])
+                            for ch in range(nc if typ == 2 else 0):
+                                tmp = f32(tmp * Z[kid[ch]][(w >> (4 * ch)) & 15])
+                            total = f32(total + tmp)
+                        assert outo + c < self.seq_elems
+                        arena[outo + c] = total
+                        if not np.isnan(total):
+                            mx = max(mx, total)
+                e = math.frexp(float(mx))[1] - 1 if mx > 0 else 0
+                if -6 <= e < 6:
+                    e = 0
+                if e:
+                    arena[outo:outo + size] = np.ldexp(arena[outo:outo + size], -e).astype(f32)
+                    E += e
+                h += 32 + 32 * nblk
+            out.append(math.log(float(arena[last])) + E * math.log(2.0))
         return np.array(out)

@@ -123,9 +123,11 @@ def test_program_ped150_terms_and_bounds(tmp_path):
 @pytest.mark.parametrize("name", GOLDEN_CASES)
 def test_sequential_program_fp64_bit_exact_and_fp32_within_bar(name):
     """The lane kernel's sequential form: every table alive at once has its
-    own arena cells, and executed with the kernel's arithmetic (one evaluation
-    per lane) it gives the oracle's likelihood bit for bit in FP64 and within
-    1e-5 LOD in FP32 with power-of-two renormalisation."""
+    own arena cells, and executed with the workgroup kernel's arithmetic in
+    that order it gives the oracle's likelihood bit for bit in FP64 and within
+    1e-5 LOD in FP32 with power-of-two renormalisation; the lane kernel's
+    descriptor stream, executed as the kernel reads it (blocks, padding cells,
+    transmission table), is within 1e-5 LOD."""
     case, fx = load_golden(name)
     P = Program(native.program_export(case))
     P.check_seq()
@@ -135,13 +137,15 @@ def test_sequential_program_fp64_bit_exact_and_fp32_within_bar(name):
             ref = [np.log(o.trait_likelihood(g, l, off)) for off in range(1, case.nlod + 1)]
             assert np.array_equal(P.run_seq(case, g, l), ref)
             assert np.abs(P.run_seq(case, g, l, fp32=True) - ref).max() / np.log(10) < 1e-5
+            assert np.abs(P.run_lane_stream(case, g, l) - ref).max() / np.log(10) < 1e-5
 
 
 def test_sequential_program_ped150_and_underflowing_scale(tmp_path):
-    """150 people: the sequential form's arena layout and FP64 values; with
-    every disease probability scaled by 2^-40 (a likelihood of about 2^-6000
-    times the original, far below FP32 and FP64 range) the FP32 renormalised
-    execution still gives ln L - 40 N ln 2 within the LOD bar."""
+    """150 people: the sequential form's arena layout and FP64 values, and the
+    lane stream within the LOD bar; with every disease probability scaled by
+    2^-40 (a likelihood of about 2^-6000 times the original, far below FP32
+    and FP64 range) the FP32 renormalised execution of both still gives
+    ln L - 40 N ln 2 within the LOD bar."""
     import dataclasses
     from swiftlink_amd import synth
     case, data = synth.make_case("ped150_snp1000", str(tmp_path))
@@ -151,6 +155,9 @@ def test_sequential_program_ped150_and_underflowing_scale(tmp_path):
     o = Oracle(case)
     ref = [np.log(o.trait_likelihood(g, 300, off)) for off in range(1, case.nlod + 1)]
     assert np.array_equal(P.run_seq(case, g, 300), ref)
+    assert np.abs(P.run_lane_stream(case, g, 300) - ref).max() / np.log(10) < 1e-4
     tiny = dataclasses.replace(case, disease_prob=np.ldexp(case.disease_prob, -40))
     got = P.run_seq(tiny, g, 300, fp32=True)
+    assert np.abs(got - (np.array(ref) - 40 * case.N * np.log(2.0))).max() / np.log(10) < 1e-4
+    got = Program(native.program_export(tiny)).run_lane_stream(tiny, g, 300)
     assert np.abs(got - (np.array(ref) - 40 * case.N * np.log(2.0))).max() / np.log(10) < 1e-4
