@@ -155,6 +155,7 @@ struct sl_ctx {
     int m_wave_bytes = 0;
     size_t m_lds = 0;       // dynamic LDS of a likelihood-kernel workgroup
     size_t m_spec_lds = 0;  // and of a speculative-kernel workgroup (0: too large, no speculation)
+    bool m_ubuf = false;    // forward-backward passes keep the step's uniforms in LDS
     bool m_ready = false;
     std::string m_why;
     std::vector<int32_t> meioses_h;
@@ -485,6 +486,10 @@ int setup_msampler(sl_ctx* c, const sl_pedigree_desc* ped) {
         return SL_OK;
     }
     if (c->m_spec_lds > 160 * 1024) c->m_spec_lds = 0;  // steps then take two launches
+    // forward-backward passes keep the step's uniforms in LDS (in the
+    // speculative kernel, in the dynamic LDS its job workgroups use)
+    c->m_ubuf = M <= sl::kFbUbufLoci;
+    if (c->m_ubuf && c->m_spec_lds) c->m_spec_lds = std::max(c->m_spec_lds, sizeof(double) * (size_t)M);
     SL_HIP(c, sl::configure_msampler_lds(c->m_lds, c->m_spec_lds));
     int rc;
     if ((rc = upload(c, &c->d_gen_off, gen_off.data(), gen_off.size())) ||
@@ -1352,6 +1357,7 @@ static sl::MsampArgs msamp_args(sl_ctx* c) {
     A.ptab_bytes = c->m_ptab_bytes;
     A.n_chains = c->n_chains;
     A.fp32 = c->precision == SL_FP32_RESCALED ? 1 : 0;
+    A.fb_ubuf = c->m_ubuf ? 1 : 0;
     return A;
 }
 

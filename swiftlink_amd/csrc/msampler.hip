@@ -252,18 +252,35 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
 // a single LDS read, and the first round that hooks nothing has read exactly
 // what the consistency check needs.  The per-root terms and their order of
 // summation are those of fag_log_likelihood, so the result is the same.
+// A job's global inputs for the register-resident likelihood, loaded at once
+// at its start: lane j's genotypes and person-table slots j, j+64, ..., and
+// the locus's minor allele frequency.
 template <int KC>
-__device__ double fag_log_likelihood_r(const MsampArgs& A, const uint2* ptab, const WaveArea& w, int l) {
+struct JobInputs {
+    int g[KC];
+    uint2 pt[KC];
+    double minor;
+    __device__ __forceinline__ void load(const MsampArgs& A, int l) {
+        const int lane = lane_id();
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            const int i = 64 * c + lane;
+            g[c] = i < A.N ? genotype_at(A, l, i) : 0;
+            pt[c] = i < A.N - A.F ? A.ptab[i] : make_uint2(0u, 0u);
+        }
+        minor = A.minor[l];
+    }
+};
+
+template <int KC>
+__device__ double fag_log_likelihood_r(const MsampArgs& A, const JobInputs<KC>& in, const WaveArea& w) {
     constexpr int KN = KC;  // node chunks: the chunk class covers 2F too
     const int lane = lane_id();
-    const int N = A.N, F = A.F, NF = A.N - A.F, nodes = 2 * A.F;
+    const int F = A.F, NF = A.N - A.F, nodes = 2 * A.F;
     int g[KC];
 #pragma unroll
-    for (int c = 0; c < KC; ++c) {
-        const int i = 64 * c + lane;
-        g[c] = i < N ? genotype_at(A, l, i) : 0;
-    }
-    const double minor = A.minor[l];
+    for (int c = 0; c < KC; ++c) g[c] = in.g[c];
+    const double minor = in.minor;
 #pragma unroll
     for (int n = 0; n < KN; ++n) {
         const int x = 64 * n + lane;
@@ -284,7 +301,7 @@ __device__ double fag_log_likelihood_r(const MsampArgs& A, const uint2* ptab, co
         sb[c] = 0;
         src0[c] = src1[c] = 0u;
         if (k < NF) {
-            const uint2 e = ptab[k];
+            const uint2 e = in.pt[c];
             const int i = (int)(e.x & 0xFFFFu), mo = (int)(e.y & 0xFFFFu), fa = (int)(e.y >> 16);
             const int b = 2 * (i - F);
             const uint32_t bm = (w.colw[b >> 5] >> (b & 31)) & 1u;
@@ -564,6 +581,9 @@ __device__ __forceinline__ int order_at(const MsampArgs& A, int chain, int k) {
 template <int KC>
 __device__ void fag_job(const MsampArgs& A, int chain, int q, const uint2* ptab, unsigned char* wbase) {
     const int lane = lane_id(), M = A.M;
+    // the chain's status, read with the job's other loads: a chain whose
+    // graph is illegal keeps its buffers as they are
+    const int status = A.status[chain];
     int l, v = 0, flip = -1, force = -1;
     bool current = false;
     if (A.with_lcur && q < M) {
@@ -581,6 +601,8 @@ __device__ void fag_job(const MsampArgs& A, int chain, int q, const uint2* ptab,
     }
     WaveArea w = wave_area(A, wbase);
     const uint32_t* src = A.graphs + ((size_t)chain * M + l) * A.W;
+    JobInputs<KC> in;
+    if constexpr (KC > 0) in.load(A, l);
     for (int k = lane; k < A.W; k += 64) {
         uint32_t x = src[k];
         if (force >= 0 && (force >> 5) == k) x = (x & ~(1u << (force & 31))) | ((uint32_t)v << (force & 31));
@@ -589,8 +611,9 @@ __device__ void fag_job(const MsampArgs& A, int chain, int q, const uint2* ptab,
     }
     wave_sync();
     double L;
-    if constexpr (KC > 0) L = fag_log_likelihood_r<KC>(A, ptab, w, l);
+    if constexpr (KC > 0) L = fag_log_likelihood_r<KC>(A, in, w);
     else L = fag_log_likelihood(A, ptab, w, l);
+    if (status != 0) return;
     if (lane == 0) {
         if (current) {
             A.lcur[(size_t)chain * M + l] = L;
@@ -619,10 +642,15 @@ struct StepView {
 // locus matrix from the two states' likelihoods scaled by the larger one:
 // T(theta_{l-1}) diag(e_l), A_0 = diag(e_0)
 template <typename T>
-__device__ __forceinline__ MatT<T> emission_matrix(const MsampArgs& A, T e0, T e1, int l) {
-    if (l == 0) return MatT<T>{e0, (T)0, (T)0, e1};
-    const T th = (T)A.theta[l - 1], an = (T)(1.0 - A.theta[l - 1]);
+__device__ __forceinline__ MatT<T> emission_matrix_th(T e0, T e1, bool first, double theta) {
+    if (first) return MatT<T>{e0, (T)0, (T)0, e1};
+    const T th = (T)theta, an = (T)(1.0 - theta);
     return MatT<T>{an * e0, th * e1, th * e0, an * e1};
+}
+
+template <typename T>
+__device__ __forceinline__ MatT<T> emission_matrix(const MsampArgs& A, T e0, T e1, int l) {
+    return emission_matrix_th(e0, e1, l == 0, l == 0 ? 0.0 : A.theta[l - 1]);
 }
 
 __device__ __forceinline__ double exp_t(double x) { return exp(x); }
@@ -664,16 +692,17 @@ __device__ __forceinline__ uint32_t locus_draws(const MsampArgs& A, T f0, T f1, 
 template <typename T>
 __device__ __forceinline__ double edge_tol() { return sizeof(T) == 4 ? 1e-5 : 1e-12; }
 
-// 4-bit decision map of locus l: bit bn = new indicator when the next locus's
-// is bn; bit 2+bn = that decision's uniform lay within edge_tol of its
-// threshold.  A drawing locus (draws, from locus_draws) uses draw index ctr of
-// step k.
+// 4-bit decision map of a locus (`last`: locus M-1; theta: of the interval to
+// the next locus): bit bn = new indicator when the next locus's is bn; bit
+// 2+bn = that decision's uniform u lay within edge_tol of its threshold.  A
+// drawing locus (draws, from locus_draws) takes u; locus_map draws it as draw
+// index ctr of step k.
 template <typename T>
-__device__ __forceinline__ uint32_t locus_map(const MsampArgs& A, int chain, int k, T f0, T f1, int l,
-                                              uint32_t ctr, bool draws, uint32_t* near) {
+__device__ __forceinline__ uint32_t locus_map_u(bool last, double theta, T f0, T f1, double u, bool draws,
+                                                uint32_t* near) {
     T w0[2], w1[2];
-    if (l < A.M - 1) {
-        const T th = (T)A.theta[l], an = (T)(1.0 - A.theta[l]);
+    if (!last) {
+        const T th = (T)theta, an = (T)(1.0 - theta);
         w0[0] = f0 * an;
         w1[0] = f1 * th;
         w0[1] = f0 * th;
@@ -690,7 +719,6 @@ __device__ __forceinline__ uint32_t locus_map(const MsampArgs& A, int chain, int
         // an FP32 underflow of a state the FP64 pass keeps is counted
         if (draws && !(w0[0] != 0.0f && w1[0] != 0.0f && w0[1] != 0.0f && w1[1] != 0.0f)) ++*near;
     }
-    const double u = draws ? philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), (uint32_t)k, ctr) : 0.0;
     uint32_t map = 0;
 #pragma unroll
     for (int bn = 0; bn < 2; ++bn) {
@@ -707,6 +735,13 @@ __device__ __forceinline__ uint32_t locus_map(const MsampArgs& A, int chain, int
         map |= (b << bn) | (edge << (2 + bn));
     }
     return map;
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t locus_map(const MsampArgs& A, int chain, int k, T f0, T f1, int l,
+                                              uint32_t ctr, bool draws, uint32_t* near) {
+    const double u = draws ? philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), (uint32_t)k, ctr) : 0.0;
+    return locus_map_u(l == A.M - 1, l < A.M - 1 ? A.theta[l] : 0.0, f0, f1, u, draws, near);
 }
 
 // exclusive left-to-right product of the threads' segment products: within
@@ -735,17 +770,56 @@ __device__ __forceinline__ MatT<T> scan_products(MatT<T> P, MatT<T>* wtot) {
     return mmul(R, mat_dpp<kDppWaveShr1>(P));
 }
 
-// sum of v over the threads after this one
-__device__ __forceinline__ uint32_t sum_above(uint32_t v, uint32_t* wcnt) {
-    const int lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    uint32_t sc = v;  // inclusive suffix sums within rows, then across them
+// within a wavefront: inclusive suffix sums of v (this lane and the lanes
+// after it)
+__device__ __forceinline__ uint32_t wave_suffix_sums(uint32_t v) {
+    const int lane = lane_id();
+    uint32_t sc = v;  // within rows, then across them
     sc += u32_dpp<dpp_row_shl(1)>(0u, sc);
     sc += u32_dpp<dpp_row_shl(2)>(0u, sc);
     sc += u32_dpp<dpp_row_shl(4)>(0u, sc);
     sc += u32_dpp<dpp_row_shl(8)>(0u, sc);
     const uint32_t t1 = readlane_u32(sc, 16), t2 = readlane_u32(sc, 32), t3 = readlane_u32(sc, 48);
     const int row = lane >> 4;
-    sc += (row < 1 ? t1 : 0u) + (row < 2 ? t2 : 0u) + (row < 3 ? t3 : 0u);
+    return sc + (row < 1 ? t1 : 0u) + (row < 2 ? t2 : 0u) + (row < 3 ? t3 : 0u);
+}
+
+// scan_products and sum_above in one exchange between wavefronts: the
+// exclusive product before this thread's segment, and in *above the sum of v
+// over the threads after this one
+template <typename T>
+__device__ __forceinline__ MatT<T> scan_products_counts(MatT<T> P, uint32_t v, MatT<T>* wtot, uint32_t* wcnt,
+                                                        uint32_t* above) {
+    const int lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    P = wave_products(P);
+    const uint32_t sc = wave_suffix_sums(v);
+    if (lane == 63) wtot[wave] = P;
+    if (lane == 0) wcnt[wave] = sc;
+    __syncthreads();
+    uint32_t r = sc - v;
+    for (int q = wave + 1; q < nw; ++q) r += wcnt[q];
+    *above = r;
+    MatT<T> R{(T)1, (T)0, (T)0, (T)1};
+    if (nw <= 4) {
+        for (int q = 0; q < wave; ++q) R = mmul(R, wtot[q]);
+    } else {
+        MatT<T> Q = lane < nw ? wtot[lane] : MatT<T>{(T)1, (T)0, (T)0, (T)1};
+        Q = mmul(mat_dpp<dpp_row_shr(1)>(Q), Q);
+        Q = mmul(mat_dpp<dpp_row_shr(2)>(Q), Q);
+        Q = mmul(mat_dpp<dpp_row_shr(4)>(Q), Q);
+        Q = mmul(mat_dpp<dpp_row_shr(8)>(Q), Q);
+        const int wu = __builtin_amdgcn_readfirstlane(wave);
+        if (wu > 0)
+            R = MatT<T>{readlane_t(Q.a, wu - 1), readlane_t(Q.b, wu - 1), readlane_t(Q.c, wu - 1),
+                        readlane_t(Q.d, wu - 1)};
+    }
+    return mmul(R, mat_dpp<kDppWaveShr1>(P));
+}
+
+// sum of v over the threads after this one
+__device__ __forceinline__ uint32_t sum_above(uint32_t v, uint32_t* wcnt) {
+    const int lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t sc = wave_suffix_sums(v);
     if (lane == 0) wcnt[wave] = sc;
     __syncthreads();
     uint32_t r = sc - v;
@@ -780,11 +854,18 @@ __device__ __forceinline__ uint32_t bit_from_above(uint32_t comp, uint32_t* wmap
 // shuffles and across them through LDS.  SM > 0: segments of at most SM loci
 // whose likelihoods, forward probabilities and maps stay in registers; SM = 0:
 // any length, through the chain's global scratch.  T: the arithmetic of the
-// forward values and thresholds.
+// forward values and thresholds.  A chain whose status is set or whose sweep
+// is an L-sweep is left as it is.  ubuf (SM > 0, may be null): LDS for M
+// uniforms, filled with draw indices 0..M-1 of the step while the loads are
+// in flight.
 template <int SM, typename T>
 __device__ void forward_backward(const MsampArgs& A, int chain, int k, MatT<T>* wtot, uint32_t* wcnt,
-                                 uint32_t* wmap) {
+                                 uint32_t* wmap, double* ubuf) {
     const int B = blockDim.x, tid = threadIdx.x, M = A.M;
+    if (!is_msweep(A, chain)) return;
+    // the status is read beside the meiosis order (an illegal chain's order
+    // may be stale) and checked before any load the order addresses
+    const int status = A.status[chain];
     const int m = order_at(A, chain, k);
     StepView sv;
     sv.A = &A;
@@ -794,6 +875,7 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int k, MatT<T>* 
     sv.bm = 1u << (m & 31);
     sv.spec = A.spec && k > 0;
     const int pm = sv.spec ? order_at(A, chain, k - 1) : 0;
+    if (status != 0) return;
     sv.pword = pm >> 5;
     sv.pbm = 1u << (pm & 31);
     const int word = sv.word;
@@ -805,9 +887,15 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int k, MatT<T>* 
     uint32_t near = 0;
     if constexpr (SM > 0) {
         // every load of the pass at once: the column words holding m_k and
-        // m_{k-1}, the current column's likelihood and both flipped variants
+        // m_{k-1}, the current column's likelihood, both flipped variants and
+        // the thetas of the segment's intervals
         uint32_t gw[SM], pw[SM];
-        double lcv[SM], lf0[SM], lf1[SM];
+        double lcv[SM], lf0[SM], lf1[SM], thv[SM + 1];
+#pragma unroll
+        for (int s = 0; s <= SM; ++s) {
+            const int i = a0 + s - 1;  // interval i joins loci i and i+1
+            thv[s] = (i >= 0 && i < M - 1 && a0 + s - 1 < a1) ? A.theta[i] : 0.0;
+        }
 #pragma unroll
         for (int s = 0; s < SM; ++s) {
             const int l = a0 + s;
@@ -823,6 +911,11 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int k, MatT<T>* 
                 }
             }
         }
+        // the step's uniforms by draw index, while the loads are in flight
+        // (the first barrier of the scans below publishes them)
+        if (ubuf)
+            for (int i = tid; i < M; i += B)
+                ubuf[i] = philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), (uint32_t)k, (uint32_t)i);
         T e0[SM], e1[SM], f0[SM], f1[SM];
         double lfv[SM];
         uint32_t curm = 0, drm = 0, both = 0;
@@ -837,29 +930,47 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int k, MatT<T>* 
         MatT<T> P{(T)1, (T)0, (T)0, (T)1};
 #pragma unroll
         for (int s = 0; s < SM; ++s)
-            if (a0 + s < a1) P = mmul(P, emission_matrix(A, e0[s], e1[s], a0 + s));
-        MatT<T> R = scan_products(P, wtot);
+            if (a0 + s < a1) P = mmul(P, emission_matrix_th(e0[s], e1[s], a0 + s == 0, thv[s]));
+        // the backward pass runs from the last locus down, so a drawing
+        // locus's draw index counts the draws of later threads and of the
+        // later loci of this segment.  FP32 draws where FP64 emissions allow
+        // both states, known before the forward pass: one exchange carries
+        // both scans
+        MatT<T> R;
+        uint32_t cnt;
+        if constexpr (sizeof(T) == 4) {
+            drm = both;
+            R = scan_products_counts(P, (uint32_t)__popc(drm), wtot, wcnt, &cnt);
+        } else {
+            R = scan_products(P, wtot);
+        }
 #pragma unroll
         for (int s = 0; s < SM; ++s) {
             f0[s] = f1[s] = (T)0;
             if (a0 + s < a1) {
-                R = mmul(R, emission_matrix(A, e0[s], e1[s], a0 + s));
+                R = mmul(R, emission_matrix_th(e0[s], e1[s], a0 + s == 0, thv[s]));
                 f0[s] = R.a + R.c;
                 f1[s] = R.b + R.d;
-                drm |= locus_draws(A, f0[s], f1[s], a0 + s, ((both >> s) & 1u) != 0u) << s;
+                if constexpr (sizeof(T) == 8) {
+                    const bool last = a0 + s == M - 1;
+                    const double t = last ? 1.0 : thv[s + 1], a = last ? 1.0 : 1.0 - thv[s + 1];
+                    drm |= ((f0[s] * a != 0.0 && f1[s] * t != 0.0) ? 1u : 0u) << s;
+                }
             }
         }
-        // the backward pass runs from the last locus down, so a drawing
-        // locus's draw index counts the draws of later threads and of the
-        // later loci of this segment
-        const uint32_t cnt = sum_above((uint32_t)__popc(drm), wcnt);
+        if constexpr (sizeof(T) == 8) cnt = sum_above((uint32_t)__popc(drm), wcnt);
         uint32_t mp[SM];
 #pragma unroll
         for (int s = 0; s < SM; ++s) {
             mp[s] = 0u;
-            if (a0 + s < a1)
-                mp[s] = locus_map(A, chain, k, f0[s], f1[s], a0 + s, cnt + (uint32_t)__popc(drm >> (s + 1)),
-                                  ((drm >> s) & 1u) != 0u, &near);
+            if (a0 + s < a1) {
+                const bool dr = ((drm >> s) & 1u) != 0u;
+                const uint32_t ctr = cnt + (uint32_t)__popc(drm >> (s + 1));
+                const double u = !dr ? 0.0
+                                 : ubuf ? ubuf[ctr]
+                                        : philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), (uint32_t)k, ctr);
+                mp[s] = locus_map_u(a0 + s == M - 1, thv[s + 1], f0[s], f1[s], u, dr, &near);
+            }
         }
         uint32_t comp = 2u;  // identity
 #pragma unroll
@@ -927,20 +1038,18 @@ __device__ void forward_backward(const MsampArgs& A, int chain, int k, MatT<T>* 
     if (near) atomicAdd(&A.near_edge[chain], near);
 }
 
-// status and sweep kind, read once per workgroup
-__device__ __forceinline__ bool skip_chain(const MsampArgs& A, int chain, int* s_skip) {
-    if (threadIdx.x == 0) *s_skip = A.status[chain] != 0 || !is_msweep(A, chain);
-    __syncthreads();
-    return *s_skip != 0;
-}
-
+// The workgroup's jobs of a chain in an M-sweep.  The register-resident
+// likelihood reads its slots of the person table with its other loads; the
+// LDS one reads a copy staged once per workgroup.
 template <int KC>
 __device__ __forceinline__ void run_jobs(const MsampArgs& A, int chain, int block, unsigned char* smem) {
+    if (!is_msweep(A, chain)) return;
     const int nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
-    // person table of the label pass, staged once per workgroup
     uint2* ptab = reinterpret_cast<uint2*>(smem);
-    for (int k = threadIdx.x; k < A.N - A.F; k += blockDim.x) ptab[k] = A.ptab[k];
-    __syncthreads();
+    if constexpr (KC == 0) {
+        for (int k = threadIdx.x; k < A.N - A.F; k += blockDim.x) ptab[k] = A.ptab[k];
+        __syncthreads();
+    }
     const int q = block * nw + wave;
     if (q < A.n_jobs) fag_job<KC>(A, chain, q, ptab, smem + A.ptab_bytes + (size_t)wave * A.wave_bytes);
 }
@@ -950,19 +1059,18 @@ __device__ __forceinline__ void run_jobs(const MsampArgs& A, int chain, int bloc
 template <int KC>
 __global__ void __launch_bounds__(kMsampBlock) mstep_kernel(MsampArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int s_skip;
-    if (skip_chain(A, blockIdx.y, &s_skip)) return;
     run_jobs<KC>(A, blockIdx.y, blockIdx.x, smem);
 }
 
-// forward-backward pass of step fb_step only; one workgroup per chain
+// forward-backward pass of step fb_step only; one workgroup per chain.  The
+// dynamic LDS, when A.fb_ubuf, holds the step's uniforms.
 template <int SM, typename T>
 __global__ void __launch_bounds__(kFbBlock) mfb_kernel(MsampArgs A) {
+    extern __shared__ __align__(16) unsigned char smem[];
     __shared__ MatT<T> wtot[kFbBlock / 64];
     __shared__ uint32_t wcnt[kFbBlock / 64], wmap[kFbBlock / 64];
-    __shared__ int s_skip;
-    if (skip_chain(A, blockIdx.x, &s_skip)) return;
-    forward_backward<SM, T>(A, blockIdx.x, A.fb_step, wtot, wcnt, wmap);
+    forward_backward<SM, T>(A, blockIdx.x, A.fb_step, wtot, wcnt, wmap,
+                            A.fb_ubuf ? reinterpret_cast<double*>(smem) : nullptr);
 }
 
 // one speculative launch: workgroup 0 of each chain runs step fb_step's
@@ -972,10 +1080,11 @@ __global__ void __launch_bounds__(kSpecBlock) mspec_kernel(MsampArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ MatT<T> wtot[kSpecBlock / 64];
     __shared__ uint32_t wcnt[kSpecBlock / 64], wmap[kSpecBlock / 64];
-    __shared__ int s_skip;
-    if (skip_chain(A, blockIdx.y, &s_skip)) return;
-    if (blockIdx.x == 0) forward_backward<SM, T>(A, blockIdx.y, A.fb_step, wtot, wcnt, wmap);
-    else run_jobs<KC>(A, blockIdx.y, blockIdx.x - 1, smem);
+    if (blockIdx.x == 0)
+        forward_backward<SM, T>(A, blockIdx.y, A.fb_step, wtot, wcnt, wmap,
+                                A.fb_ubuf ? reinterpret_cast<double*>(smem) : nullptr);
+    else
+        run_jobs<KC>(A, blockIdx.y, blockIdx.x - 1, smem);
 }
 
 // this sweep's meiosis order: the given order, or a Fisher-Yates shuffle of the
@@ -1138,11 +1247,12 @@ template <typename T>
 void launch_mfb_t(const MsampArgs& A, hipStream_t s) {
     const int seg = (A.M + kFbBlock - 1) / kFbBlock;
     const dim3 grid(A.n_chains), block(kFbBlock);
-    if (seg <= 1) hipLaunchKernelGGL((mfb_kernel<1, T>), grid, block, 0, s, A);
-    else if (seg <= 2) hipLaunchKernelGGL((mfb_kernel<2, T>), grid, block, 0, s, A);
-    else if (seg <= 4) hipLaunchKernelGGL((mfb_kernel<4, T>), grid, block, 0, s, A);
-    else if (seg <= 8) hipLaunchKernelGGL((mfb_kernel<8, T>), grid, block, 0, s, A);
-    else hipLaunchKernelGGL((mfb_kernel<0, T>), grid, block, 0, s, A);
+    const size_t lds = A.fb_ubuf ? sizeof(double) * (size_t)A.M : 0;
+    if (seg <= 1) hipLaunchKernelGGL((mfb_kernel<1, T>), grid, block, lds, s, A);
+    else if (seg <= 2) hipLaunchKernelGGL((mfb_kernel<2, T>), grid, block, lds, s, A);
+    else if (seg <= 4) hipLaunchKernelGGL((mfb_kernel<4, T>), grid, block, lds, s, A);
+    else if (seg <= 8) hipLaunchKernelGGL((mfb_kernel<8, T>), grid, block, lds, s, A);
+    else hipLaunchKernelGGL((mfb_kernel<0, T>), grid, block, lds, s, A);
 }
 
 }  // namespace

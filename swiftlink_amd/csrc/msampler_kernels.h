@@ -54,7 +54,12 @@ struct MsampArgs {
                                   // that many people and 32x that many founders; 0: in LDS, any size
     int n_chains;
     int fp32;                     // 1: the forward-backward pass in FP32 (FP32-rescaled contexts)
+    int fb_ubuf;                  // 1: forward-backward launches carry M doubles of dynamic LDS for the
+                                  //    step's uniforms (mspec: within its dynamic LDS)
 };
+
+// Loci up to which a forward-backward pass keeps the step's uniforms in LDS
+constexpr int kFbUbufLoci = 4096;
 
 bool msampler_supported(int N, int F);
 int msampler_people_chunks(int N, int F);  // register chunks for N people, F founders; 0 when too many
