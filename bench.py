@@ -58,6 +58,19 @@ def profile_traffic(kernel: str, workload: str):
     return float(m.group(1)) if m else None
 
 
+def cpu_threads() -> int:
+    """The host cores this process may use: OMP_NUM_THREADS when set (the
+    share of host cores a GPU is granted), else the CPUs in its affinity
+    mask."""
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if env > 0:
+        return env
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(tmpdir: str, base: str, case, threads: int):
     """Times the reference CPU path on a bounded sample of the same schedule:
     L-sweeps (LocusSampler::step over both parities), M-sweeps (MeiosisSampler
@@ -70,19 +83,24 @@ def cpu_baseline(tmpdir: str, base: str, case, threads: int):
         os.makedirs(out, exist_ok=True)
         subprocess.run([harness, "graphs", p, m, d, out, "5", "1", "1"], check=True, stdout=subprocess.DEVNULL,
                        stderr=subprocess.DEVNULL)
-        nl, nm, ng = 200, 200, 8 * threads
-        subprocess.run([harness, "bench", p, m, d, out, os.path.join(out, "graphs.i32"), str(threads), str(nl),
-                        str(nm), str(ng)], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
-        b = json.load(open(os.path.join(out, "bench.json")))
-        tl, tm, tg = b["l_seconds"] / nl, b["m_seconds"] / nm, b["lod_seconds"] / ng
+        # five samples; each component is the median of its five timings
+        nl, nm, ng, reps = 100, 60, 4 * threads, 5
+        runs, wall = [], 0.0
+        for _ in range(reps):
+            subprocess.run([harness, "bench", p, m, d, out, os.path.join(out, "graphs.i32"), str(threads), str(nl),
+                            str(nm), str(ng)], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            b = json.load(open(os.path.join(out, "bench.json")))
+            runs.append((b["l_seconds"] / nl, b["m_seconds"] / nm, b["lod_seconds"] / ng,
+                         b["lod_evals"] / b["lod_seconds"]))
+            wall += b["l_seconds"] + b["m_seconds"] + b["lod_seconds"]
+        tl, tm, tg, lev = (float(np.median([r[i] for r in runs])) for i in range(4))
         sweep = LPROB * tl + (1 - LPROB) * tm + tg / PERIOD
         return {"value": 1.0 / sweep, "unit": "sweeps/s", "cores": threads, "kind": "reference",
-                "lod_evals_per_s": b["lod_evals"] / b["lod_seconds"],
-                "l_sweep_ms": tl * 1e3, "m_sweep_ms": tm * 1e3, "lod_graph_ms": tg * 1e3,
-                "sample": f"{nl} L-sweeps, {nm} M-sweeps and {ng} graphs x {case.M - 1} intervals x {case.nlod} "
-                          f"positions with the reference's LocusSampler, MeiosisSampler and Peeler, FP64, "
-                          f"{threads} OpenMP threads ({b['l_seconds'] + b['m_seconds'] + b['lod_seconds']:.1f} s "
-                          f"wall)"}
+                "lod_evals_per_s": lev, "l_sweep_ms": tl * 1e3, "m_sweep_ms": tm * 1e3, "lod_graph_ms": tg * 1e3,
+                "samples": reps, "os_cpu_count": os.cpu_count(),
+                "sample": f"median of {reps} samples, each {nl} L-sweeps, {nm} M-sweeps and {ng} graphs x "
+                          f"{case.M - 1} intervals x {case.nlod} positions with the reference's LocusSampler, "
+                          f"MeiosisSampler and Peeler, FP64, {threads} OpenMP threads ({wall:.1f} s wall in all)"}
     # the reference binary is absent: the oracle restatement, one core
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
@@ -235,11 +253,12 @@ def main():
         kms, nl = ctx.profile_read(native.SL_KERNEL_LOD)
         _, _, lodv = ctx.lod_results()
         flops, _ = ctx.lod_work()
+        kern = ctx.lod_kernel_for(B)
         ctx.close()
-        return launches * evals_per_launch / dt, kms / max(nl, 1), lodv, flops
+        return launches * evals_per_launch / dt, kms / max(nl, 1), lodv, flops, kern
 
-    lod32, k32, lodv32, flops = lod(native.SL_FP32_RESCALED, 20)
-    lod64, _, lodv64, _ = lod(native.SL_FP64, 8)
+    lod32, k32, lodv32, flops, kern32 = lod(native.SL_FP32_RESCALED, 20)
+    lod64, _, lodv64, _, _ = lod(native.SL_FP64, 8)
 
     # ---- 16 pedigrees of uneven size, 4 chains each, one captured sweep per
     # context replayed side by side (sl_batch_*), shapes from the autotuner ----
@@ -319,6 +338,8 @@ def main():
     ms_avg = ms_ms / max(ms_launches, 1)
     achieved = bytes_per_launch / (ms_avg * 1e-3) / 1e9 if ms_launches else 0.0
     lod_achieved = flops * evals_per_launch / (k32 * 1e-3) / 1e12
+    lod_name = ("lane_lod_kernel (one (graph, position) evaluation per lane)" if kern32 == native.SL_LOD_LANES
+                else "lod_kernel<float,true>")
 
     res = {
         "metric": METRIC,
@@ -356,9 +377,9 @@ def main():
                      "traffic": profile_traffic("mspec_kernel", args.config),
                      "bytes_per_launch": bytes_per_launch, "launch_ms": ms_avg},
         "lod_roofline": {"bound": "mfma", "compute": "fp32 VALU (no MFMA: tables of 4^k cells)",
-                         "kernel": "lod_kernel<float,true>", "achieved": lod_achieved, "peak": FP32_PEAK_TFLOPS,
+                         "kernel": lod_name, "achieved": lod_achieved, "peak": FP32_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": lod_achieved / FP32_PEAK_TFLOPS,
-                         "traffic": profile_traffic("lod_kernel", args.config),
+                         "traffic": profile_traffic(lod_name.split("<")[0].split(" ")[0], args.config),
                          "flops_per_eval": flops, "launch_ms": k32},
     }
     if extras:
@@ -389,7 +410,7 @@ def main():
                         f"0.05 M apart), {ELOD_REPS} replicates; FP32-rescaled sampler, FP64 scoring",
             "seconds": edt, "replicates_per_s": ELOD_REPS / edt, "elod": elod_value}
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         res["cpu_baseline"] = cpu_baseline(tmp, os.path.join(tmp, args.config), case, threads)
     if not args.no_cpu and extras:
         _, ajac = native.trait_model_tables(MOD_MODEL[0], MOD_MODEL[1], acase.affection, acase.F)

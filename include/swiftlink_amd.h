@@ -196,6 +196,11 @@ int sl_profile_read(sl_ctx* ctx, int32_t kernel, double* ms, int64_t* launches);
  * position) from the compiled program: flops and LDS bytes. */
 int sl_lod_work(sl_ctx* ctx, double* flops_per_eval, double* lds_bytes_per_eval);
 
+/* The trait-peel kernel (SL_LOD_WORKGROUP or SL_LOD_LANES) that scores a
+ * launch of n_graphs graphs under the context's options and precision.  No
+ * counterpart in the reference; for tuners and measurement. */
+int sl_lod_kernel_for(sl_ctx* ctx, int32_t n_graphs, int32_t* kernel);
+
 /* ---- MCMC chains on the device (the intended GPUMarkovChain,
  * gpu_markov_chain.h:76-125) ---- */
 

@@ -966,6 +966,12 @@ int sl_lod_work(sl_ctx* c, double* flops, double* lds_bytes) {
     return SL_OK;
 }
 
+int sl_lod_kernel_for(sl_ctx* c, int32_t n_graphs, int32_t* kernel) {
+    if (!c || !kernel || n_graphs < 0) return SL_E_INVALID;
+    *kernel = lane_mode(c, n_graphs) ? SL_LOD_LANES : SL_LOD_WORKGROUP;
+    return SL_OK;
+}
+
 static int need_sampler(sl_ctx* c) {
     if (!c->samp_ready) return fail(c, SL_E_UNSUPPORTED, "sampler unavailable: " + c->samp_why);
     return SL_OK;

@@ -76,6 +76,9 @@ constexpr int kLodGlobalSlots = 2048;
 constexpr int kLaneGraphs = 64;
 hipError_t launch_lane_lod(const LodArgs& A, int precision, size_t lds, int nchunks, hipStream_t s);
 hipError_t configure_lane_lds(int precision, size_t lds);
+// XCDs of the device; the lane kernel deals runs of intervals to them
+constexpr int kXcds = 8;
+__host__ __device__ inline int lane_intervals_per_xcd(int n_intervals) { return (n_intervals + kXcds - 1) / kXcds; }
 // The lane kernel's transmission table: Z[16 codes][16 masks] and Q[16][4]
 constexpr int kZWords = 256 + 64;
 constexpr int kLaneRingWindows = 2;

@@ -673,7 +673,13 @@ __device__ __forceinline__ int lane_op(LaneStream& st, int h, unsigned char* la,
 __global__ void __launch_bounds__(kLanes) lane_lod_kernel(LodArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     if (A.gate_sweep && !gate_open(*A.gate_sweep, A.gate_burnin, A.gate_period)) return;
-    const int l = blockIdx.x, pos = blockIdx.y, chunk = blockIdx.z;
+    // workgroups are dealt to the XCDs in turn (id mod 8): XCD x takes the
+    // run of intervals [x per, (x+1) per), all positions of one interval in a
+    // row, so the workgroups that read a graph column share one L2
+    const int xcd = (int)(blockIdx.x % kXcds), slot = (int)(blockIdx.x / kXcds);
+    const int l = xcd * lane_intervals_per_xcd(A.M - 1) + slot / A.npos, pos = slot % A.npos;
+    const int chunk = blockIdx.z;
+    if (l >= A.M - 1) return;
     const int lane = (int)threadIdx.x;
     const int g = chunk * kLanes + lane;
     const bool live = g < A.G;
@@ -1112,7 +1118,7 @@ hipError_t configure_lane_lds(int precision, size_t lds) {
 
 hipError_t launch_lane_lod(const LodArgs& A, int precision, size_t lds, int nchunks, hipStream_t s) {
     if (precision == 0) return hipErrorInvalidValue;
-    dim3 grid(A.M - 1, A.npos, nchunks);
+    dim3 grid(kXcds * lane_intervals_per_xcd(A.M - 1) * A.npos, 1, nchunks);
     hipLaunchKernelGGL(lane_lod_kernel, grid, dim3(kLanes), lds, s, A);
     return hipGetLastError();
 }

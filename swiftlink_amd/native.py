@@ -90,6 +90,7 @@ _SIGS = {
     "sl_profile": (C.c_int, [C.c_void_p, C.c_int32]),
     "sl_profile_read": (C.c_int, [C.c_void_p, C.c_int32, f64p, i64p]),
     "sl_lod_work": (C.c_int, [C.c_void_p, f64p, f64p]),
+    "sl_lod_kernel_for": (C.c_int, [C.c_void_p, C.c_int32, i32p]),
     "sl_program_export": (C.c_int, [C.POINTER(PedigreeDesc), C.POINTER(PeelDesc), C.c_int32, i32p, C.c_int64,
                                     i64p]),
     "sl_mcmc_init": (C.c_int, [C.c_void_p, C.c_int32, i32p]),
@@ -412,6 +413,12 @@ class Context:
         self._check(lib().sl_mcmc_near_edge_loci(self.h, _ptr(bits, u32p)), "sl_mcmc_near_edge_loci")
         b = np.unpackbits(bits.view(np.uint8), axis=1, bitorder="little")
         return b[:, :self.case.M].astype(bool)
+
+    def lod_kernel_for(self, n_graphs: int) -> int:
+        """SL_LOD_WORKGROUP or SL_LOD_LANES: the kernel that scores n_graphs."""
+        k = C.c_int32(0)
+        self._check(lib().sl_lod_kernel_for(self.h, n_graphs, C.byref(k)), "sl_lod_kernel_for")
+        return k.value
 
     def lod_work(self):
         f = C.c_double(0)
