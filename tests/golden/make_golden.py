@@ -21,7 +21,8 @@ Each fixture holds:
   writer_*    LinkageWriter's LOD table for the graphs as one family and split
               into two, the LOD values it printed, marker names and positions
   (and peelseq_ped150.npz: the reference's peel table of the synthetic
-   150-person pedigree, see make_peelseq)
+   150-person pedigree, see make_peelseq; linkage/: the example pedigrees'
+   LINKAGE input files the loop and east fixtures were made from)
   elod_*      Elod::run's set-up and replicate loop (400 replicates, default
               model, replicate j under (seed 31, j, 0, step 0xFFFFFFFC)): the
               ELOD, accumulator, P(T), the first 6 replicates' graphs, both
@@ -155,8 +156,21 @@ def make_peelseq(tmp):
     print(f"peelseq_ped150: ops={case.ops.n} -> {os.path.getsize(path)} bytes")
 
 
+def copy_linkage_inputs():
+    """The example pedigrees' LINKAGE input files (data: .ped/.map/.dat),
+    kept beside the fixtures made from them so that the device pipeline can
+    start from the same files where the reference is not mounted."""
+    import shutil
+    dst = os.path.join(OUT, "linkage")
+    os.makedirs(dst, exist_ok=True)
+    for ex in ("loop", "east"):
+        for ext in (".ped", ".map", ".dat"):
+            shutil.copyfile(os.path.join(EXAMPLES, ex + ext), os.path.join(dst, ex + ext))
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REFSRC={REF}"], check=True)
+    copy_linkage_inputs()
     with tempfile.TemporaryDirectory() as tmp:
         make_peelseq(tmp)
         for ex in ("loop", "east"):

@@ -36,11 +36,13 @@ def test_synthetic_pedigree_matches_reference_tables():
     _compare(mine, ref)
 
 
+# the reference's example pedigrees' LINKAGE files, kept with the fixtures
+EXAMPLES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "linkage")
+
+
 @pytest.mark.parametrize("ex", ["loop", "east"])
 def test_examples_match_reference_tables(ex):
-    src = os.path.join("/root/reference/examples", ex)
-    if not os.path.exists(src + ".ped"):
-        pytest.skip("reference examples not mounted (golden fixtures still cover the oracle)")
+    src = os.path.join(EXAMPLES, ex)
     ref, fx = load_golden(ex)
     _compare(linkage.load(src + ".ped", src + ".map", src + ".dat"), ref)
 
@@ -209,9 +211,7 @@ def test_examples_with_reference_peel_search_match_reference(ex):
     """The reference's example pedigrees read with the reference's person
     order and peel search: the same persons in the same order and the same
     peel table as the reference built."""
-    src = os.path.join("/root/reference/examples", ex)
-    if not os.path.exists(src + ".ped"):
-        pytest.skip("reference examples not mounted (the golden fixtures still pin the search)")
+    src = os.path.join(EXAMPLES, ex)
     ref, fx = load_golden(ex)
     mine = linkage.load(src + ".ped", src + ".map", src + ".dat", peel="reference", peel_iterations=20000,
                         peel_seed=12345)

@@ -389,6 +389,119 @@ def test_locus_by_locus_weight_brute_force():
     assert abs(w - want) <= 1e-12 * abs(want)
 
 
+@pytest.mark.parametrize("side", ["left", "right"])
+def test_one_sided_locus_likelihood_brute_force(side):
+    """The sequential-imputation walk (start_from, locus_sampler2.cc:209-241)
+    steps away from its starting locus conditioning on one neighbour only: a
+    heterozygous parent passes the allele its indicator b selects with
+    probability (1-theta) when b equals the neighbour's indicator of that
+    meiosis and theta otherwise (sampler_rfunction.cc:102-157 normalises
+    0.5 x that factor over b), a homozygous one its only allele.  The
+    likelihood of the step is the sum over all 4^N phased-trait assignments
+    of prod_p q_p(t_p) times the kids' transmissions, enumerated here for the
+    9-person loop pedigree at every locus that has the neighbour, with the
+    two intervals given different thetas so that the interval taken for each
+    side is pinned too."""
+    case, fx = load_golden("loop")
+    case = dataclasses.replace(case, theta=np.array([0.05, 0.2]))
+    o = Oracle(case)
+    N, F, M = case.N, case.F, case.M
+    g0 = fx["graphs"][1]
+    alleles = [(0, 0), (1, 1), (1, 0), (0, 1)]  # UU, AA, AU, UA: (maternal, paternal), 1 = A
+    loci = range(1, M) if side == "left" else range(0, M - 1)
+    for l in loci:
+        nb, th = (l - 1, case.theta[l - 1]) if side == "left" else (l + 1, case.theta[l])
+        full = np.ones((4,) * N)
+        for p in range(N):
+            shape = [1] * N
+            shape[p] = 4
+            full = full * case.geno_prob[p, l].reshape(shape)
+        for k in range(F, N):
+            m, f = int(case.mother[k]), int(case.father[k])
+            T = np.zeros((4, 4, 4))
+            for tm in range(4):
+                for tf in range(4):
+                    for tk in range(4):
+                        pr = 1.0
+                        for s, tp in ((0, tm), (1, tf)):
+                            nbit = int(g0[nb, 2 * k + s])
+                            # indicator b passes the parent's allele b (0 maternal, 1 paternal)
+                            pr *= sum((1 - th if b == nbit else th) if alleles[tp][0] != alleles[tp][1] else 0.5
+                                      for b in (0, 1) if alleles[tp][b] == alleles[tk][s])
+                        T[tm, tf, tk] = pr
+            shape = [1] * N
+            shape[m], shape[f], shape[k] = 4, 4, 4
+            full = full * np.moveaxis(T, [0, 1, 2], np.argsort(np.argsort([m, f, k]))).reshape(shape)
+        want = full.sum()
+        got = o.locus_likelihood(g0, l, use_left=side == "left", use_right=side == "right")
+        assert want > 0
+        assert abs(got - want) <= 1e-12 * want, (l, got, want)
+
+
+def test_chain_is_its_schedule_of_steps():
+    """MarkovChain::run (markov_chain.cc:335-389): sweep s is an L-sweep when
+    the schedule stream's first uniform is below lprob (its second picks the
+    parity done first; every locus of that parity, then of the other, is
+    stepped) and otherwise an M-sweep over the meioses shuffled by
+    random_shuffle on the shuffle stream (swap position i with floor(u (i+1)),
+    i from the last down); a sweep at or past burnin whose index is a multiple
+    of the period is scored, as one merge of every interval's terms.  The
+    oracle's chain equals these steps composed by hand, sweep by sweep, and
+    its accumulators the log-sum of the scored graphs' terms."""
+    case, fx = load_golden("ped30_snp100")
+    o = Oracle(case)
+    seed, lprob, burnin, period, sweeps = 91, 0.5, 2, 3, 9
+    g = fx["graph_init"].copy()
+    ml = o.meioses()
+    kinds, scored = [], []
+    for sw in range(sweeps):
+        if uniform(seed, 0, sw, 0xFFFFFFFF, 0) < lprob:
+            kinds.append(0)
+            p0 = 0 if uniform(seed, 0, sw, 0xFFFFFFFF, 1) < 0.5 else 1
+            for par in (p0, 1 - p0):
+                for l in range(par, case.M, 2):
+                    g, _ = o.lstep(g, l, seed, 0, sw)
+        else:
+            kinds.append(1)
+            order = ml.copy()
+            for d, i in enumerate(range(len(order) - 1, 0, -1)):
+                j = min(int(uniform(seed, 0, sw, 0xFFFFFFFE, d) * (i + 1)), i)
+                order[i], order[j] = order[j], order[i]
+            g = o.msweep(g, order, seed, 0, sw)
+        if sw >= burnin and sw % period == 0:
+            scored.append(np.concatenate([o.lod_terms(g, l) for l in range(case.M - 1)]))
+    assert 0 < sum(kinds) < sweeps
+    out, acc, cnt, k = o.chain(fx["graph_init"], seed, 0, sweeps, burnin, period, lprob)
+    assert list(k) == kinds
+    assert np.array_equal(out, g)
+    assert cnt == len(scored) == len([s for s in range(burnin, sweeps) if s % period == 0])
+    np.testing.assert_allclose(acc, np.logaddexp.reduce(np.array(scored), axis=0), rtol=1e-12, atol=0)
+
+
+def test_recombination_term_closed_form():
+    """Peeler::process returns ln L - R - T (peeler.cc:97-99): with T = 2(N-F)
+    ln 1/2 and R = k ln theta_l + (2(N-F) - k) ln(1 - theta_l) for k
+    meioses recombining across interval l.  A graph with identical columns
+    (k = 0 everywhere) and then 3 meioses flipped at every locus past l
+    (k = 3 at l, 0 elsewhere) against the trait likelihood alone; every
+    interval gets its own theta, so the interval R is taken from is pinned."""
+    case, fx = load_golden("ped30_snp100")
+    case = dataclasses.replace(case, theta=np.random.default_rng(5).uniform(0.01, 0.3, case.M - 1))
+    o = Oracle(case)
+    N, F, M = case.N, case.F, case.M
+    nm = 2 * (N - F)
+    base = np.repeat(fx["graphs"][0][:1], M, axis=0)
+    for l in (5, 40):
+        g = base.copy()
+        flip = [2 * F + 1, 2 * F + 6, 2 * N - 2]
+        g[l + 1:, flip] ^= 1
+        for li, k in ((l, 3), (l - 1, 0), (l + 1, 0)):
+            th = case.theta[li]
+            R = k * np.log(th) + (nm - k) * np.log1p(-th)
+            want = [np.log(o.trait_likelihood(g, li, off)) - R - nm * np.log(0.5) for off in range(1, case.nlod + 1)]
+            np.testing.assert_allclose(o.lod_terms(g, li), want, rtol=1e-13, atol=1e-12)
+
+
 def test_impute_walk_covers_every_locus_once():
     """A replicate's locus walk: every column is written (no column keeps the
     zero it started with wherever a heterozygous parent fixes the bit), and
