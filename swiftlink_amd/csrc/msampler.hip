@@ -281,6 +281,8 @@ __device__ double fag_log_likelihood_r(const MsampArgs& A, const JobInputs<KC>& 
 #pragma unroll
     for (int c = 0; c < KC; ++c) g[c] = in.g[c];
     const double minor = in.minor;
+    // the allele-frequency logs of the final sum, formed while the graph is built
+    const double lmaj = log(1.0 - minor), lmin = log(minor);
 #pragma unroll
     for (int n = 0; n < KN; ++n) {
         const int x = 64 * n + lane;
@@ -415,7 +417,6 @@ __device__ double fag_log_likelihood_r(const MsampArgs& A, const JobInputs<KC>& 
         }
     }
     wave_sync();
-    const double lmaj = log(1.0 - minor), lmin = log(minor);
     double ll = 0.0;
     bool conflict = false;
 #pragma unroll
