@@ -185,6 +185,39 @@ def test_batch16_fp32_lod_within_bar(batch16):
         assert worst <= LOD_BAR
 
 
+def test_batch16_lane_kernel_covers_wide_ops(batch16):
+    """The lane kernel on the pedigrees whose programs have the widest ops
+    (an op consuming 4 tables, which the generic cell path evaluates; PARENT
+    ops over 3 children in the cutset): 64 graphs scored by lanes, against the
+    FP32 workgroup kernel (FP32 rounding) and against the FP64 oracle at about
+    60 intervals (the LOD bar).  A pedigree whose lane arena exceeds LDS is
+    scored by the workgroup kernel, as sl_lod_kernel_for reports."""
+    from program_emu import Program
+    picked = 0
+    for case, g0 in batch16:
+        P = Program(native.program_export(case))
+        wide = max(s_[1] for s_ in P.seq) >= 4 or max((s_[7] for s_ in P.seq if s_[0] == 2), default=0) >= 3
+        if not wide:
+            continue
+        picked += 1
+        graphs = _flipped(case, g0, 64, 40, case.N)
+        with native.Context(case, native.SL_FP32_RESCALED, lod_kernel=native.SL_LOD_LANES) as a, \
+                native.Context(case, native.SL_FP32_RESCALED, lod_kernel=native.SL_LOD_WORKGROUP) as b:
+            lanes = a.lod_kernel_for(64) == native.SL_LOD_LANES
+            ta, tb = a.lod_terms(graphs), b.lod_terms(graphs)
+        print(f"batch16 lanes: N={case.N} max n_prev={max(s_[1] for s_ in P.seq)} lane kernel={lanes}", flush=True)
+        assert np.all(np.isfinite(ta))
+        assert np.abs(ta - tb).max() <= 1e-4 * max(1.0, np.abs(tb).max())
+        o = Oracle(case)
+        step = max(1, (case.M - 1) // 60)
+        worst = 0.0
+        for g in (0, 63):
+            for l in range(0, case.M - 1, step):
+                worst = max(worst, np.abs(ta[g, l] - o.lod_terms(graphs[g], l)).max() / LN10)
+        assert worst <= LOD_BAR, (case.N, worst)
+    assert picked >= 3
+
+
 # ---------------------------------------------------------------- adjoint, 200 people
 
 @pytest.fixture(scope="module")
