@@ -519,6 +519,9 @@ template <int TYPE, int NP, int NC>
 __device__ __forceinline__ void lane_block(const uint32_t* blk, int dz, const float tr[4], const float qr[4],
                                            const unsigned char* const zb[4], const unsigned char* la,
                                            unsigned char* ob, float* mx) {
+    // FP32 only: each term's last product is fused into the cell's sum (the
+    // FP64 kernels keep the reference's separate roundings)
+#pragma clang fp contract(fast)
     constexpr int G = 1 + NP;
     constexpr int STRIDE = NP <= 1 ? 8 : 16;
     constexpr int CPB = 32 / STRIDE;
@@ -576,6 +579,7 @@ __device__ __forceinline__ float lane_blocks_any(LaneStream& st, const uint32_t*
                                               int np, int nc, int lane, float4 trv, float4 qrv, const uint32_t* lc,
                                               const uint32_t* zt, const unsigned char* la, unsigned char* ob,
                                               float mx) {
+#pragma clang fp contract(fast)
     const float tr[4] = {trv.x, trv.y, trv.z, trv.w}, qr[4] = {qrv.x, qrv.y, qrv.z, qrv.w};
     const int G = 1 + np;
     const int stride = __builtin_amdgcn_readfirstlane((int)hd[6]);
