@@ -50,17 +50,21 @@ typedef const __attribute__((address_space(4))) float* cflt_p;
 
 enum { PT_UU = 0, PT_AA = 1, PT_AU = 2, PT_UA = 3 };
 
-// r-th set bit (r < popcount) of a 4-bit mask
+// r-th set bit (r < popcount) of a 4-bit mask: one byte per mask holds its
+// set bits in increasing order, 2 bits each (mask 0b0110: 1 | 2 << 2)
 __device__ __forceinline__ uint32_t select_bit(uint32_t m, uint32_t r) {
-    uint32_t v = 0;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        if ((m >> b) & 1u) {
-            if (r == 0u) { v = (uint32_t)b; r = 0xFFu; }
-            else if (r != 0xFFu) --r;
-        }
-    }
-    return v;
+    const uint64_t lut = m < 8u ? 0x2409080204010000ull : 0xe439380e340d0c03ull;
+    const uint32_t byte = (uint32_t)(lut >> (8u * (m & 7u))) & 0xFFu;
+    return (byte >> (2u * r)) & 3u;
+}
+
+// quotient of a compact cell index by a digit's count of legal values (1 to
+// 4): a multiply by 2^16 / n rounded up, exact for every index below 2^15
+// (a table holds at most 4^kMaxCut cells)
+static_assert(kMaxCut <= 7, "cell indices below 2^15");
+__device__ __forceinline__ uint32_t div_by_count(uint32_t x, uint32_t n) {
+    const uint32_t mul = n == 1u ? 65536u : n == 2u ? 32768u : n == 3u ? 21846u : 16384u;
+    return __umul24(x, mul) >> 16;
 }
 
 __device__ __forceinline__ uint32_t rank_in(uint32_t m, uint32_t v) { return __popc(m & ((1u << v) - 1u)); }
@@ -90,6 +94,20 @@ __device__ __forceinline__ float wave_max(float v) {
     const float r2 = __int_as_float(__builtin_amdgcn_readlane(x, 32));
     const float r3 = __int_as_float(__builtin_amdgcn_readlane(x, 48));
     return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+}
+
+// lane CTRL/0x55 of each quad of lanes (quad_perm controls 0x00, 0x55, 0xAA,
+// 0xFF select lanes 0 to 3), for every lane of the quad
+template <typename T, int CTRL>
+__device__ __forceinline__ T quad_lane(T x) {
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+    } else {
+        const long long b = __double_as_longlong(x);
+        const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xFFFFFFFFll), CTRL, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+        return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+    }
 }
 
 // exclusive prefix sum of a[0..n) in place (LDS), whole workgroup; returns total
@@ -294,7 +312,14 @@ __device__ __forceinline__ void sampler_op(const OpEnv<T>& E) {
     if (RESCALE && E.impute && E.lane == 0 && scale_exp != 0) atomicAdd(E.scale_exp, scale_exp);
     const T* dist_peel = E.dist + 8 * (peel - F);
     float mx = 0.0f;
-    for (int c = E.lane; c < size; c += 64) {
+    // one lane per (cell, value of the peel node): lanes 4c .. 4c+3 of a quad
+    // hold cell c's four presums, summed in value order across the quad
+    const int tasks = 4 * size;
+    for (int t0 = 0; t0 < tasks; t0 += 64) {
+        const int t = t0 + E.lane;
+        const bool act = t < tasks;
+        const int c = act ? t >> 2 : 0;
+        const uint32_t v = (uint32_t)t & 3u;
         // mixed-radix decode of the compact cell into trait values (2 bits
         // per digit) and the consumed tables' base indices
         uint32_t ext = 0, rk = 0;
@@ -307,7 +332,7 @@ __device__ __forceinline__ void sampler_op(const OpEnv<T>& E) {
             if (q < k) {
                 const uint32_t m = (cmw >> (4 * q)) & 15u;
                 const int n = __popc(m);
-                const int quo = (int)(((float)rem + 0.5f) / (float)n);
+                const int quo = (int)div_by_count((uint32_t)rem, (uint32_t)n);
                 const int r = rem - quo * n;
                 rem = quo;
                 ext |= select_bit(m, (uint32_t)r) << (2 * q);
@@ -316,40 +341,14 @@ __device__ __forceinline__ void sampler_op(const OpEnv<T>& E) {
                 for (int p = 0; p < NP; ++p) base[p] += r * S[p][q];
             }
         }
-        // v-independent transmission parts: CHILD parents' traits; PARENT
-        // each child's trait and the factor of its other parent
-        const uint32_t mt_c = (ext >> (2 * mat_digit)) & 3u, pt_c = (ext >> (2 * pat_digit)) & 3u;
-        uint32_t kid_t[7];
-        T fixed[7];
-        if (type == kParent) {
-#pragma unroll
-            for (int ch = 0; ch < 7; ++ch) {
-                kid_t[ch] = 0;
-                fixed[ch] = (T)0;
-                if (ch < n_child) {
-                    const uint32_t kid = (ext >> (2 * (kinfo[ch] & 15))) & 3u;
-                    const uint32_t other = (ext >> (2 * ((kinfo[ch] >> 4) & 15))) & 3u;
-                    const T* dk = E.dist + kdist[ch];
-                    kid_t[ch] = kid;
-                    // other parent: father when the peel node is the mother
-                    fixed[ch] = (kinfo[ch] >> 8) ? parent_factor<T>(dk + 4, other, (kid == PT_AA || kid == PT_UA) ? 1u : 0u)
-                                                 : parent_factor<T>(dk, other, (kid == PT_AA || kid == PT_AU) ? 1u : 0u);
-                }
+        const bool legal = act && ((pmask >> v) & 1u) != 0u;
+        const int rv = __popc(pmask & ((1u << v) - 1u));  // rank of v among the legal values
+        T tmp = legal ? E.trt[4 * peel + v] : (T)0;
+        if (tmp != (T)0) {
+            if (type == kChild) {
+                const uint32_t mt_c = (ext >> (2 * mat_digit)) & 3u, pt_c = (ext >> (2 * pat_digit)) & 3u;
+                tmp *= trans<T>(dist_peel, mt_c, pt_c, v);
             }
-        }
-        T total = (T)0;
-        T* pc = E.pre + 4 * (size_t)(my_off + c);
-        int rv = 0;  // rank of v among the legal values
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            T tmp = tr[v];
-            const bool legal = ((pmask >> v) & 1u) != 0u;
-            if (!legal || tmp == (T)0) {
-                pc[v] = (T)0;
-                rv += legal ? 1 : 0;
-                continue;
-            }
-            if (type == kChild) tmp *= trans<T>(dist_peel, mt_c, pt_c, (uint32_t)v);
 #pragma unroll
             for (int q = 0; q < NP; ++q) tmp *= E.tab[p_off[q] + base[q] + rv * PS[q]] * p_scale[q];
             if (NP_ < 0) {
@@ -362,27 +361,33 @@ __device__ __forceinline__ void sampler_op(const OpEnv<T>& E) {
                 }
             }
             if (type == kParent) {
+                // each child's trait and the factor of its other parent, then
+                // the factor of v (trans(dist, mt, pt, kid) = md * pd)
                 T child_prob = (T)1;
 #pragma unroll
                 for (int ch = 0; ch < 7; ++ch) {
                     if (ch < n_child) {
-                        const uint32_t kid = kid_t[ch];
+                        const uint32_t kid = (ext >> (2 * (kinfo[ch] & 15))) & 3u;
+                        const uint32_t other = (ext >> (2 * ((kinfo[ch] >> 4) & 15))) & 3u;
                         const T* dk = E.dist + kdist[ch];
-                        const bool pim = (kinfo[ch] >> 8) != 0;
-                        // trans(dist, mt, pt, kid) = md * pd, one of them v's
-                        const T vf = pim ? parent_factor<T>(dk, (uint32_t)v, (kid == PT_AA || kid == PT_AU) ? 1u : 0u)
-                                         : parent_factor<T>(dk + 4, (uint32_t)v, (kid == PT_AA || kid == PT_UA) ? 1u : 0u);
-                        child_prob *= pim ? vf * fixed[ch] : fixed[ch] * vf;
+                        const bool pim = (kinfo[ch] >> 8) != 0;  // other parent: father when the peel node is the mother
+                        const uint32_t am = (kid == PT_AA || kid == PT_AU) ? 1u : 0u;
+                        const uint32_t ap = (kid == PT_AA || kid == PT_UA) ? 1u : 0u;
+                        const T fixed = pim ? parent_factor<T>(dk + 4, other, ap) : parent_factor<T>(dk, other, am);
+                        const T vf = pim ? parent_factor<T>(dk, v, am) : parent_factor<T>(dk + 4, v, ap);
+                        child_prob *= pim ? vf * fixed : fixed * vf;
                     }
                 }
                 tmp *= child_prob;
             }
-            pc[v] = tmp;
-            total += tmp;
-            ++rv;
         }
-        E.tab[my_off + c] = total;
-        if (RESCALE) mx = fmaxf(mx, (float)total);
+        if (act) E.pre[4 * (size_t)(my_off + c) + v] = tmp;
+        // the cell's sum in value order (skipped values add exact zeros)
+        const T s0 = quad_lane<T, 0x00>(tmp), s1 = quad_lane<T, 0x55>(tmp);
+        const T s2 = quad_lane<T, 0xAA>(tmp), s3 = quad_lane<T, 0xFF>(tmp);
+        const T total = ((s0 + s1) + s2) + s3;
+        if (act && v == 0u) E.tab[my_off + c] = total;
+        if (RESCALE && act) mx = fmaxf(mx, (float)total);
     }
     if (RESCALE) {
         const float m = wave_max(mx);
