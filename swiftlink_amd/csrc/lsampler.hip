@@ -258,28 +258,22 @@ __device__ __forceinline__ void sampler_op(const OpEnv<T>& E) {
     const uint32_t os = (uint32_t)uni((int)rec[1]);
     const int my_off = (int)(os & 0xFFFFu), size = (int)(os >> 16);
     const uint32_t pmask = (cmw >> (4 * k)) & 15u;
-    T tr[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) tr[v] = E.trt[4 * peel + v];
     // consumed tables: offset, scale, and the stride of each of this op's
-    // digits in the table's radix (0 where the digit is not in the table)
-    int p_off[NP > 0 ? NP : 1], S[NP > 0 ? NP : 1][kMaxCut + 1], PS[NP > 0 ? NP : 1];
+    // digits in the table's radix (0 where the digit is not in the table),
+    // 16 bits each; read by every lane, so they stay in vector registers
+    int p_off[NP > 0 ? NP : 1], PS[NP > 0 ? NP : 1];
+    uint32_t sw[NP > 0 ? NP : 1][4];
     T p_scale[NP > 0 ? NP : 1];
     int scale_exp = 0;
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
         const uint32_t* rq = rec + 2 + 4 * q;
-        const uint32_t op_ps = (uint32_t)uni((int)rq[0]);
+        const uint32_t op_ps = rq[0];
         p_off[q] = (int)(op_ps & 0xFFFFu);
         PS[q] = (int)(op_ps >> 16);
 #pragma unroll
-        for (int w = 0; w < 3; ++w) {
-            const uint32_t pair = (uint32_t)uni((int)rq[1 + w]);
-            S[q][2 * w] = (int)(pair & 0xFFFFu);
-            S[q][2 * w + 1] = (int)(pair >> 16);
-        }
-        S[q][6] = (int)((((uint32_t)uni((int)rec[14 + q / 2])) >> (16 * (q & 1))) & 0xFFFFu);
-        S[q][7] = 0;  // digit 7 exists only as the peel node of a 7-person cutset
+        for (int w = 0; w < 3; ++w) sw[q][w] = rq[1 + w];
+        sw[q][3] = rec[14 + q / 2] >> (16 * (q & 1));  // digit 6
         p_scale[q] = (T)1;
         if (RESCALE) {
             const uint32_t mb = E.maxbits[d[16 + q]];  // SampOp words 16..: consumed ops
@@ -338,7 +332,7 @@ __device__ __forceinline__ void sampler_op(const OpEnv<T>& E) {
                 ext |= select_bit(m, (uint32_t)r) << (2 * q);
                 rk |= (uint32_t)r << (2 * q);
 #pragma unroll
-                for (int p = 0; p < NP; ++p) base[p] += r * S[p][q];
+                for (int p = 0; p < NP; ++p) base[p] += r * (int)((sw[p][q >> 1] >> (16 * (q & 1))) & 0xFFFFu);
             }
         }
         const bool legal = act && ((pmask >> v) & 1u) != 0u;
