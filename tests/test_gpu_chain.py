@@ -66,6 +66,35 @@ def test_chains_merge_like_replicates(name, mode):
     np.testing.assert_allclose(acc, np.logaddexp.reduce(np.array(accs), axis=0), rtol=0, atol=ACC_TOL)
 
 
+@pytest.mark.parametrize("mode", [native.SL_MSAMPLER_SPECULATIVE, native.SL_MSAMPLER_TWO_LAUNCH])
+@pytest.mark.parametrize("chains", [2, 4])
+@pytest.mark.parametrize("precision", [native.SL_FP64, native.SL_FP32_RESCALED])
+def test_captured_sweeps_match_launched_sweeps(precision, chains, mode):
+    """With several chains sl_mcmc_run replays captured graphs of its sweeps
+    (the sweep index on the device) unless profiling is on, when every kernel
+    is launched from the stream with events around it.  Over two runs, with
+    LOD scoring launched between the replays, both give the same graphs, LOD
+    accumulators, sweep counts and near-edge counts; the chains mix L- and
+    M-sweeps within sweeps."""
+    case, fx = load_golden("ped30_snp100")
+    starts = np.stack([fx["graph_init"], fx["graphs"][0], fx["graphs"][1], fx["graph_init"]])[:chains]
+    res = []
+    for prof in (False, True):
+        with native.Context(case, precision, msampler_mode=mode) as ctx:
+            ctx.profile(prof)
+            ctx.mcmc_init(starts)
+            ctx.lod_reset()
+            st = [ctx.mcmc_run(23, 4, 3, 0.5, 17), ctx.mcmc_run(9, 4, 3, 0.5, 17, first_sweep=23)]
+            res.append((st, ctx.mcmc_graphs(), ctx.lod_results(), ctx.mcmc_near_edge()))
+    (st_a, g_a, (acc_a, n_a, _), ne_a), (st_b, g_b, (acc_b, n_b, _), ne_b) = res
+    assert st_a == st_b
+    assert np.array_equal(g_a, g_b)
+    assert n_a == n_b > 0
+    assert np.array_equal(acc_a, acc_b)
+    assert np.array_equal(ne_a, ne_b)
+    assert st_a[0]["l_sweeps"] > 0 and st_a[0]["m_sweeps"] > 0
+
+
 def test_run_continues_from_first_sweep():
     case, fx = load_golden("ped30_snp100")
     starts = np.stack([fx["graph_init"], fx["graphs"][0]])
