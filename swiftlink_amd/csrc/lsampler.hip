@@ -63,7 +63,7 @@ __device__ __forceinline__ uint32_t select_bit(uint32_t m, uint32_t r) {
 // (a table holds at most 4^kMaxCut cells)
 static_assert(kMaxCut <= 7, "cell indices below 2^15");
 __device__ __forceinline__ uint32_t div_by_count(uint32_t x, uint32_t n) {
-    const uint32_t mul = n == 1u ? 65536u : n == 2u ? 32768u : n == 3u ? 21846u : 16384u;
+    const uint32_t mul = n == 3u ? 21846u : 0x10000u >> (n >> 1);  // n = 1, 2, 4: 2^16 >> 0, 1, 2
     return __umul24(x, mul) >> 16;
 }
 
