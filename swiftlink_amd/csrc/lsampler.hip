@@ -110,30 +110,39 @@ __device__ __forceinline__ T quad_lane(T x) {
     }
 }
 
-// exclusive prefix sum of a[0..n) in place (LDS), whole workgroup; returns total
+// inclusive prefix sum over the 64 lanes of a wavefront (DPP: shifts within
+// rows of 16, then each row's total broadcast into the rows above it)
+__device__ __forceinline__ int wave_inclusive_sum(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15 into rows 1 and 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31 into rows 2 and 3
+    return x;
+}
+
+// exclusive prefix sum of a[0..n) in place (LDS) by wavefront 0, each lane
+// over a run of ceil(n/64) entries; returns the total to every thread.  The
+// caller's barrier precedes it (a[] written); one barrier ends it.
 __device__ __forceinline__ int block_exclusive_scan(int* a, int n, int* tmp) {
-    const int T = blockDim.x, t = threadIdx.x;
-    const int per = (n + T - 1) / T;
-    const int b0 = min(n, t * per), b1 = min(n, b0 + per);
-    int s = 0;
-    for (int i = b0; i < b1; ++i) s += a[i];
-    tmp[t] = s;
-    __syncthreads();
-    for (int off = 1; off < T; off <<= 1) {
-        int v = t >= off ? tmp[t - off] : 0;
-        __syncthreads();
-        tmp[t] += v;
-        __syncthreads();
-    }
-    int run = t > 0 ? tmp[t - 1] : 0;
-    const int total = tmp[T - 1];
-    for (int i = b0; i < b1; ++i) {
-        int x = a[i];
-        a[i] = run;
-        run += x;
+    const int t = threadIdx.x;
+    if (t < 64) {
+        const int per = (n + 63) / 64;
+        const int b0 = min(n, t * per), b1 = min(n, b0 + per);
+        int s = 0;
+        for (int i = b0; i < b1; ++i) s += a[i];
+        const int incl = wave_inclusive_sum(s);
+        int run = incl - s;
+        for (int i = b0; i < b1; ++i) {
+            const int x = a[i];
+            a[i] = run;
+            run += x;
+        }
+        if (t == 63) tmp[0] = incl;
     }
     __syncthreads();
-    return total;
+    return tmp[0];
 }
 
 // Locus of step t of one sequential-imputation replicate (mode: the impute
@@ -578,16 +587,46 @@ __global__ void __launch_bounds__(kLsamplerBlockWide) lsampler_kernel(SampArgs A
         A.imp_weight[chain] += w;
     }
 
-    // 3. reverse sampling by reverse levels
+    // 3. reverse sampling by reverse levels.  A thread's first op of the next
+    // level, its cutset size, peel node and cutset (static descriptors) are
+    // loaded while the current level runs.
+    struct RevOp {
+        int op, k, peel, cut[kMaxCut];
+    };
+    auto fetch_rev = [&](int R, RevOp* ro) {
+        ro->op = -1;
+        if (R >= A.n_rlevels) return;
+        const int i = ((cint_p)A.rlevel_off)[R] + tid;
+        if (i >= ((cint_p)A.rlevel_off)[R + 1]) return;
+        ro->op = A.rlevel_ops[i];
+        const SampOp* dp = A.ops + ro->op;
+        ro->k = dp->k;
+        ro->peel = dp->peel;
+#pragma unroll
+        for (int q = 0; q < kMaxCut; ++q) ro->cut[q] = dp->cut[q];
+    };
+    RevOp cur;
+    fetch_rev(0, &cur);
     for (int R = 0; R < A.n_rlevels; ++R) {
         const int r0 = ((cint_p)A.rlevel_off)[R], r1 = ((cint_p)A.rlevel_off)[R + 1];
+        RevOp nxt;
+        fetch_rev(R + 1, &nxt);
         for (int i = r0 + tid; i < r1; i += blockDim.x) {
-            const int op = A.rlevel_ops[i];
-            const SampOp* dp = A.ops + op;
-            const int k = dp->k, peel = dp->peel;
+            RevOp ro = cur;
+            if (i != r0 + tid) {  // past the first op of this thread (levels wider than the workgroup)
+                ro.op = A.rlevel_ops[i];
+                const SampOp* dq = A.ops + ro.op;
+                ro.k = dq->k;
+                ro.peel = dq->peel;
+#pragma unroll
+                for (int q = 0; q < kMaxCut; ++q) ro.cut[q] = dq->cut[q];
+            }
+            const int op = ro.op, k = ro.k, peel = ro.peel;
             int base = 0, st = 1;
-            for (int q = 0; q < k; ++q) {
-                const int who = dp->cut[q];
+#pragma unroll
+            for (int q = 0; q < kMaxCut; ++q) {
+                if (q >= k) break;
+                const int who = ro.cut[q];
                 const uint32_t m = mask[who];
                 base += (int)rank_in(m, pmk[who]) * st;
                 st *= __popc(m);
@@ -622,6 +661,7 @@ __global__ void __launch_bounds__(kLsamplerBlockWide) lsampler_kernel(SampArgs A
             }
         }
         __syncthreads();
+        cur = nxt;
     }
 
     // 4. meiosis indicators
