@@ -166,6 +166,45 @@ def test_batch16_largest_pedigree_matches_oracle(batch16):
     assert np.array_equal(mout, o.msweep(g, order, 5, 0, 1))
 
 
+def _sampler_widths(case):
+    """Most tables any op consumes, and per locus the cells of its largest op
+    (the product of its cutset members' legal trait counts)."""
+    pc = np.array([bin(x).count("1") for x in range(16)])
+    o = case.ops
+    legal = np.asarray(case.legal)
+    cells = np.ones(case.M, np.int64)
+    for k in range(o.n):
+        s = np.ones(case.M, np.int64)
+        for p in o.cut[o.cut_off[k]:o.cut_off[k + 1]]:
+            s *= pc[legal[p]]
+        cells = np.maximum(cells, s)
+    return int(np.diff(o.prev_off).max()), cells
+
+
+def test_batch16_sampler_wide_ops_match_oracle(batch16):
+    """The L-sampler's generic cell path (an op consuming 4 tables) and its
+    several passes over ops of more than 16 cells (4 lanes per cell, 64 lanes
+    a pass): a whole FP64 L-sweep of the pedigrees that have them against the
+    oracle's chain."""
+    picked = 0
+    for case, g0 in batch16:
+        most, cells = _sampler_widths(case)
+        if most <= 3 and (cells > 16).sum() <= 100:
+            continue
+        picked += 1
+        print(f"batch16 sampler: N={case.N} M={case.M} most consumed {most}, "
+              f"loci with an op over 16 cells {(cells > 16).sum()}", flush=True)
+        with native.Context(case, native.SL_FP64) as ctx:
+            ctx.mcmc_init(g0[None])
+            ctx.mcmc_run(1, 100, 1, 1.0, 515)
+            out = ctx.mcmc_graphs()[0]
+            assert ctx.mcmc_near_edge()[0] == 0
+        g, _, _, kinds = Oracle(case).chain(g0, 515, 0, 1, 100, 1, 1.0)
+        assert np.all(kinds == 0)
+        assert np.array_equal(out, g)
+    assert picked >= 2
+
+
 def test_batch16_fp32_lod_within_bar(batch16):
     """FP32-rescaled LOD terms against the FP64 oracle at about 150 intervals
     spread over the map of the smallest, middle and largest pedigree, for two
