@@ -299,7 +299,11 @@ typedef struct {
  * 0, every chain's graph is scored into the LOD accumulators
  * (markov_chain.cc:366-389; several chains merge as -R does).  Returns
  * SL_E_ZERO_LIKELIHOOD or SL_E_ILLEGAL_GRAPH when a chain failed (its graph
- * stops changing at the failing step; sl_mcmc_status says which). */
+ * stops changing at the failing step; sl_mcmc_status says which), and
+ * SL_E_UNSUPPORTED before any sweep runs when the schedule draws a sampler
+ * the context lacks.  Runs of several chains and 8 or more sweeps replay a
+ * captured graph of each sweep's launches unless profiling is on (same
+ * results either way). */
 int sl_mcmc_run(sl_ctx* ctx, const sl_schedule* schedule, sl_mcmc_stats* stats);
 
 /* CODA trace (options -T/-P, markov_chain.cc:83-91, 362-370): with enable

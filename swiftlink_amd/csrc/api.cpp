@@ -1476,14 +1476,18 @@ int sl_msweep(sl_ctx* c, uint64_t seed, uint32_t sweep, const int32_t* order, in
 
 namespace {
 
-// An instantiated graph of one sweep's launches, destroyed with its owner.
+// An instantiated graph of one sweep's launches, destroyed with its owner
+// once the stream it was launched on has drained (an early error return may
+// leave replays in flight).
 struct CapturedSweep {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
     CapturedSweep() = default;
     CapturedSweep(const CapturedSweep&) = delete;
     CapturedSweep& operator=(const CapturedSweep&) = delete;
     ~CapturedSweep() {
+        if (exec && stream) (void)hipStreamSynchronize(stream);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
     }
@@ -1499,6 +1503,7 @@ int capture_sweep(sl_ctx* c, CapturedSweep* out, Fn body) {
     if (rc) return rc;
     if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
     SL_HIP(c, hipGraphInstantiate(&out->exec, out->graph, nullptr, nullptr, 0));
+    out->stream = c->stream;
     return SL_OK;
 }
 
