@@ -71,6 +71,32 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// value of the lane CTRL selects (DPP, within row mask ROWS), 0 where there
+// is no source
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_or_zero(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROWS, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// sum over the wavefront's lanes, for every lane: an inclusive DPP scan
+// (shifts within rows of 16, then each row's total added into the rows
+// above it), read from lane 63
+__device__ __forceinline__ double wave_sum(double x) {
+    x += dpp_or_zero<0x111, 0xf>(x);  // row_shr:1
+    x += dpp_or_zero<0x112, 0xf>(x);  // row_shr:2
+    x += dpp_or_zero<0x114, 0xf>(x);  // row_shr:4
+    x += dpp_or_zero<0x118, 0xf>(x);  // row_shr:8
+    x += dpp_or_zero<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+    x += dpp_or_zero<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), 63);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // A wave's LDS: column words, union-find words, per-node accumulators, edges
 // and founder-allele labels.
 struct WaveArea {
@@ -239,7 +265,7 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
         else ll += fmax(l0, l1) + log1p(exp(-fabs(l0 - l1)));
     }
     if (__ballot(conflict)) return -INFINITY;
-    for (int off = 32; off > 0; off >>= 1) ll += __shfl_xor(ll, off);
+    ll = wave_sum(ll);
     return ll;
 }
 
@@ -435,7 +461,7 @@ __device__ double fag_log_likelihood_r(const MsampArgs& A, const JobInputs<KC>& 
         else ll += fmax(l0, l1) + log1p(exp(-fabs(l0 - l1)));
     }
     if (__ballot(conflict)) return -INFINITY;
-    for (int off = 32; off > 0; off >>= 1) ll += __shfl_xor(ll, off);
+    ll = wave_sum(ll);
     return ll;
 }
 
