@@ -376,6 +376,10 @@ __device__ double fag_log_likelihood_r(const MsampArgs& A, const JobInputs<KC>& 
     // rounds of hooking and compression until a round hooks nothing (each
     // hooking round removes a component, so at most 2F rounds hook)
     uint32_t ux[KC], uy[KC];
+    // every node's compressed word (root | parity << 16), as the LDS holds it
+    uint32_t r[KN];
+#pragma unroll
+    for (int n = 0; n < KN; ++n) r[n] = (uint32_t)(64 * n + lane);
     bool converged = false;
     for (int round = 0; round <= nodes; ++round) {
         wave_sync();
@@ -403,11 +407,9 @@ __device__ double fag_log_likelihood_r(const MsampArgs& A, const JobInputs<KC>& 
             break;
         }
         wave_sync();
-        uint32_t r[KN];
 #pragma unroll
         for (int n = 0; n < KN; ++n) {
             const int x = 64 * n + lane;
-            r[n] = 0u;
             if (x < nodes) {
                 uint32_t p;
                 r[n] = uf_find(w.uf, (uint32_t)x, &p) | (p << 16);
@@ -432,25 +434,32 @@ __device__ double fag_log_likelihood_r(const MsampArgs& A, const JobInputs<KC>& 
         }
     }
     if (__ballot(illegal)) return -INFINITY;
-    uint32_t u[KN];
+    // touched marks of every chunk read at once, then each touched node
+    // counted at its root by parity
+    uint32_t t[KN];
 #pragma unroll
     for (int n = 0; n < KN; ++n) {
         const int x = 64 * n + lane;
-        u[n] = 0xFFFFFFFFu;
-        if (x < nodes) {
-            u[n] = w.uf[x];
-            if (w.acc[x] & 4u) atomicAdd(&w.acc[u[n] & 0xFFFFu], ((u[n] >> 16) & 1u) ? (1u << 14) : (1u << 3));
-        }
+        t[n] = 0u;
+        if (x < nodes) t[n] = w.acc[x];
     }
+#pragma unroll
+    for (int n = 0; n < KN; ++n)
+        if (t[n] & 4u) atomicAdd(&w.acc[r[n] & 0xFFFFu], ((r[n] >> 16) & 1u) ? (1u << 14) : (1u << 3));
     wave_sync();
+    uint32_t ar[KN];
+#pragma unroll
+    for (int n = 0; n < KN; ++n) {
+        const int x = 64 * n + lane;
+        ar[n] = 0u;
+        if (x < nodes && r[n] == (uint32_t)x) ar[n] = w.acc[x];  // roots only
+    }
     double ll = 0.0;
     bool conflict = false;
 #pragma unroll
     for (int n = 0; n < KN; ++n) {
-        const int x = 64 * n + lane;
-        if (x >= nodes || (u[n] & 0xFFFFu) != (uint32_t)x) continue;  // not a root
-        const uint32_t a = w.acc[x];
-        if (!(a & 4u)) continue;  // untouched
+        const uint32_t a = ar[n];
+        if (!(a & 4u)) continue;  // not a root, or untouched
         const double n0 = (double)((a >> 3) & 0x7FFu), n1 = (double)((a >> 14) & 0x7FFu);
         const double l0 = n0 * lmaj + n1 * lmin;
         const double l1 = n1 * lmaj + n0 * lmin;
