@@ -164,7 +164,7 @@ struct sl_ctx {
     uint2* d_ptab = nullptr;
     int m_ptab_bytes = 0;
     uint32_t* d_geno2 = nullptr;
-    double* d_minor = nullptr;
+    double* d_lfreq = nullptr;     // [M][2] ln(1 - minor), ln(minor): the founder-allele terms
     int32_t* d_meioses = nullptr;
     int32_t* d_order = nullptr;
     size_t order_cap = 0;
@@ -390,7 +390,7 @@ int release_all(sl_ctx* c) {
                     c->d_ssched_ops_w, c->d_rlevel_off,
                     c->d_rlevel_ops, c->d_mother, c->d_father, c->d_legal_t, c->d_gprob_d, c->d_gprob_f,
                     c->d_theta, c->d_chains, c->d_status,
-                    c->d_gen_off, c->d_ptab, c->d_geno2, c->d_minor, c->d_meioses, c->d_order,
+                    c->d_gen_off, c->d_ptab, c->d_geno2, c->d_lfreq, c->d_meioses, c->d_order,
                     c->d_fixed_order, c->d_lcur, c->d_lflip, c->d_fbuf, c->d_maps, c->d_near, c->d_near_loci,
                     c->d_ops, c->d_prevs, c->d_children, c->d_tasks, c->d_tinfo, c->d_idx, c->d_level_base,
                     c->d_sched_off,
@@ -491,11 +491,17 @@ int setup_msampler(sl_ctx* c, const sl_pedigree_desc* ped) {
     c->m_ubuf = M <= sl::kFbUbufLoci;
     if (c->m_ubuf && c->m_spec_lds) c->m_spec_lds = std::max(c->m_spec_lds, sizeof(double) * (size_t)M);
     SL_HIP(c, sl::configure_msampler_lds(c->m_lds, c->m_spec_lds));
+    // the founder-allele log terms of every locus, formed once here
+    std::vector<double> lfreq(2 * (size_t)M);
+    for (int l = 0; l < M; ++l) {
+        lfreq[2 * (size_t)l] = std::log(1.0 - c->minor[l]);
+        lfreq[2 * (size_t)l + 1] = std::log(c->minor[l]);
+    }
     int rc;
     if ((rc = upload(c, &c->d_gen_off, gen_off.data(), gen_off.size())) ||
         (rc = upload(c, &c->d_ptab, ptab.data(), ptab.size())) ||
         (rc = upload(c, &c->d_geno2, gt.data(), gt.size())) ||
-        (rc = upload(c, &c->d_minor, c->minor.data(), c->minor.size())) ||
+        (rc = upload(c, &c->d_lfreq, lfreq.data(), lfreq.size())) ||
         (rc = upload(c, &c->d_meioses, c->meioses_h.data(), c->meioses_h.size())))
         return rc;
     SL_HIP(c, hipStreamSynchronize(c->stream));  // the host tables above go out of scope
@@ -1345,7 +1351,7 @@ static sl::MsampArgs msamp_args(sl_ctx* c) {
     A.n_gen = c->n_gen;
     A.geno2 = c->d_geno2;
     A.GW = (c->N + 15) / 16;
-    A.minor = c->d_minor;
+    A.lfreq = c->d_lfreq;
     A.theta = c->d_theta;
     A.meioses = c->d_meioses;
     A.n_meioses = (int)c->meioses_h.size();

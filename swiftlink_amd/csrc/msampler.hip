@@ -248,8 +248,7 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
         }
     }
     wave_sync();
-    const double minor = A.minor[l];
-    const double lmaj = log(1.0 - minor), lmin = log(minor);
+    const double lmaj = A.lfreq[2 * (size_t)l], lmin = A.lfreq[2 * (size_t)l + 1];
     double ll = 0.0;
     bool conflict = false;
     for (int x = lane; x < nodes; x += 64) {
@@ -280,12 +279,12 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
 // summation are those of fag_log_likelihood, so the result is the same.
 // A job's global inputs for the register-resident likelihood, loaded at once
 // at its start: lane j's genotypes and person-table slots j, j+64, ..., and
-// the locus's minor allele frequency.
+// the locus's log allele-frequency terms.
 template <int KC>
 struct JobInputs {
     int g[KC];
     uint2 pt[KC];
-    double minor;
+    double lmaj, lmin;
     __device__ __forceinline__ void load(const MsampArgs& A, int l) {
         const int lane = lane_id();
 #pragma unroll
@@ -294,7 +293,8 @@ struct JobInputs {
             g[c] = i < A.N ? genotype_at(A, l, i) : 0;
             pt[c] = i < A.N - A.F ? A.ptab[i] : make_uint2(0u, 0u);
         }
-        minor = A.minor[l];
+        lmaj = A.lfreq[2 * (size_t)l];
+        lmin = A.lfreq[2 * (size_t)l + 1];
     }
 };
 
@@ -306,9 +306,7 @@ __device__ double fag_log_likelihood_r(const MsampArgs& A, const JobInputs<KC>& 
     int g[KC];
 #pragma unroll
     for (int c = 0; c < KC; ++c) g[c] = in.g[c];
-    const double minor = in.minor;
-    // the allele-frequency logs of the final sum, formed while the graph is built
-    const double lmaj = log(1.0 - minor), lmin = log(minor);
+    const double lmaj = in.lmaj, lmin = in.lmin;
 #pragma unroll
     for (int n = 0; n < KN; ++n) {
         const int x = 64 * n + lane;

@@ -27,7 +27,7 @@ struct MsampArgs {
     const uint32_t* geno2;        // [M][GW] marker genotypes, 2 bits per person (bits 2(i%16) of word
                                   // i/16; 0 untyped, 1 heterozygous, 2/3 homozygous A/B)
     int GW;                       // ceil(N/16)
-    const double* minor;          // [M] minor allele frequency
+    const double* lfreq;          // [M][2] ln(1 - minor allele frequency), ln(minor allele frequency)
     const double* theta;          // [M-1]
     const int32_t* meioses;       // [n_meioses] non-ignorable meioses (bit index 2(p-F)+parent)
     int n_meioses;                // meioses sampled by this sweep
