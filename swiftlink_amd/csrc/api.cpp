@@ -165,6 +165,7 @@ struct sl_ctx {
     int m_ptab_bytes = 0;
     uint32_t* d_geno2 = nullptr;
     double* d_lfreq = nullptr;     // [M][2] ln(1 - minor), ln(minor): the founder-allele terms
+    double* d_lsum = nullptr;      // [M][kLsumK] a free component's term by allele-count difference
     int32_t* d_meioses = nullptr;
     int32_t* d_order = nullptr;
     size_t order_cap = 0;
@@ -390,7 +391,7 @@ int release_all(sl_ctx* c) {
                     c->d_ssched_ops_w, c->d_rlevel_off,
                     c->d_rlevel_ops, c->d_mother, c->d_father, c->d_legal_t, c->d_gprob_d, c->d_gprob_f,
                     c->d_theta, c->d_chains, c->d_status,
-                    c->d_gen_off, c->d_ptab, c->d_geno2, c->d_lfreq, c->d_meioses, c->d_order,
+                    c->d_gen_off, c->d_ptab, c->d_geno2, c->d_lfreq, c->d_lsum, c->d_meioses, c->d_order,
                     c->d_fixed_order, c->d_lcur, c->d_lflip, c->d_fbuf, c->d_maps, c->d_near, c->d_near_loci,
                     c->d_ops, c->d_prevs, c->d_children, c->d_tasks, c->d_tinfo, c->d_idx, c->d_level_base,
                     c->d_sched_off,
@@ -491,17 +492,25 @@ int setup_msampler(sl_ctx* c, const sl_pedigree_desc* ped) {
     c->m_ubuf = M <= sl::kFbUbufLoci;
     if (c->m_ubuf && c->m_spec_lds) c->m_spec_lds = std::max(c->m_spec_lds, sizeof(double) * (size_t)M);
     SL_HIP(c, sl::configure_msampler_lds(c->m_lds, c->m_spec_lds));
-    // the founder-allele log terms of every locus, formed once here
-    std::vector<double> lfreq(2 * (size_t)M);
+    // the founder-allele log terms of every locus, formed once here: ln of
+    // each allele's frequency, and the term ln(e^l0 + e^l1) - max(l0, l1) =
+    // ln(1 + e^-|l0 - l1|) of a component free to take either assignment,
+    // where l0 - l1 = (n0 - n1)(ln(1 - q) - ln q) for its allele counts n0, n1
+    std::vector<double> lfreq(2 * (size_t)M), lsum((size_t)M * sl::kLsumK);
     for (int l = 0; l < M; ++l) {
-        lfreq[2 * (size_t)l] = std::log(1.0 - c->minor[l]);
-        lfreq[2 * (size_t)l + 1] = std::log(c->minor[l]);
+        const double lmaj = std::log(1.0 - c->minor[l]), lmin = std::log(c->minor[l]);
+        lfreq[2 * (size_t)l] = lmaj;
+        lfreq[2 * (size_t)l + 1] = lmin;
+        const double step = std::fabs(lmaj - lmin);
+        lsum[(size_t)l * sl::kLsumK] = std::log(2.0);
+        for (int k = 1; k < sl::kLsumK; ++k) lsum[(size_t)l * sl::kLsumK + k] = std::log1p(std::exp(-(double)k * step));
     }
     int rc;
     if ((rc = upload(c, &c->d_gen_off, gen_off.data(), gen_off.size())) ||
         (rc = upload(c, &c->d_ptab, ptab.data(), ptab.size())) ||
         (rc = upload(c, &c->d_geno2, gt.data(), gt.size())) ||
         (rc = upload(c, &c->d_lfreq, lfreq.data(), lfreq.size())) ||
+        (rc = upload(c, &c->d_lsum, lsum.data(), lsum.size())) ||
         (rc = upload(c, &c->d_meioses, c->meioses_h.data(), c->meioses_h.size())))
         return rc;
     SL_HIP(c, hipStreamSynchronize(c->stream));  // the host tables above go out of scope
@@ -1352,6 +1361,7 @@ static sl::MsampArgs msamp_args(sl_ctx* c) {
     A.geno2 = c->d_geno2;
     A.GW = (c->N + 15) / 16;
     A.lfreq = c->d_lfreq;
+    A.lsum = c->d_lsum;
     A.theta = c->d_theta;
     A.meioses = c->d_meioses;
     A.n_meioses = (int)c->meioses_h.size();

@@ -97,9 +97,10 @@ __device__ __forceinline__ double wave_sum(double x) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// A wave's LDS: column words, union-find words, per-node accumulators, edges
-// and founder-allele labels.
+// A wave's LDS: the locus's free-component terms, column words, union-find
+// words, per-node accumulators, edges and founder-allele labels.
 struct WaveArea {
+    double* lsum;     // [kLsumK] the locus's row of MsampArgs::lsum
     uint32_t* colw;   // [W]
     uint32_t* uf;     // [2F] parent (bits 0-15) | parity to parent (bit 16)
     uint32_t* acc;    // [2F] root allele A needed (bit 0), B needed (bit 1), touched (bit 2),
@@ -110,7 +111,8 @@ struct WaveArea {
 
 __device__ __forceinline__ WaveArea wave_area(const MsampArgs& A, unsigned char* base) {
     WaveArea w;
-    w.colw = reinterpret_cast<uint32_t*>(base);
+    w.lsum = reinterpret_cast<double*>(base);
+    w.colw = reinterpret_cast<uint32_t*>(base + sizeof(double) * kLsumK);
     w.uf = w.colw + A.W;
     w.acc = w.uf + 2 * A.F;
     w.edges = w.acc + 2 * A.F;
@@ -254,14 +256,15 @@ __device__ double fag_log_likelihood(const MsampArgs& A, const uint2* ptab, cons
     for (int x = lane; x < nodes; x += 64) {
         const uint32_t a = w.acc[x];
         if (!(a & 4u) || (w.uf[x] & 0xFFFFu) != (uint32_t)x) continue;  // untouched, or not a root
-        const double n0 = (double)((a >> 3) & 0x7FFu), n1 = (double)((a >> 14) & 0x7FFu);
+        const uint32_t c0 = (a >> 3) & 0x7FFu, c1 = (a >> 14) & 0x7FFu, kd = c0 > c1 ? c0 - c1 : c1 - c0;
+        const double n0 = (double)c0, n1 = (double)c1;
         const double l0 = n0 * lmaj + n1 * lmin;  // root allele A
         const double l1 = n1 * lmaj + n0 * lmin;  // root allele B
         const uint32_t need = a & 3u;
         if (need == 3u) conflict = true;
         else if (need == 1u) ll += l0;
         else if (need == 2u) ll += l1;
-        else ll += fmax(l0, l1) + log1p(exp(-fabs(l0 - l1)));
+        else ll += fmax(l0, l1) + (kd < (uint32_t)kLsumK ? w.lsum[kd] : log1p(exp(-fabs(l0 - l1))));
     }
     if (__ballot(conflict)) return -INFINITY;
     ll = wave_sum(ll);
@@ -458,14 +461,15 @@ __device__ double fag_log_likelihood_r(const MsampArgs& A, const JobInputs<KC>& 
     for (int n = 0; n < KN; ++n) {
         const uint32_t a = ar[n];
         if (!(a & 4u)) continue;  // not a root, or untouched
-        const double n0 = (double)((a >> 3) & 0x7FFu), n1 = (double)((a >> 14) & 0x7FFu);
+        const uint32_t c0 = (a >> 3) & 0x7FFu, c1 = (a >> 14) & 0x7FFu, kd = c0 > c1 ? c0 - c1 : c1 - c0;
+        const double n0 = (double)c0, n1 = (double)c1;
         const double l0 = n0 * lmaj + n1 * lmin;
         const double l1 = n1 * lmaj + n0 * lmin;
         const uint32_t need = a & 3u;
         if (need == 3u) conflict = true;
         else if (need == 1u) ll += l0;
         else if (need == 2u) ll += l1;
-        else ll += fmax(l0, l1) + log1p(exp(-fabs(l0 - l1)));
+        else ll += fmax(l0, l1) + (kd < (uint32_t)kLsumK ? w.lsum[kd] : log1p(exp(-fabs(l0 - l1))));
     }
     if (__ballot(conflict)) return -INFINITY;
     ll = wave_sum(ll);
@@ -637,12 +641,18 @@ __device__ void fag_job(const MsampArgs& A, int chain, int q, const uint2* ptab,
     const uint32_t* src = A.graphs + ((size_t)chain * M + l) * A.W;
     JobInputs<KC> in;
     if constexpr (KC > 0) in.load(A, l);
+    static_assert(kLsumK % 64 == 0, "the term row is copied 64 entries at a time");
+    double ls[kLsumK / 64];
+#pragma unroll
+    for (int j = 0; j < kLsumK / 64; ++j) ls[j] = A.lsum[(size_t)l * kLsumK + 64 * j + lane];
     for (int k = lane; k < A.W; k += 64) {
         uint32_t x = src[k];
         if (force >= 0 && (force >> 5) == k) x = (x & ~(1u << (force & 31))) | ((uint32_t)v << (force & 31));
         if (flip >= 0 && (flip >> 5) == k) x ^= 1u << (flip & 31);
         w.colw[k] = x;
     }
+#pragma unroll
+    for (int j = 0; j < kLsumK / 64; ++j) w.lsum[64 * j + lane] = ls[j];
     wave_sync();
     double L;
     if constexpr (KC > 0) L = fag_log_likelihood_r<KC>(A, in, w);
@@ -1223,7 +1233,8 @@ int msampler_people_chunks(int N, int F) {
 }
 
 size_t msampler_wave_bytes(int N, int F, int W) {
-    const size_t b = 4 * (size_t)W + 2 * 4 * (size_t)(2 * F) + 4 * (size_t)N + 2 * 2 * (size_t)(N - F);
+    const size_t b = sizeof(double) * kLsumK + 4 * (size_t)W + 2 * 4 * (size_t)(2 * F) + 4 * (size_t)N +
+                     2 * 2 * (size_t)(N - F);
     return (b + 15) / 16 * 16;
 }
 

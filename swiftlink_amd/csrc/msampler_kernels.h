@@ -28,6 +28,9 @@ struct MsampArgs {
                                   // i/16; 0 untyped, 1 heterozygous, 2/3 homozygous A/B)
     int GW;                       // ceil(N/16)
     const double* lfreq;          // [M][2] ln(1 - minor allele frequency), ln(minor allele frequency)
+    const double* lsum;           // [M][kLsumK] ln(1 + exp(-k |ln(1 - q) - ln q|)), q the minor allele
+                                  // frequency: the term of a component whose founder alleles may take
+                                  // either assignment, k the difference of its two allele counts
     const double* theta;          // [M-1]
     const int32_t* meioses;       // [n_meioses] non-ignorable meioses (bit index 2(p-F)+parent)
     int n_meioses;                // meioses sampled by this sweep
@@ -60,6 +63,8 @@ struct MsampArgs {
 
 // Loci up to which a forward-backward pass keeps the step's uniforms in LDS
 constexpr int kFbUbufLoci = 4096;
+// Entries per locus of MsampArgs::lsum (count differences past it are computed)
+constexpr int kLsumK = 64;
 
 bool msampler_supported(int N, int F);
 int msampler_people_chunks(int N, int F);  // register chunks for N people, F founders; 0 when too many
