@@ -310,14 +310,8 @@ __device__ double fag_log_likelihood_r(const MsampArgs& A, const JobInputs<KC>& 
 #pragma unroll
     for (int c = 0; c < KC; ++c) g[c] = in.g[c];
     const double lmaj = in.lmaj, lmin = in.lmin;
-#pragma unroll
-    for (int n = 0; n < KN; ++n) {
-        const int x = 64 * n + lane;
-        if (x < nodes) {
-            w.uf[x] = (uint32_t)x;
-            w.acc[x] = 0u;
-        }
-    }
+    // (the union-find words and accumulators were reset by fag_job while
+    // the job's loads were in flight)
     // label slots (reset, founder_allele_graph4.cc:543-569): the source of
     // each of a slot's two labels, a founder allele (bit 31 set) or the label
     // slot of the parent it came from
@@ -640,7 +634,19 @@ __device__ void fag_job(const MsampArgs& A, int chain, int q, const uint2* ptab,
     WaveArea w = wave_area(A, wbase);
     const uint32_t* src = A.graphs + ((size_t)chain * M + l) * A.W;
     JobInputs<KC> in;
-    if constexpr (KC > 0) in.load(A, l);
+    if constexpr (KC > 0) {
+        in.load(A, l);
+        // every node its own root with nothing counted, written while the
+        // loads above are in flight
+#pragma unroll
+        for (int n = 0; n < KC; ++n) {
+            const int x = 64 * n + lane;
+            if (x < 2 * A.F) {
+                w.uf[x] = (uint32_t)x;
+                w.acc[x] = 0u;
+            }
+        }
+    }
     static_assert(kLsumK % 64 == 0, "the term row is copied 64 entries at a time");
     double ls[kLsumK / 64];
 #pragma unroll
