@@ -633,6 +633,8 @@ __device__ void fag_job(const MsampArgs& A, int chain, int q, const uint2* ptab,
     }
     WaveArea w = wave_area(A, wbase);
     const uint32_t* src = A.graphs + ((size_t)chain * M + l) * A.W;
+    // where the likelihood goes, formed before the work that produces it
+    double* const dst = current ? A.lcur + (size_t)chain * M + l : A.lflip + lflip_index(A, A.step, v, chain, l);
     JobInputs<KC> in;
     if constexpr (KC > 0) {
         in.load(A, l);
@@ -665,12 +667,8 @@ __device__ void fag_job(const MsampArgs& A, int chain, int q, const uint2* ptab,
     else L = fag_log_likelihood(A, ptab, w, l);
     if (status != 0) return;
     if (lane == 0) {
-        if (current) {
-            A.lcur[(size_t)chain * M + l] = L;
-            if (L == -INFINITY) A.status[chain] = kIllegalGraph;
-        } else {
-            A.lflip[lflip_index(A, A.step, v, chain, l)] = L;
-        }
+        *dst = L;
+        if (current && L == -INFINITY) A.status[chain] = kIllegalGraph;
     }
 }
 
