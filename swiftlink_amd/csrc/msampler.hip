@@ -1137,11 +1137,24 @@ __global__ void __launch_bounds__(kSpecBlock) mspec_kernel(MsampArgs A) {
 
 // this sweep's meiosis order: the given order, or a Fisher-Yates shuffle of the
 // non-ignorable meioses (random_shuffle, markov_chain.cc:346) driven by the
-// chain's draws at step kStepShuffle.  Position i (from the top) swaps with
-// floor(u_d (i+1)), d = n-1-i; the uniforms are drawn in parallel, the swaps
-// done in order.  One workgroup per chain; dynamic LDS holds n swap targets.
-__global__ void morder_kernel(MsampArgs A, const int32_t* fixed_order) {
-    extern __shared__ int32_t jdx[];
+// chain's draws at step kStepShuffle.  Position i (from the top, i = n-1 .. 1)
+// swaps with floor(u_d (i+1)), d = n-1-i.  The swap partners are drawn in
+// parallel into LDS.  Up to kOrderChunk meioses, every output position follows
+// its element back through the swaps (last swap first), all positions at once:
+// n^2 compares instead of a chain of n dependent memory round trips.  Larger
+// orders swap in global memory one after another, kOrderChunk partners per pass.
+// One workgroup per chain.
+constexpr int kOrderBlock = 1024;
+constexpr int kOrderChunk = 1024;
+
+__device__ __forceinline__ int shuffle_partner(const MsampArgs& A, int chain, int n, int i) {
+    const double u = philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), kStepShuffle, (uint32_t)(n - 1 - i));
+    const int j = (int)(u * (double)(i + 1));
+    return j > i ? i : j;
+}
+
+__global__ void __launch_bounds__(kOrderBlock) morder_kernel(MsampArgs A, const int32_t* fixed_order) {
+    __shared__ int32_t jdx[kOrderChunk];
     __shared__ int s_skip;
     const int chain = blockIdx.x;
     if (threadIdx.x == 0) s_skip = A.status[chain] != 0 || !is_msweep(A, chain);
@@ -1153,21 +1166,34 @@ __global__ void morder_kernel(MsampArgs A, const int32_t* fixed_order) {
         for (int i = threadIdx.x; i < n; i += blockDim.x) o[i] = fixed_order[i];
         return;
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        o[i] = A.meioses[i];
-        if (i > 0) {
-            const double u = philox_uniform(A.seed, (uint32_t)chain, sweep_of(A), kStepShuffle, (uint32_t)(n - 1 - i));
-            const int j = (int)(u * (double)(i + 1));
-            jdx[i] = j > i ? i : j;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = n - 1; i > 0; --i) {
+    if (n <= kOrderChunk) {
+        for (int i = threadIdx.x + 1; i < n; i += blockDim.x) jdx[i] = shuffle_partner(A, chain, n, i);
+        __syncthreads();
+        const int q = threadIdx.x;
+        if (q >= n) return;
+        // swaps apply for i = n-1 down to 1; undo them from i = 1 up
+        int p = q;
+#pragma unroll 8
+        for (int i = 1; i < n; ++i) {
             const int j = jdx[i];
-            const int32_t t = o[i];
-            o[i] = o[j];
-            o[j] = t;
+            p = p == i ? j : p == j ? i : p;
+        }
+        o[q] = A.meioses[p];
+        return;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) o[i] = A.meioses[i];
+    for (int hi = n - 1; hi > 0; hi -= kOrderChunk) {
+        const int lo = hi - kOrderChunk + 1 > 1 ? hi - kOrderChunk + 1 : 1;
+        __syncthreads();  // o written; the previous pass's partners consumed
+        for (int i = lo + (int)threadIdx.x; i <= hi; i += blockDim.x) jdx[i - lo] = shuffle_partner(A, chain, n, i);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = hi; i >= lo; --i) {
+                const int j = jdx[i - lo];
+                const int32_t t = o[i];
+                o[i] = o[j];
+                o[j] = t;
+            }
         }
     }
 }
@@ -1319,8 +1345,7 @@ hipError_t configure_msampler_lds(size_t job_lds, size_t spec_lds) {
 }
 
 hipError_t launch_morder(const MsampArgs& A, const int32_t* fixed_order, hipStream_t s) {
-    hipLaunchKernelGGL(morder_kernel, dim3(A.n_chains), dim3(256), sizeof(int32_t) * (size_t)A.n_meioses, s, A,
-                       fixed_order);
+    hipLaunchKernelGGL(morder_kernel, dim3(A.n_chains), dim3(kOrderBlock), 0, s, A, fixed_order);
     return hipGetLastError();
 }
 

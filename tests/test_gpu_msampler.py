@@ -243,6 +243,33 @@ def test_msweep_long_map_matches_oracle(long_map, mode):
         assert np.array_equal(out[c], o.msweep(starts[c], shuffled_order(o, 606, c, 2), 606, c, 2)), c
 
 
+@pytest.fixture(scope="module")
+def ped1100(tmp_path_factory):
+    """1100 people (301 founders) over 6 SNPs: 1541 non-ignorable meioses, more
+    than one pass of the shuffle's swap partners (1024 per pass), and
+    founder-allele graphs in LDS (over 1024 people)."""
+    d = tmp_path_factory.mktemp("ped1100")
+    case, data = synth.make_case("ped600_snp20000", str(d), n_people=1100, n_markers=6, n_generations=10,
+                                 n_loops=6)
+    return case, synth.true_graph(data, case), Oracle(case)
+
+
+def test_msweep_shuffle_over_1024_meioses(ped1100):
+    case, g0, o = ped1100
+    assert len(o.meioses()) > 1024
+    starts = np.stack([g0, g0])
+    print(f"ped1100: N={case.N} F={case.F} meioses={len(o.meioses())}", flush=True)
+    with native.Context(case, native.SL_FP64) as ctx:
+        ctx.mcmc_init(starts)
+        ctx.msweep(99, 3)
+        out = ctx.mcmc_graphs()
+        assert np.all(ctx.mcmc_status() == 0)
+        near = ctx.mcmc_near_edge()
+    assert np.all(near == 0)
+    for c in range(2):
+        assert np.array_equal(out[c], o.msweep(starts[c], shuffled_order(o, 99, c, 3), 99, c, 3)), c
+
+
 def _fp32_vs_fp64_steps(case, g0, o, mode, steps, seed):
     """One M-step at a time (sl_msweep with a one-meiosis order) in an FP64 and
     an FP32-rescaled context from the same graph: the FP32 forward-backward
