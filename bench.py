@@ -23,6 +23,17 @@ on one core.  `sequential_imputation` times 1000 replicates of the chains'
 starting-graph search on the bench pedigree, and `elod` 262140 ELOD
 replicates of it under the default ELOD model, each beside the reference's
 own code on one core.
+
+`python bench.py --large` instead runs the large combined workload
+ped600_snp20000 (600 people over 8 generations with 10 consanguineous loops,
+20000 SNPs at 0.1 cM, 20% missing genotypes, 5 positions per interval): 256
+replicate chains from the gene-drop graph, L-sampler:M-sampler 1:1, a scored
+graph every 100th sweep.  It times sweeps of all 256 chains together and one
+scoring pass over 256 resident graphs, and reports sweeps/s over the 256
+chains with the scoring's share of 1/100, LOD evals/s in FP32-rescaled and
+FP64, and over every position of 4 graphs the non-finite LODs of each (the
+unscaled FP64 peel underflows at this size) and the largest |dLOD| where both
+are finite.
 """
 from __future__ import annotations
 
@@ -47,6 +58,7 @@ LPROB, PERIOD, SEED = 0.5, 10, 20250101
 ADJ_CONFIG = "ped200_snp2000"  # the differentiation workload: 200 people, 7 generations, 4 loops, 2000 SNPs
 SI_REPS = 1000          # sequential-imputation replicates (the reference's -s default)
 ELOD_REPS = 4 * 65535   # ELOD replicates (four launches' worth)
+LARGE_CONFIG, LARGE_CHAINS, LARGE_PERIOD = "ped600_snp20000", 256, 100  # the large combined workload
 
 
 def profile_traffic(kernel: str, workload: str):
@@ -165,11 +177,117 @@ def cpu_adjoint_baseline(case, graph, jac, positions: int):
                       f"gradient, 1 core, {dt:.1f} s"}
 
 
+def large_workload(args):
+    """The large combined workload: LARGE_CHAINS chains of LARGE_CONFIG
+    through the sweep schedule on one device, one sweep per call so that each
+    prints its time, then the trait peel over resident graphs."""
+    import torch
+    from swiftlink_amd import native, synth
+    from swiftlink_amd.case import pack_graphs
+
+    def note(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    case, data = synth.make_case(LARGE_CONFIG, tempfile.mkdtemp(prefix="sllarge_"))
+    g0 = synth.true_graph(data, case)
+    note(f"{LARGE_CONFIG}: {case.N} people ({case.F} founders), {case.M} SNPs, made in "
+         f"{time.perf_counter() - t0:.1f} s")
+    C = LARGE_CHAINS
+    sched = (0, LARGE_PERIOD, LPROB, SEED)
+    with native.Context(case, native.SL_FP32_RESCALED) as ctx:
+        init = np.repeat(g0[None], C, axis=0)  # the reference layout: C x 96 MB at this size
+        ctx.mcmc_init(init)
+        del init
+        # sweep 0 is a scored one (0 mod the period): the warm-up sweeps start there
+        for s in range(args.warmup):
+            t0 = time.perf_counter()
+            ctx.mcmc_run(1, *sched, first_sweep=s)
+            ctx.sync()
+            note(f"warm-up sweep {s}: {time.perf_counter() - t0:.2f} s")
+        kinds = {"l_sweeps": 0, "m_sweeps": 0, "scored": 0}
+        times = []
+        for s in range(args.warmup, args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = ctx.mcmc_run(1, *sched, first_sweep=s)
+            ctx.sync()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+            for k in kinds:
+                kinds[k] += st[k]
+            note(f"sweep {s}: {times[-1]:.2f} s, L-sweep chains {st['l_sweeps']}, M-sweep chains {st['m_sweeps']}")
+        near = int(ctx.mcmc_near_edge().sum())
+        ok = bool(np.all(ctx.mcmc_status() == 0))
+
+    # the scoring of a graph of every chain: the trait peel over C resident graphs
+    packed = np.repeat(pack_graphs(case, g0[None]), C, axis=0)  # packed once, then tiled
+    packed = torch.from_numpy(packed.reshape(-1).view(np.int32)).cuda()
+    torch.cuda.synchronize()
+    evals = C * (case.M - 1) * case.nlod
+
+    def lod(precision, graphs, launches):
+        with native.Context(case, precision) as lctx:
+            lctx.lod_accumulate_packed(packed.data_ptr(), graphs)
+            lctx.sync()
+            lctx.lod_reset()
+            t0 = time.perf_counter()
+            for _ in range(launches):
+                lctx.lod_accumulate_packed(packed.data_ptr(), graphs)
+            lctx.sync()
+            dt = (time.perf_counter() - t0) / launches
+            _, _, lodv = lctx.lod_results()
+        return dt, lodv
+
+    score_s, _ = lod(native.SL_FP32_RESCALED, C, 2)
+    note(f"scoring {C} graphs (FP32): {score_s:.2f} s")
+    DG = 4
+    d32, lod32 = lod(native.SL_FP32_RESCALED, DG, 1)
+    d64, lod64 = lod(native.SL_FP64, DG, 1)
+    note(f"{DG} graphs: FP32 {d32:.2f} s, FP64 {d64:.2f} s")
+    ok32, ok64 = np.isfinite(lod32), np.isfinite(lod64)
+    both = ok32 & ok64
+    sweep_s = float(np.mean(times))
+    per_sweep = sweep_s + score_s / LARGE_PERIOD
+    res = {
+        "metric": METRIC,
+        "value": C / per_sweep,
+        "unit": "sweeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": per_sweep * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded pedigree and gene-drop genotypes; every chain starts from the gene-drop graph)",
+        "config": {"workload": f"{LARGE_CONFIG}: {case.N} people ({case.F} founders), {case.M} SNPs, "
+                               f"{case.nlod} positions/interval; {C} chains, L:M 1:1, scored every "
+                               f"{LARGE_PERIOD}th sweep",
+                   "chains": C, "lprob": LPROB, "period": LARGE_PERIOD},
+        "sweeps": dict(kinds, near_edge_draws=near, chains_legal=ok),
+        "sweep_s": sweep_s,
+        "score_s": score_s,
+        "lod_evals_per_s": evals / score_s,
+        "fp64_lod_evals_per_s": DG * (case.M - 1) * case.nlod / d64,
+        "lod_check": {
+            "graphs": DG, "positions": int(lod32.size),
+            "fp32_nonfinite": int(np.count_nonzero(~ok32)), "fp64_nonfinite": int(np.count_nonzero(~ok64)),
+            "max_abs_dlod_where_both_finite": float(np.abs(lod32[both] - lod64[both]).max()) if both.any() else None,
+            "note": "the unscaled FP64 peel underflows at this size (ln L below about -745) as the reference's "
+                    "does, and loses bits in subnormals just above it; the FP32-rescaled peel stays finite"},
+        "note": f"value = {C} / (mean sweep time of all {C} chains + one scoring pass over {C} graphs / "
+                f"{LARGE_PERIOD})",
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200, help="timed sweeps")
-    ap.add_argument("--warmup", type=int, default=20, help="untimed sweeps")
+    ap.add_argument("--steps", type=int, default=None, help="timed sweeps (default 200; 2 with --large)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed sweeps (default 20; 1 with --large)")
     ap.add_argument("--batch", type=int, default=64, help="descent graphs per LOD-throughput launch")
     ap.add_argument("--config", default="ped150_snp1000")
     ap.add_argument("--no-cpu", action="store_true")
@@ -177,7 +295,13 @@ def main():
                     help="skip the 16-pedigree batch and the adjoint workloads (profiling passes)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (reference for HBM byte counters)")
+    ap.add_argument("--large", action="store_true",
+                    help=f"only the large combined workload ({LARGE_CONFIG}, {LARGE_CHAINS} chains)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 2 if args.large else 200
+    if args.warmup is None:
+        args.warmup = 1 if args.large else 20
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
@@ -192,6 +316,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": 4 * n, "launches": 5}))
+        return
+    if args.large:
+        large_workload(args)
         return
     from swiftlink_amd import native, synth
     from swiftlink_amd.case import pack_graphs
