@@ -335,16 +335,24 @@ __device__ double fag_log_likelihood_r(const MsampArgs& A, const JobInputs<KC>& 
             src1[c] = (fa < F ? (0x80000000u | (uint32_t)(2 * fa)) : (uint32_t)(2 * (fa - F))) | bp;
         }
     }
-    for (int gen = 0; gen < A.n_gen; ++gen) {
+    // the slots are in generation order and a parent's generation is below
+    // its child's, so a chunk's slots only read slots of earlier chunks or of
+    // earlier generations of their own chunk: each chunk runs just the
+    // generations it spans (its first and last slot's)
 #pragma unroll
-        for (int c = 0; c < KC; ++c) {
-            if (sgen[c] == gen) {
-                const uint32_t l0 = (src0[c] >> 31) ? (src0[c] & 0xFFFFu) : (uint32_t)w.lab[src0[c]];
-                const uint32_t l1 = (src1[c] >> 31) ? (src1[c] & 0xFFFFu) : (uint32_t)w.lab[src1[c]];
-                *reinterpret_cast<uint32_t*>(w.lab + sb[c]) = l0 | (l1 << 16);
+    for (int c = 0; c < KC; ++c) {
+        if (64 * c < NF) {
+            const int last = NF - 64 * c > 64 ? 63 : NF - 1 - 64 * c;
+            const int g1 = __builtin_amdgcn_readlane(sgen[c], last);
+            for (int gen = __builtin_amdgcn_readlane(sgen[c], 0); gen <= g1; ++gen) {
+                if (sgen[c] == gen) {
+                    const uint32_t l0 = (src0[c] >> 31) ? (src0[c] & 0xFFFFu) : (uint32_t)w.lab[src0[c]];
+                    const uint32_t l1 = (src1[c] >> 31) ? (src1[c] & 0xFFFFu) : (uint32_t)w.lab[src1[c]];
+                    *reinterpret_cast<uint32_t*>(w.lab + sb[c]) = l0 | (l1 << 16);
+                }
+                wave_sync();
             }
         }
-        wave_sync();
     }
     // edges of the typed people; the founder alleles they touch are marked
     uint32_t e[KC];
